@@ -1,0 +1,222 @@
+/*
+ * pyroffi_amd — C ABI of the MI355X (gfx950) batched-kinematics kernels.
+ *
+ * Every entry point takes a HIP stream (as `void*`, i.e. a `hipStream_t`),
+ * device pointers that the caller owns, and plain sizes.  Nothing here
+ * allocates, frees or synchronises: outputs are preallocated by the caller,
+ * work is enqueued on `stream`, and the call returns as soon as the launch is
+ * queued.  This is the ownership model of an XLA FFI custom call (XLA owns all
+ * buffers and the stream), so each function below can sit behind a ~20-line
+ * FFI handler that forwards `Buffer::typed_data()` pointers and dimensions.
+ *
+ * Conventions
+ *   - poses are float32 [w, x, y, z, tx, ty, tz] ("wxyz_xyz", Hamilton
+ *     quaternion then translation);
+ *   - twists are float32 [vx, vy, vz, wx, wy, wz] (linear part first);
+ *   - index tables are int32; -1 means "none" (root parent, fixed joint ...);
+ *   - signed distances are positive when separated, negative in penetration.
+ *
+ * Return value: PK_OK, or PK_ERR_INVALID / PK_ERR_HIP with a message that
+ * pk_last_error() returns (thread-local).  An empty batch is a successful
+ * no-op, as in the reference handlers.
+ *
+ * Robot tables ("the 8 robot buffers") are the reference's JointInfo arrays,
+ * cast to float32/int32 (src/pyroffi/_robot_urdf_parser.py:15-68):
+ *   twists (J,6), parent_tf (J,7), parent_idx (J), act_idx (J),
+ *   mimic_mul (J), mimic_off (J), mimic_act_idx (J), topo_inv (J).
+ */
+#ifndef PYROFFI_AMD_H_
+#define PYROFFI_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PK_OK 0
+#define PK_ERR_INVALID 1
+#define PK_ERR_HIP 2
+
+#define PK_MAX_JOINTS 64
+#define PK_MAX_ACT 16
+#define PK_MAX_EE 4
+
+/* Message of the last failing call on this thread ("" when none). */
+const char* pk_last_error(void);
+
+/* ABI version; bumped whenever a signature below changes. */
+int32_t pk_abi_version(void);
+
+/*
+ * Forward kinematics: world pose of every joint.
+ *
+ * Replaces the XLA FFI target FkCudaFfi
+ * (src/pyroffi/cuda_kernels/_fk_cuda_kernel.cu:456-471, bound in
+ * src/pyroffi/cuda_kernels/_fk_cuda.py:109-124) with the same buffers in the
+ * same order.  fk_level_starts/fk_level_joints are accepted for signature
+ * parity and not read: one lane evaluates one configuration's whole chain in
+ * topological order (topo_inv), so the depth schedule is not needed.
+ *
+ *   cfg        (batch, n_act)        float32
+ *   out        (batch, n_joints, 7)  float32, joint order of the tables
+ * Errors: n_joints > PK_MAX_JOINTS or n_act > PK_MAX_ACT -> PK_ERR_INVALID.
+ */
+int pk_fk_f32(void* stream, const float* cfg, int64_t batch, int32_t n_act,
+              const float* twists, const float* parent_tf,
+              const int32_t* parent_idx, const int32_t* act_idx,
+              const float* mimic_mul, const float* mimic_off,
+              const int32_t* mimic_act_idx, const int32_t* topo_inv,
+              const int32_t* fk_level_starts, const int32_t* fk_level_joints,
+              int32_t n_joints, int32_t n_levels, float* out);
+
+/*
+ * Forward kinematics straight to link poses: Robot.forward_kinematics
+ * (src/pyroffi/_robot.py:65-109) = FkCudaFfi followed by the link gather of
+ * _link_poses_from_joint_poses (src/pyroffi/_robot.py:180-196), fused so the
+ * joint poses never reach HBM.  link_parent_joint (n_links) int32 is
+ * LinkInfo.parent_joint_indices; links whose entry is -1 get the identity.
+ *
+ *   out_links  (batch, n_links, 7)   float32
+ */
+int pk_fk_links_f32(void* stream, const float* cfg, int64_t batch, int32_t n_act,
+                    const float* twists, const float* parent_tf,
+                    const int32_t* parent_idx, const int32_t* act_idx,
+                    const float* mimic_mul, const float* mimic_off,
+                    const int32_t* mimic_act_idx, const int32_t* topo_inv,
+                    int32_t n_joints, const int32_t* link_parent_joint,
+                    int32_t n_links, float* out_links);
+
+/*
+ * World collision, sphere robot vs sphere world, reduced over each link's S
+ * spheres: out[b, n, m] = min_s (|c[b, s*N+n] - o_m| - r[b, s*N+n] - R_m).
+ * Sphere slots with r < 0 are padding and skipped; a link with no real sphere
+ * reports 1e9.
+ *
+ * Replaces CollisionWorldSphereReducedFfi
+ * (src/pyroffi/cuda_kernels/_collision_cuda_kernel.cu:673-839, bound in
+ * src/pyroffi/cuda_kernels/_collision_cuda_ffi.py:109-151).  Only sphere
+ * worlds are supported: n_world_capsules, n_world_boxes and
+ * n_world_halfspaces must be 0 (PK_ERR_INVALID otherwise).
+ *
+ *   sphere_centers (3, batch, k) float32 SoA,  k = S * n_links
+ *   sphere_radii   (batch, k)    float32
+ *   world_spheres  (Ms, 4)       float32 [x, y, z, r]
+ *   out            (batch, n_links, Ms) float32
+ */
+int pk_collision_world_sphere_reduced_f32(
+    void* stream, const float* sphere_centers, const float* sphere_radii,
+    int64_t batch, int32_t k, const float* world_spheres, int32_t n_world_spheres,
+    const float* world_capsules, int32_t n_world_capsules,
+    const float* world_boxes, int32_t n_world_boxes,
+    const float* world_halfspaces, int32_t n_world_halfspaces,
+    int32_t n_links, float* out);
+
+/*
+ * Self collision between link pairs: out[b, p] = min over all sphere pairs
+ * (si of link pair_i[p], sj of link pair_j[p]) of the sphere distance, padding
+ * (r < 0) skipped, 1e9 when either link has no real sphere.
+ *
+ * Replaces CollisionSelfSphereFfi
+ * (src/pyroffi/cuda_kernels/_collision_cuda_kernel.cu:1006-1110, bound in
+ * src/pyroffi/cuda_kernels/_collision_cuda_ffi.py:191-216).
+ *
+ *   centers (batch, S, N, 3) float32,  radii (batch, S, N) float32
+ *   pair_i, pair_j (n_pairs) int32,    out (batch, n_pairs) float32
+ */
+int pk_collision_self_sphere_f32(void* stream, const float* centers,
+                                 const float* radii, int64_t batch, int32_t s,
+                                 int32_t n_links, const int32_t* pair_i,
+                                 const int32_t* pair_j, int32_t n_pairs,
+                                 float* out);
+
+/*
+ * Fused sphere sweep from link poses (no reference counterpart as one call:
+ * it fuses what CUDARobotCollisionChecker does in three steps -- the vmapped
+ * sphere transform, AoS->SoA transposes and the two distance kernels
+ * (src/pyroffi/collision/_cuda_collision.py:358-540) -- for the spherized
+ * model of src/pyroffi/collision/_robot_collision.py:445-549).
+ *
+ *   link_poses        (batch, n_links, 7) float32   (pk_fk_links_f32 output)
+ *   spheres_local     (n_spheres, 4) float32, [x, y, z, r] in the frame of
+ *                     their link, grouped by link (padding removed)
+ *   link_sphere_start (n_links + 1) int32: spheres of link n are
+ *                     [link_sphere_start[n], link_sphere_start[n+1])
+ *   world_spheres     (Ms, 4) float32
+ *   self_entries      (n_entries) uint32: one per sphere pair to test,
+ *                     (a | b << 10 | p << 20): sphere indices a, b < 1024
+ *                     into spheres_local and active pair p < 4096
+ *   outputs (any may be NULL):
+ *     out_world       (batch, n_links, Ms)  per-link min over its spheres
+ *                     (1e9 for links without spheres)  == compute_world_collision_distance
+ *     out_world_min   (batch)               min over all links and obstacles
+ *     out_self        (batch, n_pairs)      == compute_self_collision_distance
+ *                     (full S_i x S_j minimum, the reference CUDA semantics)
+ */
+int pk_sphere_sweep_f32(void* stream, const float* link_poses, int64_t batch,
+                        int32_t n_links, const float* spheres_local,
+                        const int32_t* link_sphere_start, int32_t n_spheres,
+                        const float* world_spheres, int32_t n_world_spheres,
+                        const uint32_t* self_entries, int32_t n_entries,
+                        int32_t n_pairs, float* out_world,
+                        float* out_world_min, float* out_self);
+
+/*
+ * Multi-seed Levenberg-Marquardt IK, one seed per lane.
+ *
+ * Replaces LsIkCudaFfi (src/pyroffi/cuda_kernels/_ls_ik_cuda_kernel.cu:400-529,
+ * bound in src/pyroffi/cuda_kernels/_ls_ik_cuda.py:152-182): same buffers,
+ * attributes and outputs, same algorithm (weighted geometric-Jacobian LM with
+ * Jacobi column scaling, norm-clipped trust region, 5-alpha line search,
+ * all-time-best tracking, merit-only collision penalty).  The damped normal
+ * equations are solved in FP32 (the reference uses FP64).  Only sphere worlds
+ * are supported (other world counts must be 0).
+ *
+ *   seeds        (n_problems, n_seeds, n_act)
+ *   target_jnts  (n_ee) int32, ancestor_masks (n_ee, n_joints) int32
+ *   target_T     (n_problems, n_ee, 7)
+ *   robot_spheres_local (n_robot_spheres, 4) in joint frames,
+ *   robot_sphere_joint_idx (n_robot_spheres) int32
+ *   lower, upper (n_act) float32, fixed_mask (n_act) int32
+ *   out_cfgs (n_problems, n_seeds, n_act), out_err (n_problems, n_seeds)
+ */
+int pk_ls_ik_f32(void* stream, const float* seeds, int32_t n_problems,
+                 int32_t n_seeds, int32_t n_act, const float* twists,
+                 const float* parent_tf, const int32_t* parent_idx,
+                 const int32_t* act_idx, const float* mimic_mul,
+                 const float* mimic_off, const int32_t* mimic_act_idx,
+                 const int32_t* topo_inv, int32_t n_joints,
+                 const int32_t* target_jnts, const int32_t* ancestor_masks,
+                 int32_t n_ee, const float* target_T,
+                 const float* robot_spheres_local,
+                 const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres,
+                 const float* world_spheres, int32_t n_world_spheres,
+                 const float* world_capsules, int32_t n_world_capsules,
+                 const float* world_boxes, int32_t n_world_boxes,
+                 const float* world_halfspaces, int32_t n_world_halfspaces,
+                 const float* lower, const float* upper, const int32_t* fixed_mask,
+                 int32_t max_iter, float pos_weight, float ori_weight,
+                 float lambda_init, float eps_pos, float eps_ori,
+                 int32_t enable_collision, float collision_weight,
+                 float collision_margin, float* out_cfgs, float* out_err);
+
+/*
+ * Winner selection after pk_ls_ik_f32: per problem, the seed minimising
+ * err + continuity_weight * |cfg - previous|^2 (first index on ties).
+ * Replaces the argmin of _ls_ik_solve_cuda_batch_jit
+ * (src/pyroffi/optimization_engines/_ls_ik.py:974-999).
+ *
+ *   cfgs (n_problems, n_seeds, n_act), errs (n_problems, n_seeds),
+ *   previous (n_problems, n_act)
+ *   out_best (n_problems, n_act), out_idx (n_problems) int32 or NULL
+ */
+int pk_ls_ik_select_f32(void* stream, const float* cfgs, const float* errs,
+                        const float* previous, int32_t n_problems,
+                        int32_t n_seeds, int32_t n_act, float continuity_weight,
+                        float* out_best, int32_t* out_idx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYROFFI_AMD_H_ */

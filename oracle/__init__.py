@@ -1,0 +1,206 @@
+"""FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+A double-precision restatement of the reference algorithms (``oracle.c``,
+each function citing the reference file:line it follows), used as the checker
+for the HIP kernels.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package; the
+``pyroffi_amd`` package never does.
+
+Parity pins (what independently checks each function) are listed in
+``oracle/README.md``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_d = ctypes.POINTER(ctypes.c_double)
+_f = ctypes.POINTER(ctypes.c_float)
+_i = ctypes.POINTER(ctypes.c_int32)
+_I = ctypes.c_int
+_L = ctypes.c_int64
+_D = ctypes.c_double
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        h = ctypes.CDLL(LIB)
+        h.or_fk.argtypes = [_d, _L, _I, _f, _f, _i, _i, _f, _f, _i, _i, _I, _d, _I]
+        h.or_link_poses.argtypes = [_d, _L, _I, _i, _I, _d]
+        h.or_world_dist.argtypes = [_d, _L, _I, _d, _I, _d, _I, _I, _d, _d, _I]
+        h.or_self_dist.argtypes = [_d, _L, _I, _d, _I, _i, _i, _I, _I, _d, _I]
+        h.or_ls_ik.argtypes = (
+            [_d, _I, _I, _I, _f, _f, _i, _i, _f, _f, _i, _i, _I, _i, _i, _I, _d, _f, _i, _I, _f, _I,
+             _d, _d, _i, _I, _D, _D, _D, _D, _D, _I, _D, _D, _d, _d, _I]
+        )
+        h.or_res_jac.argtypes = [_d, _I, _f, _f, _i, _i, _f, _f, _i, _i, _I, _i, _i, _I, _d, _d, _d]
+        h.or_lm_direction.argtypes = [_d, _d, _I, _I, _d, _D, _i, _d]
+        h.or_lm_direction.restype = ctypes.c_int
+        for fn in (h.or_fk, h.or_link_poses, h.or_world_dist, h.or_self_dist, h.or_ls_ik, h.or_res_jac):
+            fn.restype = None
+        _lib = h
+    return _lib
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ct)
+
+
+class Tables:
+    """float32/int32 robot tables exactly as the kernels receive them."""
+
+    def __init__(self, joints, links):
+        self.twists = _c(joints.twists, np.float32)
+        self.parent_tf = _c(joints.parent_transforms, np.float32)
+        self.parent_idx = _c(joints.parent_indices, np.int32)
+        self.act_idx = _c(joints.actuated_indices, np.int32)
+        self.mimic_mul = _c(joints.mimic_multiplier, np.float32)
+        self.mimic_off = _c(joints.mimic_offset, np.float32)
+        self.mimic_act_idx = _c(joints.mimic_act_indices, np.int32)
+        self.topo_inv = _c(joints._topo_sort_inv, np.int32)
+        self.link_parent_joint = _c(links.parent_joint_indices, np.int32)
+        self.J = int(joints.num_joints)
+        self.A = int(joints.num_actuated_joints)
+        self.L = int(links.num_links)
+
+    @staticmethod
+    def of(robot) -> "Tables":
+        return Tables(robot.joints, robot.links)
+
+    def ptrs(self):
+        return (
+            _p(self.twists, _f), _p(self.parent_tf, _f), _p(self.parent_idx, _i), _p(self.act_idx, _i),
+            _p(self.mimic_mul, _f), _p(self.mimic_off, _f), _p(self.mimic_act_idx, _i), _p(self.topo_inv, _i),
+        )
+
+
+def fk_joints(tables: Tables, cfg, threads: int = 0) -> np.ndarray:
+    """World pose of every joint, FP64, ``(*batch, J, 7)``."""
+    cfg = _c(cfg, np.float64)
+    batch = cfg.shape[:-1]
+    flat = cfg.reshape(-1, tables.A)
+    out = np.empty((flat.shape[0], tables.J, 7))
+    t = tables.ptrs()
+    lib().or_fk(_p(flat, _d), flat.shape[0], tables.A, t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7],
+                tables.J, _p(out, _d), threads)
+    return out.reshape(*batch, tables.J, 7)
+
+
+def fk_links(tables: Tables, cfg, threads: int = 0) -> np.ndarray:
+    """World pose of every link, FP64, ``(*batch, L, 7)``."""
+    jp = fk_joints(tables, cfg, threads)
+    batch = jp.shape[:-2]
+    flat = _c(jp.reshape(-1, tables.J, 7), np.float64)
+    out = np.empty((flat.shape[0], tables.L, 7))
+    lib().or_link_poses(_p(flat, _d), flat.shape[0], tables.J, _p(tables.link_parent_joint, _i), tables.L,
+                        _p(out, _d))
+    return out.reshape(*batch, tables.L, 7)
+
+
+def padded_spheres(coll) -> np.ndarray:
+    """``(L, S, 4)`` link-frame sphere table with the model's -1e9 padding."""
+    c = np.asarray(coll.coll.pose.translation(), dtype=np.float64)
+    r = np.asarray(coll.coll.radius, dtype=np.float64)
+    return _c(np.concatenate([c, r[..., None]], axis=-1), np.float64)
+
+
+def world_dist(link_poses, spheres, world, bias: bool = False, want_full: bool = True, threads: int = 0):
+    """``(out (B, L, M) | None, per-config min (B,))`` from FP64 link poses."""
+    lp = _c(link_poses, np.float64)
+    B, L, _ = lp.shape
+    sp = _c(spheres, np.float64)
+    S = sp.shape[1]
+    w = _c(world, np.float64).reshape(-1, 4)
+    out = np.empty((B, L, w.shape[0])) if want_full else None
+    mins = np.empty(B)
+    lib().or_world_dist(_p(lp, _d), B, L, _p(sp, _d), S, _p(w, _d), w.shape[0], int(bias),
+                        None if out is None else _p(out, _d), _p(mins, _d), threads)
+    return out, mins
+
+
+def self_dist(link_poses, spheres, pair_i, pair_j, mode: int = 0, threads: int = 0) -> np.ndarray:
+    """``(B, P)``; mode 0 = CUDA semantics (all sphere pairs), 1 = JAX diagonal pairs."""
+    lp = _c(link_poses, np.float64)
+    B, L, _ = lp.shape
+    sp = _c(spheres, np.float64)
+    pi = _c(pair_i, np.int32)
+    pj = _c(pair_j, np.int32)
+    out = np.empty((B, pi.shape[0]))
+    lib().or_self_dist(_p(lp, _d), B, L, _p(sp, _d), sp.shape[1], _p(pi, _i), _p(pj, _i), pi.shape[0], mode,
+                       _p(out, _d), threads)
+    return out
+
+
+def ls_ik(tables: Tables, seeds, target_jnts, ancestor_masks, target_T, lower, upper, fixed_mask=None, *,
+          max_iter=60, pos_weight=50.0, ori_weight=10.0, lambda_init=5e-3, eps_pos=1e-8, eps_ori=1e-8,
+          robot_spheres=None, sphere_joint=None, world=None, enable_collision=False, collision_weight=1e4,
+          collision_margin=0.02, threads: int = 0):
+    """FP64 restatement of the LM kernel; returns ``(cfgs (P, S, A), errs (P, S))``."""
+    sd = _c(seeds, np.float64)
+    P, S, A = sd.shape
+    tj = _c(target_jnts, np.int32).reshape(-1)
+    n_ee = tj.shape[0]
+    am = _c(ancestor_masks, np.int32).reshape(n_ee, tables.J)
+    tT = _c(target_T, np.float64).reshape(P, n_ee, 7)
+    rs = _c(np.zeros((0, 4)) if robot_spheres is None else robot_spheres, np.float32).reshape(-1, 4)
+    rj = _c(np.zeros(0) if sphere_joint is None else sphere_joint, np.int32).reshape(-1)
+    ws = _c(np.zeros((0, 4)) if world is None else world, np.float32).reshape(-1, 4)
+    lo = _c(lower, np.float64)
+    hi = _c(upper, np.float64)
+    fm = _c(np.zeros(A) if fixed_mask is None else fixed_mask, np.int32)
+    out = np.empty((P, S, A))
+    err = np.empty((P, S))
+    t = tables.ptrs()
+    lib().or_ls_ik(_p(sd, _d), P, S, A, *t, tables.J, _p(tj, _i), _p(am, _i), n_ee, _p(tT, _d), _p(rs, _f),
+                   _p(rj, _i), rs.shape[0], _p(ws, _f), ws.shape[0], _p(lo, _d), _p(hi, _d), _p(fm, _i),
+                   int(max_iter), pos_weight, ori_weight, lambda_init, eps_pos, eps_ori, int(bool(enable_collision)),
+                   collision_weight, collision_margin, _p(out, _d), _p(err, _d), threads)
+    return out, err
+
+
+def res_jac(tables: Tables, q, target_jnts, ancestor_masks, target_T):
+    """Residual ``(6 n_ee,)`` and geometric Jacobian ``(6 n_ee, A)`` at one configuration."""
+    qq = _c(q, np.float64)
+    tj = _c(target_jnts, np.int32).reshape(-1)
+    n_ee = tj.shape[0]
+    am = _c(ancestor_masks, np.int32).reshape(n_ee, tables.J)
+    tT = _c(target_T, np.float64).reshape(n_ee, 7)
+    r = np.empty(6 * n_ee)
+    jm = np.empty((6 * n_ee, tables.A))
+    lib().or_res_jac(_p(qq, _d), tables.A, *tables.ptrs(), tables.J, _p(tj, _i), _p(am, _i), n_ee, _p(tT, _d),
+                     _p(r, _d), _p(jm, _d))
+    return r, jm
+
+
+def lm_direction(r, jac, weights6, lam, fixed_mask=None):
+    """One LM direction (before the trust region) and the Cholesky success flag."""
+    rr = _c(r, np.float64)
+    jm = _c(jac, np.float64)
+    R, A = jm.shape
+    w = _c(weights6, np.float64)
+    fm = _c(np.zeros(A) if fixed_mask is None else fixed_mask, np.int32)
+    out = np.empty(A)
+    ok = lib().or_lm_direction(_p(rr, _d), _p(jm, _d), R, A, _p(w, _d), float(lam), _p(fm, _i), _p(out, _d))
+    return out, bool(ok)

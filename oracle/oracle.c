@@ -1,0 +1,560 @@
+/*
+ * FP64 CPU oracle for the batched kinematics path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This file restates the reference algorithms in double precision so the HIP
+ * kernels can be checked against it.  It is compiled into oracle/_build/ and
+ * is loaded only by tests/, by __graft_entry__.smoke() and by the cpu_baseline
+ * leg of bench.py -- never by the pyroffi_amd package.
+ *
+ * Functions and the reference code they follow:
+ *   or_fk            src/pyroffi/_robot.py:198-263 (JAX FK: fori_loop over the
+ *                    topological order) and _fk_cuda_helpers.cuh:158-208; SE(3)
+ *                    exp/compose per jaxlie 1.5.0 (FP64 Taylor branch below
+ *                    theta^2 < 1e-10, as jaxlie does for float64)
+ *   or_link_poses    src/pyroffi/_robot.py:180-196
+ *   or_world_dist    CUDA semantics: _collision_cuda_kernel.cu:169-201 (padding
+ *                    skipped, 1e9 when a link has no sphere); JAX semantics
+ *                    (bias=1): _robot_collision.py:830-895 with the +1e-6 bias
+ *                    of collision/_utils.py:45 and padding evaluated as r=-1e9
+ *   or_self_dist     CUDA: _collision_cuda_kernel.cu:425-459 (full S x S);
+ *                    JAX (mode 1): _robot_collision.py:796-828 (same-index
+ *                    "diagonal" sphere pairs, +1e-6 bias)
+ *   or_ls_ik         _ls_ik_cuda_kernel.cu:64-394 with the residual/Jacobian
+ *                    of _ik_cuda_helpers.cuh:433-631 and the Cholesky of
+ *                    _ik_cuda_helpers.cuh:207-243
+ *
+ * Parity status of each function is stated in oracle/README.md.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_MAXJ 64
+#define OR_MAXA 16
+#define OR_MAXEE 4
+
+typedef struct {
+  double w, x, y, z, tx, ty, tz;
+} se3;
+
+static se3 se3_load_f(const float* p) {
+  se3 o = {p[0], p[1], p[2], p[3], p[4], p[5], p[6]};
+  return o;
+}
+
+static void qmul(double aw, double ax, double ay, double az, double bw, double bx, double by, double bz,
+                 double* o) {
+  o[0] = aw * bw - ax * bx - ay * by - az * bz;
+  o[1] = aw * bx + ax * bw + ay * bz - az * by;
+  o[2] = aw * by - ax * bz + ay * bw + az * bx;
+  o[3] = aw * bz + ax * by - ay * bx + az * bw;
+}
+
+/* Rotate v by the rotation matrix of unit quaternion q. */
+static void qrot(const se3* q, const double* v, double* o) {
+  const double w = q->w, x = q->x, y = q->y, z = q->z;
+  const double r00 = 1 - 2 * (y * y + z * z), r01 = 2 * (x * y - w * z), r02 = 2 * (x * z + w * y);
+  const double r10 = 2 * (x * y + w * z), r11 = 1 - 2 * (x * x + z * z), r12 = 2 * (y * z - w * x);
+  const double r20 = 2 * (x * z - w * y), r21 = 2 * (y * z + w * x), r22 = 1 - 2 * (x * x + y * y);
+  o[0] = r00 * v[0] + r01 * v[1] + r02 * v[2];
+  o[1] = r10 * v[0] + r11 * v[1] + r12 * v[2];
+  o[2] = r20 * v[0] + r21 * v[1] + r22 * v[2];
+}
+
+static se3 se3_mul(const se3* a, const se3* b) {
+  se3 o;
+  double q[4], t[3], v[3] = {b->tx, b->ty, b->tz};
+  qmul(a->w, a->x, a->y, a->z, b->w, b->x, b->y, b->z, q);
+  qrot(a, v, t);
+  o.w = q[0]; o.x = q[1]; o.y = q[2]; o.z = q[3];
+  o.tx = t[0] + a->tx; o.ty = t[1] + a->ty; o.tz = t[2] + a->tz;
+  return o;
+}
+
+/* jaxlie SE3.exp of tangent [v; w]. */
+static se3 se3_exp(const double* tg) {
+  const double vx = tg[0], vy = tg[1], vz = tg[2], wx = tg[3], wy = tg[4], wz = tg[5];
+  const double th2 = wx * wx + wy * wy + wz * wz;
+  se3 o;
+  const int taylor = th2 < 1e-10;
+  double re, im;
+  if (taylor) {
+    re = 1.0 - th2 / 8.0 + th2 * th2 / 384.0;
+    im = 0.5 - th2 / 48.0 + th2 * th2 / 3840.0;
+  } else {
+    const double th = sqrt(th2);
+    re = cos(0.5 * th);
+    im = sin(0.5 * th) / th;
+  }
+  o.w = re; o.x = im * wx; o.y = im * wy; o.z = im * wz;
+  double V[9];
+  if (taylor) {
+    const double x = o.x, y = o.y, z = o.z, w = o.w;
+    V[0] = 1 - 2 * (y * y + z * z); V[1] = 2 * (x * y - w * z); V[2] = 2 * (x * z + w * y);
+    V[3] = 2 * (x * y + w * z); V[4] = 1 - 2 * (x * x + z * z); V[5] = 2 * (y * z - w * x);
+    V[6] = 2 * (x * z - w * y); V[7] = 2 * (y * z + w * x); V[8] = 1 - 2 * (x * x + y * y);
+  } else {
+    const double th = sqrt(th2);
+    const double b = (1.0 - cos(th)) / th2, c = (th - sin(th)) / (th2 * th);
+    /* V = I + b [w]x + c [w]x^2 ; [w]x^2 = w w^T - th2 I */
+    const double K[9] = {0, -wz, wy, wz, 0, -wx, -wy, wx, 0};
+    const double W[3] = {wx, wy, wz};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        V[3 * i + j] = (i == j ? 1.0 : 0.0) + b * K[3 * i + j] + c * (W[i] * W[j] - (i == j ? th2 : 0.0));
+  }
+  o.tx = V[0] * vx + V[1] * vy + V[2] * vz;
+  o.ty = V[3] * vx + V[4] * vy + V[5] * vz;
+  o.tz = V[6] * vx + V[7] * vy + V[8] * vz;
+  return o;
+}
+
+typedef struct {
+  int J, A;
+  const float *twists, *parent_tf, *mimic_mul, *mimic_off;
+  const int32_t *parent_idx, *act_idx, *mimic_act_idx, *topo_inv;
+} tables_t;
+
+/* World poses of all joints for one configuration (joint order of the tables). */
+static void fk_one(const tables_t* t, const double* q_act, se3* Tw) {
+  for (int i = 0; i < t->J; ++i) {
+    const int j = t->topo_inv[i];
+    const int m = t->mimic_act_idx[j];
+    const int src = (m != -1) ? m : t->act_idx[j];
+    const double qref = (src < 0) ? 0.0 : q_act[src];
+    const double q = qref * (double)t->mimic_mul[j] + (double)t->mimic_off[j];
+    double tg[6];
+    for (int c = 0; c < 6; ++c) tg[c] = (double)t->twists[6 * j + c] * q;
+    const se3 ptf = se3_load_f(t->parent_tf + 7 * j);
+    const se3 d = se3_exp(tg);
+    const se3 tpc = se3_mul(&ptf, &d);
+    const int p = t->parent_idx[j];
+    Tw[j] = (p < 0) ? tpc : se3_mul(&Tw[p], &tpc);
+  }
+}
+
+static void set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+void or_fk(const double* cfg, int64_t B, int A, const float* twists, const float* parent_tf,
+           const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul,
+           const float* mimic_off, const int32_t* mimic_act_idx, const int32_t* topo_inv, int J,
+           double* out, int threads) {
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  set_threads(threads);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < B; ++b) {
+    se3 Tw[OR_MAXJ];
+    fk_one(&t, cfg + b * A, Tw);
+    for (int j = 0; j < J; ++j) {
+      double* o = out + (b * J + j) * 7;
+      o[0] = Tw[j].w; o[1] = Tw[j].x; o[2] = Tw[j].y; o[3] = Tw[j].z;
+      o[4] = Tw[j].tx; o[5] = Tw[j].ty; o[6] = Tw[j].tz;
+    }
+  }
+}
+
+void or_link_poses(const double* joint_poses, int64_t B, int J, const int32_t* link_parent_joint, int L,
+                   double* out) {
+  for (int64_t b = 0; b < B; ++b)
+    for (int l = 0; l < L; ++l) {
+      const int j = link_parent_joint[l];
+      double* o = out + (b * L + l) * 7;
+      if (j < 0) {
+        o[0] = 1; o[1] = o[2] = o[3] = o[4] = o[5] = o[6] = 0;
+      } else {
+        memcpy(o, joint_poses + (b * J + j) * 7, 7 * sizeof(double));
+      }
+    }
+}
+
+/* World-frame sphere centre from a link pose and a link-frame centre. */
+static void place(const double* pose, const double* c, double* o) {
+  se3 T = {pose[0], pose[1], pose[2], pose[3], pose[4], pose[5], pose[6]};
+  qrot(&T, c, o);
+  o[0] += T.tx; o[1] += T.ty; o[2] += T.tz;
+}
+
+static double sdist(const double* a, double ra, const double* b, double rb, int bias) {
+  const double e = bias ? 1e-6 : 0.0;
+  const double dx = b[0] - a[0] + e, dy = b[1] - a[1] + e, dz = b[2] - a[2] + e;
+  return sqrt(dx * dx + dy * dy + dz * dz) - (ra + rb);
+}
+
+/* spheres: padded (L, S, 4) link-frame table, r < 0 = padding.
+ * out (B, L, M) may be NULL; out_min (B) may be NULL. */
+void or_world_dist(const double* link_poses, int64_t B, int L, const double* spheres, int S,
+                   const double* world, int M, int bias, double* out, double* out_min, int threads) {
+  set_threads(threads);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < B; ++b) {
+    double gmin = 1e9;
+    for (int n = 0; n < L; ++n) {
+      double cw[64][3];
+      const int Su = S < 64 ? S : 64;
+      for (int s = 0; s < Su; ++s) place(link_poses + (b * L + n) * 7, spheres + (n * S + s) * 4, cw[s]);
+      for (int m = 0; m < M; ++m) {
+        double best = bias ? INFINITY : 1e9;
+        for (int s = 0; s < Su; ++s) {
+          const double r = spheres[(n * S + s) * 4 + 3];
+          if (!bias && r < 0) continue;
+          const double d = sdist(cw[s], r, world + 4 * m, world[4 * m + 3], bias);
+          if (d < best) best = d;
+        }
+        if (out) out[(b * L + n) * M + m] = best;
+        if (best < gmin) gmin = best;
+      }
+    }
+    if (out_min) out_min[b] = gmin;
+  }
+}
+
+/* mode 0: CUDA semantics (all S x S pairs, padding skipped, 1e9 sentinel);
+ * mode 1: JAX semantics (sphere s of link i vs sphere s of link j, +1e-6 bias). */
+void or_self_dist(const double* link_poses, int64_t B, int L, const double* spheres, int S,
+                  const int32_t* pair_i, const int32_t* pair_j, int P, int mode, double* out, int threads) {
+  set_threads(threads);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < B; ++b) {
+    for (int p = 0; p < P; ++p) {
+      const int li = pair_i[p], lj = pair_j[p];
+      double best = mode ? INFINITY : 1e9;
+      for (int si = 0; si < S; ++si) {
+        const double ri = spheres[(li * S + si) * 4 + 3];
+        if (!mode && ri < 0) continue;
+        double ci[3];
+        place(link_poses + (b * L + li) * 7, spheres + (li * S + si) * 4, ci);
+        for (int sj = mode ? si : 0; sj < (mode ? si + 1 : S); ++sj) {
+          const double rj = spheres[(lj * S + sj) * 4 + 3];
+          if (!mode && rj < 0) continue;
+          double cj[3];
+          place(link_poses + (b * L + lj) * 7, spheres + (lj * S + sj) * 4, cj);
+          const double d = sdist(ci, ri, cj, rj, mode);
+          if (d < best) best = d;
+        }
+      }
+      out[b * P + p] = best;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ LM IK */
+
+typedef struct {
+  const tables_t* t;
+  int n_ee;
+  const int32_t* target_jnts;
+  const int32_t* anc;  /* (n_ee, J) */
+  const double* tgt;   /* (n_ee, 7) */
+  const float* rs;     /* (K, 4) joint-frame spheres */
+  const int32_t* rj;
+  int K;
+  const float* ws;     /* (M, 4) */
+  int M;
+  int coll;
+  double wc, margin;
+} ikp_t;
+
+static void residual_ee(const se3* T, const double* tg, double* r) {
+  r[0] = T->tx - tg[4]; r[1] = T->ty - tg[5]; r[2] = T->tz - tg[6];
+  double e[4];
+  qmul(T->w, T->x, T->y, T->z, tg[0], -tg[1], -tg[2], -tg[3], e);
+  if (e[0] < 0) { e[0] = -e[0]; e[1] = -e[1]; e[2] = -e[2]; e[3] = -e[3]; }
+  const double sh = sqrt(e[1] * e[1] + e[2] * e[2] + e[3] * e[3]);
+  if (sh > 1e-6) {
+    const double k = 2.0 * atan2(sh, e[0]) / sh;
+    r[3] = e[1] * k; r[4] = e[2] * k; r[5] = e[3] * k;
+  } else {
+    r[3] = 2 * e[1]; r[4] = 2 * e[2]; r[5] = 2 * e[3];
+  }
+}
+
+static double penalty(const ikp_t* p, const se3* Tw) {
+  if (!p->coll || p->K <= 0) return 0.0;
+  double pen = 0.0;
+  for (int i = 0; i < p->K; ++i) {
+    const int j = p->rj[i];
+    if (j < 0 || j >= p->t->J) continue;
+    const double c[3] = {p->rs[4 * i], p->rs[4 * i + 1], p->rs[4 * i + 2]};
+    double w[3];
+    qrot(&Tw[j], c, w);
+    w[0] += Tw[j].tx; w[1] += Tw[j].ty; w[2] += Tw[j].tz;
+    for (int m = 0; m < p->M; ++m) {
+      const double o[3] = {p->ws[4 * m], p->ws[4 * m + 1], p->ws[4 * m + 2]};
+      const double d = sdist(w, p->rs[4 * i + 3], o, p->ws[4 * m + 3], 0);
+      if (d < p->margin) pen += (d - p->margin) * (d - p->margin);
+    }
+  }
+  return p->wc * pen;
+}
+
+static double weighted_err(const ikp_t* p, const double* q, const double* W, se3* Tw, double* r) {
+  fk_one(p->t, q, Tw);
+  double e = 0;
+  for (int k = 0; k < p->n_ee; ++k) {
+    residual_ee(&Tw[p->target_jnts[k]], p->tgt + 7 * k, r + 6 * k);
+    for (int c = 0; c < 6; ++c) e += (r[6 * k + c] * W[c]) * (r[6 * k + c] * W[c]);
+  }
+  return e + penalty(p, Tw);
+}
+
+/* Geometric Jacobian rows (6 n_ee) x A for the poses in Tw (_ik_cuda_helpers.cuh:501-566). */
+static void jacobian(const ikp_t* p, const se3* Tw, double* Jm) {
+  const tables_t* t = p->t;
+  const int A = t->A;
+  memset(Jm, 0, sizeof(double) * 6 * p->n_ee * A);
+  for (int j = 0; j < t->J; ++j) {
+    const int a1 = t->act_idx[j], a2 = t->mimic_act_idx[j];
+    if (a1 < 0 && a2 < 0) continue;
+    const float* tw = t->twists + 6 * j;
+    const double ang = (double)tw[3] * tw[3] + (double)tw[4] * tw[4] + (double)tw[5] * tw[5];
+    const double lin = (double)tw[0] * tw[0] + (double)tw[1] * tw[1] + (double)tw[2] * tw[2];
+    int rev;
+    double ax[3], z[3];
+    if (ang > 1e-6) {
+      rev = 1;
+      for (int c = 0; c < 3; ++c) ax[c] = tw[3 + c] / sqrt(ang);
+    } else if (lin > 1e-6) {
+      rev = 0;
+      for (int c = 0; c < 3; ++c) ax[c] = tw[c] / sqrt(lin);
+    } else {
+      continue;
+    }
+    qrot(&Tw[j], ax, z);
+    for (int e = 0; e < p->n_ee; ++e) {
+      if (!p->anc[e * t->J + j]) continue;
+      const se3* Te = &Tw[p->target_jnts[e]];
+      double g[6];
+      if (rev) {
+        const double arm[3] = {Te->tx - Tw[j].tx, Te->ty - Tw[j].ty, Te->tz - Tw[j].tz};
+        g[0] = z[1] * arm[2] - z[2] * arm[1];
+        g[1] = z[2] * arm[0] - z[0] * arm[2];
+        g[2] = z[0] * arm[1] - z[1] * arm[0];
+        g[3] = z[0]; g[4] = z[1]; g[5] = z[2];
+      } else {
+        g[0] = z[0]; g[1] = z[1]; g[2] = z[2]; g[3] = g[4] = g[5] = 0;
+      }
+      const double s = t->mimic_mul[j];
+      for (int c = 0; c < 6; ++c) {
+        if (a1 >= 0) Jm[(6 * e + c) * A + a1] += s * g[c];
+        if (a2 >= 0) Jm[(6 * e + c) * A + a2] += s * g[c];
+      }
+    }
+  }
+}
+
+/* Cholesky solve of A x = b in place (b <- x); zero step on a non-positive pivot. */
+static int chol_solve(double* A, double* b, int n) {
+  for (int k = 0; k < n; ++k) {
+    double s = A[k * n + k];
+    for (int q = 0; q < k; ++q) s -= A[k * n + q] * A[k * n + q];
+    if (s <= 0.0) {
+      for (int i = 0; i < n; ++i) b[i] = 0.0;
+      return 0;
+    }
+    const double l = sqrt(s);
+    A[k * n + k] = l;
+    for (int i = k + 1; i < n; ++i) {
+      double v = A[i * n + k];
+      for (int q = 0; q < k; ++q) v -= A[i * n + q] * A[k * n + q];
+      A[i * n + k] = v / l;
+    }
+    for (int j = k + 1; j < n; ++j) A[k * n + j] = 0.0;
+  }
+  double y[OR_MAXA];
+  for (int i = 0; i < n; ++i) {
+    double s = b[i];
+    for (int q = 0; q < i; ++q) s -= A[i * n + q] * y[q];
+    y[i] = s / A[i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = y[i];
+    for (int q = i + 1; q < n; ++q) s -= A[q * n + i] * b[q];
+    b[i] = s / A[i * n + i];
+  }
+  return 1;
+}
+
+static void solve_seed(const ikp_t* p, const double* seed, const double* lo, const double* hi,
+                       const int32_t* fixed, int max_iter, const double* W, double lam0,
+                       double eps_pos, double eps_ori, double* out_q, double* out_err) {
+  const int A = p->t->A, R = 6 * p->n_ee;
+  double q[OR_MAXA] = {0}, best_q[OR_MAXA] = {0}, r[6 * OR_MAXEE], Jm[6 * OR_MAXEE * OR_MAXA];
+  se3 Tw[OR_MAXJ];
+  for (int a = 0; a < A; ++a) q[a] = best_q[a] = seed[a];
+  double best_err = weighted_err(p, q, W, Tw, r);
+  double lam = lam0;
+  static const double alphas[5] = {1.0, 0.5, 0.25, 0.1, 0.025};
+  for (int it = 0; it < max_iter; ++it) {
+    double cur = weighted_err(p, q, W, Tw, r);
+    int conv = 1;
+    double max_p = 0, max_o = 0;
+    for (int e = 0; e < p->n_ee; ++e) {
+      const double np = sqrt(r[6 * e] * r[6 * e] + r[6 * e + 1] * r[6 * e + 1] + r[6 * e + 2] * r[6 * e + 2]);
+      const double no = sqrt(r[6 * e + 3] * r[6 * e + 3] + r[6 * e + 4] * r[6 * e + 4] + r[6 * e + 5] * r[6 * e + 5]);
+      if (np >= eps_pos || no >= eps_ori) conv = 0;
+      if (np > max_p) max_p = np;
+      if (no > max_o) max_o = no;
+    }
+    if (conv) break;
+    jacobian(p, Tw, Jm);
+    double fw[6 * OR_MAXEE];
+    for (int k = 0; k < R; ++k) {
+      fw[k] = r[k] * W[k % 6];
+      for (int a = 0; a < A; ++a) Jm[k * A + a] *= W[k % 6];
+    }
+    double cs[OR_MAXA];
+    for (int a = 0; a < A; ++a) {
+      double s2 = 0;
+      for (int k = 0; k < R; ++k) s2 += Jm[k * A + a] * Jm[k * A + a];
+      cs[a] = sqrt(s2) + 1e-8;
+    }
+    for (int k = 0; k < R; ++k)
+      for (int a = 0; a < A; ++a) Jm[k * A + a] /= cs[a];
+    double M[OR_MAXA * OR_MAXA], rhs[OR_MAXA];
+    for (int i = 0; i < A; ++i) {
+      for (int j = 0; j < A; ++j) {
+        double acc = 0;
+        for (int k = 0; k < R; ++k) acc += Jm[k * A + i] * Jm[k * A + j];
+        M[i * A + j] = acc;
+      }
+      double rb = 0;
+      for (int k = 0; k < R; ++k) rb += Jm[k * A + i] * fw[k];
+      rhs[i] = -rb;
+      M[i * A + i] += lam;
+    }
+    for (int a = 0; a < A; ++a) {
+      if (!fixed[a]) continue;
+      for (int j = 0; j < A; ++j) M[a * A + j] = M[j * A + a] = 0.0;
+      M[a * A + a] = 1.0;
+      rhs[a] = 0.0;
+    }
+    chol_solve(M, rhs, A);
+    double delta[OR_MAXA], dn = 0;
+    for (int a = 0; a < A; ++a) {
+      delta[a] = rhs[a] / cs[a];
+      dn += delta[a] * delta[a];
+    }
+    double Rt;
+    if (max_p > 1e-2 || max_o > 0.6) Rt = 0.38;
+    else if (max_p > 1e-3 || max_o > 0.25) Rt = 0.22;
+    else if (max_p > 2e-4 || max_o > 0.08) Rt = 0.12;
+    else Rt = 0.05;
+    dn = sqrt(dn);
+    if (dn > Rt) {
+      const double s = Rt / (dn + 1e-18);
+      for (int a = 0; a < A; ++a) delta[a] *= s;
+    }
+    double best_a_err = 1e30;
+    int best_a = 0;
+    double qt[OR_MAXA], rt[6 * OR_MAXEE];
+    for (int ai = 0; ai < 5; ++ai) {
+      for (int a = 0; a < A; ++a) {
+        const double v = q[a] + alphas[ai] * delta[a];
+        qt[a] = v < lo[a] ? lo[a] : (v > hi[a] ? hi[a] : v);
+      }
+      const double et = weighted_err(p, qt, W, Tw, rt);
+      if (et < best_a_err) { best_a_err = et; best_a = ai; }
+    }
+    for (int a = 0; a < A; ++a) {
+      const double v = q[a] + alphas[best_a] * delta[a];
+      qt[a] = v < lo[a] ? lo[a] : (v > hi[a] ? hi[a] : v);
+    }
+    if (best_a_err < cur * (1.0 - 1e-4)) {
+      for (int a = 0; a < A; ++a) q[a] = qt[a];
+      lam = lam * 0.5 > 1e-10 ? lam * 0.5 : 1e-10;
+    } else {
+      lam = lam * 3.0 < 1e6 ? lam * 3.0 : 1e6;
+    }
+    if (best_a_err < best_err) {
+      best_err = best_a_err;
+      for (int a = 0; a < A; ++a) best_q[a] = qt[a];
+    }
+  }
+  for (int a = 0; a < A; ++a) out_q[a] = best_q[a];
+  *out_err = best_err;
+}
+
+void or_ls_ik(const double* seeds, int P, int S, int A, const float* twists, const float* parent_tf,
+              const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul,
+              const float* mimic_off, const int32_t* mimic_act_idx, const int32_t* topo_inv, int J,
+              const int32_t* target_jnts, const int32_t* ancestor_masks, int n_ee, const double* target_T,
+              const float* robot_spheres, const int32_t* sphere_joint, int K, const float* world, int M,
+              const double* lower, const double* upper, const int32_t* fixed_mask, int max_iter,
+              double pos_weight, double ori_weight, double lambda_init, double eps_pos, double eps_ori,
+              int enable_collision, double collision_weight, double collision_margin, double* out_cfgs,
+              double* out_err, int threads) {
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  const double W[6] = {pos_weight, pos_weight, pos_weight, ori_weight, ori_weight, ori_weight};
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+  for (int pi = 0; pi < P; ++pi)
+    for (int si = 0; si < S; ++si) {
+      ikp_t p = {&t, n_ee, target_jnts, ancestor_masks, target_T + (int64_t)pi * n_ee * 7, robot_spheres,
+                 sphere_joint, K, world, M, enable_collision, collision_weight, collision_margin};
+      const int64_t g = (int64_t)pi * S + si;
+      solve_seed(&p, seeds + g * A, lower, upper, fixed_mask, max_iter, W, lambda_init, eps_pos, eps_ori,
+                 out_cfgs + g * A, out_err + g);
+    }
+}
+
+/* Residual (6 n_ee) and weighted-free geometric Jacobian (6 n_ee x A) at q. */
+void or_res_jac(const double* q, int A, const float* twists, const float* parent_tf, const int32_t* parent_idx,
+                const int32_t* act_idx, const float* mimic_mul, const float* mimic_off,
+                const int32_t* mimic_act_idx, const int32_t* topo_inv, int J, const int32_t* target_jnts,
+                const int32_t* ancestor_masks, int n_ee, const double* target_T, double* out_r, double* out_J) {
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  ikp_t p = {&t, n_ee, target_jnts, ancestor_masks, target_T, NULL, NULL, 0, NULL, 0, 0, 0.0, 0.0};
+  se3 Tw[OR_MAXJ];
+  fk_one(&t, q, Tw);
+  for (int e = 0; e < n_ee; ++e) residual_ee(&Tw[target_jnts[e]], target_T + 7 * e, out_r + 6 * e);
+  jacobian(&p, Tw, out_J);
+}
+
+/* Damped Gauss-Newton direction of one LM step before the trust region:
+ * delta = S^-1 (Js^T Js + lam I)^-1 (-Js^T W r), Js = W J S^-1 (column scaling S). */
+int or_lm_direction(const double* r, const double* Jm, int R, int A, const double* W, double lam,
+                    const int32_t* fixed, double* delta) {
+  double Jw[6 * OR_MAXEE * OR_MAXA], fw[6 * OR_MAXEE], cs[OR_MAXA], M[OR_MAXA * OR_MAXA], rhs[OR_MAXA];
+  for (int k = 0; k < R; ++k) {
+    fw[k] = r[k] * W[k % 6];
+    for (int a = 0; a < A; ++a) Jw[k * A + a] = Jm[k * A + a] * W[k % 6];
+  }
+  for (int a = 0; a < A; ++a) {
+    double s2 = 0;
+    for (int k = 0; k < R; ++k) s2 += Jw[k * A + a] * Jw[k * A + a];
+    cs[a] = sqrt(s2) + 1e-8;
+  }
+  for (int k = 0; k < R; ++k)
+    for (int a = 0; a < A; ++a) Jw[k * A + a] /= cs[a];
+  for (int i = 0; i < A; ++i) {
+    for (int j = 0; j < A; ++j) {
+      double acc = 0;
+      for (int k = 0; k < R; ++k) acc += Jw[k * A + i] * Jw[k * A + j];
+      M[i * A + j] = acc;
+    }
+    double rb = 0;
+    for (int k = 0; k < R; ++k) rb += Jw[k * A + i] * fw[k];
+    rhs[i] = -rb;
+    M[i * A + i] += lam;
+  }
+  for (int a = 0; a < A; ++a) {
+    if (!fixed[a]) continue;
+    for (int j = 0; j < A; ++j) M[a * A + j] = M[j * A + a] = 0.0;
+    M[a * A + a] = 1.0;
+    rhs[a] = 0.0;
+  }
+  const int ok = chol_solve(M, rhs, A);
+  for (int a = 0; a < A; ++a) delta[a] = rhs[a] / cs[a];
+  return ok;
+}

@@ -1,0 +1,37 @@
+"""pyroffi_amd: pyroffi's batched kinematics hot path, native to MI355X (gfx950).
+
+Public API (same names and arguments as pyroffi):
+
+* ``Robot.from_urdf`` / ``Robot.forward_kinematics``
+* ``collision.RobotCollisionSpherized``, ``collision.CUDARobotCollisionChecker``,
+  ``collision.colldist_from_sdf``
+* ``ls_ik_solve_cuda`` / ``ls_ik_solve_cuda_batch``
+
+All computation runs in the HIP library ``libpyroffi_amd.so`` (C ABI in
+``include/pyroffi_amd.h``); torch supplies device memory and the stream.
+"""
+
+from . import collision, kernels
+from ._robot import Robot
+from ._robot_urdf_parser import JointInfo, LinkInfo, RobotURDFParser
+from ._se3 import SE3, SO3
+from ._urdf import URDF, bundled_robot_names, bundled_srdf_path, bundled_urdf_path, load_bundled_urdf
+from .optimization_engines import ls_ik_solve_cuda, ls_ik_solve_cuda_batch
+
+__all__ = [
+    "SE3",
+    "SO3",
+    "URDF",
+    "JointInfo",
+    "LinkInfo",
+    "Robot",
+    "RobotURDFParser",
+    "bundled_robot_names",
+    "bundled_srdf_path",
+    "bundled_urdf_path",
+    "collision",
+    "kernels",
+    "load_bundled_urdf",
+    "ls_ik_solve_cuda",
+    "ls_ik_solve_cuda_batch",
+]

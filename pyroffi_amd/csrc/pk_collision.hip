@@ -1,0 +1,347 @@
+// Sphere-robot collision distances for gfx950.
+//
+// Work decomposition (one 64-lane workgroup = one wave):
+//   * the wave loops over configurations (grid-stride);
+//   * for one configuration it places every robot sphere in the world frame
+//     into LDS (one float4 per sphere: centre, radius);
+//   * lanes run across world obstacles -- each lane keeps T obstacles in
+//     registers for the whole launch -- and every robot sphere is read once
+//     from LDS as a broadcast and tested against the lane's T obstacles;
+//   * per-link minima are written as coalesced rows (lanes = obstacles), the
+//     per-configuration minimum is a wavefront min-reduction, and self pairs
+//     are reduced with LDS atomic-min over an order-preserving integer key.
+//
+// Distance of two spheres: |c1 - c2| - r1 - r2 (positive = separated), the
+// reference's sphere_sphere_dist (_collision_cuda_helpers.cuh:14-19).
+#include "pk_model.h"
+
+#include <math.h>
+
+#include <algorithm>
+
+namespace pk {
+
+constexpr float kBig = 1e9f;   // the reference's "no sphere" sentinel
+
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+struct SweepArgs {
+  // source A: link poses + compact local sphere table
+  const float* link_poses;   // (B, L, 7)
+  const float* spheres_local;  // (K, 4)
+  const int* link_sphere_start;  // (L + 1)
+  // source B: world-frame sphere slots k = s * L + n
+  const float* centers_soa;  // (3, B, K)
+  const float* radii;        // (B, K)
+  int s_per_link;            // S
+  // common
+  long long batch;
+  int n_links;
+  int n_spheres;             // K (compact for A, S * L for B)
+  const float* world;        // (M, 4)
+  int n_world;
+  const unsigned* self_entries;
+  int n_entries;
+  int n_pairs;
+  float* out_world;          // (B, L, M)
+  float* out_min;            // (B)
+  float* out_self;           // (B, P)
+};
+
+// SRC = 0: link poses + local spheres (fused path); SRC = 1: precomputed
+// world-frame sphere slots (the reference FFI layout).  T = obstacles per lane.
+template <int SRC, int T>
+__global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x;
+  const int L = a.n_links;
+  const int K = a.n_spheres;
+  const int M = a.n_world;
+  // LDS carve-up.
+  float4* sc = reinterpret_cast<float4*>(smem);               // K world spheres
+  float4* sl = sc + K;                                        // K local spheres (SRC 0)
+  int* lstart = reinterpret_cast<int*>(sl + (SRC == 0 ? K : 0));  // L + 1
+  int* slink = lstart + (L + 1);                              // K: link of sphere (SRC 0)
+  float* ptile = reinterpret_cast<float*>(slink + (SRC == 0 ? K : 0));  // L * 7 poses
+  unsigned* pmin = reinterpret_cast<unsigned*>(ptile + (SRC == 0 ? L * 7 : 0));  // P
+  unsigned* ents = pmin + a.n_pairs;                          // n_entries
+
+  // Per-launch tables.
+  if (SRC == 0) {
+    for (int k = lane; k < K; k += kWave) {
+      const float* s = a.spheres_local + 4 * k;
+      sl[k] = make_float4(s[0], s[1], s[2], s[3]);
+    }
+    // Clamp the CSR offsets into a monotone [0, K] sequence.
+    if (lane == 0) {
+      int prev = 0;
+      for (int n = 0; n <= L; ++n) {
+        const int v = (n == L) ? K : a.link_sphere_start[n];
+        prev = max(prev, min(v, K));
+        lstart[n] = (n == 0) ? 0 : prev;
+      }
+    }
+    for (int k = lane; k < K; k += kWave) slink[k] = 0;
+  } else {
+    for (int n = lane; n <= L; n += kWave) lstart[n] = n * a.s_per_link;
+  }
+  // Self entries that point outside the tables are dropped.
+  for (int e = lane; e < a.n_entries; e += kWave) {
+    const unsigned en = a.self_entries[e];
+    const bool ok = static_cast<int>(en & 1023u) < K && static_cast<int>((en >> 10) & 1023u) < K &&
+                    static_cast<int>(en >> 20) < a.n_pairs;
+    ents[e] = ok ? en : 0xffffffffu;
+  }
+  __syncthreads();
+  if (SRC == 0) {
+    for (int n = lane; n < L; n += kWave)
+      for (int k = lstart[n]; k < lstart[n + 1]; ++k) slink[k] = n;
+  }
+
+  // This lane's obstacles (first chunk stays resident when M <= 64 * T).
+  const int chunk = kWave * T;
+  const int n_chunks = (M + chunk - 1) / chunk;
+  float ox[T], oy[T], oz[T], orad[T];
+  auto load_obstacles = [&](int c) {
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int m = c * chunk + t * kWave + lane;
+      if (m < M) {
+        ox[t] = a.world[4 * m]; oy[t] = a.world[4 * m + 1];
+        oz[t] = a.world[4 * m + 2]; orad[t] = a.world[4 * m + 3];
+      } else {  // never the minimum: infinitely far
+        ox[t] = 0.f; oy[t] = 0.f; oz[t] = 0.f; orad[t] = -INFINITY;
+      }
+    }
+  };
+  if (n_chunks > 0) load_obstacles(0);
+  __syncthreads();
+
+  for (long long b = blockIdx.x; b < a.batch; b += gridDim.x) {
+    // 1. Robot spheres of configuration b in the world frame -> LDS.
+    if (SRC == 0) {
+      const float* pb = a.link_poses + b * L * 7;
+      for (int f = lane; f < L * 7; f += kWave) ptile[f] = pb[f];
+      __syncthreads();
+      for (int k = lane; k < K; k += kWave) {
+        const float* p = ptile + 7 * slink[k];
+        const float4 s = sl[k];
+        float wx, wy, wz;
+        quat_rotate(p[0], p[1], p[2], p[3], s.x, s.y, s.z, wx, wy, wz);
+        sc[k] = make_float4(wx + p[4], wy + p[5], wz + p[6], s.w);
+      }
+    } else {
+      const long long BK = a.batch * K;
+      for (int k = lane; k < K; k += kWave) {
+        // slot k = s * L + n is stored at LDS position n * S + s
+        const int n = k % L, s = k / L;
+        const long long g = b * K + k;
+        const float r = a.radii[g];
+        sc[n * a.s_per_link + s] =
+            make_float4(a.centers_soa[g], a.centers_soa[BK + g], a.centers_soa[2 * BK + g],
+                        r < 0.f ? -INFINITY : r);
+      }
+    }
+    for (int p = lane; p < a.n_pairs; p += kWave) pmin[p] = float_to_ordered(kBig);
+    __syncthreads();
+
+    // 2. World sweep.
+    float gmin = INFINITY;
+    if (a.out_world != nullptr || a.out_min != nullptr) {
+      for (int c = 0; c < n_chunks; ++c) {
+        if (c > 0) load_obstacles(c);
+        for (int n = 0; n < L; ++n) {
+          float acc[T];
+#pragma unroll
+          for (int t = 0; t < T; ++t) acc[t] = INFINITY;
+          const int k1 = lstart[n + 1];
+          for (int k = lstart[n]; k < k1; ++k) {
+            const float4 s = sc[k];
+            if (SRC == 1 && s.w == -INFINITY) continue;  // padding slot (wave-uniform)
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+              const float dx = s.x - ox[t], dy = s.y - oy[t], dz = s.z - oz[t];
+              const float d = fast_sqrt(dx * dx + dy * dy + dz * dz) - s.w;
+              acc[t] = fminf(acc[t], d);
+            }
+          }
+#pragma unroll
+          for (int t = 0; t < T; ++t) {
+            const float v = (acc[t] == INFINITY) ? kBig : acc[t] - orad[t];
+            const int m = c * chunk + t * kWave + lane;
+            if (m < M) {
+              if (a.out_world != nullptr) a.out_world[(b * L + n) * M + m] = v;
+              gmin = fminf(gmin, v);
+            }
+          }
+        }
+      }
+      if (a.out_min != nullptr) {
+        const float w = wave_min(gmin);
+        if (lane == 0) a.out_min[b] = (w == INFINITY) ? kBig : w;
+      }
+      if (n_chunks > 1) load_obstacles(0);
+    }
+
+    // 3. Self pairs.
+    if (a.out_self != nullptr) {
+      for (int e = lane; e < a.n_entries; e += kWave) {
+        const unsigned en = ents[e];
+        if (en == 0xffffffffu) continue;
+        const float4 s1 = sc[en & 1023u];
+        const float4 s2 = sc[(en >> 10) & 1023u];
+        const float dx = s1.x - s2.x, dy = s1.y - s2.y, dz = s1.z - s2.z;
+        const float d = fast_sqrt(dx * dx + dy * dy + dz * dz) - s1.w - s2.w;
+        atomicMin(&pmin[en >> 20], float_to_ordered(d));
+      }
+      __syncthreads();
+      for (int p = lane; p < a.n_pairs; p += kWave) a.out_self[b * a.n_pairs + p] = ordered_to_float(pmin[p]);
+    }
+    __syncthreads();
+  }
+}
+
+// Reference-layout self distances: one lane per (configuration, pair), the
+// full S x S minimum with padding skipped (_collision_cuda_kernel.cu:425-459).
+__global__ __launch_bounds__(256) void self_sphere_aos_kernel(
+    const float* __restrict__ centers, const float* __restrict__ radii, long long batch,
+    int S, int N, const int* __restrict__ pair_i, const int* __restrict__ pair_j, int P,
+    float* __restrict__ out) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= batch * P) return;
+  const int p = static_cast<int>(idx % P);
+  const long long b = idx / P;
+  const int li = pair_i[p], lj = pair_j[p];
+  float best = kBig;
+  if (li >= 0 && li < N && lj >= 0 && lj < N) {
+    for (int si = 0; si < S; ++si) {
+      const long long ki = (b * S + si) * N + li;
+      const float ri = radii[ki];
+      if (ri < 0.f) continue;
+      const float ax = centers[3 * ki], ay = centers[3 * ki + 1], az = centers[3 * ki + 2];
+      for (int sj = 0; sj < S; ++sj) {
+        const long long kj = (b * S + sj) * N + lj;
+        const float rj = radii[kj];
+        if (rj < 0.f) continue;
+        const float dx = ax - centers[3 * kj], dy = ay - centers[3 * kj + 1],
+                    dz = az - centers[3 * kj + 2];
+        best = fminf(best, fast_sqrt(dx * dx + dy * dy + dz * dz) - (ri + rj));
+      }
+    }
+  }
+  out[idx] = best;
+}
+
+static int pick_t(int M) {
+  if (M <= 64) return 1;
+  if (M <= 128) return 2;
+  if (M <= 256) return 4;
+  return 8;
+}
+
+template <int SRC>
+static int launch_sweep(void* stream, const SweepArgs& a) {
+  const int K = a.n_spheres, L = a.n_links;
+  size_t lds = sizeof(float4) * K + sizeof(int) * (L + 1) + sizeof(unsigned) * (a.n_pairs + a.n_entries);
+  if (SRC == 0) lds += sizeof(float4) * K + sizeof(int) * K + sizeof(float) * L * 7;
+  PK_REQUIRE(lds <= 150 * 1024, "sphere tables need %zu bytes of LDS (limit 150 KiB)", lds);
+  // Enough waves to cover every CU several times; each loops over configs.
+  const long long want = (a.batch + 7) / 8;
+  const unsigned grid = static_cast<unsigned>(std::max(1LL, std::min(want, 4096LL)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (pick_t(a.n_world)) {
+    case 1: hipLaunchKernelGGL((sweep_kernel<SRC, 1>), dim3(grid), dim3(kWave), lds, s, a); break;
+    case 2: hipLaunchKernelGGL((sweep_kernel<SRC, 2>), dim3(grid), dim3(kWave), lds, s, a); break;
+    case 4: hipLaunchKernelGGL((sweep_kernel<SRC, 4>), dim3(grid), dim3(kWave), lds, s, a); break;
+    default: hipLaunchKernelGGL((sweep_kernel<SRC, 8>), dim3(grid), dim3(kWave), lds, s, a); break;
+  }
+  return check_launch("sweep_kernel");
+}
+
+}  // namespace pk
+
+extern "C" int pk_collision_world_sphere_reduced_f32(
+    void* stream, const float* sphere_centers, const float* sphere_radii, int64_t batch,
+    int32_t k, const float* world_spheres, int32_t n_world_spheres,
+    const float* world_capsules, int32_t n_world_capsules, const float* world_boxes,
+    int32_t n_world_boxes, const float* world_halfspaces, int32_t n_world_halfspaces,
+    int32_t n_links, float* out) {
+  (void)world_capsules; (void)world_boxes; (void)world_halfspaces;
+  PK_REQUIRE(n_world_capsules == 0 && n_world_boxes == 0 && n_world_halfspaces == 0,
+             "only sphere worlds are supported (got %d capsules, %d boxes, %d halfspaces)",
+             n_world_capsules, n_world_boxes, n_world_halfspaces);
+  PK_REQUIRE(batch >= 0 && k >= 0 && n_world_spheres >= 0 && n_links >= 0, "negative size");
+  if (batch == 0 || n_links == 0 || n_world_spheres == 0) return PK_OK;
+  PK_REQUIRE(k % n_links == 0, "k=%d is not a multiple of n_links=%d", k, n_links);
+  PK_REQUIRE(sphere_centers && sphere_radii && world_spheres && out, "null pointer");
+  pk::SweepArgs a{};
+  a.centers_soa = sphere_centers;
+  a.radii = sphere_radii;
+  a.s_per_link = k / n_links;
+  a.batch = batch;
+  a.n_links = n_links;
+  a.n_spheres = k;
+  a.world = world_spheres;
+  a.n_world = n_world_spheres;
+  a.out_world = out;
+  return pk::launch_sweep<1>(stream, a);
+}
+
+extern "C" int pk_collision_self_sphere_f32(void* stream, const float* centers,
+                                            const float* radii, int64_t batch, int32_t s,
+                                            int32_t n_links, const int32_t* pair_i,
+                                            const int32_t* pair_j, int32_t n_pairs,
+                                            float* out) {
+  PK_REQUIRE(batch >= 0 && s >= 0 && n_links >= 0 && n_pairs >= 0, "negative size");
+  if (batch == 0 || n_pairs == 0) return PK_OK;
+  PK_REQUIRE(centers && radii && pair_i && pair_j && out, "null pointer");
+  const long long total = batch * static_cast<long long>(n_pairs);
+  const long long blocks = (total + 255) / 256;
+  PK_REQUIRE(blocks <= 0x7fffffffLL, "batch too large");
+  hipLaunchKernelGGL(pk::self_sphere_aos_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), centers, radii, static_cast<long long>(batch),
+                     s, n_links, pair_i, pair_j, n_pairs, out);
+  return pk::check_launch("self_sphere_aos_kernel");
+}
+
+extern "C" int pk_sphere_sweep_f32(void* stream, const float* link_poses, int64_t batch,
+                                   int32_t n_links, const float* spheres_local,
+                                   const int32_t* link_sphere_start, int32_t n_spheres,
+                                   const float* world_spheres, int32_t n_world_spheres,
+                                   const uint32_t* self_entries, int32_t n_entries,
+                                   int32_t n_pairs, float* out_world, float* out_world_min,
+                                   float* out_self) {
+  PK_REQUIRE(batch >= 0 && n_links >= 0 && n_spheres >= 0 && n_world_spheres >= 0 &&
+                 n_entries >= 0 && n_pairs >= 0,
+             "negative size");
+  PK_REQUIRE(n_spheres <= 1024 && n_pairs <= 4096,
+             "self-entry encoding needs n_spheres <= 1024 and n_pairs <= 4096");
+  PK_REQUIRE(n_links <= pk::kMaxJoints + 1, "n_links=%d too large", n_links);
+  if (batch == 0) return PK_OK;
+  PK_REQUIRE(link_poses && link_sphere_start && (n_spheres == 0 || spheres_local),
+             "null pointer");
+  if (out_world || out_world_min) PK_REQUIRE(n_world_spheres == 0 || world_spheres, "null world");
+  if (out_self) PK_REQUIRE(n_entries == 0 || self_entries, "null self_entries");
+  pk::SweepArgs a{};
+  a.link_poses = link_poses;
+  a.spheres_local = spheres_local;
+  a.link_sphere_start = link_sphere_start;
+  a.batch = batch;
+  a.n_links = n_links;
+  a.n_spheres = n_spheres;
+  a.world = world_spheres;
+  a.n_world = n_world_spheres;
+  a.self_entries = out_self ? self_entries : nullptr;
+  a.n_entries = out_self ? n_entries : 0;
+  a.n_pairs = out_self ? n_pairs : 0;
+  a.out_world = out_world;
+  a.out_min = out_world_min;
+  a.out_self = out_self;
+  return pk::launch_sweep<0>(stream, a);
+}

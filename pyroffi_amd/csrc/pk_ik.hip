@@ -1,0 +1,736 @@
+// Multi-seed Levenberg-Marquardt IK for gfx950: one lane = one seed.
+//
+// Algorithm (the reference's ls_ik_lm_kernel, _ls_ik_cuda_kernel.cu:64-394,
+// with the residual/Jacobian of _ik_cuda_helpers.cuh:433-631):
+//   r_pos = p_ee - p_tgt;  r_ori = log(q_ee * q_tgt^-1) (hemisphere-folded)
+//   J     = geometric Jacobian ([z x (p_ee - p_j); z] revolute, [z; 0]
+//           prismatic), times the mimic multiplier, ancestor-masked per EE
+//   per iteration: stop if every EE has |r_pos| < eps_pos and |r_ori| < eps_ori;
+//   fw = W r, Jw = W J, cur = |fw|^2 + penalty(q)
+//   Jacobi column scaling s_a = |Jw[:,a]| + 1e-8; (Js^T Js + lam I) p = -Js^T fw
+//   (fixed joints: zero row/col, unit diagonal); delta = p / s
+//   trust region: |delta| <= R(max pos err, max ori err) in {0.38, 0.22, 0.12, 0.05}
+//   5-point line search alpha in {1, .5, .25, .1, .025} on clamp(q + alpha delta),
+//   all evaluated, best kept; accept if err < cur (1 - 1e-4): lam *= 0.5 (>= 1e-10),
+//   else lam *= 3 (<= 1e6); track the all-time best (err, q).
+//   penalty(q) = w_coll * sum over robot spheres x world spheres of
+//                min(d - margin, 0)^2.
+//
+// gfx950 layout: the workgroup prologue folds every fixed joint into its
+// nearest moving ancestor, producing a tree of "nodes" (the joints whose
+// transform depends on q).  A lane holds only the node poses in registers
+// (n_act of them for a robot without mimic joints), and the EE frames and
+// sphere centres are constant offsets from a node.  All per-seed arrays are
+// indexed by compile-time constants (MAXA = padded DOF, NEE = padded EE
+// count); padded DOF are frozen joints with zero limits, padded EEs have zero
+// residual, so they change nothing.  The damped normal equations are solved
+// with an FP32 Cholesky in registers.
+#include "pk_model.h"
+
+#include <math.h>
+
+namespace pk {
+
+struct NodeRec {
+  float ptf[7];     // folded constant: parent node frame -> this joint (before motion)
+  float lin[3], ang[3];
+  float mul, off;
+  int src;          // actuated index driving this node, -1 none
+  int parent;       // parent node, -1 = world
+  int kind;         // joint_exp kind (1..3)
+  int jac_type;     // 1 revolute, 2 prismatic, 0 no Jacobian column
+  float ax[3];      // unit joint axis in the joint frame (Jacobian)
+  unsigned ee_mask; // bit e set when the node is an ancestor of EE e
+  int pad[2];
+};
+
+struct IkArgs {
+  const float* seeds;
+  int n_problems, n_seeds, n_act;
+  RobotTables tables;
+  const int* target_jnts;
+  const int* ancestor_masks;
+  int n_ee;
+  const float* target_T;
+  const float* robot_spheres;
+  const int* robot_sphere_joint;
+  int n_robot_spheres;
+  const float* world;
+  int n_world;
+  const float* lower;
+  const float* upper;
+  const int* fixed_mask;
+  int max_iter;
+  float pos_weight, ori_weight, lambda_init, eps_pos, eps_ori;
+  int enable_collision;
+  float collision_weight, collision_margin;
+  float* out_cfgs;
+  float* out_err;
+};
+
+// Shared-memory image of one problem, built by the prologue.
+template <int MAXA, int NEE>
+struct IkShared {
+  NodeRec nodes[MAXA + kMaxJoints];   // worst case: every joint moves
+  int n_nodes;
+  int ee_node[NEE];
+  float ee_rel[NEE][7];               // node frame -> EE frame
+  float tgt[NEE][7];
+  float lo[MAXA], hi[MAXA];
+  int fixed[MAXA];
+  int sph_start[MAXA + kMaxJoints + 2];  // spheres grouped by node; group 0 = world-fixed
+  int cnt[MAXA + kMaxJoints + 2];
+  int n_sph;
+  int too_many_nodes;                 // moving joints exceed the padded width MAXA
+};
+
+__device__ __forceinline__ Pose pose_from(const float* p) { return load_pose(p); }
+__device__ __forceinline__ void pose_to(float* p, const Pose& v) {
+  p[0] = v.w; p[1] = v.x; p[2] = v.y; p[3] = v.z; p[4] = v.tx; p[5] = v.ty; p[6] = v.tz;
+}
+
+// Single-lane tree folding (J <= 64 joints, runs once per workgroup).
+template <int MAXA, int NEE>
+__device__ void build_problem(const IkArgs& a, const JointRec* recs, const int* slot_of,
+                              IkShared<MAXA, NEE>& sh, int* anchor, float* rel,
+                              float4* sph, int* sph_node_tmp) {
+  const int J = a.tables.n_joints;
+  int nn = 0;
+  for (int i = 0; i < J; ++i) {
+    const JointRec& r = recs[i];
+    int base = -1;
+    Pose bt = pose_identity();
+    if (r.parent_slot >= 0) {
+      base = anchor[r.parent_slot];
+      bt = pose_from(rel + 7 * r.parent_slot);
+    }
+    if (r.kind == 0) {
+      anchor[i] = base;
+      pose_to(rel + 7 * i, compose(bt, load_pose(r.ptf)));
+    } else {
+      NodeRec& n = sh.nodes[nn];
+      pose_to(n.ptf, compose(bt, load_pose(r.ptf)));
+      for (int c = 0; c < 3; ++c) { n.lin[c] = r.lin[c]; n.ang[c] = r.ang[c]; }
+      n.mul = r.mul; n.off = r.off; n.src = r.src; n.parent = base; n.kind = r.kind;
+      const float a2 = r.ang[0] * r.ang[0] + r.ang[1] * r.ang[1] + r.ang[2] * r.ang[2];
+      const float l2 = r.lin[0] * r.lin[0] + r.lin[1] * r.lin[1] + r.lin[2] * r.lin[2];
+      n.jac_type = 0;
+      n.ax[0] = n.ax[1] = n.ax[2] = 0.f;
+      if (r.src >= 0) {
+        if (a2 > 1e-6f) {
+          const float inv = 1.f / sqrtf(a2);
+          n.jac_type = 1;
+          for (int c = 0; c < 3; ++c) n.ax[c] = r.ang[c] * inv;
+        } else if (l2 > 1e-6f) {
+          const float inv = 1.f / sqrtf(l2);
+          n.jac_type = 2;
+          for (int c = 0; c < 3; ++c) n.ax[c] = r.lin[c] * inv;
+        }
+      }
+      unsigned m = 0;
+      for (int e = 0; e < a.n_ee; ++e)
+        if (a.ancestor_masks[e * J + r.joint]) m |= 1u << e;
+      n.ee_mask = m;
+      anchor[i] = nn;
+      pose_to(rel + 7 * i, pose_identity());
+      ++nn;
+    }
+  }
+  sh.n_nodes = nn;
+  sh.too_many_nodes = nn > MAXA;
+  for (int e = 0; e < NEE; ++e) {
+    sh.ee_node[e] = -2;  // padded EE: zero residual
+    if (e < a.n_ee) {
+      int tj = a.target_jnts[e];
+      const int s = (tj >= 0 && tj < J) ? slot_of[tj] : -1;
+      if (s >= 0) {
+        sh.ee_node[e] = anchor[s];
+        for (int c = 0; c < 7; ++c) sh.ee_rel[e][c] = rel[7 * s + c];
+      } else {
+        sh.ee_node[e] = -1;
+        Pose id = pose_identity();
+        pose_to(sh.ee_rel[e], id);
+      }
+    }
+  }
+  // Robot spheres, regrouped by node (group g = node + 1; group 0 = world-fixed).
+  const int K = a.enable_collision ? a.n_robot_spheres : 0;
+  int* count = sh.cnt;
+  for (int g = 0; g <= nn; ++g) count[g] = 0;
+  for (int k = 0; k < K; ++k) {
+    const int jidx = a.robot_sphere_joint[k];
+    int g = -1;
+    if (jidx >= 0 && jidx < J && slot_of[jidx] >= 0) g = anchor[slot_of[jidx]] + 1;
+    sph_node_tmp[k] = g;
+    if (g >= 0) ++count[g];
+  }
+  int acc = 0;
+  for (int g = 0; g <= nn; ++g) { sh.sph_start[g] = acc; acc += count[g]; count[g] = sh.sph_start[g]; }
+  sh.sph_start[nn + 1] = acc;
+  sh.n_sph = acc;
+  for (int k = 0; k < K; ++k) {
+    const int g = sph_node_tmp[k];
+    if (g < 0) continue;
+    const int jidx = a.robot_sphere_joint[k];
+    const int s = slot_of[jidx];
+    const Pose rp = pose_from(rel + 7 * s);
+    const float* sp = a.robot_spheres + 4 * k;
+    float cx, cy, cz;
+    quat_rotate(rp.w, rp.x, rp.y, rp.z, sp[0], sp[1], sp[2], cx, cy, cz);
+    sph[count[g]++] = make_float4(cx + rp.tx, cy + rp.ty, cz + rp.tz, sp[3]);
+  }
+}
+
+// Forward kinematics of the node tree for one lane.  Node parents are
+// wave-uniform, so the parent lookup is a chain of scalar branches.
+template <int MAXA, int NEE>
+__device__ __forceinline__ void node_fk(const IkShared<MAXA, NEE>& sh, const float* q,
+                                        Pose* P) {
+#pragma unroll
+  for (int n = 0; n < MAXA; ++n) {
+    if (n < sh.n_nodes) {
+      const NodeRec& r = sh.nodes[n];
+      const int src = __builtin_amdgcn_readfirstlane(r.src);
+      float qs = 0.f;
+#pragma unroll
+      for (int a = 0; a < MAXA; ++a)
+        if (src == a) qs = q[a];
+      const float qj = qs * r.mul + r.off;
+      const Pose tl = compose(load_pose(r.ptf),
+                              joint_exp(r.kind, qj, r.lin[0], r.lin[1], r.lin[2], r.ang[0],
+                                        r.ang[1], r.ang[2]));
+      const int par = __builtin_amdgcn_readfirstlane(r.parent);
+      Pose pp = pose_identity();
+      bool has_parent = false;
+#pragma unroll
+      for (int m = 0; m < n; ++m)
+        if (par == m) { pp = P[m]; has_parent = true; }
+      P[n] = has_parent ? compose(pp, tl) : tl;
+    }
+  }
+}
+
+template <int MAXA, int NEE>
+__device__ __forceinline__ Pose ee_pose(const IkShared<MAXA, NEE>& sh, const Pose* P, int e) {
+  const int node = __builtin_amdgcn_readfirstlane(sh.ee_node[e]);
+  const Pose rel = load_pose(sh.ee_rel[e]);
+  Pose base = pose_identity();
+  bool found = false;
+#pragma unroll
+  for (int n = 0; n < MAXA; ++n)
+    if (node == n) { base = P[n]; found = true; }
+  return found ? compose(base, rel) : rel;
+}
+
+// Residual of one EE: [p - p_t; log(q q_t^-1)] (_ik_cuda_helpers.cuh:471-494).
+__device__ __forceinline__ void ee_residual(const Pose& T, const float* tgt, float* r) {
+  r[0] = T.tx - tgt[4];
+  r[1] = T.ty - tgt[5];
+  r[2] = T.tz - tgt[6];
+  float ew, ex, ey, ez;
+  quat_mul(T.w, T.x, T.y, T.z, tgt[0], -tgt[1], -tgt[2], -tgt[3], ew, ex, ey, ez);
+  if (ew < 0.f) { ew = -ew; ex = -ex; ey = -ey; ez = -ez; }
+  const float sh = sqrtf(ex * ex + ey * ey + ez * ez);
+  if (sh > 1e-6f) {
+    const float k = 2.f * atan2f(sh, ew) / sh;
+    r[3] = ex * k; r[4] = ey * k; r[5] = ez * k;
+  } else {
+    r[3] = 2.f * ex; r[4] = 2.f * ey; r[5] = 2.f * ez;
+  }
+}
+
+template <int MAXA, int NEE>
+__device__ __forceinline__ void residuals(const IkShared<MAXA, NEE>& sh, const Pose* P,
+                                          float* r, Pose* Tee) {
+#pragma unroll
+  for (int e = 0; e < NEE; ++e) {
+    if (sh.ee_node[e] == -2) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) r[6 * e + c] = 0.f;
+      Tee[e] = pose_identity();
+    } else {
+      Tee[e] = ee_pose(sh, P, e);
+      ee_residual(Tee[e], sh.tgt[e], r + 6 * e);
+    }
+  }
+}
+
+// Collision merit penalty sum of min(|c - o| - r - R - margin, 0)^2 (unweighted).
+template <int MAXA, int NEE>
+__device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, const Pose* P,
+                                                const float4* sph, const float4* ws, int M) {
+  float pen = 0.f;
+#pragma unroll
+  for (int g = 0; g <= MAXA; ++g) {
+    if (g <= sh.n_nodes) {
+      const int k0 = sh.sph_start[g], k1 = sh.sph_start[g + 1];
+      if (k1 > k0) {
+        const Pose T = (g == 0) ? pose_identity() : P[g > 0 ? g - 1 : 0];
+        const Rot3 R = quat_to_rot(T.w, T.x, T.y, T.z);
+        for (int k = k0; k < k1; ++k) {
+          const float4 s = sph[k];
+          const float cx = R.m00 * s.x + R.m01 * s.y + R.m02 * s.z + T.tx;
+          const float cy = R.m10 * s.x + R.m11 * s.y + R.m12 * s.z + T.ty;
+          const float cz = R.m20 * s.x + R.m21 * s.y + R.m22 * s.z + T.tz;
+          float p0 = 0.f, p1 = 0.f;
+          int m = 0;
+          for (; m + 1 < M; m += 2) {
+            const float4 o0 = ws[m], o1 = ws[m + 1];
+            const float ax = cx - o0.x, ay = cy - o0.y, az = cz - o0.z;
+            const float bx = cx - o1.x, by = cy - o1.y, bz = cz - o1.z;
+            const float e0 = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o0.w - s.w, 0.f);
+            const float e1 = fminf(__builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz) - o1.w - s.w, 0.f);
+            p0 = fmaf(e0, e0, p0);
+            p1 = fmaf(e1, e1, p1);
+          }
+          if (m < M) {
+            const float4 o0 = ws[m];
+            const float ax = cx - o0.x, ay = cy - o0.y, az = cz - o0.z;
+            const float e0 = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o0.w - s.w, 0.f);
+            p0 = fmaf(e0, e0, p0);
+          }
+          pen += p0 + p1;
+        }
+      }
+    }
+  }
+  return pen;
+}
+
+struct IkLds {
+  size_t recs, slot_of, anchor, rel, sph_tmp, sph, ws, total;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~static_cast<size_t>(15); }
+
+// Byte offsets of the IK kernel's LDS regions (float4 regions 16-byte aligned).
+__host__ __device__ inline IkLds ik_lds_layout(size_t shared_bytes, int J, int K, int M) {
+  IkLds o;
+  o.recs = align16(shared_bytes);
+  o.slot_of = align16(o.recs + sizeof(JointRec) * J);
+  o.anchor = o.slot_of + sizeof(int) * J;
+  o.rel = o.anchor + sizeof(int) * J;
+  o.sph_tmp = o.rel + sizeof(float) * 7 * J;
+  o.sph = align16(o.sph_tmp + sizeof(int) * K);
+  o.ws = o.sph + sizeof(float4) * K;
+  o.total = o.ws + sizeof(float4) * M;
+  return o;
+}
+
+__device__ __forceinline__ float ls_alpha(int i) {
+  return i == 0 ? 1.f : (i == 1 ? 0.5f : (i == 2 ? 0.25f : (i == 3 ? 0.1f : 0.025f)));
+}
+
+template <int MAXA, int NEE>
+__global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
+  extern __shared__ float4 smem4[];
+  using Sh = IkShared<MAXA, NEE>;
+  const int J = a.tables.n_joints;
+  const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
+  char* base = reinterpret_cast<char*>(smem4);
+  const IkLds off = ik_lds_layout(sizeof(Sh), J, Kin, a.enable_collision ? a.n_world : 0);
+  Sh& sh = *reinterpret_cast<Sh*>(base);
+  JointRec* recs = reinterpret_cast<JointRec*>(base + off.recs);
+  int* slot_of = reinterpret_cast<int*>(base + off.slot_of);
+  int* anchor = reinterpret_cast<int*>(base + off.anchor);
+  float* rel = reinterpret_cast<float*>(base + off.rel);
+  int* sph_tmp = reinterpret_cast<int*>(base + off.sph_tmp);
+  float4* sph = reinterpret_cast<float4*>(base + off.sph);
+  float4* ws = reinterpret_cast<float4*>(base + off.ws);
+
+  const int lane = threadIdx.x;
+  const int p = blockIdx.y;
+  stage_joint_records(a.tables, recs, slot_of);
+  if (lane == 0) build_problem<MAXA, NEE>(a, recs, slot_of, sh, anchor, rel, sph, sph_tmp);
+  for (int i = lane; i < NEE * 7; i += kWave) {
+    const int e = i / 7, c = i - e * 7;
+    sh.tgt[e][c] = (e < a.n_ee) ? a.target_T[(static_cast<long long>(p) * a.n_ee + e) * 7 + c]
+                                : (c == 0 ? 1.f : 0.f);
+  }
+  for (int i = lane; i < MAXA; i += kWave) {
+    const bool real = i < a.n_act;
+    sh.lo[i] = real ? a.lower[i] : 0.f;
+    sh.hi[i] = real ? a.upper[i] : 0.f;
+    sh.fixed[i] = real ? (a.fixed_mask[i] != 0) : 1;
+  }
+  const int M = (a.enable_collision && Kin > 0) ? a.n_world : 0;
+  for (int m = lane; m < M; m += kWave) {
+    const float* o = a.world + 4 * m;
+    ws[m] = make_float4(o[0], o[1], o[2], o[3] + a.collision_margin);
+  }
+  __syncthreads();
+
+  const int s = blockIdx.x * kWave + lane;
+  if (s >= a.n_seeds) return;
+  const long long gs = static_cast<long long>(p) * a.n_seeds + s;
+  if (sh.too_many_nodes) {  // reported to the caller as NaN errors, seeds unchanged
+    for (int i = 0; i < a.n_act; ++i) a.out_cfgs[gs * a.n_act + i] = a.seeds[gs * a.n_act + i];
+    a.out_err[gs] = NAN;
+    return;
+  }
+  const bool coll = M > 0 && sh.n_sph > 0;
+  const float wc = a.collision_weight;
+  float W[6] = {a.pos_weight, a.pos_weight, a.pos_weight, a.ori_weight, a.ori_weight, a.ori_weight};
+
+  float q[MAXA], best_q[MAXA];
+#pragma unroll
+  for (int i = 0; i < MAXA; ++i) {
+    q[i] = (i < a.n_act) ? a.seeds[gs * a.n_act + i] : 0.f;
+    best_q[i] = q[i];
+  }
+
+  Pose P[MAXA];
+  Pose Tee[NEE];
+  float r[6 * NEE];
+  node_fk(sh, q, P);
+  residuals(sh, P, r, Tee);
+  float best_err = 0.f;
+#pragma unroll
+  for (int i = 0; i < 6 * NEE; ++i) { const float v = r[i] * W[i % 6]; best_err += v * v; }
+  if (coll) best_err += wc * sphere_penalty(sh, P, sph, ws, M);
+
+  float lam = a.lambda_init;
+
+  for (int it = 0; it < a.max_iter; ++it) {
+    node_fk(sh, q, P);
+    residuals(sh, P, r, Tee);
+    bool conv = true;
+    float max_p = 0.f, max_o = 0.f;
+#pragma unroll
+    for (int e = 0; e < NEE; ++e) {
+      const float np = sqrtf(r[6 * e] * r[6 * e] + r[6 * e + 1] * r[6 * e + 1] + r[6 * e + 2] * r[6 * e + 2]);
+      const float no = sqrtf(r[6 * e + 3] * r[6 * e + 3] + r[6 * e + 4] * r[6 * e + 4] + r[6 * e + 5] * r[6 * e + 5]);
+      if (e < a.n_ee && (np >= a.eps_pos || no >= a.eps_ori)) conv = false;
+      max_p = fmaxf(max_p, np);
+      max_o = fmaxf(max_o, no);
+    }
+    if (conv) break;
+
+    float cur = 0.f;
+    float fw[6 * NEE];
+#pragma unroll
+    for (int i = 0; i < 6 * NEE; ++i) { fw[i] = r[i] * W[i % 6]; cur += fw[i] * fw[i]; }
+    if (coll) cur += wc * sphere_penalty(sh, P, sph, ws, M);
+
+    // Weighted geometric Jacobian, column a from the node(s) driven by a.
+    float Jm[6 * NEE][MAXA];
+#pragma unroll
+    for (int i = 0; i < 6 * NEE; ++i)
+#pragma unroll
+      for (int c = 0; c < MAXA; ++c) Jm[i][c] = 0.f;
+#pragma unroll
+    for (int n = 0; n < MAXA; ++n) {
+      if (n < sh.n_nodes) {
+        const NodeRec& nr = sh.nodes[n];
+        const int jt = __builtin_amdgcn_readfirstlane(nr.jac_type);
+        if (jt != 0) {
+          float zx, zy, zz;
+          quat_rotate(P[n].w, P[n].x, P[n].y, P[n].z, nr.ax[0], nr.ax[1], nr.ax[2], zx, zy, zz);
+          const int src = __builtin_amdgcn_readfirstlane(nr.src);
+          const unsigned em = __builtin_amdgcn_readfirstlane(nr.ee_mask);
+          const float mul = nr.mul;
+#pragma unroll
+          for (int e = 0; e < NEE; ++e) {
+            if (em & (1u << e)) {
+              float g[6];
+              if (jt == 1) {
+                const float ax = Tee[e].tx - P[n].tx, ay = Tee[e].ty - P[n].ty, az = Tee[e].tz - P[n].tz;
+                g[0] = zy * az - zz * ay; g[1] = zz * ax - zx * az; g[2] = zx * ay - zy * ax;
+                g[3] = zx; g[4] = zy; g[5] = zz;
+              } else {
+                g[0] = zx; g[1] = zy; g[2] = zz; g[3] = 0.f; g[4] = 0.f; g[5] = 0.f;
+              }
+#pragma unroll
+              for (int c = 0; c < MAXA; ++c)
+                if (src == c) {
+#pragma unroll
+                  for (int k = 0; k < 6; ++k) Jm[6 * e + k][c] += mul * g[k] * W[k];
+                }
+            }
+          }
+        }
+      }
+    }
+
+    // Column scaling and normal equations (packed lower triangle).
+    float cs[MAXA];
+#pragma unroll
+    for (int c = 0; c < MAXA; ++c) {
+      float sq = 0.f;
+#pragma unroll
+      for (int i = 0; i < 6 * NEE; ++i) sq += Jm[i][c] * Jm[i][c];
+      cs[c] = sqrtf(sq) + 1e-8f;
+    }
+#pragma unroll
+    for (int i = 0; i < 6 * NEE; ++i)
+#pragma unroll
+      for (int c = 0; c < MAXA; ++c) Jm[i][c] /= cs[c];
+    constexpr int NT = MAXA * (MAXA + 1) / 2;
+    float A[NT];
+    float rhs[MAXA];
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i) {
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6 * NEE; ++k) acc += Jm[k][i] * Jm[k][j];
+        A[i * (i + 1) / 2 + j] = acc;
+      }
+      float rb = 0.f;
+#pragma unroll
+      for (int k = 0; k < 6 * NEE; ++k) rb += Jm[k][i] * fw[k];
+      rhs[i] = -rb;
+      A[i * (i + 1) / 2 + i] += lam;
+    }
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i) {
+      if (sh.fixed[i]) {
+#pragma unroll
+        for (int j = 0; j < MAXA; ++j) {
+          if (j <= i) A[i * (i + 1) / 2 + j] = 0.f;
+          else A[j * (j + 1) / 2 + i] = 0.f;
+        }
+        A[i * (i + 1) / 2 + i] = 1.f;
+        rhs[i] = 0.f;
+      }
+    }
+    // Cholesky A = L L^T in place, then two triangular solves.
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < MAXA; ++k) {
+      float sd = A[k * (k + 1) / 2 + k];
+#pragma unroll
+      for (int pp = 0; pp < k; ++pp) sd -= A[k * (k + 1) / 2 + pp] * A[k * (k + 1) / 2 + pp];
+      if (sd <= 0.f) ok = false;
+      const float lkk = sqrtf(fmaxf(sd, 1e-30f));
+      A[k * (k + 1) / 2 + k] = lkk;
+      const float inv = 1.f / lkk;
+#pragma unroll
+      for (int i = k + 1; i < MAXA; ++i) {
+        float t = A[i * (i + 1) / 2 + k];
+#pragma unroll
+        for (int pp = 0; pp < k; ++pp) t -= A[i * (i + 1) / 2 + pp] * A[k * (k + 1) / 2 + pp];
+        A[i * (i + 1) / 2 + k] = t * inv;
+      }
+    }
+    float y[MAXA];
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i) {
+      float t = rhs[i];
+#pragma unroll
+      for (int pp = 0; pp < i; ++pp) t -= A[i * (i + 1) / 2 + pp] * y[pp];
+      y[i] = t / A[i * (i + 1) / 2 + i];
+    }
+    float delta[MAXA];
+#pragma unroll
+    for (int i = MAXA - 1; i >= 0; --i) {
+      float t = y[i];
+#pragma unroll
+      for (int pp = i + 1; pp < MAXA; ++pp) t -= A[pp * (pp + 1) / 2 + i] * delta[pp];
+      delta[i] = t / A[i * (i + 1) / 2 + i];
+    }
+    float dn = 0.f;
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i) {
+      delta[i] = ok ? delta[i] / cs[i] : 0.f;
+      dn += delta[i] * delta[i];
+    }
+    // Trust region.
+    float R;
+    if (max_p > 1e-2f || max_o > 0.6f) R = 0.38f;
+    else if (max_p > 1e-3f || max_o > 0.25f) R = 0.22f;
+    else if (max_p > 2e-4f || max_o > 0.08f) R = 0.12f;
+    else R = 0.05f;
+    dn = sqrtf(dn);
+    if (dn > R) {
+      const float sc = R / (dn + 1e-18f);
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) delta[i] *= sc;
+    }
+
+    // Line search: evaluate all five steps, keep the best.
+    float best_a_err = 1e30f;
+    int best_a = 0;
+#pragma unroll 1
+    for (int ai = 0; ai < 5; ++ai) {
+      float qt[MAXA];
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], q[i] + ls_alpha(ai) * delta[i]));
+      node_fk(sh, qt, P);
+      residuals(sh, P, r, Tee);
+      float et = 0.f;
+#pragma unroll
+      for (int i = 0; i < 6 * NEE; ++i) { const float v = r[i] * W[i % 6]; et += v * v; }
+      if (coll) et += wc * sphere_penalty(sh, P, sph, ws, M);
+      if (et < best_a_err) { best_a_err = et; best_a = ai; }
+    }
+    const float al = ls_alpha(best_a);
+    float qt[MAXA];
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], q[i] + al * delta[i]));
+    if (best_a_err < cur * (1.f - 1e-4f)) {
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) q[i] = qt[i];
+      lam = fmaxf(lam * 0.5f, 1e-10f);
+    } else {
+      lam = fminf(lam * 3.f, 1e6f);
+    }
+    if (best_a_err < best_err) {
+      best_err = best_a_err;
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) best_q[i] = qt[i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MAXA; ++i)
+    if (i < a.n_act) a.out_cfgs[gs * a.n_act + i] = best_q[i];
+  a.out_err[gs] = best_err;
+}
+
+template <int MAXA, int NEE>
+static int launch_ik(void* stream, const IkArgs& a) {
+  using Sh = IkShared<MAXA, NEE>;
+  const int J = a.tables.n_joints;
+  const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
+  const int M = a.enable_collision ? a.n_world : 0;
+  const size_t lds = ik_lds_layout(sizeof(Sh), J, Kin, M).total;
+  PK_REQUIRE(lds <= 150 * 1024, "IK tables need %zu bytes of LDS (limit 150 KiB)", lds);
+  dim3 grid((a.n_seeds + kWave - 1) / kWave, a.n_problems);
+  hipLaunchKernelGGL((ls_ik_kernel<MAXA, NEE>), grid, dim3(kWave), lds,
+                     static_cast<hipStream_t>(stream), a);
+  return check_launch("ls_ik_kernel");
+}
+
+// Per problem: argmin over seeds of err + w |q - prev|^2, first index on ties.
+__global__ __launch_bounds__(64) void select_kernel(const float* __restrict__ cfgs,
+                                                    const float* __restrict__ errs,
+                                                    const float* __restrict__ prev, int S, int A,
+                                                    float cw, float* __restrict__ out_best,
+                                                    int* __restrict__ out_idx) {
+  const int p = blockIdx.x;
+  const int lane = threadIdx.x;
+  float bv = INFINITY;
+  int bi = 0x7fffffff;
+  for (int s = lane; s < S; s += kWave) {
+    const long long g = static_cast<long long>(p) * S + s;
+    float v = errs[g];
+    if (cw != 0.f) {
+      float d2 = 0.f;
+      for (int i = 0; i < A; ++i) {
+        const float d = cfgs[g * A + i] - prev[static_cast<long long>(p) * A + i];
+        d2 += d * d;
+      }
+      v += cw * d2;
+    }
+    if (v < bv || (v == bv && s < bi) || (v != v && bi == 0x7fffffff)) { bv = v; bi = s; }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float ov = __shfl_xor(bv, off, kWave);
+    const int oi = __shfl_xor(bi, off, kWave);
+    if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+  }
+  if (bi == 0x7fffffff) bi = 0;
+  for (int i = lane; i < A; i += kWave)
+    out_best[static_cast<long long>(p) * A + i] = cfgs[(static_cast<long long>(p) * S + bi) * A + i];
+  if (out_idx != nullptr && lane == 0) out_idx[p] = bi;
+}
+
+}  // namespace pk
+
+extern "C" int pk_ls_ik_f32(
+    void* stream, const float* seeds, int32_t n_problems, int32_t n_seeds, int32_t n_act,
+    const float* twists, const float* parent_tf, const int32_t* parent_idx,
+    const int32_t* act_idx, const float* mimic_mul, const float* mimic_off,
+    const int32_t* mimic_act_idx, const int32_t* topo_inv, int32_t n_joints,
+    const int32_t* target_jnts, const int32_t* ancestor_masks, int32_t n_ee,
+    const float* target_T, const float* robot_spheres_local,
+    const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres,
+    const float* world_spheres, int32_t n_world_spheres, const float* world_capsules,
+    int32_t n_world_capsules, const float* world_boxes, int32_t n_world_boxes,
+    const float* world_halfspaces, int32_t n_world_halfspaces, const float* lower,
+    const float* upper, const int32_t* fixed_mask, int32_t max_iter, float pos_weight,
+    float ori_weight, float lambda_init, float eps_pos, float eps_ori,
+    int32_t enable_collision, float collision_weight, float collision_margin,
+    float* out_cfgs, float* out_err) {
+  (void)world_capsules; (void)world_boxes; (void)world_halfspaces;
+  PK_REQUIRE(n_problems >= 0 && n_seeds >= 0 && n_act >= 0 && n_ee >= 0, "negative size");
+  PK_REQUIRE(n_joints >= 0 && n_joints <= pk::kMaxJoints, "n_joints=%d outside [0, %d]", n_joints,
+             pk::kMaxJoints);
+  PK_REQUIRE(n_act <= pk::kMaxAct, "n_act=%d exceeds %d", n_act, pk::kMaxAct);
+  PK_REQUIRE(n_ee <= pk::kMaxEE, "n_ee=%d exceeds %d", n_ee, pk::kMaxEE);
+  PK_REQUIRE(n_problems <= 65535, "n_problems=%d exceeds the 65535 grid limit", n_problems);
+  PK_REQUIRE(max_iter >= 0, "max_iter must be >= 0");
+  const bool coll = enable_collision != 0 && n_robot_spheres > 0;
+  if (coll)
+    PK_REQUIRE(n_world_capsules == 0 && n_world_boxes == 0 && n_world_halfspaces == 0,
+               "only sphere worlds are supported in the LM collision penalty");
+  PK_REQUIRE(n_robot_spheres >= 0 && n_robot_spheres <= 4096, "n_robot_spheres=%d outside [0, 4096]",
+             n_robot_spheres);
+  PK_REQUIRE(n_world_spheres >= 0 && n_world_spheres <= 4096, "n_world_spheres=%d outside [0, 4096]",
+             n_world_spheres);
+  if (n_problems == 0 || n_seeds == 0) return PK_OK;
+  PK_REQUIRE(seeds && out_cfgs && out_err && lower && upper && fixed_mask, "null pointer");
+  PK_REQUIRE(n_ee == 0 || (target_jnts && ancestor_masks && target_T), "null target pointer");
+  if (coll) PK_REQUIRE(robot_spheres_local && robot_sphere_joint_idx &&
+                           (n_world_spheres == 0 || world_spheres),
+                       "null collision pointer");
+  pk::IkArgs a{};
+  a.seeds = seeds;
+  a.n_problems = n_problems;
+  a.n_seeds = n_seeds;
+  a.n_act = n_act;
+  a.tables = pk::RobotTables{twists, parent_tf, parent_idx, act_idx, mimic_mul,
+                             mimic_off, mimic_act_idx, topo_inv, n_joints, n_act};
+  a.target_jnts = target_jnts;
+  a.ancestor_masks = ancestor_masks;
+  a.n_ee = n_ee;
+  a.target_T = target_T;
+  a.robot_spheres = robot_spheres_local;
+  a.robot_sphere_joint = robot_sphere_joint_idx;
+  a.n_robot_spheres = coll ? n_robot_spheres : 0;
+  a.world = world_spheres;
+  a.n_world = coll ? n_world_spheres : 0;
+  a.lower = lower;
+  a.upper = upper;
+  a.fixed_mask = fixed_mask;
+  a.max_iter = max_iter;
+  a.pos_weight = pos_weight;
+  a.ori_weight = ori_weight;
+  a.lambda_init = lambda_init;
+  a.eps_pos = eps_pos;
+  a.eps_ori = eps_ori;
+  a.enable_collision = coll ? 1 : 0;
+  a.collision_weight = collision_weight;
+  a.collision_margin = collision_margin;
+  a.out_cfgs = out_cfgs;
+  a.out_err = out_err;
+  // Padded widths: the moving-joint count (actuated + mimic + offset-only
+  // joints) must fit; a robot that exceeds it gets NaN errors (see header).
+  const int nee = n_ee <= 1 ? 1 : (n_ee <= 2 ? 2 : 4);
+  const int wide = n_act <= 8 ? 8 : 16;
+  if (wide == 8) {
+    if (nee == 1) return pk::launch_ik<8, 1>(stream, a);
+    if (nee == 2) return pk::launch_ik<8, 2>(stream, a);
+    return pk::launch_ik<8, 4>(stream, a);
+  }
+  if (nee == 1) return pk::launch_ik<16, 1>(stream, a);
+  if (nee == 2) return pk::launch_ik<16, 2>(stream, a);
+  return pk::launch_ik<16, 4>(stream, a);
+}
+
+extern "C" int pk_ls_ik_select_f32(void* stream, const float* cfgs, const float* errs,
+                                   const float* previous, int32_t n_problems, int32_t n_seeds,
+                                   int32_t n_act, float continuity_weight, float* out_best,
+                                   int32_t* out_idx) {
+  PK_REQUIRE(n_problems >= 0 && n_seeds >= 0 && n_act >= 0, "negative size");
+  if (n_problems == 0) return PK_OK;
+  PK_REQUIRE(n_seeds > 0, "n_seeds must be > 0");
+  PK_REQUIRE(cfgs && errs && out_best && (continuity_weight == 0.f || previous), "null pointer");
+  PK_REQUIRE(n_problems <= 0x7fffffff, "too many problems");
+  hipLaunchKernelGGL(pk::select_kernel, dim3(n_problems), dim3(pk::kWave), 0,
+                     static_cast<hipStream_t>(stream), cfgs, errs, previous, n_seeds, n_act,
+                     continuity_weight, out_best, out_idx);
+  return pk::check_launch("select_kernel");
+}

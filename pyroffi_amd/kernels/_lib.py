@@ -1,0 +1,88 @@
+"""ctypes binding of ``libpyroffi_amd.so`` (declared in ``include/pyroffi_amd.h``).
+
+This is the ROCm counterpart of the reference's loader
+(``src/pyroffi/cuda_kernels/_fk_cuda.py:26-49``): the reference opens its
+``.so`` with ``ctypes.CDLL`` and hands the handler symbol to XLA; here the
+same library is called directly with device pointers taken from torch tensors
+and the current HIP stream.  A missing library raises ``RuntimeError`` -- there
+is no fallback implementation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import functools
+import os
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "libpyroffi_amd.so")
+
+_p = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f32 = ctypes.c_float
+
+_SIGNATURES = {
+    "pk_last_error": (ctypes.c_char_p, []),
+    "pk_abi_version": (_i32, []),
+    "pk_fk_f32": (_i32, [_p, _p, _i64, _i32] + [_p] * 10 + [_i32, _i32, _p]),
+    "pk_fk_links_f32": (_i32, [_p, _p, _i64, _i32] + [_p] * 8 + [_i32, _p, _i32, _p]),
+    "pk_collision_world_sphere_reduced_f32": (
+        _i32,
+        [_p, _p, _p, _i64, _i32, _p, _i32, _p, _i32, _p, _i32, _p, _i32, _i32, _p],
+    ),
+    "pk_collision_self_sphere_f32": (_i32, [_p, _p, _p, _i64, _i32, _i32, _p, _p, _i32, _p]),
+    "pk_sphere_sweep_f32": (
+        _i32,
+        [_p, _p, _i64, _i32, _p, _p, _i32, _p, _i32, _p, _i32, _i32, _p, _p, _p],
+    ),
+    "pk_ls_ik_f32": (
+        _i32,
+        [_p, _p, _i32, _i32, _i32]  # stream, seeds, P, S, A
+        + [_p] * 8 + [_i32]  # robot tables, n_joints
+        + [_p, _p, _i32, _p]  # target_jnts, ancestor_masks, n_ee, target_T
+        + [_p, _p, _i32]  # robot spheres
+        + [_p, _i32, _p, _i32, _p, _i32, _p, _i32]  # world primitives
+        + [_p, _p, _p]  # lower, upper, fixed_mask
+        + [_i32, _f32, _f32, _f32, _f32, _f32, _i32, _f32, _f32]  # attributes
+        + [_p, _p],  # outputs
+    ),
+    "pk_ls_ik_select_f32": (_i32, [_p, _p, _p, _p, _i32, _i32, _i32, _f32, _p, _p]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+@functools.lru_cache(maxsize=1)
+def lib() -> ctypes.CDLL:
+    """Load the library once; raise ``RuntimeError`` when it is not built."""
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(
+            f"HIP library not found at {_LIB_PATH}.\n"
+            "Build it first:  python -m pyroffi_amd._build"
+        )
+    handle = ctypes.CDLL(_LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    return handle
+
+
+class PkError(RuntimeError):
+    """A C-ABI call failed with a HIP error (the reference's kInternal)."""
+
+
+class PkInvalidArgument(PkError, ValueError):
+    """A C-ABI call rejected its arguments (the reference's kInvalidArgument)."""
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = lib().pk_last_error().decode(errors="replace")
+        if status == 1:
+            raise PkInvalidArgument(f"{what}: invalid argument: {msg}")
+        raise PkError(f"{what}: HIP error: {msg}")

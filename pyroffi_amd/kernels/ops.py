@@ -1,0 +1,224 @@
+"""Tensor-level wrappers of the C ABI, mirroring the reference's FFI wrappers.
+
+Each function takes torch tensors that live on the GPU (any float/int dtype is
+cast on the device, like the reference's ``astype`` calls), allocates its
+outputs with torch, and enqueues one library call on torch's current HIP
+stream.  Names and argument order follow the reference:
+
+* ``fk_cuda``                      -- ``src/pyroffi/cuda_kernels/_fk_cuda.py:52-126``
+* ``collision_world_sphere_reduced`` -- ``src/pyroffi/cuda_kernels/_collision_cuda_ffi.py:109-151``
+* ``collision_self_sphere``        -- ``src/pyroffi/cuda_kernels/_collision_cuda_ffi.py:191-216``
+* ``ls_ik_cuda``                   -- ``src/pyroffi/cuda_kernels/_ls_ik_cuda.py:76-183``
+
+``fk_links`` and ``sphere_sweep`` are the fused entry points the
+higher-level API uses.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import torch
+
+from ._lib import check, lib
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev(x, dtype: torch.dtype, device=None) -> torch.Tensor:
+    """Contiguous device tensor of `dtype` (host inputs are uploaded)."""
+    if not isinstance(x, torch.Tensor):
+        x = torch.as_tensor(x)
+    if not x.is_cuda:
+        x = x.to(device if device is not None else "cuda")
+    if x.dtype != dtype:
+        x = x.to(dtype)
+    return x.contiguous()
+
+
+def _ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _tables(twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, mimic_act_idx, topo_inv, device):
+    f, i = torch.float32, torch.int32
+    return (
+        _dev(twists, f, device),
+        _dev(parent_tf, f, device),
+        _dev(parent_idx, i, device),
+        _dev(act_idx, i, device),
+        _dev(mimic_mul, f, device),
+        _dev(mimic_off, f, device),
+        _dev(mimic_act_idx, i, device),
+        _dev(topo_inv, i, device),
+    )
+
+
+def fk_hip(cfg, twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, mimic_act_idx,
+           topo_inv, fk_level_starts, fk_level_joints) -> torch.Tensor:
+    """World pose of every joint, ``(*batch, n_joints, 7)`` wxyz_xyz."""
+    cfg = _dev(cfg, torch.float32)
+    dev = cfg.device
+    batch_axes = tuple(cfg.shape[:-1])
+    n_act = cfg.shape[-1]
+    batch = math.prod(batch_axes)
+    tb = _tables(twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, mimic_act_idx, topo_inv, dev)
+    n_joints = tb[0].shape[0]
+    starts = _dev(fk_level_starts, torch.int32, dev)
+    joints = _dev(fk_level_joints, torch.int32, dev)
+    out = torch.empty((batch, n_joints, 7), dtype=torch.float32, device=dev)
+    st = lib().pk_fk_f32(
+        _stream(), _ptr(cfg), batch, n_act, *[_ptr(t) for t in tb], _ptr(starts), _ptr(joints),
+        n_joints, max(0, starts.shape[0] - 1), _ptr(out),
+    )
+    check(st, "pk_fk_f32")
+    return out.reshape(*batch_axes, n_joints, 7)
+
+
+fk_cuda = fk_hip
+
+
+def fk_links(cfg, twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, mimic_act_idx,
+             topo_inv, link_parent_joint) -> torch.Tensor:
+    """World pose of every link, ``(*batch, n_links, 7)`` (FK fused with the link gather)."""
+    cfg = _dev(cfg, torch.float32)
+    dev = cfg.device
+    batch_axes = tuple(cfg.shape[:-1])
+    n_act = cfg.shape[-1]
+    batch = math.prod(batch_axes)
+    tb = _tables(twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, mimic_act_idx, topo_inv, dev)
+    lpj = _dev(link_parent_joint, torch.int32, dev)
+    n_links = lpj.shape[0]
+    out = torch.empty((batch, n_links, 7), dtype=torch.float32, device=dev)
+    st = lib().pk_fk_links_f32(
+        _stream(), _ptr(cfg), batch, n_act, *[_ptr(t) for t in tb], tb[0].shape[0], _ptr(lpj),
+        n_links, _ptr(out),
+    )
+    check(st, "pk_fk_links_f32")
+    return out.reshape(*batch_axes, n_links, 7)
+
+
+def collision_world_sphere_reduced(sphere_centers, sphere_radii, world_spheres, world_capsules,
+                                   world_boxes, world_halfspaces, n: int) -> torch.Tensor:
+    """``[B, N, M]`` per-link world distances from SoA centres ``[3, B, K]``."""
+    sc = _dev(sphere_centers, torch.float32)
+    dev = sc.device
+    sr = _dev(sphere_radii, torch.float32, dev)
+    ws = _dev(world_spheres, torch.float32, dev).reshape(-1, 4)
+    wc = _dev(world_capsules, torch.float32, dev).reshape(-1, 7)
+    wb = _dev(world_boxes, torch.float32, dev).reshape(-1, 15)
+    wh = _dev(world_halfspaces, torch.float32, dev).reshape(-1, 6)
+    _, B, K = sc.shape
+    M = ws.shape[0] + wc.shape[0] + wb.shape[0] + wh.shape[0]
+    out = torch.empty((B, n, M), dtype=torch.float32, device=dev)
+    st = lib().pk_collision_world_sphere_reduced_f32(
+        _stream(), _ptr(sc), _ptr(sr), B, K, _ptr(ws), ws.shape[0], _ptr(wc), wc.shape[0],
+        _ptr(wb), wb.shape[0], _ptr(wh), wh.shape[0], n, _ptr(out),
+    )
+    check(st, "pk_collision_world_sphere_reduced_f32")
+    return out
+
+
+def collision_self_sphere(centers, radii, pair_i, pair_j) -> torch.Tensor:
+    """``[B, P]`` link-pair distances from AoS centres ``[B, S, N, 3]``."""
+    c = _dev(centers, torch.float32)
+    dev = c.device
+    r = _dev(radii, torch.float32, dev)
+    pi = _dev(pair_i, torch.int32, dev)
+    pj = _dev(pair_j, torch.int32, dev)
+    B, S, N, _ = c.shape
+    P = pi.shape[0]
+    out = torch.empty((B, P), dtype=torch.float32, device=dev)
+    st = lib().pk_collision_self_sphere_f32(
+        _stream(), _ptr(c), _ptr(r), B, S, N, _ptr(pi), _ptr(pj), P, _ptr(out)
+    )
+    check(st, "pk_collision_self_sphere_f32")
+    return out
+
+
+def sphere_sweep(link_poses, spheres_local, link_sphere_start, world_spheres, self_entries,
+                 n_pairs: int, *, world: bool = False, world_min: bool = False,
+                 self_pairs: bool = False):
+    """Fused distances from link poses ``(B, L, 7)``.
+
+    Returns ``(out_world (B, L, M) | None, out_world_min (B,) | None,
+    out_self (B, P) | None)``.
+    """
+    lp = _dev(link_poses, torch.float32)
+    dev = lp.device
+    B, L, _ = lp.shape
+    sl = _dev(spheres_local, torch.float32, dev).reshape(-1, 4)
+    ls = _dev(link_sphere_start, torch.int32, dev)
+    ws = _dev(world_spheres, torch.float32, dev).reshape(-1, 4)
+    ents = _dev(self_entries, torch.int32, dev)
+    M = ws.shape[0]
+    ow = torch.empty((B, L, M), dtype=torch.float32, device=dev) if world else None
+    om = torch.empty((B,), dtype=torch.float32, device=dev) if world_min else None
+    osf = torch.empty((B, n_pairs), dtype=torch.float32, device=dev) if self_pairs else None
+    st = lib().pk_sphere_sweep_f32(
+        _stream(), _ptr(lp), B, L, _ptr(sl), _ptr(ls), sl.shape[0], _ptr(ws), M, _ptr(ents),
+        ents.shape[0], n_pairs, _ptr(ow), _ptr(om), _ptr(osf),
+    )
+    check(st, "pk_sphere_sweep_f32")
+    return ow, om, osf
+
+
+def ls_ik_hip(seeds, twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, mimic_act_idx,
+              topo_inv, target_jnts, ancestor_masks, target_T, robot_spheres_local,
+              robot_sphere_joint_idx, world_spheres, world_capsules, world_boxes,
+              world_halfspaces, lower, upper, fixed_mask, *, max_iter: int, pos_weight: float,
+              ori_weight: float, lambda_init: float, eps_pos: float, eps_ori: float,
+              enable_collision: bool = False, collision_weight: float = 0.0,
+              collision_margin: float = 0.02):
+    """Multi-seed LM; returns ``(cfgs (P, S, A), errs (P, S))``."""
+    sd = _dev(seeds, torch.float32)
+    dev = sd.device
+    P, S, A = sd.shape
+    tb = _tables(twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, mimic_act_idx, topo_inv, dev)
+    tj = _dev(target_jnts, torch.int32, dev).reshape(-1)
+    n_ee = tj.shape[0]
+    am = _dev(ancestor_masks, torch.int32, dev).reshape(n_ee, -1)
+    tT = _dev(target_T, torch.float32, dev).reshape(P, n_ee, 7)
+    rs = _dev(robot_spheres_local, torch.float32, dev).reshape(-1, 4)
+    rj = _dev(robot_sphere_joint_idx, torch.int32, dev).reshape(-1)
+    ws = _dev(world_spheres, torch.float32, dev).reshape(-1, 4)
+    wc = _dev(world_capsules, torch.float32, dev).reshape(-1, 7)
+    wb = _dev(world_boxes, torch.float32, dev).reshape(-1, 15)
+    wh = _dev(world_halfspaces, torch.float32, dev).reshape(-1, 6)
+    lo = _dev(lower, torch.float32, dev)
+    hi = _dev(upper, torch.float32, dev)
+    fm = _dev(fixed_mask, torch.int32, dev)
+    out = torch.empty((P, S, A), dtype=torch.float32, device=dev)
+    err = torch.empty((P, S), dtype=torch.float32, device=dev)
+    st = lib().pk_ls_ik_f32(
+        _stream(), _ptr(sd), P, S, A, *[_ptr(t) for t in tb], tb[0].shape[0],
+        _ptr(tj), _ptr(am), n_ee, _ptr(tT), _ptr(rs), _ptr(rj), rs.shape[0],
+        _ptr(ws), ws.shape[0], _ptr(wc), wc.shape[0], _ptr(wb), wb.shape[0], _ptr(wh), wh.shape[0],
+        _ptr(lo), _ptr(hi), _ptr(fm), int(max_iter), float(pos_weight), float(ori_weight),
+        float(lambda_init), float(eps_pos), float(eps_ori), int(bool(enable_collision)),
+        float(collision_weight), float(collision_margin), _ptr(out), _ptr(err),
+    )
+    check(st, "pk_ls_ik_f32")
+    return out, err
+
+
+ls_ik_cuda = ls_ik_hip
+
+
+def ls_ik_select(cfgs, errs, previous, continuity_weight: float = 0.0):
+    """Per-problem winner ``(best (P, A), index (P,))``."""
+    c = _dev(cfgs, torch.float32)
+    dev = c.device
+    e = _dev(errs, torch.float32, dev)
+    P, S, A = c.shape
+    prev = _dev(previous, torch.float32, dev).reshape(P, A)
+    best = torch.empty((P, A), dtype=torch.float32, device=dev)
+    idx = torch.empty((P,), dtype=torch.int32, device=dev)
+    st = lib().pk_ls_ik_select_f32(
+        _stream(), _ptr(c), _ptr(e), _ptr(prev), P, S, A, float(continuity_weight), _ptr(best), _ptr(idx)
+    )
+    check(st, "pk_ls_ik_select_f32")
+    return best, idx

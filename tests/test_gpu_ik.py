@@ -1,0 +1,155 @@
+"""GPU parity: multi-seed LM IK kernel vs the FP64 oracle on the same seeds.
+
+* max_iter = 0 returns the seeds and their weighted error plus collision
+  penalty: a deterministic check of residual, weights and penalty (1e-4 rel).
+* Full solves: FP32 and FP64 trajectories can part ways on ill-conditioned
+  seeds, so parity is on outcomes -- per-seed success fraction (pos <= 1 mm and
+  rot <= 0.01 rad) within 2 percentage points, per-problem success within
+  0.5 pp, winners within the joint limits.
+"""
+
+import numpy as np
+import pytest
+import torch
+from conftest import load_robot, random_cfgs
+from scipy.spatial.transform import Rotation
+
+import pyroffi_amd as pk
+from pyroffi_amd.collision import RobotCollisionSpherized
+from pyroffi_amd.kernels import ops
+from pyroffi_amd.optimization_engines._ls_ik import ee_tables, ls_ik_run, make_seeds, prepare_ls_collision_buffers
+
+pytestmark = pytest.mark.gpu
+
+
+EE_LINK = {"panda": "panda_hand", "ur5": "ee_link", "fetch": "gripper_link", "mimic_arm": "finger_c"}
+
+
+def _ee(robot, name):
+    return robot.links.names.index(EE_LINK[name])
+
+
+def _problems(oracle_mod, robot, links, n_prob, n_seeds, seed):
+    t = oracle_mod.Tables.of(robot)
+    qt = random_cfgs(robot, n_prob, seed=seed)
+    lp = oracle_mod.fk_links(t, qt)
+    tgt = np.stack([lp[:, l] for l in links], axis=1)  # (P, n_ee, 7)
+    rng = np.random.default_rng(seed + 1)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    seeds = rng.uniform(lo, hi, (n_prob, n_seeds, robot.joints.num_actuated_joints)).astype(np.float32)
+    return t, tgt, seeds
+
+
+def _errors(oracle_mod, t, cfgs, tgt, links):
+    lp = oracle_mod.fk_links(t, cfgs.astype(np.float64))
+    pos, rot = [], []
+    for e, l in enumerate(links):
+        g = lp[..., l, :]
+        T = np.broadcast_to(tgt[:, None, e], g.shape)
+        pos.append(np.linalg.norm(g[..., 4:] - T[..., 4:], axis=-1))
+        rg = Rotation.from_quat(g.reshape(-1, 7)[:, [1, 2, 3, 0]])
+        rt = Rotation.from_quat(T.reshape(-1, 7)[:, [1, 2, 3, 0]])
+        rot.append(np.linalg.norm((rt.inv() * rg).as_rotvec(), axis=-1).reshape(g.shape[:-1]))
+    return np.max(pos, axis=0), np.max(rot, axis=0)
+
+
+def _run_both(oracle_mod, robot, links, tgt, seeds, t, **kw):
+    coll = kw.pop("collision", None)
+    cfgs, errs = ls_ik_run(robot, seeds, tgt, links, collision=coll, **kw)
+    tj, masks = ee_tables(robot, links)
+    okw = dict(kw)
+    if coll is not None:
+        rs, rj, ws, w, m = coll
+        okw.update(robot_spheres=rs, sphere_joint=rj, world=ws, enable_collision=True, collision_weight=w,
+                   collision_margin=m)
+    ocfgs, oerrs = oracle_mod.ls_ik(t, seeds, tj, masks, tgt, robot.joints.lower_limits, robot.joints.upper_limits,
+                                    **okw)
+    return cfgs.cpu().numpy(), errs.cpu().numpy(), ocfgs, oerrs
+
+
+def _collision(robot, path, n_world, seed, weight=1e4, margin=0.02):
+    coll = RobotCollisionSpherized.from_urdf(path)
+    rng = np.random.default_rng(seed)
+    world = np.concatenate([rng.uniform(-1, 1, (n_world, 3)), rng.uniform(0.02, 0.15, (n_world, 1))], 1)
+    rs, rj, ws, ok = prepare_ls_collision_buffers(robot, coll, world.astype(np.float32))
+    assert ok
+    return (rs, rj, ws, weight, margin)
+
+
+@pytest.mark.parametrize("name,with_coll", [("panda", False), ("panda", True), ("ur5", False),
+                                            ("fetch", True), ("mimic_arm", False)])
+def test_initial_error_and_penalty(oracle_mod, name, with_coll):
+    robot, path = load_robot(name)
+    link = _ee(robot, name)
+    t, tgt, seeds = _problems(oracle_mod, robot, (link,), 4, 100, seed=3)
+    coll = _collision(robot, path, 256, 4) if with_coll else None
+    cfgs, errs, ocfgs, oerrs = _run_both(oracle_mod, robot, (link,), tgt, seeds, t, max_iter=0, collision=coll)
+    np.testing.assert_array_equal(cfgs, seeds)
+    np.testing.assert_allclose(errs, oerrs, rtol=1e-4, atol=1e-6)
+
+
+def test_baxter_two_end_effectors_initial_error(oracle_mod):
+    robot, _ = load_robot("baxter")
+    names = robot.links.names
+    links = (names.index("left_gripper_base"), names.index("right_gripper_base"))
+    t, tgt, seeds = _problems(oracle_mod, robot, links, 3, 64, seed=5)
+    cfgs, errs, _, oerrs = _run_both(oracle_mod, robot, links, tgt, seeds, t, max_iter=0)
+    np.testing.assert_allclose(errs, oerrs, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("name,with_coll", [("panda", False), ("panda", True), ("fetch", False),
+                                            ("mimic_arm", False)])
+def test_solve_success_parity(oracle_mod, name, with_coll):
+    robot, path = load_robot(name)
+    link = _ee(robot, name)
+    t, tgt, seeds = _problems(oracle_mod, robot, (link,), 16, 64, seed=7)
+    coll = _collision(robot, path, 64, 8) if with_coll else None
+    kw = dict(max_iter=30, lambda_init=1e-2)
+    cfgs, errs, ocfgs, oerrs = _run_both(oracle_mod, robot, (link,), tgt, seeds, t, collision=coll, **kw)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    assert np.all(cfgs >= lo - 1e-6) and np.all(cfgs <= hi + 1e-6)
+    pos, rot = _errors(oracle_mod, t, cfgs, tgt, (link,))
+    opos, orot = _errors(oracle_mod, t, ocfgs, tgt, (link,))
+    ok, ook = (pos <= 1e-3) & (rot <= 0.01), (opos <= 1e-3) & (orot <= 0.01)
+    assert abs(ok.mean() - ook.mean()) <= 0.02, (ok.mean(), ook.mean())
+    win = ok[np.arange(16), np.argmin(errs, axis=1)]
+    owin = ook[np.arange(16), np.argmin(oerrs, axis=1)]
+    assert abs(win.mean() - owin.mean()) <= 0.005, (win.mean(), owin.mean())
+    # the returned error is the all-time best, never above the seed's own error
+    _, e0, _, _ = _run_both(oracle_mod, robot, (link,), tgt, seeds, t, collision=coll, max_iter=0)
+    assert np.all(errs <= e0 * (1 + 1e-5) + 1e-6)
+
+
+def test_select_kernel_matches_numpy():
+    rng = np.random.default_rng(0)
+    cfgs = rng.normal(size=(37, 100, 7)).astype(np.float32)
+    errs = rng.uniform(size=(37, 100)).astype(np.float32)
+    errs[3, [5, 9]] = -1.0  # tie: the first index wins
+    prev = rng.normal(size=(37, 7)).astype(np.float32)
+    for cw in (0.0, 0.5):
+        best, idx = ops.ls_ik_select(torch.as_tensor(cfgs, device="cuda"), torch.as_tensor(errs, device="cuda"),
+                                     torch.as_tensor(prev, device="cuda"), cw)
+        score = errs + cw * ((cfgs - prev[:, None]) ** 2).sum(-1)
+        ref = np.argmin(score, axis=1)
+        np.testing.assert_array_equal(idx.cpu().numpy(), ref)
+        np.testing.assert_array_equal(best.cpu().numpy(), cfgs[np.arange(37), ref])
+
+
+def test_public_api_solves(oracle_mod):
+    robot, path = load_robot("panda")
+    t = oracle_mod.Tables.of(robot)
+    link = robot.links.names.index("panda_hand")
+    qt = random_cfgs(robot, 8, seed=31)
+    tgt = oracle_mod.fk_links(t, qt)[:, link]
+    prev = np.tile(robot.default_joint_cfg, (8, 1))
+    sol = pk.ls_ik_solve_cuda_batch(robot, link, pk.SE3(tgt), 0, prev, num_seeds=64, max_iter=40)
+    pos, rot = _errors(oracle_mod, t, sol[:, None], tgt[:, None], (link,))
+    assert np.mean((pos <= 1e-3) & (rot <= 0.01)) >= 0.75
+    one = pk.ls_ik_solve_cuda(robot, (link,), (pk.SE3(tgt[0]),), 1, prev[0], num_seeds=64, max_iter=40)
+    assert one.shape == (7,)
+    coll = RobotCollisionSpherized.from_urdf(path)
+    chk = pk.collision.CUDARobotCollisionChecker(coll)
+    world = np.array([[2.0, 2.0, 2.0, 0.1]], np.float32)
+    sol_c = pk.ls_ik_solve_cuda_batch(robot, link, pk.SE3(tgt), 0, prev, num_seeds=64, max_iter=40,
+                                      collision_free=True, collision_checker=chk, collision_world=world)
+    assert sol_c.shape == (8, 7)

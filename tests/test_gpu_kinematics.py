@@ -1,0 +1,166 @@
+"""GPU parity: FK and sphere distances (HIP kernels) vs the FP64 oracle.
+
+Tolerances: poses within 1e-5 (quaternion components and metres), distances
+within 1e-5 m -- FP32 arithmetic against an FP64 restatement of the same
+algorithm, on the same float32 tables.
+"""
+
+import numpy as np
+import pytest
+import torch
+from conftest import ROBOTS, load_robot, random_cfgs
+
+import pyroffi_amd as pk
+from pyroffi_amd.collision import CUDARobotCollisionChecker, RobotCollisionSpherized, Sphere
+from pyroffi_amd.kernels import ops
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _world(n, seed, lo=0.02, hi=0.15):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([rng.uniform(-1, 1, (n, 3)), rng.uniform(lo, hi, (n, 1))], axis=1).astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ROBOTS + ("mimic_arm",))
+@pytest.mark.parametrize("batch", [1, 63, 64, 65, 300])
+def test_fk_links_matches_oracle(oracle_mod, name, batch):
+    robot, _ = load_robot(name)
+    q = random_cfgs(robot, batch, seed=batch, margin=0.2).astype(np.float32)
+    got = robot.forward_kinematics(q)
+    ref = oracle_mod.fk_links(oracle_mod.Tables.of(robot), q.astype(np.float64))
+    assert got.shape == (batch, robot.links.num_links, 7)
+    np.testing.assert_allclose(got, ref, atol=TOL, rtol=0)
+
+
+@pytest.mark.parametrize("name", ("panda", "baxter", "mimic_arm"))
+def test_fk_joints_matches_oracle(oracle_mod, name):
+    robot, _ = load_robot(name)
+    q = random_cfgs(robot, 129, seed=7).astype(np.float32)
+    got = robot.forward_kinematics_joints(q)
+    ref = oracle_mod.fk_joints(oracle_mod.Tables.of(robot), q.astype(np.float64))
+    np.testing.assert_allclose(got, ref, atol=TOL, rtol=0)
+
+
+def test_fk_batch_shapes_and_empty():
+    robot, _ = load_robot("panda")
+    q = torch.zeros((2, 3, 7), device="cuda")
+    assert robot.forward_kinematics(q).shape == (2, 3, 13, 7)
+    assert robot.forward_kinematics(torch.zeros((0, 7), device="cuda")).shape == (0, 13, 7)
+    out = robot.forward_kinematics(np.zeros(7))
+    np.testing.assert_allclose(out[8, 4:], [0.088, 0, 0.926], atol=1e-6)
+
+
+def test_fk_large_batch_properties(oracle_mod):
+    """65536 configurations: unit quaternions everywhere, oracle on a 4096 subset."""
+    robot, _ = load_robot("panda")
+    q = random_cfgs(robot, 65536, seed=11).astype(np.float32)
+    got = robot.forward_kinematics(torch.as_tensor(q, device="cuda")).cpu().numpy()
+    assert np.all(np.abs(np.linalg.norm(got[..., :4], axis=-1) - 1) < 1e-5)
+    idx = np.random.default_rng(0).choice(65536, 4096, replace=False)
+    ref = oracle_mod.fk_links(oracle_mod.Tables.of(robot), q[idx].astype(np.float64))
+    np.testing.assert_allclose(got[idx], ref, atol=TOL, rtol=0)
+
+
+def test_fk_ffi_level_wrapper(oracle_mod):
+    robot, _ = load_robot("fetch")
+    t = robot.device_tables()
+    q = torch.as_tensor(random_cfgs(robot, 50, seed=3), device="cuda", dtype=torch.float32)
+    got = ops.fk_cuda(q, *t.robot_buffers(), t.fk_level_starts, t.fk_level_joints).cpu().numpy()
+    ref = oracle_mod.fk_joints(oracle_mod.Tables.of(robot), q.cpu().numpy().astype(np.float64))
+    np.testing.assert_allclose(got, ref, atol=TOL, rtol=0)
+
+
+def _oracle_distances(oracle_mod, robot, coll, q, world):
+    lp = oracle_mod.fk_links(oracle_mod.Tables.of(robot), q.astype(np.float64))
+    sp = oracle_mod.padded_spheres(coll)
+    full, mins = oracle_mod.world_dist(lp, sp, world.astype(np.float64))
+    selfd = oracle_mod.self_dist(lp, sp, coll.active_idx_i, coll.active_idx_j)
+    return full, mins, selfd
+
+
+@pytest.mark.parametrize("name", ROBOTS + ("mimic_arm",))
+@pytest.mark.parametrize("m", [1, 37, 256, 600])
+def test_world_and_self_distances(oracle_mod, name, m):
+    robot, path = load_robot(name)
+    coll = RobotCollisionSpherized.from_urdf(path)
+    q = random_cfgs(robot, 70, seed=m).astype(np.float32)
+    world = _world(m, seed=m + 1)
+    full, mins, selfd = _oracle_distances(oracle_mod, robot, coll, q, world)
+    chk = CUDARobotCollisionChecker(coll)
+    wg = Sphere.from_center_and_radius(world[:, :3], world[:, 3])
+    got = chk.compute_world_collision_distance(robot, q, wg)
+    np.testing.assert_allclose(got, full, atol=TOL, rtol=0)
+    got_min = chk.compute_min_world_distance(robot, q, wg)
+    np.testing.assert_allclose(got_min, mins, atol=TOL, rtol=0)
+    got_self = chk.compute_self_collision_distance(robot, q)
+    np.testing.assert_allclose(got_self, selfd, atol=TOL, rtol=0)
+
+
+def test_compute_all_config2_shape(oracle_mod):
+    """The fused two-launch path on 4096 configs x 256 world spheres."""
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    q = random_cfgs(robot, 4096, seed=21).astype(np.float32)
+    world = _world(256, seed=22)
+    full, mins, selfd = _oracle_distances(oracle_mod, robot, coll, q, world)
+    chk = CUDARobotCollisionChecker(coll)
+    poses, gmin, gself = chk.compute_all(robot, torch.as_tensor(q, device="cuda"), world)
+    np.testing.assert_allclose(gmin.cpu().numpy(), mins, atol=TOL, rtol=0)
+    np.testing.assert_allclose(gself.cpu().numpy(), selfd, atol=TOL, rtol=0)
+    assert poses.shape == (4096, 13, 7)
+
+
+def test_srdf_pairs_and_empty_world(oracle_mod):
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path, srdf_path=pk.bundled_srdf_path("panda"))
+    q = random_cfgs(robot, 33, seed=5).astype(np.float32)
+    _, _, selfd = _oracle_distances(oracle_mod, robot, coll, q, _world(1, 0))
+    np.testing.assert_allclose(coll.compute_self_collision_distance(robot, q), selfd, atol=TOL, rtol=0)
+    empty = coll.compute_world_collision_distance(robot, q, np.zeros((0, 4), np.float32))
+    assert empty.shape == (33, 13, 0)
+
+
+def test_reference_layout_kernels(oracle_mod):
+    """collision_world_sphere_reduced ([3,B,K] SoA) and collision_self_sphere ([B,S,N,3] AoS)."""
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    q = random_cfgs(robot, 40, seed=8)
+    world = _world(100, seed=9)
+    full, _, selfd = _oracle_distances(oracle_mod, robot, coll, q.astype(np.float32), world)
+    at = coll.at_config(robot, q.astype(np.float32))  # batch (B, S, N)
+    B, S, N = at.get_batch_axes()
+    centers = at.pose.translation().astype(np.float32)  # (B, S, N, 3)
+    radii = at.radius.astype(np.float32)
+    soa = np.ascontiguousarray(np.transpose(centers.reshape(B, S * N, 3), (2, 0, 1)))
+    e = np.zeros((0, 7), np.float32)
+    got = ops.collision_world_sphere_reduced(
+        torch.as_tensor(soa, device="cuda"), torch.as_tensor(radii.reshape(B, S * N), device="cuda"),
+        torch.as_tensor(world, device="cuda"), e, np.zeros((0, 15), np.float32), np.zeros((0, 6), np.float32), N,
+    ).cpu().numpy()
+    np.testing.assert_allclose(got, full, atol=TOL, rtol=0)
+    got_self = ops.collision_self_sphere(
+        torch.as_tensor(centers, device="cuda"), torch.as_tensor(radii, device="cuda"),
+        torch.as_tensor(coll.active_idx_i, device="cuda"), torch.as_tensor(coll.active_idx_j, device="cuda"),
+    ).cpu().numpy()
+    np.testing.assert_allclose(got_self, selfd, atol=TOL, rtol=0)
+
+
+def test_links_without_spheres_report_sentinel():
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    got = coll.compute_world_collision_distance(robot, np.zeros((2, 7), np.float32), _world(5, 3))
+    for name in ("panda_link8", "panda_grasptarget"):
+        assert np.all(got[:, robot.links.names.index(name)] == 1e9)
+
+
+def test_colldist_from_sdf_on_device():
+    from pyroffi_amd.collision import colldist_from_sdf
+
+    d = torch.tensor([-0.1, 0.0, 0.01, 0.05, 0.2], device="cuda")
+    got = colldist_from_sdf(d, 0.05).cpu().numpy()
+    m = 0.05
+    ref = [-0.1 - 0.5 * m, -0.5 / (m + 1e-6) * m * m, -0.5 / (m + 1e-6) * 0.04 ** 2, 0.0, 0.0]
+    np.testing.assert_allclose(got, ref, atol=1e-7)
