@@ -1,0 +1,298 @@
+"""Benchmark of the batched kinematics hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): Panda with its
+59-sphere model, 65536 float32 configurations uniform in the joint limits, a
+world of 256 spheres (centres uniform in the 2 m cube around the base, radii
+U[0.02, 0.15] m).  One step = forward kinematics to all 13 link poses + the
+minimum signed distance of each configuration to the world + the 66 self-pair
+distances; two kernel launches (pk_fk_links_f32, pk_sphere_sweep_f32) on
+inputs already resident in HBM.
+
+Printed as one JSON line: whole-job configs/s, the roofline of the dominant
+kernel (the sphere sweep, FP32 VALU-bound; timed with HIP events on the stream
+it runs on), the FP64 CPU oracle timed on this host's cores for a bounded
+sample of the same workload, and the LM-IK line (16 targets x 64 seeds, 30
+iterations, lambda = 1e-2; solves/s and success at 1 mm / 0.01 rad).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LM-IK solves/s @≤1mm/0.01rad fused-batched vs plain; FK+coll configs/s"
+B = 65536
+M_WORLD = 256
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# Algorithmic work per configuration (flop convention: sphere-sphere distance
+# including the min = 12; SE(3) sphere placement ~ 33 per sphere).
+K_SPHERES = 59
+SELF_SPHERE_PAIRS = 1230
+FLOP_SWEEP = K_SPHERES * M_WORLD * 12 + SELF_SPHERE_PAIRS * 12 + K_SPHERES * 33
+BYTES_FK = 7 * 4 + 13 * 7 * 4  # cfg in + link poses out
+BYTES_SWEEP = 13 * 7 * 4 + 4 + 66 * 4  # link poses in + min + self pairs out
+
+
+def make_inputs(seed: int = 0):
+    import pyroffi_amd as pk
+    from pyroffi_amd.collision import RobotCollisionSpherized
+
+    robot = pk.Robot.from_urdf(pk.bundled_urdf_path("panda"))
+    coll = RobotCollisionSpherized.from_urdf(pk.bundled_urdf_path("panda"))
+    rng = np.random.default_rng(seed)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    q = rng.uniform(lo, hi, (B, 7)).astype(np.float32)
+    world = np.concatenate([rng.uniform(-1, 1, (M_WORLD, 3)), rng.uniform(0.02, 0.15, (M_WORLD, 1))], 1)
+    return robot, coll, q, world.astype(np.float32)
+
+
+def gpu_bench(steps: int, warmup: int):
+    import torch
+
+    from pyroffi_amd.kernels.ops import fk_links, sphere_sweep
+
+    robot, coll, q_np, world_np = make_inputs()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    q = torch.as_tensor(q_np, device=dev)
+    world = torch.as_tensor(world_np, device=dev)
+    t = robot.device_tables(dev)
+    ct = coll.device_tables(dev)
+    P = coll.num_pairs
+
+    def step():
+        poses = fk_links(q, *t.robot_buffers(), t.link_parent_joint)
+        _, wmin, selfd = sphere_sweep(poses, ct["spheres_local"], ct["link_sphere_start"], world,
+                                      ct["self_entries"], P, world_min=True, self_pairs=True)
+        return poses, wmin, selfd
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / steps
+
+    # Per-kernel averages with HIP events on the launch stream.
+    poses = out[0]
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps  # ms
+
+    reps = max(20, steps)
+    fk_ms = timed(lambda: fk_links(q, *t.robot_buffers(), t.link_parent_joint), reps)
+    sweep_ms = timed(lambda: sphere_sweep(poses, ct["spheres_local"], ct["link_sphere_start"], world,
+                                          ct["self_entries"], P, world_min=True, self_pairs=True), reps)
+    return dict(ms_per_step=ms_per_step, fk_ms=fk_ms, sweep_ms=sweep_ms, results=out)
+
+
+def ik_bench(reps: int):
+    """Config 1 IK: Panda, 16 FK-generated targets x 64 uniform seeds, 30 iterations, lambda 1e-2."""
+    import torch
+
+    import oracle
+    import pyroffi_amd as pk
+    from pyroffi_amd.kernels.ops import ls_ik_select
+    from pyroffi_amd.optimization_engines._ls_ik import ee_tables, ls_ik_run
+
+    robot = pk.Robot.from_urdf(pk.bundled_urdf_path("panda"))
+    link = robot.links.names.index("panda_hand")
+    rng = np.random.default_rng(1)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    tgt_q = rng.uniform(lo, hi, (16, 7))
+    tgt = robot.forward_kinematics(tgt_q.astype(np.float32))[:, link].astype(np.float64)
+    seeds = torch.as_tensor(rng.uniform(lo, hi, (16, 64, 7)).astype(np.float32), device="cuda")
+    tT = torch.as_tensor(tgt[:, None].astype(np.float32), device="cuda")
+    prev = torch.zeros((16, 7), device="cuda")
+    kw = dict(max_iter=30, lambda_init=1e-2)
+
+    def solve():
+        cfgs, errs = ls_ik_run(robot, seeds, tT, (link,), **kw)
+        return ls_ik_select(cfgs, errs, prev, 0.0)[0], cfgs, errs
+
+    solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        best, cfgs, errs = solve()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+
+    def success(qs):
+        lp = robot.forward_kinematics(np.asarray(qs, np.float32).reshape(-1, 7))[:, link].astype(np.float64)
+        tt = np.repeat(tgt, qs.size // (16 * 7), axis=0) if qs.ndim == 3 else tgt
+        from pyroffi_amd._se3 import quat_conj, quat_mul, so3_log
+
+        pos = np.linalg.norm(lp[:, 4:] - tt[:, 4:], axis=1)
+        rot = np.linalg.norm(so3_log(quat_mul(quat_conj(tt[:, :4]), lp[:, :4])), axis=1)
+        return (pos <= 1e-3) & (rot <= 0.01), (pos <= 1e-3) & (rot <= 0.05)
+
+    win01, win05 = success(best.cpu().numpy())
+    seed01, _ = success(cfgs.cpu().numpy())
+    # FP64 oracle on the same seeds (also the CPU baseline of this line).
+    t_or = oracle.Tables.of(robot)
+    tj, masks = ee_tables(robot, (link,))
+    threads = cpu_threads()
+    c0 = time.perf_counter()
+    ocfgs, oerrs = oracle.ls_ik(t_or, seeds.cpu().numpy(), tj, masks, tgt[:, None], lo, hi, threads=threads, **kw)
+    cpu_dt = time.perf_counter() - c0
+    owin01, _ = success(ocfgs[np.arange(16), oerrs.argmin(1)])
+    oseed01, _ = success(ocfgs)
+    return {
+        "workload": "panda_lm_ik_16targets_64seeds_30iters",
+        "solves_per_s": 16 / dt,
+        "ms_per_solve_call": dt * 1e3,
+        "success_1mm_0.01rad": float(win01.mean()),
+        "success_1mm_0.05rad": float(win05.mean()),
+        "seed_success_1mm_0.01rad": float(seed01.mean()),
+        "oracle_success_1mm_0.01rad": float(owin01.mean()),
+        "oracle_seed_success_1mm_0.01rad": float(oseed01.mean()),
+        "cpu_baseline": {"value": 16 / cpu_dt, "unit": "solves/s", "cores": threads, "kind": "port",
+                         "sample": "the same 16 problems x 64 seeds, FP64 oracle"},
+    }
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(sample: int):
+    """FP64 oracle (C + OpenMP) on `sample` configurations of the same workload."""
+    import oracle
+
+    robot, coll, q, world = make_inputs()
+    q = q[:sample].astype(np.float64)
+    t = oracle.Tables.of(robot)
+    sp = oracle.padded_spheres(coll)
+    threads = cpu_threads()
+    oracle.fk_links(t, q[:64], threads)  # warm-up (thread pool, page-in)
+    t0 = time.perf_counter()
+    lp = oracle.fk_links(t, q, threads)
+    _, mins = oracle.world_dist(lp, sp, world.astype(np.float64), want_full=False, threads=threads)
+    selfd = oracle.self_dist(lp, sp, coll.active_idx_i, coll.active_idx_j, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": sample / dt, "unit": "configs/s", "cores": threads, "kind": "port",
+            "sample": f"first {sample} of the {B} configurations, FP64 oracle (FK + world min + self pairs)"}, \
+        (lp, mins, selfd)
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "traffic_sphere_sweep_panda_b65536_m256.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def profile_run():
+    import torch
+
+    g = gpu_bench(steps=5, warmup=2)
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)  # reads 256 MiB, writes 256 MiB
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": True, "ms_per_step": g["ms_per_step"], "sweep_ms": g["sweep_ms"],
+                      "fk_ms": g["fk_ms"], "copy_bytes_each_way": src.numel() * 4}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-sample", type=int, default=B)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ik", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: 5 steps of the workload plus a 256 MiB device copy "
+                         "(the byte-count calibration for the HBM counters)")
+    args = ap.parse_args()
+    if args.profile:
+        profile_run()
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this workload "
+              "shards by configuration with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    g = gpu_bench(args.steps, args.warmup)
+    value = B / (g["ms_per_step"] * 1e-3)
+    achieved = FLOP_SWEEP * B / (g["sweep_ms"] * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "configs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": g["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "panda_fk_sphere_collision_b65536_world256",
+            "batch": B,
+            "world_spheres": M_WORLD,
+            "robot_spheres": K_SPHERES,
+            "self_pairs": 66,
+            "outputs": "link poses (B,13,7) + per-config world min (B) + self pair distances (B,66)",
+            "parallelism": "single_gpu",
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "sweep_kernel (pk_sphere_sweep_f32)",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": load_traffic(),
+            "flop_per_config": FLOP_SWEEP,
+            "avg_launch_ms": g["sweep_ms"],
+        },
+        "fk_kernel": {
+            "bound": "hbm",
+            "avg_launch_ms": g["fk_ms"],
+            "achieved": BYTES_FK * B / (g["fk_ms"] * 1e-3) / 1e9,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+        },
+    }
+    if not args.no_cpu_baseline:
+        cb, _ = cpu_baseline(args.cpu_sample)
+        line["cpu_baseline"] = cb
+    if not args.no_ik:
+        line["ik"] = ik_bench(reps=10)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

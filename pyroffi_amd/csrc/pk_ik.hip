@@ -8,6 +8,7 @@
 //   per iteration: stop if every EE has |r_pos| < eps_pos and |r_ori| < eps_ori;
 //   fw = W r, Jw = W J, cur = |fw|^2 + penalty(q)
 //   Jacobi column scaling s_a = |Jw[:,a]| + 1e-8; (Js^T Js + lam I) p = -Js^T fw
+//   (solved in FP32 with the damping floored at 1e-6, see kDampFloor)
 //   (fixed joints: zero row/col, unit diagonal); delta = p / s
 //   trust region: |delta| <= R(max pos err, max ori err) in {0.38, 0.22, 0.12, 0.05}
 //   5-point line search alpha in {1, .5, .25, .1, .025} on clamp(q + alpha delta),
@@ -317,6 +318,8 @@ __host__ __device__ inline IkLds ik_lds_layout(size_t shared_bytes, int J, int K
   return o;
 }
 
+constexpr float kDampFloor = 1e-6f;
+
 __device__ __forceinline__ float ls_alpha(int i) {
   return i == 0 ? 1.f : (i == 1 ? 0.5f : (i == 2 ? 0.25f : (i == 3 ? 0.1f : 0.025f)));
 }
@@ -481,7 +484,12 @@ __global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
 #pragma unroll
       for (int k = 0; k < 6 * NEE; ++k) rb += Jm[k][i] * fw[k];
       rhs[i] = -rb;
-      A[i * (i + 1) / 2 + i] += lam;
+      // Columns are unit-scaled, so the diagonal of Js^T Js is ~1 and an FP32
+      // Cholesky resolves eigenvalues down to ~1e-6.  On redundant arms
+      // Js^T Js is singular and lam decays to 1e-10, which FP64 (the reference)
+      // survives and FP32 does not; the damping used inside the solve is
+      // floored at 1e-6 while the lam schedule itself is left unchanged.
+      A[i * (i + 1) / 2 + i] += fmaxf(lam, kDampFloor);
     }
 #pragma unroll
     for (int i = 0; i < MAXA; ++i) {

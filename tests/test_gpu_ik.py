@@ -98,11 +98,12 @@ def test_baxter_two_end_effectors_initial_error(oracle_mod):
 
 
 @pytest.mark.parametrize("name,with_coll", [("panda", False), ("panda", True), ("fetch", False),
-                                            ("mimic_arm", False)])
+                                            ("ur5", False), ("mimic_arm", False)])
 def test_solve_success_parity(oracle_mod, name, with_coll):
     robot, path = load_robot(name)
     link = _ee(robot, name)
-    t, tgt, seeds = _problems(oracle_mod, robot, (link,), 16, 64, seed=7)
+    P = 64
+    t, tgt, seeds = _problems(oracle_mod, robot, (link,), P, 64, seed=7)
     coll = _collision(robot, path, 64, 8) if with_coll else None
     kw = dict(max_iter=30, lambda_init=1e-2)
     cfgs, errs, ocfgs, oerrs = _run_both(oracle_mod, robot, (link,), tgt, seeds, t, collision=coll, **kw)
@@ -111,13 +112,30 @@ def test_solve_success_parity(oracle_mod, name, with_coll):
     pos, rot = _errors(oracle_mod, t, cfgs, tgt, (link,))
     opos, orot = _errors(oracle_mod, t, ocfgs, tgt, (link,))
     ok, ook = (pos <= 1e-3) & (rot <= 0.01), (opos <= 1e-3) & (orot <= 0.01)
-    assert abs(ok.mean() - ook.mean()) <= 0.02, (ok.mean(), ook.mean())
-    win = ok[np.arange(16), np.argmin(errs, axis=1)]
-    owin = ook[np.arange(16), np.argmin(oerrs, axis=1)]
-    assert abs(win.mean() - owin.mean()) <= 0.005, (win.mean(), owin.mean())
+    print(f"{name} coll={with_coll}: seed success {ok.mean():.4f} vs oracle {ook.mean():.4f}")
+    assert abs(ok.mean() - ook.mean()) <= 0.01, (ok.mean(), ook.mean())
+    win = ok[np.arange(P), np.argmin(errs, axis=1)]
+    owin = ook[np.arange(P), np.argmin(oerrs, axis=1)]
+    assert abs(win.sum() - owin.sum()) <= 1, (win.mean(), owin.mean())
     # the returned error is the all-time best, never above the seed's own error
     _, e0, _, _ = _run_both(oracle_mod, robot, (link,), tgt, seeds, t, collision=coll, max_iter=0)
     assert np.all(errs <= e0 * (1 + 1e-5) + 1e-6)
+
+
+def test_config1_problem_success(oracle_mod):
+    """Panda, 16 FK-generated targets x 64 seeds, 30 iterations, lambda = 1e-2."""
+    robot, _ = load_robot("panda")
+    link = _ee(robot, "panda")
+    t, tgt, seeds = _problems(oracle_mod, robot, (link,), 16, 64, seed=1)
+    cfgs, errs, ocfgs, oerrs = _run_both(oracle_mod, robot, (link,), tgt, seeds, t, max_iter=30, lambda_init=1e-2)
+    best = cfgs[np.arange(16), errs.argmin(1)][:, None]
+    obest = ocfgs[np.arange(16), oerrs.argmin(1)][:, None]
+    pos, rot = _errors(oracle_mod, t, best, tgt, (link,))
+    opos, orot = _errors(oracle_mod, t, obest, tgt, (link,))
+    win, owin = (pos <= 1e-3) & (rot <= 0.01), (opos <= 1e-3) & (orot <= 0.01)
+    print(f"config1 problem success {win.mean():.4f} vs oracle {owin.mean():.4f}")
+    assert win.mean() >= 0.9
+    assert abs(win.sum() - owin.sum()) <= 1
 
 
 def test_select_kernel_matches_numpy():
