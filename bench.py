@@ -202,10 +202,12 @@ def cpu_baseline(sample: int):
 
 
 def load_traffic():
-    path = os.path.join(ROOT, "profiles", "traffic_sphere_sweep_panda_b65536_m256.json")
+    """HBM bytes per sweep launch from the committed rocprofv3 counter summary (or None)."""
+    path = os.path.join(ROOT, "profiles", "sphere_sweep_fk_panda_b65536_m256.md")
     if os.path.exists(path):
-        with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+        for line in open(path):
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":")[1])
     return None
 
 
