@@ -139,7 +139,9 @@ int pk_collision_self_sphere_f32(void* stream, const float* centers,
  *
  *   link_poses        (batch, n_links, 7) float32   (pk_fk_links_f32 output)
  *   spheres_local     (n_spheres, 4) float32, [x, y, z, r] in the frame of
- *                     their link, grouped by link (padding removed)
+ *                     their link, grouped by link (padding removed); adjacent
+ *                     spheres of one link with equal radius share one square
+ *                     root per obstacle, so order each link's spheres by radius
  *   link_sphere_start (n_links + 1) int32: spheres of link n are
  *                     [link_sphere_start[n], link_sphere_start[n+1])
  *   world_spheres     (Ms, 4) float32

@@ -110,7 +110,13 @@ class RobotCollisionSpherized:
             ai, aj = RobotCollisionSpherized._compute_active_pair_indices(
                 names, urdf, user_ignore_pairs, ignore_immediate_adjacents
             )
-        compact = [np.concatenate([c[: len(s)], r[: len(s), None]], axis=1) for c, r, s in zip(centers, radii, link_spheres)]
+        # Kernel table: per link, spheres ordered by radius so that equal radii
+        # are adjacent (the sweep kernel shares one square root per obstacle
+        # across such a run).  The padded `coll` keeps the URDF order.
+        compact = []
+        for c, r, s in zip(centers, radii, link_spheres):
+            rows = np.concatenate([c[: len(s)], r[: len(s), None]], axis=1)
+            compact.append(rows[np.argsort(-rows[:, 3], kind="stable")])
         return RobotCollisionSpherized(n, names, coll, ai, aj, compact)
 
     @staticmethod

@@ -68,7 +68,9 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
   float4* sl = sc + K;                                        // K local spheres (SRC 0)
   int* lstart = reinterpret_cast<int*>(sl + (SRC == 0 ? K : 0));  // L + 1
   int* slink = lstart + (L + 1);                              // K: link of sphere (SRC 0)
-  float* ptile = reinterpret_cast<float*>(slink + (SRC == 0 ? K : 0));  // L * 7 poses
+  int* gstart = slink + (SRC == 0 ? K : 0);                   // K + 1: sphere groups
+  int* lgroup = gstart + (K + 1);                             // L + 1: groups of each link
+  float* ptile = reinterpret_cast<float*>(lgroup + (L + 1));  // L * 7 poses (SRC 0)
   unsigned* pmin = reinterpret_cast<unsigned*>(ptile + (SRC == 0 ? L * 7 : 0));  // P
   unsigned* ents = pmin + a.n_pairs;                          // n_entries
 
@@ -103,6 +105,20 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
     for (int n = lane; n < L; n += kWave)
       for (int k = lstart[n]; k < lstart[n + 1]; ++k) slink[k] = n;
   }
+  // Sphere groups: runs of equal radius inside a link (SRC 0), or single
+  // slots (SRC 1, whose radii are per configuration).  For one obstacle,
+  // min_k(|c_k - o|) - r = sqrt(min_k |c_k - o|^2) - r inside a group, so a
+  // group costs one square root per obstacle.
+  if (lane == 0) {
+    int g = 0;
+    for (int n = 0; n < L; ++n) {
+      lgroup[n] = g;
+      for (int k = lstart[n]; k < lstart[n + 1]; ++k)
+        if (SRC == 1 || k == lstart[n] || sl[k].w != sl[k - 1].w) gstart[g++] = k;
+    }
+    lgroup[L] = g;
+    gstart[g] = K;
+  }
 
   // This lane's obstacles (first chunk stays resident when M <= 64 * T).
   const int chunk = kWave * T;
@@ -123,7 +139,12 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
   if (n_chunks > 0) load_obstacles(0);
   __syncthreads();
 
-  for (long long b = blockIdx.x; b < a.batch; b += gridDim.x) {
+  // Each workgroup owns a contiguous block of configurations, so the pose
+  // rows it reads and the distance rows it writes share cache lines.
+  const long long per = (a.batch + gridDim.x - 1) / gridDim.x;
+  const long long b_begin = static_cast<long long>(blockIdx.x) * per;
+  const long long b_end = min(a.batch, b_begin + per);
+  for (long long b = b_begin; b < b_end; ++b) {
     // 1. Robot spheres of configuration b in the world frame -> LDS.
     if (SRC == 0) {
       const float* pb = a.link_poses + b * L * 7;
@@ -160,16 +181,24 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
           float acc[T];
 #pragma unroll
           for (int t = 0; t < T; ++t) acc[t] = INFINITY;
-          const int k1 = lstart[n + 1];
-          for (int k = lstart[n]; k < k1; ++k) {
-            const float4 s = sc[k];
-            if (SRC == 1 && s.w == -INFINITY) continue;  // padding slot (wave-uniform)
+          const int g1 = lgroup[n + 1];
+          for (int g = lgroup[n]; g < g1; ++g) {
+            const int k0 = gstart[g], k1 = gstart[g + 1];
+            const float rg = sc[k0].w;
+            if (SRC == 1 && rg == -INFINITY) continue;  // padding slot (wave-uniform)
+            float m2[T];
 #pragma unroll
-            for (int t = 0; t < T; ++t) {
-              const float dx = s.x - ox[t], dy = s.y - oy[t], dz = s.z - oz[t];
-              const float d = fast_sqrt(dx * dx + dy * dy + dz * dz) - s.w;
-              acc[t] = fminf(acc[t], d);
+            for (int t = 0; t < T; ++t) m2[t] = INFINITY;
+            for (int k = k0; k < k1; ++k) {
+              const float4 s = sc[k];
+#pragma unroll
+              for (int t = 0; t < T; ++t) {
+                const float dx = s.x - ox[t], dy = s.y - oy[t], dz = s.z - oz[t];
+                m2[t] = fminf(m2[t], dx * dx + dy * dy + dz * dz);
+              }
             }
+#pragma unroll
+            for (int t = 0; t < T; ++t) acc[t] = fminf(acc[t], fast_sqrt(m2[t]) - rg);
           }
 #pragma unroll
           for (int t = 0; t < T; ++t) {
@@ -189,17 +218,30 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
       if (n_chunks > 1) load_obstacles(0);
     }
 
-    // 3. Self pairs.
+    // 3. Self pairs.  Entries are grouped by pair; lane l reduces the
+    // contiguous run [l E / 64, (l + 1) E / 64) in registers and publishes one
+    // atomic min per pair it touches.
     if (a.out_self != nullptr) {
-      for (int e = lane; e < a.n_entries; e += kWave) {
+      const int E = a.n_entries;
+      const int e0 = static_cast<int>((static_cast<long long>(E) * lane) / kWave);
+      const int e1 = static_cast<int>((static_cast<long long>(E) * (lane + 1)) / kWave);
+      unsigned cur_p = 0xffffffffu;
+      float cur = INFINITY;
+      for (int e = e0; e < e1; ++e) {
         const unsigned en = ents[e];
         if (en == 0xffffffffu) continue;
+        const unsigned p = en >> 20;
+        if (p != cur_p) {
+          if (cur_p != 0xffffffffu) atomicMin(&pmin[cur_p], float_to_ordered(cur));
+          cur_p = p;
+          cur = INFINITY;
+        }
         const float4 s1 = sc[en & 1023u];
         const float4 s2 = sc[(en >> 10) & 1023u];
         const float dx = s1.x - s2.x, dy = s1.y - s2.y, dz = s1.z - s2.z;
-        const float d = fast_sqrt(dx * dx + dy * dy + dz * dz) - s1.w - s2.w;
-        atomicMin(&pmin[en >> 20], float_to_ordered(d));
+        cur = fminf(cur, fast_sqrt(dx * dx + dy * dy + dz * dz) - s1.w - s2.w);
       }
+      if (cur_p != 0xffffffffu) atomicMin(&pmin[cur_p], float_to_ordered(cur));
       __syncthreads();
       for (int p = lane; p < a.n_pairs; p += kWave) a.out_self[b * a.n_pairs + p] = ordered_to_float(pmin[p]);
     }
@@ -248,7 +290,8 @@ static int pick_t(int M) {
 template <int SRC>
 static int launch_sweep(void* stream, const SweepArgs& a) {
   const int K = a.n_spheres, L = a.n_links;
-  size_t lds = sizeof(float4) * K + sizeof(int) * (L + 1) + sizeof(unsigned) * (a.n_pairs + a.n_entries);
+  size_t lds = sizeof(float4) * K + sizeof(int) * (L + 1) + sizeof(int) * (K + 1) + sizeof(int) * (L + 1) +
+               sizeof(unsigned) * (a.n_pairs + a.n_entries);
   if (SRC == 0) lds += sizeof(float4) * K + sizeof(int) * K + sizeof(float) * L * 7;
   PK_REQUIRE(lds <= 150 * 1024, "sphere tables need %zu bytes of LDS (limit 150 KiB)", lds);
   // Enough waves to cover every CU several times; each loops over configs.

@@ -141,5 +141,11 @@ def test_sphere_table_matches_urdf():
     for n, l in enumerate(links):
         s = spheres[l]
         k0, k1 = c.link_sphere_start[n], c.link_sphere_start[n + 1]
-        np.testing.assert_allclose(c.spheres_local[k0:k1], s, atol=1e-7)
+        # kernel table: same spheres, equal radii adjacent (sorted by radius)
+        got = c.spheres_local[k0:k1]
+        assert np.all(np.diff(got[:, 3]) <= 0)
+        key = lambda a: a[np.lexsort(a.T[::-1])]
+        np.testing.assert_allclose(key(got.astype(np.float64)), key(s), atol=1e-7)
+        # padded table keeps the URDF order
+        np.testing.assert_allclose(c.coll.pose.translation()[n, : len(s)], s[:, :3], atol=1e-7)
         assert np.all(c.coll.radius[n, len(s):] == -1e9)
