@@ -240,8 +240,8 @@ def main():
         profile_run()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this workload "
-              "shards by configuration with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the runs here have one "
+              "GPU (the workload shards by configuration with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
 
     g = gpu_bench(args.steps, args.warmup)

@@ -6,8 +6,8 @@ for the HIP kernels.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` may import this package; the
 ``pyroffi_amd`` package never does.
 
-Parity pins (what independently checks each function) are listed in
-``oracle/README.md``.
+Each function is pinned by ``tests/test_oracle.py`` against checks that share
+no code with it.
 """
 
 from __future__ import annotations

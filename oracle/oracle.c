@@ -23,7 +23,8 @@
  *                    of _ik_cuda_helpers.cuh:433-631 and the Cholesky of
  *                    _ik_cuda_helpers.cuh:207-243
  *
- * Parity status of each function is stated in oracle/README.md.
+ * Every function here is pinned by tests/test_oracle.py against checks that
+ * share no code with it (known answers, scipy, finite differences).
  */
 #include <math.h>
 #include <stdint.h>

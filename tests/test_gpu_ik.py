@@ -5,7 +5,7 @@
 * Full solves: FP32 and FP64 trajectories can part ways on ill-conditioned
   seeds, so parity is on outcomes -- per-seed success fraction (pos <= 1 mm and
   rot <= 0.01 rad) within 2 percentage points, per-problem success within
-  0.5 pp, winners within the joint limits.
+  one problem, winners within the joint limits.
 """
 
 import numpy as np
