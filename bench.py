@@ -173,6 +173,80 @@ def ik_bench(reps: int):
     }
 
 
+def grad_bench(reps: int, cpu_sample: int):
+    """Config 3: Panda, 16384 configs, pose + hinge (margin 0.05) against 512 world spheres,
+    cost and dcost/dq through the hand-written adjoint (autograd Function)."""
+    import torch
+
+    import oracle
+    import pyroffi_amd as pk
+    from pyroffi_amd.collision import RobotCollisionSpherized
+    from pyroffi_amd.costs import pose_hinge_cost
+    from pyroffi_amd.kernels.ops import fk_links, pose_hinge_cost as raw_cost
+
+    nb, m = 16384, 512
+    robot = pk.Robot.from_urdf(pk.bundled_urdf_path("panda"))
+    coll = RobotCollisionSpherized.from_urdf(pk.bundled_urdf_path("panda"))
+    link = robot.links.names.index("panda_hand")
+    rng = np.random.default_rng(3)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    q_np = rng.uniform(lo, hi, (nb, 7)).astype(np.float32)
+    tq = rng.uniform(lo, hi, (nb, 7)).astype(np.float32)
+    world_np = np.concatenate([rng.uniform(-1, 1, (m, 3)), rng.uniform(0.02, 0.15, (m, 1))], 1).astype(np.float32)
+    q = torch.as_tensor(q_np, device="cuda")
+    tgt = robot.forward_kinematics(torch.as_tensor(tq, device="cuda"))[:, link].contiguous()
+    world = torch.as_tensor(world_np, device="cuda")
+
+    def step():
+        qq = q.detach().requires_grad_(True)
+        c = pose_hinge_cost(robot, coll, qq, tgt, world, margin=0.05, ee_link=link)
+        (g,) = torch.autograd.grad(c.sum(), qq)
+        return c, g
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    # The adjoint kernel alone (HIP events on the launch stream).
+    t = robot.device_tables()
+    ct = coll.device_tables(q.device)
+    poses = fk_links(q, *t.robot_buffers(), t.link_parent_joint)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    run = lambda: raw_cost(poses, ct["spheres_local"], ct["link_sphere_start"], world, 0.05, link, tgt, t.twists,
+                           t.parent_idx, t.act_idx, t.mimic_act_idx, t.mimic_mul, t.topo_inv, t.link_parent_joint, 7)
+    run()
+    e0.record(stream)
+    for _ in range(reps):
+        run()
+    e1.record(stream)
+    e1.synchronize()
+    kms = e0.elapsed_time(e1) / reps
+    flop = K_SPHERES * m * 13 * 3  # forward 13 flop per pair, backward ~2x forward
+    # CPU: FP64 oracle cost + central-difference gradient (15 cost evaluations per config).
+    t_or = oracle.Tables.of(robot)
+    sp = oracle.padded_spheres(coll)
+    tg = tgt.cpu().numpy().astype(np.float64)
+    threads = cpu_threads()
+    c0 = time.perf_counter()
+    oracle.pose_hinge_cost(t_or, q_np[:cpu_sample], sp, world_np, 0.05, link, tg[:cpu_sample], threads=threads)
+    oracle.pose_hinge_fd_grad(t_or, q_np[:cpu_sample], sp, world_np, 0.05, link, tg[:cpu_sample], threads=threads)
+    cdt = time.perf_counter() - c0
+    return {
+        "workload": "panda_pose_hinge_cost_and_grad_b16384_world512",
+        "configs_per_s": nb / dt,
+        "ms_per_step": dt * 1e3,
+        "adjoint_kernel_ms": kms,
+        "adjoint_kernel_tflops": flop * nb / (kms * 1e-3) / 1e12,
+        "flop_per_config": flop,
+        "cpu_baseline": {"value": cpu_sample / cdt, "unit": "configs/s", "cores": threads, "kind": "port",
+                         "sample": f"{cpu_sample} configs: FP64 cost + central-difference gradient (h=1e-4)"},
+    }
+
+
 def cpu_threads() -> int:
     try:
         n = len(os.sched_getaffinity(0))
@@ -293,6 +367,7 @@ def main():
         line["cpu_baseline"] = cb
     if not args.no_ik:
         line["ik"] = ik_bench(reps=10)
+        line["grad"] = grad_bench(reps=10, cpu_sample=256)
     print(json.dumps(line), flush=True)
 
 

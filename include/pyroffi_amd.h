@@ -164,6 +164,37 @@ int pk_sphere_sweep_f32(void* stream, const float* link_poses, int64_t batch,
                         float* out_world_min, float* out_self);
 
 /*
+ * Pose + hinge collision cost and its gradient (hand-written adjoint).
+ *
+ *   cost[b] = sum_c (W_c r_c)^2 + sum_{k,m} max(0, margin - d_km)^2
+ *   r = [p_ee - p_t; log(q_ee q_t^-1)] of link ee_link against targets[b]
+ *   (ee_link = -1 drops the pose term), W = (pos_weight x3, ori_weight x3),
+ *   d_km = |c_k - o_m| - r_k - R_m over robot spheres and world spheres.
+ *   out_grad[b] = dcost/dq (n_act), computed by reverse accumulation of link
+ *   wrenches over the joint tree -- no Jacobian is formed; NULL skips it.
+ *
+ * No single reference counterpart: the reference differentiates the CHOMP
+ * hinge^2 cost (src/pyroffi/optimization_engines/_chomp_optimization.py:140-154)
+ * with JAX AD and its CUDA trajopt kernels use finite differences.  This is
+ * the backward of a custom VJP over FK + collision.
+ *
+ *   link_poses (batch, n_links, 7) from pk_fk_links_f32; spheres as in
+ *   pk_sphere_sweep_f32; targets (batch, 7); joint tables as in pk_fk_f32 plus
+ *   link_parent_joint (n_links).
+ */
+int pk_pose_hinge_cost_f32(void* stream, const float* link_poses, int64_t batch,
+                           int32_t n_links, const float* spheres_local,
+                           const int32_t* link_sphere_start, int32_t n_spheres,
+                           const float* world_spheres, int32_t n_world_spheres,
+                           float margin, int32_t ee_link, const float* targets,
+                           float pos_weight, float ori_weight, const float* twists,
+                           const int32_t* parent_idx, const int32_t* act_idx,
+                           const int32_t* mimic_act_idx, const float* mimic_mul,
+                           const int32_t* topo_inv, const int32_t* link_parent_joint,
+                           int32_t n_joints, int32_t n_act, float* out_cost,
+                           float* out_grad);
+
+/*
  * Multi-seed Levenberg-Marquardt IK, one seed per lane.
  *
  * Replaces LsIkCudaFfi (src/pyroffi/cuda_kernels/_ls_ik_cuda_kernel.cu:400-529,

@@ -54,6 +54,9 @@ def lib() -> ctypes.CDLL:
         h.or_res_jac.argtypes = [_d, _I, _f, _f, _i, _i, _f, _f, _i, _i, _I, _i, _i, _I, _d, _d, _d]
         h.or_lm_direction.argtypes = [_d, _d, _I, _I, _d, _D, _i, _d]
         h.or_lm_direction.restype = ctypes.c_int
+        h.or_pose_hinge_cost.argtypes = [_d, _L, _I, _f, _f, _i, _i, _f, _f, _i, _i, _I, _i, _I, _d, _I, _d, _I,
+                                         _D, _I, _d, _D, _D, _d, _I]
+        h.or_pose_hinge_cost.restype = None
         for fn in (h.or_fk, h.or_link_poses, h.or_world_dist, h.or_self_dist, h.or_ls_ik, h.or_res_jac):
             fn.restype = None
         _lib = h
@@ -204,3 +207,31 @@ def lm_direction(r, jac, weights6, lam, fixed_mask=None):
     out = np.empty(A)
     ok = lib().or_lm_direction(_p(rr, _d), _p(jm, _d), R, A, _p(w, _d), float(lam), _p(fm, _i), _p(out, _d))
     return out, bool(ok)
+
+
+def pose_hinge_cost(tables: Tables, cfg, spheres, world, margin, ee_link, targets, pos_weight=1.0, ori_weight=1.0,
+                    threads: int = 0) -> np.ndarray:
+    """FP64 pose + hinge cost ``(B,)``; ``spheres`` is the padded ``(L, S, 4)`` table."""
+    q = _c(cfg, np.float64).reshape(-1, tables.A)
+    sp = _c(spheres, np.float64)
+    w = _c(world, np.float64).reshape(-1, 4)
+    tg = _c(np.zeros((q.shape[0], 7)) if targets is None else targets, np.float64).reshape(-1, 7)
+    out = np.empty(q.shape[0])
+    lib().or_pose_hinge_cost(_p(q, _d), q.shape[0], tables.A, *tables.ptrs(), tables.J,
+                             _p(tables.link_parent_joint, _i), tables.L, _p(sp, _d), sp.shape[1], _p(w, _d),
+                             w.shape[0], float(margin), int(ee_link), _p(tg, _d), float(pos_weight),
+                             float(ori_weight), _p(out, _d), threads)
+    return out
+
+
+def pose_hinge_fd_grad(tables: Tables, cfg, spheres, world, margin, ee_link, targets, h=1e-4, **kw) -> np.ndarray:
+    """FP64 central finite-difference gradient ``(B, A)`` of ``pose_hinge_cost``."""
+    q = _c(cfg, np.float64).reshape(-1, tables.A)
+    B, A = q.shape
+    pert = np.repeat(q[:, None, None, :], 2, axis=1).repeat(A, axis=2)  # (B, 2, A, A)
+    eye = np.eye(A) * h
+    pert[:, 0] += eye
+    pert[:, 1] -= eye
+    tg = np.repeat(np.asarray(targets, np.float64).reshape(B, 1, 7), 2 * A, axis=1).reshape(-1, 7)
+    c = pose_hinge_cost(tables, pert.reshape(-1, A), spheres, world, margin, ee_link, tg, **kw).reshape(B, 2, A)
+    return (c[:, 0] - c[:, 1]) / (2 * h)

@@ -559,3 +559,50 @@ int or_lm_direction(const double* r, const double* Jm, int R, int A, const doubl
   for (int a = 0; a < A; ++a) delta[a] = rhs[a] / cs[a];
   return ok;
 }
+
+/* Pose + hinge cost per configuration (FP64), the function whose gradient the
+ * adjoint kernel computes:
+ *   sum (W r)^2 over the EE residual of link ee_link (-1: none)
+ * + sum over real robot spheres x world spheres of max(0, margin - d)^2.
+ * Hinge^2 as in src/pyroffi/optimization_engines/_chomp_optimization.py:140-154;
+ * residual as in _ik_cuda_helpers.cuh:471-494. */
+void or_pose_hinge_cost(const double* cfg, int64_t B, int A, const float* twists, const float* parent_tf,
+                        const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul,
+                        const float* mimic_off, const int32_t* mimic_act_idx, const int32_t* topo_inv, int J,
+                        const int32_t* link_parent_joint, int L, const double* spheres, int S,
+                        const double* world, int M, double margin, int ee_link, const double* targets,
+                        double pos_weight, double ori_weight, double* out, int threads) {
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  set_threads(threads);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < B; ++b) {
+    se3 Tw[OR_MAXJ];
+    fk_one(&t, cfg + b * A, Tw);
+    double cost = 0.0;
+    for (int l = 0; l < L; ++l) {
+      const int j = link_parent_joint[l];
+      se3 T = (j < 0) ? (se3){1, 0, 0, 0, 0, 0, 0} : Tw[j];
+      for (int s = 0; s < S; ++s) {
+        const double* sp = spheres + (l * S + s) * 4;
+        if (sp[3] < 0) continue;
+        double c[3];
+        qrot(&T, sp, c);
+        c[0] += T.tx; c[1] += T.ty; c[2] += T.tz;
+        for (int m = 0; m < M; ++m) {
+          const double d = sdist(c, sp[3], world + 4 * m, world[4 * m + 3], 0);
+          const double h = margin - d;
+          if (h > 0) cost += h * h;
+        }
+      }
+    }
+    if (ee_link >= 0) {
+      const int j = link_parent_joint[ee_link];
+      se3 T = (j < 0) ? (se3){1, 0, 0, 0, 0, 0, 0} : Tw[j];
+      double r[6];
+      residual_ee(&T, targets + 7 * b, r);
+      for (int c = 0; c < 3; ++c) cost += pos_weight * pos_weight * r[c] * r[c];
+      for (int c = 3; c < 6; ++c) cost += ori_weight * ori_weight * r[c] * r[c];
+    }
+    out[b] = cost;
+  }
+}

@@ -11,7 +11,7 @@ All computation runs in the HIP library ``libpyroffi_amd.so`` (C ABI in
 ``include/pyroffi_amd.h``); torch supplies device memory and the stream.
 """
 
-from . import collision, kernels
+from . import collision, costs, kernels
 from ._robot import Robot
 from ._robot_urdf_parser import JointInfo, LinkInfo, RobotURDFParser
 from ._se3 import SE3, SO3
@@ -30,6 +30,7 @@ __all__ = [
     "bundled_srdf_path",
     "bundled_urdf_path",
     "collision",
+    "costs",
     "kernels",
     "load_bundled_urdf",
     "ls_ik_solve_cuda",

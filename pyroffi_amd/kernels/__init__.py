@@ -10,6 +10,7 @@ from .ops import (
     ls_ik_cuda,
     ls_ik_hip,
     ls_ik_select,
+    pose_hinge_cost,
     sphere_sweep,
 )
 
@@ -27,5 +28,6 @@ __all__ = [
     "ls_ik_cuda",
     "ls_ik_hip",
     "ls_ik_select",
+    "pose_hinge_cost",
     "sphere_sweep",
 ]

@@ -47,6 +47,11 @@ _SIGNATURES = {
         + [_p, _p],  # outputs
     ),
     "pk_ls_ik_select_f32": (_i32, [_p, _p, _p, _p, _i32, _i32, _i32, _f32, _p, _p]),
+    "pk_pose_hinge_cost_f32": (
+        _i32,
+        [_p, _p, _i64, _i32, _p, _p, _i32, _p, _i32, _f32, _i32, _p, _f32, _f32]
+        + [_p] * 7 + [_i32, _i32, _p, _p],
+    ),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

@@ -222,3 +222,31 @@ def ls_ik_select(cfgs, errs, previous, continuity_weight: float = 0.0):
     )
     check(st, "pk_ls_ik_select_f32")
     return best, idx
+
+
+def pose_hinge_cost(link_poses, spheres_local, link_sphere_start, world_spheres, margin: float,
+                    ee_link: int, targets, twists, parent_idx, act_idx, mimic_act_idx, mimic_mul,
+                    topo_inv, link_parent_joint, n_act: int, *, pos_weight: float = 1.0,
+                    ori_weight: float = 1.0, want_grad: bool = True):
+    """Cost ``(B,)`` and, when ``want_grad``, ``dcost/dq (B, n_act)`` from link poses."""
+    lp = _dev(link_poses, torch.float32)
+    dev = lp.device
+    B, L, _ = lp.shape
+    sl = _dev(spheres_local, torch.float32, dev).reshape(-1, 4)
+    ls = _dev(link_sphere_start, torch.int32, dev)
+    ws = _dev(world_spheres, torch.float32, dev).reshape(-1, 4)
+    tg = _dev(targets, torch.float32, dev).reshape(-1, 7) if ee_link >= 0 else None
+    tw = _dev(twists, torch.float32, dev)
+    tabs = [_dev(x, torch.int32, dev) for x in (parent_idx, act_idx, mimic_act_idx)]
+    mm = _dev(mimic_mul, torch.float32, dev)
+    ti = _dev(topo_inv, torch.int32, dev)
+    lpj = _dev(link_parent_joint, torch.int32, dev)
+    cost = torch.empty((B,), dtype=torch.float32, device=dev)
+    grad = torch.empty((B, n_act), dtype=torch.float32, device=dev) if want_grad else None
+    st = lib().pk_pose_hinge_cost_f32(
+        _stream(), _ptr(lp), B, L, _ptr(sl), _ptr(ls), sl.shape[0], _ptr(ws), ws.shape[0], float(margin),
+        int(ee_link), _ptr(tg), float(pos_weight), float(ori_weight), _ptr(tw), _ptr(tabs[0]), _ptr(tabs[1]),
+        _ptr(tabs[2]), _ptr(mm), _ptr(ti), _ptr(lpj), tw.shape[0], int(n_act), _ptr(cost), _ptr(grad),
+    )
+    check(st, "pk_pose_hinge_cost_f32")
+    return cost, grad

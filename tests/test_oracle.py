@@ -273,3 +273,49 @@ def test_lm_success_on_reachable_targets(oracle_mod):
     # the all-time best error never exceeds the seed's own error
     e0 = oracle_mod.ls_ik(t, seeds, tj, masks, tgt[:, None], lo, hi, max_iter=0)[1]
     assert np.all(errs <= e0 + 1e-12)
+
+
+def test_pose_hinge_cost_matches_independent(oracle_mod):
+    """Cost = |r|^2 (residual pinned above) + sum max(0, m - d)^2 with d from scipy cdist."""
+    from scipy.spatial import distance
+
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    t = oracle_mod.Tables.of(robot)
+    link = robot.links.names.index("panda_hand")
+    tj, masks = ee_tables(robot, (link,))
+    rng = np.random.default_rng(21)
+    world = np.concatenate([rng.uniform(-1, 1, (300, 3)), rng.uniform(0.02, 0.15, (300, 1))], 1)
+    q = random_cfgs(robot, 5, seed=22)
+    tgt = oracle_mod.fk_links(t, random_cfgs(robot, 5, seed=23))[:, link]
+    margin = 0.05
+    cost = oracle_mod.pose_hinge_cost(t, q, oracle_mod.padded_spheres(coll), world, margin, link, tgt,
+                                      pos_weight=2.0, ori_weight=0.5)
+    for b in range(5):
+        r, _ = oracle_mod.res_jac(t, q[b], tj, masks, tgt[b])
+        pose = 4.0 * np.sum(r[:3] ** 2) + 0.25 * np.sum(r[3:] ** 2)
+        links, ws = independent.world_spheres(path, q[b])
+        hinge = 0.0
+        for l in links:
+            s = ws[l]
+            if len(s):
+                d = distance.cdist(s[:, :3], world[:, :3]) - s[:, 3:4] - world[None, :, 3]
+                hinge += np.sum(np.maximum(margin - d, 0.0) ** 2)
+        # float32 tables (as the kernels take them) vs float64 URDF values
+        assert cost[b] == pytest.approx(pose + hinge, rel=1e-6, abs=1e-9)
+
+
+def test_pose_hinge_fd_gradient_sanity(oracle_mod):
+    """The FD helper reproduces an exact gradient: the pose-only cost's position part
+    2 J_pos^T r_pos (+ the rotation part) equals FD of |r|^2 near the target."""
+    robot, _ = load_robot("panda")
+    t = oracle_mod.Tables.of(robot)
+    link = robot.links.names.index("panda_hand")
+    tj, masks = ee_tables(robot, (link,))
+    q = random_cfgs(robot, 3, seed=24)
+    tgt = oracle_mod.fk_links(t, q + 0.01)[:, link]
+    fd = oracle_mod.pose_hinge_fd_grad(t, q, np.zeros((13, 1, 4)) - 1e9, np.zeros((0, 4)), 0.05, link, tgt,
+                                       ori_weight=0.0)
+    for b in range(3):
+        r, Jm = oracle_mod.res_jac(t, q[b], tj, masks, tgt[b])
+        np.testing.assert_allclose(fd[b], 2 * Jm[:3].T @ r[:3], rtol=1e-6, atol=1e-9)
