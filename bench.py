@@ -173,6 +173,118 @@ def ik_bench(reps: int):
     }
 
 
+UNEVEN_EE = {"panda": ("panda_hand",), "ur5": ("ee_link",), "fetch": ("gripper_link",),
+             "baxter": ("left_gripper_base", "right_gripper_base")}
+
+
+def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32, cpu_per_robot: int = 2):
+    """Config 4: 4096 targets over Panda/UR5/Fetch/Baxter (two EEs), seeds per target
+    log-uniform in [16, 256], 50 LM iterations, collision penalty against 256 world
+    spheres.  Fused = one autotuned persistent launch per robot over flat seeds,
+    grid-culled penalty, all launches captured in one graph.  Plain = one
+    single-problem launch per target against every world sphere (timed on the
+    first ``plain_per_robot`` targets of each robot; per-call cost is additive).
+    """
+    import torch
+
+    import oracle
+    import pyroffi_amd as pk
+    from pyroffi_amd._se3 import quat_conj, quat_mul, so3_log
+    from pyroffi_amd.collision import RobotCollisionSpherized
+    from pyroffi_amd.optimization_engines._ls_ik import ee_tables
+    from pyroffi_amd.optimization_engines._ls_ik_batched import (RobotGroup, UnevenLsIkSolver,
+                                                                 log_uniform_seed_counts, make_group)
+
+    rng = np.random.default_rng(4)
+    world = np.concatenate([rng.uniform(-1, 1, (M_WORLD, 3)), rng.uniform(0.02, 0.15, (M_WORLD, 1))], 1)
+    world = world.astype(np.float32)
+    groups, samples = [], []
+    for name, links in UNEVEN_EE.items():
+        path = pk.bundled_urdf_path(name)
+        robot = pk.Robot.from_urdf(path)
+        li = [robot.links.names.index(n) for n in links]
+        lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+        tq = rng.uniform(lo, hi, (per_robot, len(lo))).astype(np.float32)
+        lp = robot.forward_kinematics(tq)
+        tgt = np.stack([lp[:, l] for l in li], axis=1).astype(np.float64)
+        counts = log_uniform_seed_counts(rng, per_robot, 16, 256)
+        g = make_group(robot, li, tgt, counts, rng, collision_checker=RobotCollisionSpherized.from_urdf(path),
+                       world=world)
+        groups.append(g)
+        n = plain_per_robot
+        samples.append(RobotGroup(robot=robot, target_link_indices=g.target_link_indices, targets=g.targets[:n],
+                                  seeds=g.seeds[:g.seed_start[n]], seed_start=g.seed_start[:n + 1],
+                                  collision=g.collision))
+    n_seeds = int(sum(g.seeds.shape[0] for g in groups))
+    solver = UnevenLsIkSolver(groups, max_iter=50)
+    report = solver.autotune(reps=1, cache=False)
+    solver.capture()
+    solver.solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = solver.solve()
+    torch.cuda.synchronize()
+    fused_dt = (time.perf_counter() - t0) / reps
+    winners = [w.cpu().numpy() for w, _ in out]
+
+    plain = UnevenLsIkSolver(samples, max_iter=50)
+    plain.solve_per_call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_w = [q.cpu().numpy() for q in plain.solve_per_call()]
+    plain_dt = time.perf_counter() - t0
+    n_plain = sum(len(s.seed_start) - 1 for s in samples)
+
+    def success(g, q):
+        lp = g.robot.forward_kinematics(np.ascontiguousarray(q, np.float32)).astype(np.float64)
+        ok = np.ones(len(q), bool)
+        for e, l in enumerate(g.target_link_indices):
+            tt = g.targets[:len(q), e]
+            pos = np.linalg.norm(lp[:, l, 4:] - tt[:, 4:], axis=1)
+            rot = np.linalg.norm(so3_log(quat_mul(quat_conj(tt[:, :4]), lp[:, l, :4])), axis=1)
+            ok &= (pos <= 1e-3) & (rot <= 0.01)
+        return ok
+
+    fused_ok = [success(g, w) for g, w in zip(groups, winners)]
+    plain_ok = [success(s, w) for s, w in zip(samples, plain_w)]
+    fused_on_sample = [ok[:plain_per_robot] for ok in fused_ok]
+
+    # FP64 oracle on a bounded sample (the first targets of each robot, their own seeds).
+    threads = cpu_threads()
+    c0 = time.perf_counter()
+    n_cpu = 0
+    for g in groups:
+        t_or = oracle.Tables.of(g.robot)
+        tj, masks = ee_tables(g.robot, g.target_link_indices)
+        rs, rj, ws = g.collision
+        for p in range(cpu_per_robot):
+            s0, s1 = g.seed_start[p], g.seed_start[p + 1]
+            oracle.ls_ik(t_or, g.seeds[None, s0:s1], tj, masks, g.targets[p:p + 1], g.robot.joints.lower_limits,
+                         g.robot.joints.upper_limits, max_iter=50, robot_spheres=rs, sphere_joint=rj, world=ws,
+                         enable_collision=True, collision_weight=1e4, collision_margin=0.02, threads=threads)
+            n_cpu += 1
+    cpu_dt = time.perf_counter() - c0
+    names = list(UNEVEN_EE)
+    return {
+        "workload": "uneven_lm_ik_4robots_4096targets_seeds16to256_50iters_world256",
+        "targets": per_robot * len(groups),
+        "seeds_total": n_seeds,
+        "fused_solves_per_s": per_robot * len(groups) / fused_dt,
+        "fused_ms_per_batch": fused_dt * 1e3,
+        "plain_solves_per_s": n_plain / plain_dt,
+        "plain_sample_targets": n_plain,
+        "speedup_fused_vs_plain": (per_robot * len(groups) / fused_dt) / (n_plain / plain_dt),
+        "fused_success_1mm_0.01rad": {n: float(ok.mean()) for n, ok in zip(names, fused_ok)},
+        "fused_success_on_plain_sample": float(np.concatenate(fused_on_sample).mean()),
+        "plain_success_on_plain_sample": float(np.concatenate(plain_ok).mean()),
+        "autotune": {n: list(g.tune) for n, g in zip(names, groups)},
+        "autotune_ms": [v.get("ms") for v in report.values()],
+        "cpu_baseline": {"value": n_cpu / cpu_dt, "unit": "solves/s", "cores": threads, "kind": "port",
+                         "sample": f"first {cpu_per_robot} targets of each robot with their seeds, FP64 oracle"},
+    }
+
+
 def grad_bench(reps: int, cpu_sample: int):
     """Config 3: Panda, 16384 configs, pose + hinge (margin 0.05) against 512 world spheres,
     cost and dcost/dq through the hand-written adjoint (autograd Function)."""
@@ -368,6 +480,7 @@ def main():
     if not args.no_ik:
         line["ik"] = ik_bench(reps=10)
         line["grad"] = grad_bench(reps=10, cpu_sample=256)
+        line["ik_uneven"] = ik_uneven_bench(reps=5)
     print(json.dumps(line), flush=True)
 
 

@@ -234,8 +234,75 @@ int pk_ls_ik_f32(void* stream, const float* seeds, int32_t n_problems,
                  float collision_margin, float* out_cfgs, float* out_err);
 
 /*
+ * Uniform grid over a sphere world for exact culling of margin-limited costs.
+ * Cell (ix, iy, iz) is the box lo + cell * [i, i+1) per axis; it lists (up to
+ * cap indices) every world sphere whose distance to the box is at most
+ * R_m + pad, and counts[c] holds the true count (> cap means "overflowed: use
+ * the whole world").  With pad = max robot sphere radius + margin and the
+ * grid box = world bounding box grown by pad, a robot sphere whose centre lies
+ * in a cell can be within the margin only of spheres listed there, and one
+ * outside the grid of none.  No reference counterpart (an acceleration
+ * structure for the LM collision penalty).
+ *   counts (nx*ny*nz) int32, items (nx*ny*nz, cap) int32
+ */
+int pk_world_grid_f32(void* stream, const float* world_spheres, int32_t n_world_spheres,
+                      float pad, float lo_x, float lo_y, float lo_z, float cell,
+                      int32_t nx, int32_t ny, int32_t nz, int32_t cap, int32_t* counts,
+                      int32_t* items);
+
+/*
+ * Uneven batched LM-IK: many problems of one robot with different seed
+ * counts in one persistent launch.  Same algorithm and buffers as
+ * pk_ls_ik_f32, except
+ *   seeds            (total_seeds, n_act), problem p owns the rows
+ *                    [problem_seed_start[p], problem_seed_start[p+1])
+ *   target_T         (n_problems, n_ee, 7)
+ *   grid_*           optional world grid from pk_world_grid_f32 (NULL counts:
+ *                    every world sphere is tested) built with pad grid_pad;
+ *                    robot spheres of radius > grid_pad - collision_margin
+ *                    test every world sphere, so the penalty is the same for
+ *                    any pad
+ *   iter_begin/end   run iterations [iter_begin, iter_end) of max_iter; a
+ *                    partial range reads/writes `state` (total_seeds,
+ *                    2 n_act + 3), so a solve can be split into launches
+ *                    (e.g. nodes of a captured graph); outputs are written
+ *                    by the launch that reaches max_iter
+ *   grid_waves       number of persistent workgroups (0: one per 64-seed tile)
+ *   out_cfgs (total_seeds, n_act), out_err (total_seeds)
+ * Replaces one ls_ik_solve_cuda_batch call per robot
+ * (src/pyroffi/optimization_engines/_ls_ik.py:1002-1172), which needs the same
+ * seed count for every problem, and the per-call path for uneven ones.
+ */
+int pk_ls_ik_batched_f32(
+    void* stream, const float* seeds, const int32_t* problem_seed_start, int32_t n_problems,
+    int64_t total_seeds, int32_t n_act, const float* twists, const float* parent_tf,
+    const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul,
+    const float* mimic_off, const int32_t* mimic_act_idx, const int32_t* topo_inv,
+    int32_t n_joints, const int32_t* target_jnts, const int32_t* ancestor_masks, int32_t n_ee,
+    const float* target_T, const float* robot_spheres_local,
+    const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres, const float* world_spheres,
+    int32_t n_world_spheres, const int32_t* grid_counts, const int32_t* grid_items,
+    float grid_lo_x, float grid_lo_y, float grid_lo_z, float grid_cell, int32_t grid_nx,
+    int32_t grid_ny, int32_t grid_nz, int32_t grid_cap, float grid_pad, const float* lower,
+    const float* upper, const int32_t* fixed_mask, int32_t max_iter, int32_t iter_begin,
+    int32_t iter_end, float* state, float pos_weight, float ori_weight, float lambda_init,
+    float eps_pos, float eps_ori, int32_t enable_collision, float collision_weight,
+    float collision_margin, int32_t grid_waves, float* out_cfgs, float* out_err);
+
+/*
+ * Winner selection after pk_ls_ik_batched_f32: per problem, over its CSR seed
+ * range, the seed minimising err + continuity_weight * |cfg - previous|^2.
+ */
+int pk_ls_ik_select_segments_f32(void* stream, const float* cfgs, const float* errs,
+                                 const float* previous, const int32_t* problem_seed_start,
+                                 int32_t n_problems, int32_t n_act, float continuity_weight,
+                                 float* out_best, int32_t* out_idx);
+
+/*
  * Winner selection after pk_ls_ik_f32: per problem, the seed minimising
- * err + continuity_weight * |cfg - previous|^2 (first index on ties).
+ * err + continuity_weight * |cfg - previous|^2 (first index on ties; a NaN
+ * score ranks as +inf, where jnp.argmin would return the first NaN -- the
+ * solver only produces NaN errors for a robot it rejects, all seeds alike).
  * Replaces the argmin of _ls_ik_solve_cuda_batch_jit
  * (src/pyroffi/optimization_engines/_ls_ik.py:974-999).
  *

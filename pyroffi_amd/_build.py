@@ -14,7 +14,7 @@ import sys
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libpyroffi_amd.so")
-SOURCES = ("pk_common.cpp", "pk_fk.hip", "pk_collision.hip", "pk_ik.hip", "pk_grad.hip")
+SOURCES = ("pk_common.cpp", "pk_fk.hip", "pk_collision.hip", "pk_ik.hip", "pk_grad.hip", "pk_grid.hip")
 HEADERS = ("pk_common.h", "pk_se3.h", "pk_model.h")
 
 # Packed FP32 instructions are disabled for all device code; hipcc's host pass

@@ -30,6 +30,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace pk {
 
 struct NodeRec {
@@ -43,6 +45,14 @@ struct NodeRec {
   float ax[3];      // unit joint axis in the joint frame (Jacobian)
   unsigned ee_mask; // bit e set when the node is an ancestor of EE e
   int pad[2];
+};
+
+struct GridArgs {
+  const int* counts;   // (n_cells) or null: evaluate every world sphere
+  const int* items;    // (n_cells, cap)
+  float lo_x, lo_y, lo_z, inv_cell;
+  int nx, ny, nz, cap;
+  float r_max;         // robot spheres of larger radius test every world sphere
 };
 
 struct IkArgs {
@@ -67,6 +77,12 @@ struct IkArgs {
   float collision_weight, collision_margin;
   float* out_cfgs;
   float* out_err;
+  // Batched launch only.
+  const int* problem_seed_start;  // (n_problems + 1): seeds of problem p
+  long long total_seeds;
+  float* state;                   // (total_seeds, 2 * n_act + 3) or null
+  int iter_begin, iter_end;
+  GridArgs grid;
 };
 
 // Shared-memory image of one problem, built by the prologue.
@@ -242,7 +258,7 @@ __device__ __forceinline__ void ee_residual(const Pose& T, const float* tgt, flo
 
 template <int MAXA, int NEE>
 __device__ __forceinline__ void residuals(const IkShared<MAXA, NEE>& sh, const Pose* P,
-                                          float* r, Pose* Tee) {
+                                          const float (&tg)[NEE][7], float* r, Pose* Tee) {
 #pragma unroll
   for (int e = 0; e < NEE; ++e) {
     if (sh.ee_node[e] == -2) {
@@ -251,15 +267,67 @@ __device__ __forceinline__ void residuals(const IkShared<MAXA, NEE>& sh, const P
       Tee[e] = pose_identity();
     } else {
       Tee[e] = ee_pose(sh, P, e);
-      ee_residual(Tee[e], sh.tgt[e], r + 6 * e);
+      ee_residual(Tee[e], tg[e], r + 6 * e);
     }
   }
 }
 
-// Collision merit penalty sum of min(|c - o| - r - R - margin, 0)^2 (unweighted).
+// sum over world spheres m of min(|c - o_m| - r - R'_m, 0)^2, R' = R + margin.
+__device__ __forceinline__ float world_penalty_all(float cx, float cy, float cz, float r,
+                                                   const float4* ws, int M) {
+  float p0 = 0.f, p1 = 0.f;
+  int m = 0;
+  for (; m + 1 < M; m += 2) {
+    const float4 o0 = ws[m], o1 = ws[m + 1];
+    const float ax = cx - o0.x, ay = cy - o0.y, az = cz - o0.z;
+    const float bx = cx - o1.x, by = cy - o1.y, bz = cz - o1.z;
+    const float e0 = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o0.w - r, 0.f);
+    const float e1 = fminf(__builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz) - o1.w - r, 0.f);
+    p0 = fmaf(e0, e0, p0);
+    p1 = fmaf(e1, e1, p1);
+  }
+  if (m < M) {
+    const float4 o0 = ws[m];
+    const float ax = cx - o0.x, ay = cy - o0.y, az = cz - o0.z;
+    const float e0 = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o0.w - r, 0.f);
+    p0 = fmaf(e0, e0, p0);
+  }
+  return p0 + p1;
+}
+
+// Same sum over the grid cell holding c: exact, since every sphere that can
+// be within the margin of a robot sphere of radius <= r_max (= grid pad -
+// margin) is listed (pk_grid.hip); a larger robot sphere or an overflowing
+// cell falls back to the whole world and a centre outside the grid has no such
+// sphere.
+__device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz, float r,
+                                                    const float4* ws, int M, const GridArgs& g) {
+  if (r > g.r_max) return world_penalty_all(cx, cy, cz, r, ws, M);
+  const int ix = static_cast<int>(floorf((cx - g.lo_x) * g.inv_cell));
+  const int iy = static_cast<int>(floorf((cy - g.lo_y) * g.inv_cell));
+  const int iz = static_cast<int>(floorf((cz - g.lo_z) * g.inv_cell));
+  if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nx || iy >= g.ny || iz >= g.nz) return 0.f;
+  const long long cell = (static_cast<long long>(iz) * g.ny + iy) * g.nx + ix;
+  const int n = g.counts[cell];
+  if (n > g.cap) return world_penalty_all(cx, cy, cz, r, ws, M);
+  const int* it = g.items + cell * g.cap;
+  float pen = 0.f;
+  for (int i = 0; i < n; ++i) {
+    const int m = it[i];
+    if (m < 0 || m >= M) continue;
+    const float4 o = ws[m];
+    const float ax = cx - o.x, ay = cy - o.y, az = cz - o.z;
+    const float e = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o.w - r, 0.f);
+    pen = fmaf(e, e, pen);
+  }
+  return pen;
+}
+
+// Collision merit penalty (unweighted) over every robot sphere.
 template <int MAXA, int NEE>
 __device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, const Pose* P,
-                                                const float4* sph, const float4* ws, int M) {
+                                                const float4* sph, const float4* ws, int M,
+                                                const GridArgs& grid) {
   float pen = 0.f;
 #pragma unroll
   for (int g = 0; g <= MAXA; ++g) {
@@ -273,24 +341,8 @@ __device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, c
           const float cx = R.m00 * s.x + R.m01 * s.y + R.m02 * s.z + T.tx;
           const float cy = R.m10 * s.x + R.m11 * s.y + R.m12 * s.z + T.ty;
           const float cz = R.m20 * s.x + R.m21 * s.y + R.m22 * s.z + T.tz;
-          float p0 = 0.f, p1 = 0.f;
-          int m = 0;
-          for (; m + 1 < M; m += 2) {
-            const float4 o0 = ws[m], o1 = ws[m + 1];
-            const float ax = cx - o0.x, ay = cy - o0.y, az = cz - o0.z;
-            const float bx = cx - o1.x, by = cy - o1.y, bz = cz - o1.z;
-            const float e0 = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o0.w - s.w, 0.f);
-            const float e1 = fminf(__builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz) - o1.w - s.w, 0.f);
-            p0 = fmaf(e0, e0, p0);
-            p1 = fmaf(e1, e1, p1);
-          }
-          if (m < M) {
-            const float4 o0 = ws[m];
-            const float ax = cx - o0.x, ay = cy - o0.y, az = cz - o0.z;
-            const float e0 = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o0.w - s.w, 0.f);
-            p0 = fmaf(e0, e0, p0);
-          }
-          pen += p0 + p1;
+          pen += (grid.counts != nullptr) ? world_penalty_grid(cx, cy, cz, s.w, ws, M, grid)
+                                          : world_penalty_all(cx, cy, cz, s.w, ws, M);
         }
       }
     }
@@ -304,7 +356,7 @@ struct IkLds {
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~static_cast<size_t>(15); }
 
-// Byte offsets of the IK kernel's LDS regions (float4 regions 16-byte aligned).
+// Byte offsets of the IK kernels' LDS regions (float4 regions 16-byte aligned).
 __host__ __device__ inline IkLds ik_lds_layout(size_t shared_bytes, int J, int K, int M) {
   IkLds o;
   o.recs = align16(shared_bytes);
@@ -324,79 +376,50 @@ __device__ __forceinline__ float ls_alpha(int i) {
   return i == 0 ? 1.f : (i == 1 ? 0.5f : (i == 2 ? 0.25f : (i == 3 ? 0.1f : 0.025f)));
 }
 
-template <int MAXA, int NEE>
-__global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
-  extern __shared__ float4 smem4[];
-  using Sh = IkShared<MAXA, NEE>;
-  const int J = a.tables.n_joints;
-  const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
-  char* base = reinterpret_cast<char*>(smem4);
-  const IkLds off = ik_lds_layout(sizeof(Sh), J, Kin, a.enable_collision ? a.n_world : 0);
-  Sh& sh = *reinterpret_cast<Sh*>(base);
-  JointRec* recs = reinterpret_cast<JointRec*>(base + off.recs);
-  int* slot_of = reinterpret_cast<int*>(base + off.slot_of);
-  int* anchor = reinterpret_cast<int*>(base + off.anchor);
-  float* rel = reinterpret_cast<float*>(base + off.rel);
-  int* sph_tmp = reinterpret_cast<int*>(base + off.sph_tmp);
-  float4* sph = reinterpret_cast<float4*>(base + off.sph);
-  float4* ws = reinterpret_cast<float4*>(base + off.ws);
-
-  const int lane = threadIdx.x;
-  const int p = blockIdx.y;
-  stage_joint_records(a.tables, recs, slot_of);
-  if (lane == 0) build_problem<MAXA, NEE>(a, recs, slot_of, sh, anchor, rel, sph, sph_tmp);
-  for (int i = lane; i < NEE * 7; i += kWave) {
-    const int e = i / 7, c = i - e * 7;
-    sh.tgt[e][c] = (e < a.n_ee) ? a.target_T[(static_cast<long long>(p) * a.n_ee + e) * 7 + c]
-                                : (c == 0 ? 1.f : 0.f);
-  }
-  for (int i = lane; i < MAXA; i += kWave) {
-    const bool real = i < a.n_act;
-    sh.lo[i] = real ? a.lower[i] : 0.f;
-    sh.hi[i] = real ? a.upper[i] : 0.f;
-    sh.fixed[i] = real ? (a.fixed_mask[i] != 0) : 1;
-  }
-  const int M = (a.enable_collision && Kin > 0) ? a.n_world : 0;
-  for (int m = lane; m < M; m += kWave) {
-    const float* o = a.world + 4 * m;
-    ws[m] = make_float4(o[0], o[1], o[2], o[3] + a.collision_margin);
-  }
-  __syncthreads();
-
-  const int s = blockIdx.x * kWave + lane;
-  if (s >= a.n_seeds) return;
-  const long long gs = static_cast<long long>(p) * a.n_seeds + s;
-  if (sh.too_many_nodes) {  // reported to the caller as NaN errors, seeds unchanged
-    for (int i = 0; i < a.n_act; ++i) a.out_cfgs[gs * a.n_act + i] = a.seeds[gs * a.n_act + i];
-    a.out_err[gs] = NAN;
-    return;
-  }
-  const bool coll = M > 0 && sh.n_sph > 0;
-  const float wc = a.collision_weight;
-  float W[6] = {a.pos_weight, a.pos_weight, a.pos_weight, a.ori_weight, a.ori_weight, a.ori_weight};
-
+// Per-seed solver state (registers).
+template <int MAXA>
+struct LmState {
   float q[MAXA], best_q[MAXA];
-#pragma unroll
-  for (int i = 0; i < MAXA; ++i) {
-    q[i] = (i < a.n_act) ? a.seeds[gs * a.n_act + i] : 0.f;
-    best_q[i] = q[i];
-  }
+  float best_err, lam;
+  int done;
+};
 
+// Problem tables staged in LDS by the workgroup prologue.
+template <int MAXA, int NEE>
+struct IkView {
+  const IkShared<MAXA, NEE>* sh;
+  const float4* sph;
+  const float4* ws;
+  int M;
+  bool coll;
+  float wc;
+};
+
+template <int MAXA, int NEE>
+__device__ __forceinline__ float weighted_error(const IkView<MAXA, NEE>& v, const float (&tg)[NEE][7],
+                                                const float* W, const float* q, Pose* P, float* r,
+                                                Pose* Tee, const GridArgs& grid) {
+  node_fk(*v.sh, q, P);
+  residuals(*v.sh, P, tg, r, Tee);
+  float e = 0.f;
+#pragma unroll
+  for (int i = 0; i < 6 * NEE; ++i) { const float x = r[i] * W[i % 6]; e += x * x; }
+  if (v.coll) e += v.wc * sphere_penalty(*v.sh, P, v.sph, v.ws, v.M, grid);
+  return e;
+}
+
+// LM iterations [it0, it1) of one seed (the algorithm in the file header).
+template <int MAXA, int NEE>
+__device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const float (&tg)[NEE][7],
+                                           const float* W, const IkArgs& a, LmState<MAXA>& st,
+                                           int it0, int it1, const GridArgs& grid) {
+  const IkShared<MAXA, NEE>& sh = *v.sh;
   Pose P[MAXA];
   Pose Tee[NEE];
   float r[6 * NEE];
-  node_fk(sh, q, P);
-  residuals(sh, P, r, Tee);
-  float best_err = 0.f;
-#pragma unroll
-  for (int i = 0; i < 6 * NEE; ++i) { const float v = r[i] * W[i % 6]; best_err += v * v; }
-  if (coll) best_err += wc * sphere_penalty(sh, P, sph, ws, M);
-
-  float lam = a.lambda_init;
-
-  for (int it = 0; it < a.max_iter; ++it) {
-    node_fk(sh, q, P);
-    residuals(sh, P, r, Tee);
+  for (int it = it0; it < it1 && !st.done; ++it) {
+    node_fk(sh, st.q, P);
+    residuals(sh, P, tg, r, Tee);
     bool conv = true;
     float max_p = 0.f, max_o = 0.f;
 #pragma unroll
@@ -407,13 +430,13 @@ __global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
       max_p = fmaxf(max_p, np);
       max_o = fmaxf(max_o, no);
     }
-    if (conv) break;
+    if (conv) { st.done = 1; break; }
 
     float cur = 0.f;
     float fw[6 * NEE];
 #pragma unroll
     for (int i = 0; i < 6 * NEE; ++i) { fw[i] = r[i] * W[i % 6]; cur += fw[i] * fw[i]; }
-    if (coll) cur += wc * sphere_penalty(sh, P, sph, ws, M);
+    if (v.coll) cur += v.wc * sphere_penalty(sh, P, v.sph, v.ws, v.M, grid);
 
     // Weighted geometric Jacobian, column a from the node(s) driven by a.
     float Jm[6 * NEE][MAXA];
@@ -489,7 +512,7 @@ __global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
       // Js^T Js is singular and lam decays to 1e-10, which FP64 (the reference)
       // survives and FP32 does not; the damping used inside the solve is
       // floored at 1e-6 while the lam schedule itself is left unchanged.
-      A[i * (i + 1) / 2 + i] += fmaxf(lam, kDampFloor);
+      A[i * (i + 1) / 2 + i] += fmaxf(st.lam, kDampFloor);
     }
 #pragma unroll
     for (int i = 0; i < MAXA; ++i) {
@@ -564,64 +587,247 @@ __global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
     for (int ai = 0; ai < 5; ++ai) {
       float qt[MAXA];
 #pragma unroll
-      for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], q[i] + ls_alpha(ai) * delta[i]));
-      node_fk(sh, qt, P);
-      residuals(sh, P, r, Tee);
-      float et = 0.f;
-#pragma unroll
-      for (int i = 0; i < 6 * NEE; ++i) { const float v = r[i] * W[i % 6]; et += v * v; }
-      if (coll) et += wc * sphere_penalty(sh, P, sph, ws, M);
+      for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + ls_alpha(ai) * delta[i]));
+      const float et = weighted_error(v, tg, W, qt, P, r, Tee, grid);
       if (et < best_a_err) { best_a_err = et; best_a = ai; }
     }
     const float al = ls_alpha(best_a);
     float qt[MAXA];
 #pragma unroll
-    for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], q[i] + al * delta[i]));
+    for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + al * delta[i]));
     if (best_a_err < cur * (1.f - 1e-4f)) {
 #pragma unroll
-      for (int i = 0; i < MAXA; ++i) q[i] = qt[i];
-      lam = fmaxf(lam * 0.5f, 1e-10f);
+      for (int i = 0; i < MAXA; ++i) st.q[i] = qt[i];
+      st.lam = fmaxf(st.lam * 0.5f, 1e-10f);
     } else {
-      lam = fminf(lam * 3.f, 1e6f);
+      st.lam = fminf(st.lam * 3.f, 1e6f);
     }
-    if (best_a_err < best_err) {
-      best_err = best_a_err;
+    if (best_a_err < st.best_err) {
+      st.best_err = best_a_err;
 #pragma unroll
-      for (int i = 0; i < MAXA; ++i) best_q[i] = qt[i];
+      for (int i = 0; i < MAXA; ++i) st.best_q[i] = qt[i];
     }
   }
-#pragma unroll
-  for (int i = 0; i < MAXA; ++i)
-    if (i < a.n_act) a.out_cfgs[gs * a.n_act + i] = best_q[i];
-  a.out_err[gs] = best_err;
+}
+
+// Workgroup prologue shared by both kernels: stage the robot, limits and world.
+template <int MAXA, int NEE>
+__device__ __forceinline__ IkView<MAXA, NEE> ik_prologue(const IkArgs& a, char* base) {
+  using Sh = IkShared<MAXA, NEE>;
+  const int J = a.tables.n_joints;
+  const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
+  const IkLds off = ik_lds_layout(sizeof(Sh), J, Kin, a.enable_collision ? a.n_world : 0);
+  Sh& sh = *reinterpret_cast<Sh*>(base);
+  JointRec* recs = reinterpret_cast<JointRec*>(base + off.recs);
+  int* slot_of = reinterpret_cast<int*>(base + off.slot_of);
+  int* anchor = reinterpret_cast<int*>(base + off.anchor);
+  float* rel = reinterpret_cast<float*>(base + off.rel);
+  int* sph_tmp = reinterpret_cast<int*>(base + off.sph_tmp);
+  float4* sph = reinterpret_cast<float4*>(base + off.sph);
+  float4* ws = reinterpret_cast<float4*>(base + off.ws);
+  const int lane = threadIdx.x;
+  stage_joint_records(a.tables, recs, slot_of);
+  if (lane == 0) build_problem<MAXA, NEE>(a, recs, slot_of, sh, anchor, rel, sph, sph_tmp);
+  for (int i = lane; i < MAXA; i += kWave) {
+    const bool real = i < a.n_act;
+    sh.lo[i] = real ? a.lower[i] : 0.f;
+    sh.hi[i] = real ? a.upper[i] : 0.f;
+    sh.fixed[i] = real ? (a.fixed_mask[i] != 0) : 1;
+  }
+  const int M = (a.enable_collision && Kin > 0) ? a.n_world : 0;
+  for (int m = lane; m < M; m += kWave) {
+    const float* o = a.world + 4 * m;
+    ws[m] = make_float4(o[0], o[1], o[2], o[3] + a.collision_margin);
+  }
+  __syncthreads();
+  IkView<MAXA, NEE> v;
+  v.sh = &sh;
+  v.sph = sph;
+  v.ws = ws;
+  v.M = M;
+  v.coll = M > 0 && sh.n_sph > 0;
+  v.wc = a.collision_weight;
+  return v;
 }
 
 template <int MAXA, int NEE>
-static int launch_ik(void* stream, const IkArgs& a) {
+__device__ __forceinline__ void load_targets(const IkArgs& a, long long problem, float (&tg)[NEE][7]) {
+#pragma unroll
+  for (int e = 0; e < NEE; ++e)
+#pragma unroll
+    for (int c = 0; c < 7; ++c)
+      tg[e][c] = (e < a.n_ee) ? a.target_T[(problem * a.n_ee + e) * 7 + c] : (c == 0 ? 1.f : 0.f);
+}
+
+template <int MAXA, int NEE>
+__global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
+  extern __shared__ float4 smem4[];
+  const IkView<MAXA, NEE> v = ik_prologue<MAXA, NEE>(a, reinterpret_cast<char*>(smem4));
+  const int lane = threadIdx.x;
+  const int p = blockIdx.y;
+  const int s = blockIdx.x * kWave + lane;
+  if (s >= a.n_seeds) return;
+  const long long gs = static_cast<long long>(p) * a.n_seeds + s;
+  if (v.sh->too_many_nodes) {  // reported to the caller as NaN errors, seeds unchanged
+    for (int i = 0; i < a.n_act; ++i) a.out_cfgs[gs * a.n_act + i] = a.seeds[gs * a.n_act + i];
+    a.out_err[gs] = NAN;
+    return;
+  }
+  float tg[NEE][7];
+  load_targets<MAXA, NEE>(a, p, tg);
+  const float W[6] = {a.pos_weight, a.pos_weight, a.pos_weight, a.ori_weight, a.ori_weight, a.ori_weight};
+  LmState<MAXA> st;
+#pragma unroll
+  for (int i = 0; i < MAXA; ++i) {
+    st.q[i] = (i < a.n_act) ? a.seeds[gs * a.n_act + i] : 0.f;
+    st.best_q[i] = st.q[i];
+  }
+  Pose P[MAXA];
+  Pose Tee[NEE];
+  float r[6 * NEE];
+  st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid);
+  st.lam = a.lambda_init;
+  st.done = 0;
+  lm_iterate(v, tg, W, a, st, 0, a.max_iter, a.grid);
+#pragma unroll
+  for (int i = 0; i < MAXA; ++i)
+    if (i < a.n_act) a.out_cfgs[gs * a.n_act + i] = st.best_q[i];
+  a.out_err[gs] = st.best_err;
+}
+
+// Uneven batch: seeds of all problems in one flat list (problem p owns
+// [problem_seed_start[p], problem_seed_start[p+1])).  A workgroup takes tiles
+// of 64 consecutive seeds round-robin (tiles may straddle problems: the target
+// is per lane).  Iterations [iter_begin, iter_end) run per launch; with a state
+// buffer the solve resumes across launches (a captured graph of chunks).
+template <int MAXA, int NEE>
+__global__ __launch_bounds__(64) void ls_ik_batched_kernel(IkArgs a) {
+  extern __shared__ float4 smem4[];
+  const IkView<MAXA, NEE> v = ik_prologue<MAXA, NEE>(a, reinterpret_cast<char*>(smem4));
+  const int lane = threadIdx.x;
+  const long long n_tiles = (a.total_seeds + kWave - 1) / kWave;
+  const int A = a.n_act;
+  const int stride = 2 * A + 3;
+  const float W[6] = {a.pos_weight, a.pos_weight, a.pos_weight, a.ori_weight, a.ori_weight, a.ori_weight};
+  for (long long tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const long long s = tile * kWave + lane;
+    if (s >= a.total_seeds) continue;
+    if (v.sh->too_many_nodes) {
+      if (a.iter_end >= a.max_iter) {
+        for (int i = 0; i < A; ++i) a.out_cfgs[s * A + i] = a.seeds[s * A + i];
+        a.out_err[s] = NAN;
+      }
+      continue;
+    }
+    // Problem of this seed: binary search in the CSR offsets.
+    int lo = 0, hi = a.n_problems - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.problem_seed_start[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    float tg[NEE][7];
+    load_targets<MAXA, NEE>(a, lo, tg);
+    LmState<MAXA> st;
+    if (a.iter_begin == 0 || a.state == nullptr) {
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) {
+        st.q[i] = (i < A) ? a.seeds[s * A + i] : 0.f;
+        st.best_q[i] = st.q[i];
+      }
+      Pose P[MAXA];
+      Pose Tee[NEE];
+      float r[6 * NEE];
+      st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid);
+      st.lam = a.lambda_init;
+      st.done = 0;
+    } else {
+      const float* sp = a.state + s * stride;
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) {
+        st.q[i] = (i < A) ? sp[i] : 0.f;
+        st.best_q[i] = (i < A) ? sp[A + i] : 0.f;
+      }
+      st.best_err = sp[2 * A];
+      st.lam = sp[2 * A + 1];
+      st.done = sp[2 * A + 2] != 0.f;
+    }
+    lm_iterate(v, tg, W, a, st, a.iter_begin, a.iter_end, a.grid);
+    if (a.state != nullptr && a.iter_end < a.max_iter) {
+      float* sp = a.state + s * stride;
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i)
+        if (i < A) { sp[i] = st.q[i]; sp[A + i] = st.best_q[i]; }
+      sp[2 * A] = st.best_err;
+      sp[2 * A + 1] = st.lam;
+      sp[2 * A + 2] = st.done ? 1.f : 0.f;
+    } else {
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i)
+        if (i < A) a.out_cfgs[s * A + i] = st.best_q[i];
+      a.out_err[s] = st.best_err;
+    }
+  }
+}
+
+template <int MAXA, int NEE>
+static int launch_ik(void* stream, const IkArgs& a, bool batched, int grid_waves) {
   using Sh = IkShared<MAXA, NEE>;
   const int J = a.tables.n_joints;
   const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
   const int M = a.enable_collision ? a.n_world : 0;
   const size_t lds = ik_lds_layout(sizeof(Sh), J, Kin, M).total;
   PK_REQUIRE(lds <= 150 * 1024, "IK tables need %zu bytes of LDS (limit 150 KiB)", lds);
+  if (batched) {
+    const long long n_tiles = (a.total_seeds + kWave - 1) / kWave;
+    long long g = grid_waves > 0 ? grid_waves : n_tiles;
+    g = std::max(1LL, std::min(g, n_tiles));
+    hipLaunchKernelGGL((ls_ik_batched_kernel<MAXA, NEE>), dim3(static_cast<unsigned>(g)), dim3(kWave), lds,
+                       static_cast<hipStream_t>(stream), a);
+    return check_launch("ls_ik_batched_kernel");
+  }
   dim3 grid((a.n_seeds + kWave - 1) / kWave, a.n_problems);
   hipLaunchKernelGGL((ls_ik_kernel<MAXA, NEE>), grid, dim3(kWave), lds,
                      static_cast<hipStream_t>(stream), a);
   return check_launch("ls_ik_kernel");
 }
 
-// Per problem: argmin over seeds of err + w |q - prev|^2, first index on ties.
+template <int MAXA, int NEE>
+static int launch_ik_width(void* stream, const IkArgs& a, bool batched, int grid_waves) {
+  return launch_ik<MAXA, NEE>(stream, a, batched, grid_waves);
+}
+
+static int dispatch_ik(void* stream, const IkArgs& a, bool batched, int grid_waves) {
+  // Padded widths: the moving-joint count (actuated + mimic + offset-only
+  // joints) must fit; a robot that exceeds it gets NaN errors (see header).
+  const int nee = a.n_ee <= 1 ? 1 : (a.n_ee <= 2 ? 2 : 4);
+  if (a.n_act <= 8) {
+    if (nee == 1) return launch_ik_width<8, 1>(stream, a, batched, grid_waves);
+    if (nee == 2) return launch_ik_width<8, 2>(stream, a, batched, grid_waves);
+    return launch_ik_width<8, 4>(stream, a, batched, grid_waves);
+  }
+  if (nee == 1) return launch_ik_width<16, 1>(stream, a, batched, grid_waves);
+  if (nee == 2) return launch_ik_width<16, 2>(stream, a, batched, grid_waves);
+  return launch_ik_width<16, 4>(stream, a, batched, grid_waves);
+}
+
+// Per problem: argmin over its seeds of err + w |q - prev|^2, first index on
+// ties.  Seeds of problem p are [p S, (p + 1) S) or, with seed_start, the CSR
+// range [seed_start[p], seed_start[p + 1]).
 __global__ __launch_bounds__(64) void select_kernel(const float* __restrict__ cfgs,
                                                     const float* __restrict__ errs,
-                                                    const float* __restrict__ prev, int S, int A,
+                                                    const float* __restrict__ prev,
+                                                    const int* __restrict__ seed_start, int S, int A,
                                                     float cw, float* __restrict__ out_best,
                                                     int* __restrict__ out_idx) {
   const int p = blockIdx.x;
   const int lane = threadIdx.x;
+  const long long s0 = seed_start ? seed_start[p] : static_cast<long long>(p) * S;
+  const long long s1 = seed_start ? seed_start[p + 1] : s0 + S;
+  const int n = static_cast<int>(max(0LL, s1 - s0));
   float bv = INFINITY;
   int bi = 0x7fffffff;
-  for (int s = lane; s < S; s += kWave) {
-    const long long g = static_cast<long long>(p) * S + s;
+  for (int s = lane; s < n; s += kWave) {
+    const long long g = s0 + s;
     float v = errs[g];
     if (cw != 0.f) {
       float d2 = 0.f;
@@ -631,7 +837,8 @@ __global__ __launch_bounds__(64) void select_kernel(const float* __restrict__ cf
       }
       v += cw * d2;
     }
-    if (v < bv || (v == bv && s < bi) || (v != v && bi == 0x7fffffff)) { bv = v; bi = s; }
+    if (v != v) v = INFINITY;  // a NaN error never wins over a number
+    if (v < bv || (v == bv && s < bi)) { bv = v; bi = s; }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -640,35 +847,34 @@ __global__ __launch_bounds__(64) void select_kernel(const float* __restrict__ cf
     if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
   }
   if (bi == 0x7fffffff) bi = 0;
-  for (int i = lane; i < A; i += kWave)
-    out_best[static_cast<long long>(p) * A + i] = cfgs[(static_cast<long long>(p) * S + bi) * A + i];
+  if (n > 0)
+    for (int i = lane; i < A; i += kWave)
+      out_best[static_cast<long long>(p) * A + i] = cfgs[(s0 + bi) * A + i];
   if (out_idx != nullptr && lane == 0) out_idx[p] = bi;
 }
 
 }  // namespace pk
 
-extern "C" int pk_ls_ik_f32(
-    void* stream, const float* seeds, int32_t n_problems, int32_t n_seeds, int32_t n_act,
-    const float* twists, const float* parent_tf, const int32_t* parent_idx,
-    const int32_t* act_idx, const float* mimic_mul, const float* mimic_off,
-    const int32_t* mimic_act_idx, const int32_t* topo_inv, int32_t n_joints,
-    const int32_t* target_jnts, const int32_t* ancestor_masks, int32_t n_ee,
-    const float* target_T, const float* robot_spheres_local,
-    const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres,
-    const float* world_spheres, int32_t n_world_spheres, const float* world_capsules,
-    int32_t n_world_capsules, const float* world_boxes, int32_t n_world_boxes,
-    const float* world_halfspaces, int32_t n_world_halfspaces, const float* lower,
-    const float* upper, const int32_t* fixed_mask, int32_t max_iter, float pos_weight,
-    float ori_weight, float lambda_init, float eps_pos, float eps_ori,
-    int32_t enable_collision, float collision_weight, float collision_margin,
-    float* out_cfgs, float* out_err) {
-  (void)world_capsules; (void)world_boxes; (void)world_halfspaces;
-  PK_REQUIRE(n_problems >= 0 && n_seeds >= 0 && n_act >= 0 && n_ee >= 0, "negative size");
+namespace {
+
+int fill_ik_args(pk::IkArgs& a, const float* seeds, int32_t n_problems, int32_t n_act,
+                 const float* twists, const float* parent_tf, const int32_t* parent_idx,
+                 const int32_t* act_idx, const float* mimic_mul, const float* mimic_off,
+                 const int32_t* mimic_act_idx, const int32_t* topo_inv, int32_t n_joints,
+                 const int32_t* target_jnts, const int32_t* ancestor_masks, int32_t n_ee,
+                 const float* target_T, const float* robot_spheres_local,
+                 const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres,
+                 const float* world_spheres, int32_t n_world_spheres, int32_t n_world_capsules,
+                 int32_t n_world_boxes, int32_t n_world_halfspaces, const float* lower,
+                 const float* upper, const int32_t* fixed_mask, int32_t max_iter, float pos_weight,
+                 float ori_weight, float lambda_init, float eps_pos, float eps_ori,
+                 int32_t enable_collision, float collision_weight, float collision_margin,
+                 float* out_cfgs, float* out_err) {
+  PK_REQUIRE(n_problems >= 0 && n_act >= 0 && n_ee >= 0, "negative size");
   PK_REQUIRE(n_joints >= 0 && n_joints <= pk::kMaxJoints, "n_joints=%d outside [0, %d]", n_joints,
              pk::kMaxJoints);
   PK_REQUIRE(n_act <= pk::kMaxAct, "n_act=%d exceeds %d", n_act, pk::kMaxAct);
   PK_REQUIRE(n_ee <= pk::kMaxEE, "n_ee=%d exceeds %d", n_ee, pk::kMaxEE);
-  PK_REQUIRE(n_problems <= 65535, "n_problems=%d exceeds the 65535 grid limit", n_problems);
   PK_REQUIRE(max_iter >= 0, "max_iter must be >= 0");
   const bool coll = enable_collision != 0 && n_robot_spheres > 0;
   if (coll)
@@ -678,16 +884,13 @@ extern "C" int pk_ls_ik_f32(
              n_robot_spheres);
   PK_REQUIRE(n_world_spheres >= 0 && n_world_spheres <= 4096, "n_world_spheres=%d outside [0, 4096]",
              n_world_spheres);
-  if (n_problems == 0 || n_seeds == 0) return PK_OK;
   PK_REQUIRE(seeds && out_cfgs && out_err && lower && upper && fixed_mask, "null pointer");
   PK_REQUIRE(n_ee == 0 || (target_jnts && ancestor_masks && target_T), "null target pointer");
   if (coll) PK_REQUIRE(robot_spheres_local && robot_sphere_joint_idx &&
                            (n_world_spheres == 0 || world_spheres),
                        "null collision pointer");
-  pk::IkArgs a{};
   a.seeds = seeds;
   a.n_problems = n_problems;
-  a.n_seeds = n_seeds;
   a.n_act = n_act;
   a.tables = pk::RobotTables{twists, parent_tf, parent_idx, act_idx, mimic_mul,
                              mimic_off, mimic_act_idx, topo_inv, n_joints, n_act};
@@ -714,18 +917,91 @@ extern "C" int pk_ls_ik_f32(
   a.collision_margin = collision_margin;
   a.out_cfgs = out_cfgs;
   a.out_err = out_err;
-  // Padded widths: the moving-joint count (actuated + mimic + offset-only
-  // joints) must fit; a robot that exceeds it gets NaN errors (see header).
-  const int nee = n_ee <= 1 ? 1 : (n_ee <= 2 ? 2 : 4);
-  const int wide = n_act <= 8 ? 8 : 16;
-  if (wide == 8) {
-    if (nee == 1) return pk::launch_ik<8, 1>(stream, a);
-    if (nee == 2) return pk::launch_ik<8, 2>(stream, a);
-    return pk::launch_ik<8, 4>(stream, a);
+  a.iter_begin = 0;
+  a.iter_end = max_iter;
+  return PK_OK;
+}
+
+}  // namespace
+
+extern "C" int pk_ls_ik_f32(
+    void* stream, const float* seeds, int32_t n_problems, int32_t n_seeds, int32_t n_act,
+    const float* twists, const float* parent_tf, const int32_t* parent_idx,
+    const int32_t* act_idx, const float* mimic_mul, const float* mimic_off,
+    const int32_t* mimic_act_idx, const int32_t* topo_inv, int32_t n_joints,
+    const int32_t* target_jnts, const int32_t* ancestor_masks, int32_t n_ee,
+    const float* target_T, const float* robot_spheres_local,
+    const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres,
+    const float* world_spheres, int32_t n_world_spheres, const float* world_capsules,
+    int32_t n_world_capsules, const float* world_boxes, int32_t n_world_boxes,
+    const float* world_halfspaces, int32_t n_world_halfspaces, const float* lower,
+    const float* upper, const int32_t* fixed_mask, int32_t max_iter, float pos_weight,
+    float ori_weight, float lambda_init, float eps_pos, float eps_ori,
+    int32_t enable_collision, float collision_weight, float collision_margin,
+    float* out_cfgs, float* out_err) {
+  (void)world_capsules; (void)world_boxes; (void)world_halfspaces;
+  PK_REQUIRE(n_seeds >= 0, "negative size");
+  PK_REQUIRE(n_problems <= 65535, "n_problems=%d exceeds the 65535 grid limit", n_problems);
+  if (n_problems == 0 || n_seeds == 0) return PK_OK;
+  pk::IkArgs a{};
+  const int st = fill_ik_args(a, seeds, n_problems, n_act, twists, parent_tf, parent_idx, act_idx,
+                              mimic_mul, mimic_off, mimic_act_idx, topo_inv, n_joints, target_jnts,
+                              ancestor_masks, n_ee, target_T, robot_spheres_local,
+                              robot_sphere_joint_idx, n_robot_spheres, world_spheres,
+                              n_world_spheres, n_world_capsules, n_world_boxes, n_world_halfspaces,
+                              lower, upper, fixed_mask, max_iter, pos_weight, ori_weight,
+                              lambda_init, eps_pos, eps_ori, enable_collision, collision_weight,
+                              collision_margin, out_cfgs, out_err);
+  if (st != PK_OK) return st;
+  a.n_seeds = n_seeds;
+  return pk::dispatch_ik(stream, a, false, 0);
+}
+
+extern "C" int pk_ls_ik_batched_f32(
+    void* stream, const float* seeds, const int32_t* problem_seed_start, int32_t n_problems,
+    int64_t total_seeds, int32_t n_act, const float* twists, const float* parent_tf,
+    const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul,
+    const float* mimic_off, const int32_t* mimic_act_idx, const int32_t* topo_inv,
+    int32_t n_joints, const int32_t* target_jnts, const int32_t* ancestor_masks, int32_t n_ee,
+    const float* target_T, const float* robot_spheres_local,
+    const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres, const float* world_spheres,
+    int32_t n_world_spheres, const int32_t* grid_counts, const int32_t* grid_items,
+    float grid_lo_x, float grid_lo_y, float grid_lo_z, float grid_cell, int32_t grid_nx,
+    int32_t grid_ny, int32_t grid_nz, int32_t grid_cap, float grid_pad, const float* lower,
+    const float* upper, const int32_t* fixed_mask, int32_t max_iter, int32_t iter_begin,
+    int32_t iter_end, float* state, float pos_weight, float ori_weight, float lambda_init,
+    float eps_pos, float eps_ori, int32_t enable_collision, float collision_weight,
+    float collision_margin, int32_t grid_waves, float* out_cfgs, float* out_err) {
+  PK_REQUIRE(total_seeds >= 0, "negative size");
+  if (n_problems == 0 || total_seeds == 0) return PK_OK;
+  PK_REQUIRE(problem_seed_start != nullptr, "null problem_seed_start");
+  PK_REQUIRE(0 <= iter_begin && iter_begin <= iter_end && iter_end <= max_iter,
+             "iteration range [%d, %d) outside [0, %d]", iter_begin, iter_end, max_iter);
+  PK_REQUIRE(state != nullptr || (iter_begin == 0 && iter_end == max_iter),
+             "a partial iteration range needs a state buffer");
+  pk::IkArgs a{};
+  const int st = fill_ik_args(a, seeds, n_problems, n_act, twists, parent_tf, parent_idx, act_idx,
+                              mimic_mul, mimic_off, mimic_act_idx, topo_inv, n_joints, target_jnts,
+                              ancestor_masks, n_ee, target_T, robot_spheres_local,
+                              robot_sphere_joint_idx, n_robot_spheres, world_spheres,
+                              n_world_spheres, 0, 0, 0, lower, upper, fixed_mask, max_iter,
+                              pos_weight, ori_weight, lambda_init, eps_pos, eps_ori,
+                              enable_collision, collision_weight, collision_margin, out_cfgs,
+                              out_err);
+  if (st != PK_OK) return st;
+  a.problem_seed_start = problem_seed_start;
+  a.total_seeds = total_seeds;
+  a.state = state;
+  a.iter_begin = iter_begin;
+  a.iter_end = iter_end;
+  if (grid_counts != nullptr && a.enable_collision) {
+    PK_REQUIRE(grid_items && grid_cell > 0.f && grid_nx > 0 && grid_ny > 0 && grid_nz > 0 && grid_cap > 0,
+               "invalid world grid");
+    PK_REQUIRE(grid_pad >= 0.f, "negative grid pad");
+    a.grid = pk::GridArgs{grid_counts, grid_items, grid_lo_x, grid_lo_y, grid_lo_z, 1.f / grid_cell,
+                          grid_nx, grid_ny, grid_nz, grid_cap, grid_pad - collision_margin};
   }
-  if (nee == 1) return pk::launch_ik<16, 1>(stream, a);
-  if (nee == 2) return pk::launch_ik<16, 2>(stream, a);
-  return pk::launch_ik<16, 4>(stream, a);
+  return pk::dispatch_ik(stream, a, true, grid_waves);
 }
 
 extern "C" int pk_ls_ik_select_f32(void* stream, const float* cfgs, const float* errs,
@@ -736,9 +1012,23 @@ extern "C" int pk_ls_ik_select_f32(void* stream, const float* cfgs, const float*
   if (n_problems == 0) return PK_OK;
   PK_REQUIRE(n_seeds > 0, "n_seeds must be > 0");
   PK_REQUIRE(cfgs && errs && out_best && (continuity_weight == 0.f || previous), "null pointer");
-  PK_REQUIRE(n_problems <= 0x7fffffff, "too many problems");
   hipLaunchKernelGGL(pk::select_kernel, dim3(n_problems), dim3(pk::kWave), 0,
-                     static_cast<hipStream_t>(stream), cfgs, errs, previous, n_seeds, n_act,
+                     static_cast<hipStream_t>(stream), cfgs, errs, previous, nullptr, n_seeds, n_act,
                      continuity_weight, out_best, out_idx);
+  return pk::check_launch("select_kernel");
+}
+
+extern "C" int pk_ls_ik_select_segments_f32(void* stream, const float* cfgs, const float* errs,
+                                            const float* previous, const int32_t* problem_seed_start,
+                                            int32_t n_problems, int32_t n_act,
+                                            float continuity_weight, float* out_best,
+                                            int32_t* out_idx) {
+  PK_REQUIRE(n_problems >= 0 && n_act >= 0, "negative size");
+  if (n_problems == 0) return PK_OK;
+  PK_REQUIRE(cfgs && errs && out_best && problem_seed_start && (continuity_weight == 0.f || previous),
+             "null pointer");
+  hipLaunchKernelGGL(pk::select_kernel, dim3(n_problems), dim3(pk::kWave), 0,
+                     static_cast<hipStream_t>(stream), cfgs, errs, previous, problem_seed_start, 0,
+                     n_act, continuity_weight, out_best, out_idx);
   return pk::check_launch("select_kernel");
 }

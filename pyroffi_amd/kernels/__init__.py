@@ -8,10 +8,13 @@ from .ops import (
     fk_hip,
     fk_links,
     ls_ik_cuda,
+    ls_ik_batched,
     ls_ik_hip,
     ls_ik_select,
+    ls_ik_select_segments,
     pose_hinge_cost,
     sphere_sweep,
+    world_grid,
 )
 
 __all__ = [
@@ -26,8 +29,11 @@ __all__ = [
     "lib",
     "library_path",
     "ls_ik_cuda",
+    "ls_ik_batched",
     "ls_ik_hip",
     "ls_ik_select",
+    "ls_ik_select_segments",
     "pose_hinge_cost",
     "sphere_sweep",
+    "world_grid",
 ]

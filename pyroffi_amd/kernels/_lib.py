@@ -47,6 +47,21 @@ _SIGNATURES = {
         + [_p, _p],  # outputs
     ),
     "pk_ls_ik_select_f32": (_i32, [_p, _p, _p, _p, _i32, _i32, _i32, _f32, _p, _p]),
+    "pk_world_grid_f32": (_i32, [_p, _p, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _i32, _p, _p]),
+    "pk_ls_ik_batched_f32": (
+        _i32,
+        [_p, _p, _p, _i32, _i64, _i32]  # stream, seeds, seed_start, P, total, A
+        + [_p] * 8 + [_i32]  # robot tables, n_joints
+        + [_p, _p, _i32, _p]  # targets
+        + [_p, _p, _i32]  # robot spheres
+        + [_p, _i32]  # world
+        + [_p, _p, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _i32, _f32]  # grid
+        + [_p, _p, _p]  # limits
+        + [_i32, _i32, _i32, _p]  # max_iter, iter range, state
+        + [_f32, _f32, _f32, _f32, _f32, _i32, _f32, _f32, _i32]  # attrs, grid_waves
+        + [_p, _p],
+    ),
+    "pk_ls_ik_select_segments_f32": (_i32, [_p, _p, _p, _p, _p, _i32, _i32, _f32, _p, _p]),
     "pk_pose_hinge_cost_f32": (
         _i32,
         [_p, _p, _i64, _i32, _p, _p, _i32, _p, _i32, _f32, _i32, _p, _f32, _f32]

@@ -250,3 +250,107 @@ def pose_hinge_cost(link_poses, spheres_local, link_sphere_start, world_spheres,
     )
     check(st, "pk_pose_hinge_cost_f32")
     return cost, grad
+
+
+def world_grid(world_spheres, pad: float, cell: float | None = None, cap: int = 32, max_cells: int = 48):
+    """Build the exact-culling grid of a sphere world (see ``pk_world_grid_f32``).
+
+    ``pad`` = the robot sphere radius the grid serves + cost margin (the LM
+    kernel tests larger robot spheres against every world sphere).  The grid box
+    is the world's bounding box grown by ``pad``; ``cell`` defaults to the box's longest
+    side / ``max_cells``.  Returns a dict of device tensors and scalars.
+    """
+    import numpy as np
+
+    ws = _dev(world_spheres, torch.float32).reshape(-1, 4)
+    dev = ws.device
+    w = ws.detach().cpu().numpy().astype(np.float64)
+    if w.shape[0] == 0:
+        lo, hi = np.zeros(3), np.ones(3)
+    else:
+        lo = (w[:, :3] - w[:, 3:4]).min(0) - pad
+        hi = (w[:, :3] + w[:, 3:4]).max(0) + pad
+    if cell is None:
+        cell = float(max(hi - lo)) / max_cells
+    dims = np.maximum(1, np.ceil((hi - lo) / cell).astype(np.int64))
+    n_cells = int(np.prod(dims))
+    counts = torch.empty((n_cells,), dtype=torch.int32, device=dev)
+    items = torch.empty((n_cells, cap), dtype=torch.int32, device=dev)
+    st = lib().pk_world_grid_f32(
+        _stream(), _ptr(ws), ws.shape[0], float(pad), float(lo[0]), float(lo[1]), float(lo[2]), float(cell),
+        int(dims[0]), int(dims[1]), int(dims[2]), int(cap), _ptr(counts), _ptr(items),
+    )
+    check(st, "pk_world_grid_f32")
+    return {"counts": counts, "items": items, "pad": float(pad), "lo": tuple(float(v) for v in lo),
+            "cell": float(cell),
+            "dims": tuple(int(v) for v in dims), "cap": int(cap)}
+
+
+def ls_ik_batched(seeds, problem_seed_start, tables, target_jnts, ancestor_masks, target_T,
+                  robot_spheres_local, robot_sphere_joint_idx, world_spheres, lower, upper, fixed_mask, *,
+                  max_iter: int, iter_begin: int = 0, iter_end: int | None = None, state=None,
+                  grid: dict | None = None, pos_weight: float = 50.0, ori_weight: float = 10.0,
+                  lambda_init: float = 5e-3, eps_pos: float = 1e-8, eps_ori: float = 1e-8,
+                  enable_collision: bool = False, collision_weight: float = 1e4,
+                  collision_margin: float = 0.02, grid_waves: int = 0, out_cfgs=None, out_err=None):
+    """Uneven batched LM over flat seeds ``(S_total, A)``; ``tables`` = the 8 robot buffers.
+
+    Writes into (or allocates) ``out_cfgs (S_total, A)`` and ``out_err (S_total,)``.
+    """
+    sd = _dev(seeds, torch.float32)
+    dev = sd.device
+    S_total, A = sd.shape
+    ss = _dev(problem_seed_start, torch.int32, dev)
+    P = ss.shape[0] - 1
+    tb = [_dev(x, torch.float32 if x.dtype.is_floating_point else torch.int32, dev) for x in tables]
+    tj = _dev(target_jnts, torch.int32, dev).reshape(-1)
+    n_ee = tj.shape[0]
+    am = _dev(ancestor_masks, torch.int32, dev).reshape(n_ee, -1)
+    tT = _dev(target_T, torch.float32, dev).reshape(P, n_ee, 7)
+    rs = _dev(robot_spheres_local, torch.float32, dev).reshape(-1, 4)
+    rj = _dev(robot_sphere_joint_idx, torch.int32, dev).reshape(-1)
+    ws = _dev(world_spheres, torch.float32, dev).reshape(-1, 4)
+    lo = _dev(lower, torch.float32, dev)
+    hi = _dev(upper, torch.float32, dev)
+    fm = _dev(fixed_mask, torch.int32, dev)
+    if out_cfgs is None:
+        out_cfgs = torch.empty((S_total, A), dtype=torch.float32, device=dev)
+    if out_err is None:
+        out_err = torch.empty((S_total,), dtype=torch.float32, device=dev)
+    iter_end = max_iter if iter_end is None else iter_end
+    g = grid or {}
+    gl = g.get("lo", (0.0, 0.0, 0.0))
+    gd = g.get("dims", (0, 0, 0))
+    st = lib().pk_ls_ik_batched_f32(
+        _stream(), _ptr(sd), _ptr(ss), P, S_total, A, *[_ptr(t) for t in tb], tb[0].shape[0],
+        _ptr(tj), _ptr(am), n_ee, _ptr(tT), _ptr(rs), _ptr(rj), rs.shape[0], _ptr(ws), ws.shape[0],
+        _ptr(g.get("counts")), _ptr(g.get("items")), gl[0], gl[1], gl[2], float(g.get("cell", 1.0)),
+        gd[0], gd[1], gd[2], int(g.get("cap", 0)), float(g.get("pad", 0.0)), _ptr(lo), _ptr(hi), _ptr(fm),
+        int(max_iter),
+        int(iter_begin), int(iter_end), _ptr(state), float(pos_weight), float(ori_weight), float(lambda_init),
+        float(eps_pos), float(eps_ori), int(bool(enable_collision)), float(collision_weight),
+        float(collision_margin), int(grid_waves), _ptr(out_cfgs), _ptr(out_err),
+    )
+    check(st, "pk_ls_ik_batched_f32")
+    return out_cfgs, out_err
+
+
+def ls_ik_select_segments(cfgs, errs, problem_seed_start, previous=None, continuity_weight: float = 0.0,
+                          out_best=None, out_idx=None):
+    """Per-problem winner over CSR seed ranges: ``(best (P, A), index within the problem (P,))``."""
+    c = _dev(cfgs, torch.float32)
+    dev = c.device
+    e = _dev(errs, torch.float32, dev)
+    ss = _dev(problem_seed_start, torch.int32, dev)
+    P, A = ss.shape[0] - 1, c.shape[1]
+    prev = None if previous is None else _dev(previous, torch.float32, dev).reshape(P, A)
+    if out_best is None:
+        out_best = torch.empty((P, A), dtype=torch.float32, device=dev)
+    if out_idx is None:
+        out_idx = torch.empty((P,), dtype=torch.int32, device=dev)
+    st = lib().pk_ls_ik_select_segments_f32(
+        _stream(), _ptr(c), _ptr(e), _ptr(prev), _ptr(ss), P, A, float(continuity_weight), _ptr(out_best),
+        _ptr(out_idx),
+    )
+    check(st, "pk_ls_ik_select_segments_f32")
+    return out_best, out_idx

@@ -1,0 +1,265 @@
+"""Uneven multi-robot LM-IK in persistent launches, captured in one graph.
+
+The reference solves a batch of IK problems with ``ls_ik_solve_cuda_batch``
+(``src/pyroffi/optimization_engines/_ls_ik.py:1002-1172``): one robot, one
+end effector, the same seed count for every problem, one kernel launch per
+call.  Problems that differ in robot, DOF, end-effector count or seed count
+need one call each.  ``UnevenLsIkSolver`` takes any mix:
+
+* problems are grouped by robot; each group's seeds are one flat list with
+  CSR offsets per problem (``pk_ls_ik_batched_f32``), so seed counts can differ
+  per problem and tiles of 64 seeds may straddle problems;
+* each group runs as a persistent launch (a fixed number of workgroups
+  walking the tiles), its iterations optionally split into chunks that resume
+  from a per-seed state buffer;
+* the collision penalty is evaluated over an exact world grid
+  (``pk_world_grid_f32``) instead of every world sphere;
+* all chunk launches and the per-problem winner selections of all groups are
+  captured once into a graph (``torch.cuda.CUDAGraph`` drives HIP stream
+  capture) and replayed;
+* an autotuner times candidate (workgroups, chunk length) pairs per group on
+  the actual batch and caches the winners by batch signature.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from ..kernels.ops import ls_ik_batched, ls_ik_hip, ls_ik_select, ls_ik_select_segments, world_grid
+from ._ls_ik import ee_tables, prepare_ls_collision_buffers
+
+_CACHE_PATH = os.path.join(os.path.expanduser("~"), ".cache", "pyroffi_amd", "autotune_ls_ik.json")
+
+# Persistent-workgroup counts tried by the autotuner (0 = one workgroup per tile).
+GRID_CANDIDATES = (0, 1024, 2048, 4096)
+CHUNK_CANDIDATES = (0, 10, 25)  # 0 = all iterations in one launch
+
+
+@dataclass
+class RobotGroup:
+    """All problems of one robot: flat seeds (S_total, A) and CSR offsets."""
+
+    robot: object
+    target_link_indices: tuple
+    targets: np.ndarray  # (P, n_ee, 7)
+    seeds: np.ndarray  # (S_total, A) float32
+    seed_start: np.ndarray  # (P + 1,) int32
+    collision: tuple | None = None  # (robot spheres (K,4), joint idx (K,), world (M,4))
+    device: dict = field(default_factory=dict)
+    tune: tuple = (0, 0)
+
+
+class UnevenLsIkSolver:
+    """Multi-seed LM over problems of several robots with per-problem seed counts."""
+
+    def __init__(self, groups: list[RobotGroup], *, max_iter: int = 50, pos_weight: float = 50.0,
+                 ori_weight: float = 10.0, lambda_init: float = 5e-3, eps_pos: float = 1e-8,
+                 eps_ori: float = 1e-8, collision_weight: float = 1e4, collision_margin: float = 0.02,
+                 use_grid: bool = True):
+        self.groups = groups
+        self.kw = dict(max_iter=max_iter, pos_weight=pos_weight, ori_weight=ori_weight,
+                       lambda_init=lambda_init, eps_pos=eps_pos, eps_ori=eps_ori,
+                       collision_weight=collision_weight, collision_margin=collision_margin)
+        self.use_grid = use_grid
+        self.graph = None
+        for g in groups:
+            self._upload(g)
+
+    # ------------------------------------------------------------------ setup
+    def _upload(self, g: RobotGroup) -> None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+        t = g.robot.device_tables(dev)
+        tj, masks = ee_tables(g.robot, g.target_link_indices)
+        d = {
+            "tables": t.robot_buffers(),
+            "lower": t.lower,
+            "upper": t.upper,
+            "seeds": torch.as_tensor(np.ascontiguousarray(g.seeds, np.float32), device=dev),
+            "seed_start": torch.as_tensor(np.asarray(g.seed_start, np.int32), device=dev),
+            "tj": torch.as_tensor(tj, device=dev),
+            "masks": torch.as_tensor(masks, device=dev),
+            "targets": torch.as_tensor(np.ascontiguousarray(g.targets, np.float32), device=dev),
+            "fixed": torch.zeros(g.robot.joints.num_actuated_joints, dtype=torch.int32, device=dev),
+        }
+        S, A = g.seeds.shape
+        P = len(g.seed_start) - 1
+        d["out_cfgs"] = torch.empty((S, A), dtype=torch.float32, device=dev)
+        d["out_err"] = torch.empty((S,), dtype=torch.float32, device=dev)
+        d["state"] = torch.empty((S, 2 * A + 3), dtype=torch.float32, device=dev)
+        d["best"] = torch.empty((P, A), dtype=torch.float32, device=dev)
+        d["best_idx"] = torch.empty((P,), dtype=torch.int32, device=dev)
+        if g.collision is not None:
+            rs, rj, ws = g.collision
+            d["rs"] = torch.as_tensor(rs, device=dev)
+            d["rj"] = torch.as_tensor(rj, device=dev)
+            d["ws"] = torch.as_tensor(ws, device=dev)
+            if self.use_grid and len(rs) and len(ws):
+                # Reach of the grid: the 90th-percentile robot sphere radius + margin.
+                # The few larger robot spheres (a torso, a base plate) test every
+                # world sphere in the kernel, so any choice here is exact.
+                pad = float(np.quantile(rs[:, 3], 0.9)) + self.kw["collision_margin"]
+                d["grid"] = world_grid(d["ws"], pad)
+        else:
+            d["rs"] = torch.zeros((0, 4), device=dev)
+            d["rj"] = torch.zeros((0,), dtype=torch.int32, device=dev)
+            d["ws"] = torch.zeros((0, 4), device=dev)
+        g.device = d
+
+    # ---------------------------------------------------------------- running
+    def _run_group(self, g: RobotGroup, grid_waves: int, chunk: int) -> None:
+        d = g.device
+        kw = self.kw
+        max_iter = kw["max_iter"]
+        bounds = [0, max_iter] if chunk <= 0 or chunk >= max_iter else list(range(0, max_iter, chunk)) + [max_iter]
+        for b0, b1 in zip(bounds[:-1], bounds[1:]):
+            ls_ik_batched(
+                d["seeds"], d["seed_start"], d["tables"], d["tj"], d["masks"], d["targets"], d["rs"], d["rj"],
+                d["ws"], d["lower"], d["upper"], d["fixed"], max_iter=max_iter, iter_begin=b0, iter_end=b1,
+                state=d["state"] if len(bounds) > 2 else None, grid=d.get("grid"),
+                pos_weight=kw["pos_weight"], ori_weight=kw["ori_weight"], lambda_init=kw["lambda_init"],
+                eps_pos=kw["eps_pos"], eps_ori=kw["eps_ori"], enable_collision=g.collision is not None,
+                collision_weight=kw["collision_weight"], collision_margin=kw["collision_margin"],
+                grid_waves=grid_waves, out_cfgs=d["out_cfgs"], out_err=d["out_err"],
+            )
+        ls_ik_select_segments(d["out_cfgs"], d["out_err"], d["seed_start"], out_best=d["best"],
+                              out_idx=d["best_idx"])
+
+    def run_eager(self) -> None:
+        for g in self.groups:
+            self._run_group(g, *g.tune)
+
+    def _signature(self, g: RobotGroup) -> str:
+        n = g.robot.joints.num_actuated_joints
+        return json.dumps([n, g.robot.joints.num_joints, len(g.target_link_indices), int(g.seeds.shape[0]),
+                           int(len(g.seed_start) - 1), g.collision is not None, self.kw["max_iter"],
+                           self.use_grid])
+
+    def autotune(self, reps: int = 2, cache: bool = True) -> dict:
+        """Time every (workgroups, chunk) candidate per group; keep the fastest."""
+        saved = {}
+        if cache and os.path.exists(_CACHE_PATH):
+            try:
+                with open(_CACHE_PATH) as f:
+                    saved = json.load(f)
+            except (OSError, ValueError):
+                saved = {}
+        report = {}
+        for g in self.groups:
+            key = self._signature(g)
+            if key in saved:
+                g.tune = tuple(saved[key])
+                report[key] = {"choice": g.tune, "cached": True}
+                continue
+            timings = {}
+            for gw in GRID_CANDIDATES:
+                for ch in CHUNK_CANDIDATES:
+                    self._run_group(g, gw, ch)  # warm-up
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    for _ in range(reps):
+                        self._run_group(g, gw, ch)
+                    torch.cuda.synchronize()
+                    timings[(gw, ch)] = (time.perf_counter() - t0) / reps
+            g.tune = min(timings, key=timings.get)
+            saved[key] = list(g.tune)
+            report[key] = {"choice": g.tune, "ms": {f"{k[0]}/{k[1]}": v * 1e3 for k, v in timings.items()}}
+        if cache:
+            try:
+                os.makedirs(os.path.dirname(_CACHE_PATH), exist_ok=True)
+                with open(_CACHE_PATH, "w") as f:
+                    json.dump(saved, f)
+            except OSError:
+                pass
+        self.graph = None
+        return report
+
+    def capture(self) -> None:
+        """Capture every group's chunk launches and selections into one graph."""
+        self.run_eager()  # warm-up outside the capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run_eager()
+        self.graph = g
+
+    def solve(self) -> list[tuple[torch.Tensor, torch.Tensor]]:
+        """Run (replay the graph if captured); per group ``(winners (P, A), winner index (P,))``."""
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self.run_eager()
+        return [(g.device["best"], g.device["best_idx"]) for g in self.groups]
+
+    def solve_per_call(self) -> list[torch.Tensor]:
+        """The plain path: one single-problem LM launch plus one selection per problem.
+
+        Each problem's seeds go through ``pk_ls_ik_f32`` against every world
+        sphere (no grid), as ``ls_ik_solve_cuda`` does per call; the inputs are
+        already on the device, so this is the launch path without host work.
+        Returns per group the winners ``(P, A)``.
+        """
+        kw = self.kw
+        outs = []
+        for g in self.groups:
+            d = g.device
+            A = g.seeds.shape[1]
+            starts = [int(v) for v in g.seed_start]
+            no = torch.zeros((0, 1), device=d["seeds"].device)
+            best = torch.empty((len(starts) - 1, A), dtype=torch.float32, device=d["seeds"].device)
+            for p in range(len(starts) - 1):
+                cfgs, errs = ls_ik_hip(
+                    d["seeds"][starts[p]:starts[p + 1]].unsqueeze(0), *d["tables"], d["tj"], d["masks"],
+                    d["targets"][p:p + 1], d["rs"], d["rj"], d["ws"], no, no, no, d["lower"], d["upper"],
+                    d["fixed"], max_iter=kw["max_iter"], pos_weight=kw["pos_weight"],
+                    ori_weight=kw["ori_weight"], lambda_init=kw["lambda_init"], eps_pos=kw["eps_pos"],
+                    eps_ori=kw["eps_ori"], enable_collision=g.collision is not None,
+                    collision_weight=kw["collision_weight"], collision_margin=kw["collision_margin"])
+                b, _ = ls_ik_select(cfgs, errs, d["seeds"][starts[p]:starts[p] + 1])
+                best[p:p + 1].copy_(b)
+            outs.append(best)
+        return outs
+
+
+def log_uniform_seed_counts(rng: np.random.Generator, n: int, lo: int = 16, hi: int = 256) -> np.ndarray:
+    """Seeds per problem: floor(exp(U(ln lo, ln hi)))."""
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), n))).astype(np.int64)
+
+
+def make_group(robot, target_link_indices, targets: np.ndarray, seed_counts: np.ndarray,
+               rng: np.random.Generator, previous: np.ndarray | None = None, collision_checker=None,
+               world=None) -> RobotGroup:
+    """One robot's uneven problems.
+
+    Seeds are uniform in the joint limits, or, given ``previous`` (P, A), half
+    N(previous, 0.05^2) clipped to the limits and half uniform (the reference's
+    warm-start rule, ``_ls_ik.py:452-489``).
+    """
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    A = robot.joints.num_actuated_joints
+    P = len(seed_counts)
+    rows = []
+    for p, n in enumerate(seed_counts):
+        if previous is None:
+            rows.append(rng.uniform(lo, hi, (int(n), A)))
+            continue
+        n_warm = max(1, int(n) // 2)
+        warm = np.clip(previous[p] + rng.standard_normal((n_warm, A)) * 0.05, lo, hi)
+        rand = rng.uniform(lo, hi, (int(n) - n_warm, A))
+        rows.append(np.concatenate([warm, rand], 0))
+    seeds = np.concatenate(rows, 0).astype(np.float32)
+    start = np.zeros(P + 1, np.int32)
+    start[1:] = np.cumsum(seed_counts)
+    coll = None
+    if collision_checker is not None and world is not None:
+        rs, rj, ws, ok = prepare_ls_collision_buffers(robot, collision_checker, world)
+        if ok:
+            coll = (rs, rj, ws)
+    return RobotGroup(robot=robot, target_link_indices=tuple(target_link_indices),
+                      targets=np.asarray(targets, np.float64).reshape(P, len(target_link_indices), 7),
+                      seeds=seeds, seed_start=start, collision=coll)

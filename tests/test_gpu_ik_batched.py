@@ -1,0 +1,254 @@
+"""GPU parity: uneven batched LM IK (flat seeds + CSR offsets), the world grid,
+chunked iterations and the captured graph.
+
+* The world grid lists a superset of the spheres that can reach a cell (checked
+  in FP64 on the host) and records true counts for overflowing cells.
+* max_iter = 0 returns each seed's weighted error + penalty: grid-culled and
+  brute-force penalties agree to 1e-5 relative, and both match the FP64 oracle
+  to 1e-4 relative.
+* The batched kernel reproduces the single-problem kernel on the same seeds.
+* Splitting the iterations into chunks that resume from the state buffer gives
+  bit-identical results to one launch.
+* Full uneven solves over four robots (Baxter with two end effectors) match the
+  FP64 oracle on per-problem success within one problem per robot.
+* A captured graph replays to the eager result, replay after replay.
+"""
+
+import numpy as np
+import pytest
+import torch
+from conftest import load_robot, random_cfgs
+from scipy.spatial.transform import Rotation
+
+from pyroffi_amd.collision import RobotCollisionSpherized
+from pyroffi_amd.kernels import ops
+from pyroffi_amd.optimization_engines._ls_ik import ee_tables, ls_ik_run, prepare_ls_collision_buffers
+from pyroffi_amd.optimization_engines._ls_ik_batched import (
+    UnevenLsIkSolver,
+    log_uniform_seed_counts,
+    make_group,
+)
+
+pytestmark = pytest.mark.gpu
+
+EE_LINKS = {"panda": ("panda_hand",), "ur5": ("ee_link",), "fetch": ("gripper_link",),
+            "baxter": ("left_gripper_base", "right_gripper_base")}
+
+
+def _world(M, seed, lo=0.02, hi=0.15):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([rng.uniform(-1, 1, (M, 3)), rng.uniform(lo, hi, (M, 1))], 1).astype(np.float32)
+
+
+def _group(oracle_mod, name, n_prob, seed, world, lo=16, hi=64):
+    robot, path = load_robot(name)
+    links = [robot.links.names.index(n) for n in EE_LINKS[name]]
+    t = oracle_mod.Tables.of(robot)
+    lp = oracle_mod.fk_links(t, random_cfgs(robot, n_prob, seed=seed))
+    tgt = np.stack([lp[:, l] for l in links], axis=1)
+    rng = np.random.default_rng(seed + 1)
+    counts = log_uniform_seed_counts(rng, n_prob, lo, hi)
+    coll = RobotCollisionSpherized.from_urdf(path) if world is not None else None
+    g = make_group(robot, links, tgt, counts, rng, collision_checker=coll, world=world)
+    return g, t, links
+
+
+def _success(oracle_mod, t, q, tgt, links):
+    lp = oracle_mod.fk_links(t, q.astype(np.float64))
+    ok = np.ones(q.shape[0], bool)
+    for e, l in enumerate(links):
+        g = lp[:, l]
+        pos = np.linalg.norm(g[:, 4:] - tgt[:, e, 4:], axis=-1)
+        rg = Rotation.from_quat(g[:, [1, 2, 3, 0]])
+        rt = Rotation.from_quat(tgt[:, e][:, [1, 2, 3, 0]])
+        rot = np.linalg.norm((rt.inv() * rg).as_rotvec(), axis=-1)
+        ok &= (pos <= 1e-3) & (rot <= 0.01)
+    return ok
+
+
+def _run(solver, g, **over):
+    """One group through ls_ik_batched with the solver's device buffers."""
+    d = g.device
+    kw = dict(solver.kw)
+    kw.update(over)
+    grid = kw.pop("grid", d.get("grid"))
+    return ops.ls_ik_batched(d["seeds"], d["seed_start"], d["tables"], d["tj"], d["masks"], d["targets"], d["rs"],
+                             d["rj"], d["ws"], d["lower"], d["upper"], d["fixed"], grid=grid,
+                             enable_collision=g.collision is not None, **kw)
+
+
+def test_world_grid_lists_every_reachable_sphere():
+    ws = _world(256, 3)
+    pad = 0.12
+    for cap, max_cells in ((64, 16), (4, 12)):
+        g = ops.world_grid(torch.as_tensor(ws, device="cuda"), pad, cap=cap, max_cells=max_cells)
+        counts = g["counts"].cpu().numpy()
+        items = g["items"].cpu().numpy()
+        nx, ny, nz = g["dims"]
+        lo, cell = np.asarray(g["lo"]), g["cell"]
+        c = np.arange(nx * ny * nz)
+        b0 = lo + np.stack([c % nx, (c // nx) % ny, c // (nx * ny)], 1) * cell
+        w = ws.astype(np.float64)
+        d = np.maximum(np.maximum(b0[:, None] - w[None, :, :3], w[None, :, :3] - (b0[:, None] + cell)), 0.0)
+        dist = np.linalg.norm(d, axis=-1)
+        reach = w[:, 3] + pad
+        must = dist <= reach[None] - 1e-5
+        may = dist <= reach[None] + 1e-3
+        assert np.all(counts >= must.sum(1)) and np.all(counts <= may.sum(1))
+        for k in np.nonzero(counts <= cap)[0]:
+            listed = items[k, :counts[k]]
+            assert np.all(np.diff(listed) > 0)
+            assert set(np.nonzero(must[k])[0]) <= set(listed.tolist()) <= set(np.nonzero(may[k])[0])
+        if cap == 4:
+            assert (counts > cap).any()  # the overflow path is exercised
+
+
+def test_initial_error_grid_vs_brute_force_vs_oracle(oracle_mod):
+    world = _world(256, 11)
+    groups, meta = [], []
+    for name, seed in (("panda", 1), ("baxter", 2)):
+        g, t, links = _group(oracle_mod, name, 5, seed, world)
+        groups.append(g)
+        meta.append((t, links))
+    solver = UnevenLsIkSolver(groups, max_iter=0)
+    for g, (t, links) in zip(groups, meta):
+        assert "grid" in g.device
+        _, e_grid = _run(solver, g)
+        _, e_all = _run(solver, g, grid=None)
+        e_grid, e_all = e_grid.cpu().numpy(), e_all.cpu().numpy()
+        np.testing.assert_allclose(e_grid, e_all, rtol=1e-5, atol=1e-7)
+        assert (e_all > 0).all()
+        rs, rj, ws = g.collision
+        tj, masks = ee_tables(g.robot, links)
+        for p in range(len(g.seed_start) - 1):
+            s0, s1 = g.seed_start[p], g.seed_start[p + 1]
+            _, oerr = oracle_mod.ls_ik(t, g.seeds[None, s0:s1], tj, masks, g.targets[p:p + 1],
+                                       g.robot.joints.lower_limits, g.robot.joints.upper_limits, max_iter=0,
+                                       robot_spheres=rs, sphere_joint=rj, world=ws, enable_collision=True,
+                                       collision_weight=1e4, collision_margin=0.02)
+            np.testing.assert_allclose(e_grid[s0:s1], oerr[0], rtol=1e-4, atol=1e-6)
+
+
+def test_batched_kernel_reproduces_single_problem_kernel(oracle_mod):
+    world = _world(128, 5)
+    robot, path = load_robot("panda")
+    links = [robot.links.names.index("panda_hand")]
+    t = oracle_mod.Tables.of(robot)
+    P, S = 6, 40
+    tgt = oracle_mod.fk_links(t, random_cfgs(robot, P, seed=21))[:, links[0]][:, None]
+    seeds = np.random.default_rng(22).uniform(robot.joints.lower_limits, robot.joints.upper_limits,
+                                              (P, S, 7)).astype(np.float32)
+    coll = RobotCollisionSpherized.from_urdf(path)
+    rs, rj, ws, ok = prepare_ls_collision_buffers(robot, coll, world)
+    cfgs1, errs1 = ls_ik_run(robot, seeds, tgt, links, max_iter=40, collision=(rs, rj, ws, 1e4, 0.02))
+    g = make_group(robot, links, tgt, np.full(P, S), np.random.default_rng(0), collision_checker=coll, world=world)
+    g.seeds = seeds.reshape(P * S, 7)
+    solver = UnevenLsIkSolver([g], max_iter=40, use_grid=False)
+    cfgs2, errs2 = _run(solver, g)
+    c1, e1 = cfgs1.cpu().numpy().reshape(P * S, 7), errs1.cpu().numpy().reshape(-1)
+    c2, e2 = cfgs2.cpu().numpy(), errs2.cpu().numpy()
+    same = np.all(np.abs(c1 - c2) <= 1e-4, axis=1)
+    assert same.mean() >= 0.95, same.mean()
+    s1 = _success(oracle_mod, t, c1, np.repeat(tgt, S, 0), links)
+    s2 = _success(oracle_mod, t, c2, np.repeat(tgt, S, 0), links)
+    assert abs(int(s1.sum()) - int(s2.sum())) <= max(2, int(0.02 * P * S))
+    np.testing.assert_allclose(np.sort(e1)[: P * S // 2], np.sort(e2)[: P * S // 2], rtol=1e-2, atol=1e-6)
+
+
+@pytest.mark.parametrize("chunk", [7, 25])
+def test_chunked_iterations_equal_one_launch(oracle_mod, chunk):
+    world = _world(256, 8)
+    groups = [_group(oracle_mod, n, 9, s, world)[0] for n, s in (("ur5", 3), ("fetch", 4))]
+    solver = UnevenLsIkSolver(groups, max_iter=50)
+    for g in groups:
+        c_one, e_one = _run(solver, g)
+        c_one, e_one = c_one.clone(), e_one.clone()
+        for b0 in range(0, 50, chunk):
+            c_ch, e_ch = _run(solver, g, iter_begin=b0, iter_end=min(b0 + chunk, 50), state=g.device["state"])
+        torch.testing.assert_close(c_ch, c_one, rtol=0, atol=0)
+        torch.testing.assert_close(e_ch, e_one, rtol=0, atol=0)
+        # A persistent launch (fewer workgroups than tiles) changes nothing either.
+        c_p, e_p = _run(solver, g, grid_waves=3)
+        torch.testing.assert_close(c_p, c_one, rtol=0, atol=0)
+
+
+def test_uneven_four_robot_solve_vs_oracle(oracle_mod):
+    world = _world(64, 13, 0.02, 0.08)
+    built = [_group(oracle_mod, n, 6, s, world) for n, s in (("panda", 31), ("ur5", 32), ("fetch", 33),
+                                                                 ("baxter", 34))]
+    solver = UnevenLsIkSolver([b[0] for b in built], max_iter=50)
+    winners = [w.cpu().numpy() for w, _ in solver.solve()]
+    for (g, t, links), q in zip(built, winners):
+        assert np.all(q >= g.robot.joints.lower_limits - 1e-5) and np.all(q <= g.robot.joints.upper_limits + 1e-5)
+        ok_gpu = _success(oracle_mod, t, q, g.targets, links)
+        rs, rj, ws = g.collision
+        tj, masks = ee_tables(g.robot, links)
+        ok_or = np.zeros(len(ok_gpu), bool)
+        for p in range(len(ok_gpu)):
+            s0, s1 = g.seed_start[p], g.seed_start[p + 1]
+            oc, oe = oracle_mod.ls_ik(t, g.seeds[None, s0:s1], tj, masks, g.targets[p:p + 1],
+                                      g.robot.joints.lower_limits, g.robot.joints.upper_limits, max_iter=50,
+                                      robot_spheres=rs, sphere_joint=rj, world=ws, enable_collision=True,
+                                      collision_weight=1e4, collision_margin=0.02)
+            best = oc[0, int(np.argmin(oe[0]))]
+            ok_or[p] = _success(oracle_mod, t, best[None], g.targets[p:p + 1], links)[0]
+        assert abs(int(ok_gpu.sum()) - int(ok_or.sum())) <= 1, (g.robot.joints.num_actuated_joints, ok_gpu, ok_or)
+
+
+def test_graph_replay_matches_eager_and_per_call(oracle_mod):
+    world = _world(256, 17)
+    built = [_group(oracle_mod, n, 8, s, world) for n, s in (("panda", 41), ("baxter", 42))]
+    solver = UnevenLsIkSolver([b[0] for b in built], max_iter=30)
+    for g in solver.groups:
+        g.tune = (2, 10)
+    eager = [w.clone() for w, _ in solver.solve()]
+    eager_idx = [i.clone() for _, i in solver.solve()]
+    solver.capture()
+    for rep in range(3):
+        for g in solver.groups:
+            g.device["best"].zero_()
+        out = solver.solve()
+        torch.cuda.synchronize()
+        for (w, i), we, ie in zip(out, eager, eager_idx):
+            torch.testing.assert_close(w, we, rtol=0, atol=0)
+            torch.testing.assert_close(i, ie, rtol=0, atol=0)
+        _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+    plain = [q.cpu().numpy() for q in solver.solve_per_call()]
+    for (g, t, links), q_plain, q_fused in zip(built, plain, eager):
+        a = _success(oracle_mod, t, q_plain, g.targets, links)
+        b = _success(oracle_mod, t, q_fused.cpu().numpy(), g.targets, links)
+        assert abs(int(a.sum()) - int(b.sum())) <= 1
+
+
+def test_autotune_picks_a_candidate(oracle_mod):
+    world = _world(256, 19)
+    g = _group(oracle_mod, "panda", 4, 51, world)[0]
+    solver = UnevenLsIkSolver([g], max_iter=10)
+    report = solver.autotune(reps=1, cache=False)
+    from pyroffi_amd.optimization_engines._ls_ik_batched import CHUNK_CANDIDATES, GRID_CANDIDATES
+
+    assert g.tune[0] in GRID_CANDIDATES and g.tune[1] in CHUNK_CANDIDATES
+    assert len(report) == 1
+
+
+def test_select_segments_matches_numpy():
+    rng = np.random.default_rng(3)
+    counts = np.array([1, 17, 64, 65, 3, 130])
+    start = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    S, A = start[-1], 7
+    cfgs = rng.standard_normal((S, A)).astype(np.float32)
+    errs = rng.uniform(0, 1, S).astype(np.float32)
+    errs[70] = np.nan
+    prev = rng.standard_normal((len(counts), A)).astype(np.float32)
+    for cw in (0.0, 0.3):
+        best, idx = ops.ls_ik_select_segments(torch.as_tensor(cfgs, device="cuda"), torch.as_tensor(errs, device="cuda"),
+                                              torch.as_tensor(start, device="cuda"),
+                                              torch.as_tensor(prev, device="cuda"), cw)
+        idx = idx.cpu().numpy()
+        for p in range(len(counts)):
+            e = errs[start[p]:start[p + 1]].astype(np.float64)
+            if cw:
+                e = e + cw * np.sum((cfgs[start[p]:start[p + 1]] - prev[p]) ** 2, axis=1)
+            e = np.where(np.isnan(e), np.inf, e)
+            assert idx[p] == int(np.argmin(e))
+        np.testing.assert_array_equal(best.cpu().numpy(), cfgs[start[:-1] + idx])
