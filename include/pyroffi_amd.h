@@ -41,6 +41,21 @@ extern "C" {
 #define PK_MAX_JOINTS 64
 #define PK_MAX_ACT 16
 #define PK_MAX_EE 4
+#define PK_MAX_GRIDS 2
+
+/*
+ * A world grid built by pk_world_grid_f32 (device arrays + its geometry),
+ * passed by host pointer to the LM entry points.
+ */
+typedef struct pk_world_grid {
+  const int32_t* counts; /* (dims[0]*dims[1]*dims[2]) true counts     */
+  const int32_t* items;  /* (dims[0]*dims[1]*dims[2], cap) indices    */
+  float lo[3];           /* corner of cell (0, 0, 0)                  */
+  float cell;            /* cell side                                 */
+  int32_t dims[3];       /* nx, ny, nz                                */
+  int32_t cap;           /* indices kept per cell                     */
+  float pad;             /* reach the grid was built with             */
+} pk_world_grid;
 
 /* Message of the last failing call on this thread ("" when none). */
 const char* pk_last_error(void);
@@ -257,11 +272,11 @@ int pk_world_grid_f32(void* stream, const float* world_spheres, int32_t n_world_
  *   seeds            (total_seeds, n_act), problem p owns the rows
  *                    [problem_seed_start[p], problem_seed_start[p+1])
  *   target_T         (n_problems, n_ee, 7)
- *   grid_*           optional world grid from pk_world_grid_f32 (NULL counts:
- *                    every world sphere is tested) built with pad grid_pad;
- *                    robot spheres of radius > grid_pad - collision_margin
- *                    test every world sphere, so the penalty is the same for
- *                    any pad
+ *   grids            0..PK_MAX_GRIDS world grids (pk_world_grid_f32) in
+ *                    increasing pad order; a robot sphere of radius r uses
+ *                    the first with pad - collision_margin >= r, and tests
+ *                    every world sphere when none qualifies, so the penalty
+ *                    is the same for any set of grids
  *   iter_begin/end   run iterations [iter_begin, iter_end) of max_iter; a
  *                    partial range reads/writes `state` (total_seeds,
  *                    2 n_act + 3), so a solve can be split into launches
@@ -281,9 +296,7 @@ int pk_ls_ik_batched_f32(
     int32_t n_joints, const int32_t* target_jnts, const int32_t* ancestor_masks, int32_t n_ee,
     const float* target_T, const float* robot_spheres_local,
     const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres, const float* world_spheres,
-    int32_t n_world_spheres, const int32_t* grid_counts, const int32_t* grid_items,
-    float grid_lo_x, float grid_lo_y, float grid_lo_z, float grid_cell, int32_t grid_nx,
-    int32_t grid_ny, int32_t grid_nz, int32_t grid_cap, float grid_pad, const float* lower,
+    int32_t n_world_spheres, const pk_world_grid* grids, int32_t n_grids, const float* lower,
     const float* upper, const int32_t* fixed_mask, int32_t max_iter, int32_t iter_begin,
     int32_t iter_end, float* state, float pos_weight, float ori_weight, float lambda_init,
     float eps_pos, float eps_ori, int32_t enable_collision, float collision_weight,

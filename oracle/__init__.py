@@ -57,7 +57,13 @@ def lib() -> ctypes.CDLL:
         h.or_pose_hinge_cost.argtypes = [_d, _L, _I, _f, _f, _i, _i, _f, _f, _i, _i, _I, _i, _I, _d, _I, _d, _I,
                                          _D, _I, _d, _D, _D, _d, _I]
         h.or_pose_hinge_cost.restype = None
-        for fn in (h.or_fk, h.or_link_poses, h.or_world_dist, h.or_self_dist, h.or_ls_ik, h.or_res_jac):
+        traj_model = [_f, _f, _i, _i, _f, _f, _i, _i, _I, _I, _i, _I, _d, _I, _i, _i, _I, _d, _I, _D, _I]
+        h.or_traj_cost.argtypes = traj_model + [_d, ctypes.POINTER(ctypes.c_int64), _L, _d, _d, _d, _d, _d, _I]
+        h.or_traj_lbfgs.argtypes = traj_model + [_d, ctypes.POINTER(ctypes.c_int64), _L, _d, _d, _i, _d, _d, _d,
+                                                 _I]
+        h.or_two_loop.argtypes = [_d, _d, _d, _d, _I, _I, _I, _I, _d]
+        for fn in (h.or_fk, h.or_link_poses, h.or_world_dist, h.or_self_dist, h.or_ls_ik, h.or_res_jac,
+                   h.or_traj_cost, h.or_traj_lbfgs, h.or_two_loop):
             fn.restype = None
         _lib = h
     return _lib
@@ -235,3 +241,73 @@ def pose_hinge_fd_grad(tables: Tables, cfg, spheres, world, margin, ee_link, tar
     tg = np.repeat(np.asarray(targets, np.float64).reshape(B, 1, 7), 2 * A, axis=1).reshape(-1, 7)
     c = pose_hinge_cost(tables, pert.reshape(-1, A), spheres, world, margin, ee_link, tg, **kw).reshape(B, 2, A)
     return (c[:, 0] - c[:, 1]) / (2 * h)
+
+
+class TrajModel:
+    """Robot + collision model of the trajectory cost (padded ``(L, S, 4)`` spheres, active pairs, world)."""
+
+    def __init__(self, tables: Tables, spheres, pair_i, pair_j, world, margin: float, mode: int):
+        self.t = tables
+        self.sph = _c(spheres, np.float64)
+        self.pi = _c(pair_i, np.int32)
+        self.pj = _c(pair_j, np.int32)
+        self.world = _c(world, np.float64).reshape(-1, 4)
+        self.margin = float(margin)
+        self.mode = int(mode)
+
+    def args(self):
+        t = self.t
+        return (*t.ptrs(), t.J, t.A, _p(t.link_parent_joint, _i), t.L, _p(self.sph, _d), self.sph.shape[1],
+                _p(self.pi, _i), _p(self.pj, _i), self.pi.shape[0], _p(self.world, _d), self.world.shape[0],
+                self.margin, self.mode)
+
+
+def _traj_flat(trajs):
+    """List of (T_b, A) arrays or a (B, T, A) array -> flat (W, A) + int64 CSR starts (B + 1)."""
+    if isinstance(trajs, np.ndarray) and trajs.ndim == 3:
+        trajs = list(trajs)
+    x = _c(np.concatenate([np.asarray(t, np.float64) for t in trajs], 0), np.float64)
+    start = np.zeros(len(trajs) + 1, np.int64)
+    start[1:] = np.cumsum([len(t) for t in trajs])
+    return x, start
+
+
+def traj_cost(model: TrajModel, trajs, lower, upper, weights, grad: bool = True, threads: int = 0):
+    """``(comps (B, 3) = smooth, limits, coll; grad (W, A) of the weighted total or None)``.
+
+    ``weights`` = (w_smooth, w_vel, w_acc, w_jerk, w_limits, w_coll).
+    """
+    x, start = _traj_flat(trajs)
+    B = len(start) - 1
+    comps = np.empty((B, 3))
+    g = np.empty_like(x) if grad else None
+    lo, hi, w = _c(lower, np.float64), _c(upper, np.float64), _c(weights, np.float64)
+    lib().or_traj_cost(*model.args(), _p(x, _d), start.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), B,
+                       _p(lo, _d), _p(hi, _d), _p(w, _d), _p(comps, _d), None if g is None else _p(g, _d), threads)
+    return comps, g
+
+
+def traj_lbfgs(model: TrajModel, trajs, lower, upper, *, n_iters=100, m_lbfgs=6, escalation_interval=20,
+               w_smooth=1.0, w_vel=1.0, w_acc=0.5, w_jerk=0.1, w_limits=1.0, w_collision=1.0,
+               w_collision_max=100.0, penalty_scale=2.0, threads: int = 0):
+    """FP64 direct L-BFGS per trajectory: ``(best (W, A) flat, best tracked cost (B,), starts (B + 1))``."""
+    x, start = _traj_flat(trajs)
+    B = len(start) - 1
+    best = np.empty_like(x)
+    cost = np.empty(B)
+    cfg = _c([n_iters, m_lbfgs, escalation_interval], np.int32)
+    fcfg = _c([w_smooth, w_vel, w_acc, w_jerk, w_limits, w_collision, w_collision_max, penalty_scale], np.float64)
+    lo, hi = _c(lower, np.float64), _c(upper, np.float64)
+    lib().or_traj_lbfgs(*model.args(), _p(x, _d), start.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), B,
+                        _p(lo, _d), _p(hi, _d), _p(cfg, _i), _p(fcfg, _d), _p(best, _d), _p(cost, _d), threads)
+    return best, cost, start
+
+
+def two_loop(g, s_buf, y_buf, rho, m_used: int, newest: int):
+    """L-BFGS direction -H g from a masked ring buffer (``(m, n)`` s and y)."""
+    gg = _c(g, np.float64)
+    sb, yb, r = _c(s_buf, np.float64), _c(y_buf, np.float64), _c(rho, np.float64)
+    m, n = sb.shape
+    out = np.empty(n)
+    lib().or_two_loop(_p(gg, _d), _p(sb, _d), _p(yb, _d), _p(r, _d), int(m_used), int(newest), m, n, _p(out, _d))
+    return out

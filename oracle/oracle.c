@@ -606,3 +606,353 @@ void or_pose_hinge_cost(const double* cfg, int64_t B, int A, const float* twists
     out[b] = cost;
   }
 }
+
+/* ----------------------------------------------------------- trajectories */
+/*
+ * Trajectory cost of the L-BFGS / CHOMP trajectory optimisers:
+ *   total = w_smooth * smooth + w_limits * limits + w_coll * coll
+ *   smooth  _sco_optimization.py:170-189: w_vel sum|vel|^2 + w_acc sum|acc|^2 +
+ *           w_jerk sum|jerk|^2, vel_t = x_{t+1} - x_t,
+ *           acc_t = (-x_t + 16 x_{t+1} - 30 x_{t+2} + 16 x_{t+3} - x_{t+4}) / 12,
+ *           jerk_t = acc_{t+1} - acc_t (CHOMP's variant, _chomp_optimization.py:
+ *           93-111, is w_vel = 0)
+ *   limits  _sco_optimization.py:192-200: sum (max(0, x-u) + max(0, l-x))^2
+ *   coll    sum over waypoints of sum f(d) over _collision_distances_all
+ *           (_sco_optimization.py:203-222): the JAX self distances (diagonal
+ *           sphere pairs, +1e-6 bias) of every active pair and the JAX world
+ *           distances d[n, m] = min over link n's spheres (+1e-6 bias, padding
+ *           r = -1e9 included) of every link n and world sphere m;
+ *           f = max(0, m - d)^2 (mode 0, CHOMP, _chomp_optimization.py:140-154)
+ *           or -min(colldist_from_sdf(d, m), 0) (mode 1, L-BFGS,
+ *           _lbfgs_trajopt_optimization.py:108-111; collision/_collision.py:136-167)
+ * The gradient is analytic: per sphere centre c, dcost/dc, pulled back to the
+ * joints with dc/dq_j = mul_j ((R_j w_j) x (c - p_j) + R_j v_j) for every
+ * moving ancestor joint j (tests pin it with central differences).
+ */
+static double act_fn(double d, double m, int mode, double* dfd) {
+  if (mode == 0) {
+    const double h = m - d;
+    if (h > 0) { *dfd = -2.0 * h; return h * h; }
+    *dfd = 0.0;
+    return 0.0;
+  }
+  const double dm = d < m ? d : m;
+  if (dm < 0) { *dfd = -1.0; return 0.5 * m - dm; }
+  if (d < m) { *dfd = (d - m) / (m + 1e-6); return 0.5 / (m + 1e-6) * (d - m) * (d - m); }
+  *dfd = 0.0;
+  return 0.0;
+}
+
+typedef struct {
+  const tables_t* t;
+  const int32_t* lpj; /* link parent joint (L) */
+  int L;
+  const double* sph;  /* (L, S, 4) link frame, r < 0 padding */
+  int S;
+  const int32_t *pi, *pj;
+  int P;
+  const double* world;
+  int M;
+  double margin;
+  int mode;
+} traj_model_t;
+
+/* Accumulate g . dc/dq into grad for a point c on link l. */
+static void pull_back(const traj_model_t* md, const se3* Tw, int l, const double* c, const double* g,
+                      double* grad) {
+  const tables_t* t = md->t;
+  int j = md->lpj[l];
+  while (j >= 0) {
+    const int mm = t->mimic_act_idx[j];
+    const int src = (mm != -1) ? mm : t->act_idx[j];
+    if (src >= 0) {
+      double w[3], v[3];
+      const double ang[3] = {t->twists[6 * j + 3], t->twists[6 * j + 4], t->twists[6 * j + 5]};
+      const double lin[3] = {t->twists[6 * j], t->twists[6 * j + 1], t->twists[6 * j + 2]};
+      qrot(&Tw[j], ang, w);
+      qrot(&Tw[j], lin, v);
+      const double r[3] = {c[0] - Tw[j].tx, c[1] - Tw[j].ty, c[2] - Tw[j].tz};
+      const double dc[3] = {w[1] * r[2] - w[2] * r[1] + v[0], w[2] * r[0] - w[0] * r[2] + v[1],
+                            w[0] * r[1] - w[1] * r[0] + v[2]};
+      grad[src] += (double)t->mimic_mul[j] * (g[0] * dc[0] + g[1] * dc[1] + g[2] * dc[2]);
+    }
+    j = t->parent_idx[j];
+  }
+}
+
+/* Collision cost of one waypoint; grad (A) is accumulated when non-NULL. */
+static double wp_coll(const traj_model_t* md, const double* q, double* grad) {
+  const int L = md->L, S = md->S;
+  se3 Tw[OR_MAXJ];
+  fk_one(md->t, q, Tw);
+  double* cw = (double*)malloc(sizeof(double) * 3 * L * S);
+  for (int l = 0; l < L; ++l) {
+    const int j = md->lpj[l];
+    se3 T = (j < 0) ? (se3){1, 0, 0, 0, 0, 0, 0} : Tw[j];
+    for (int s = 0; s < S; ++s) {
+      double* o = cw + 3 * (l * S + s);
+      qrot(&T, md->sph + (l * S + s) * 4, o);
+      o[0] += T.tx; o[1] += T.ty; o[2] += T.tz;
+    }
+  }
+  double cost = 0.0;
+  const double e = 1e-6;
+  /* world: per (link, obstacle), the link's closest sphere */
+  for (int l = 0; l < L; ++l) {
+    for (int m = 0; m < md->M; ++m) {
+      const double* o = md->world + 4 * m;
+      double best = INFINITY;
+      int arg = -1;
+      for (int s = 0; s < S; ++s) {
+        const double d = sdist(cw + 3 * (l * S + s), md->sph[(l * S + s) * 4 + 3], o, o[3], 1);
+        if (d < best) { best = d; arg = s; }
+      }
+      if (arg < 0) continue;
+      double dfd;
+      cost += act_fn(best, md->margin, md->mode, &dfd);
+      if (grad && dfd != 0.0) {
+        const double* c = cw + 3 * (l * S + arg);
+        const double x[3] = {o[0] - c[0] + e, o[1] - c[1] + e, o[2] - c[2] + e};
+        const double n = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        const double g[3] = {-dfd * x[0] / n, -dfd * x[1] / n, -dfd * x[2] / n};
+        pull_back(md, Tw, l, c, g, grad);
+      }
+    }
+  }
+  /* self: active pairs, same-slot sphere pairs */
+  for (int p = 0; p < md->P; ++p) {
+    const int li = md->pi[p], lj = md->pj[p];
+    double best = INFINITY;
+    int arg = -1;
+    for (int s = 0; s < S; ++s) {
+      const double d = sdist(cw + 3 * (li * S + s), md->sph[(li * S + s) * 4 + 3], cw + 3 * (lj * S + s),
+                             md->sph[(lj * S + s) * 4 + 3], 1);
+      if (d < best) { best = d; arg = s; }
+    }
+    if (arg < 0) continue;
+    double dfd;
+    cost += act_fn(best, md->margin, md->mode, &dfd);
+    if (grad && dfd != 0.0) {
+      const double* ci = cw + 3 * (li * S + arg);
+      const double* cj = cw + 3 * (lj * S + arg);
+      const double x[3] = {cj[0] - ci[0] + e, cj[1] - ci[1] + e, cj[2] - ci[2] + e};
+      const double n = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+      const double gj[3] = {dfd * x[0] / n, dfd * x[1] / n, dfd * x[2] / n};
+      const double gi[3] = {-gj[0], -gj[1], -gj[2]};
+      pull_back(md, Tw, lj, cj, gj, grad);
+      pull_back(md, Tw, li, ci, gi, grad);
+    }
+  }
+  free(cw);
+  return cost;
+}
+
+/* One trajectory x (T, A): comps = {smooth, limits, coll} (unweighted by
+ * w_smooth / w_limits / w_coll); grad (T, A) = gradient of the weighted total
+ * when non-NULL.  w = {w_smooth, w_vel, w_acc, w_jerk, w_limits, w_coll}. */
+static void traj_cost_one(const traj_model_t* md, const double* x, int T, const double* lower,
+                          const double* upper, const double* w, double* comps, double* grad) {
+  const int A = md->t->A;
+  double sm = 0.0, lim = 0.0, col = 0.0;
+  if (grad) memset(grad, 0, sizeof(double) * T * A);
+  const double c5[5] = {-1.0, 16.0, -30.0, 16.0, -1.0};
+  for (int a = 0; a < A; ++a) {
+    for (int t = 0; t + 1 < T; ++t) {
+      const double v = x[(t + 1) * A + a] - x[t * A + a];
+      sm += w[1] * v * v;
+      if (grad) {
+        grad[(t + 1) * A + a] += w[0] * w[1] * 2 * v;
+        grad[t * A + a] -= w[0] * w[1] * 2 * v;
+      }
+    }
+    const int na = T - 4;
+    double* acc = (double*)malloc(sizeof(double) * (na > 0 ? na : 1));
+    for (int t = 0; t < na; ++t) {
+      double s = 0;
+      for (int o = 0; o < 5; ++o) s += c5[o] * x[(t + o) * A + a];
+      acc[t] = s / 12.0;
+      sm += w[2] * acc[t] * acc[t];
+    }
+    for (int t = 0; t + 1 < na; ++t) {
+      const double jk = acc[t + 1] - acc[t];
+      sm += w[3] * jk * jk;
+    }
+    if (grad) {
+      for (int t = 0; t < na; ++t) {
+        double ga = 2 * w[2] * acc[t];
+        if (t + 1 < na) ga -= 2 * w[3] * (acc[t + 1] - acc[t]);
+        if (t >= 1) ga += 2 * w[3] * (acc[t] - acc[t - 1]);
+        for (int o = 0; o < 5; ++o) grad[(t + o) * A + a] += w[0] * ga * c5[o] / 12.0;
+      }
+    }
+    free(acc);
+    for (int t = 0; t < T; ++t) {
+      const double xv = x[t * A + a];
+      const double vu = xv > upper[a] ? xv - upper[a] : 0.0;
+      const double vl = lower[a] > xv ? lower[a] - xv : 0.0;
+      const double v = vu + vl;
+      lim += v * v;
+      if (grad) grad[t * A + a] += w[4] * 2 * v * ((xv > upper[a] ? 1.0 : 0.0) - (lower[a] > xv ? 1.0 : 0.0));
+    }
+  }
+  double gq[OR_MAXA];
+  for (int t = 0; t < T; ++t) {
+    if (grad) for (int a = 0; a < A; ++a) gq[a] = 0.0;
+    col += wp_coll(md, x + t * A, grad ? gq : NULL);
+    if (grad) for (int a = 0; a < A; ++a) grad[t * A + a] += w[5] * gq[a];
+  }
+  comps[0] = sm; comps[1] = lim; comps[2] = col;
+}
+
+void or_traj_cost(const float* twists, const float* parent_tf, const int32_t* parent_idx, const int32_t* act_idx,
+                  const float* mimic_mul, const float* mimic_off, const int32_t* mimic_act_idx,
+                  const int32_t* topo_inv, int J, int A, const int32_t* link_parent_joint, int L,
+                  const double* spheres, int S, const int32_t* pair_i, const int32_t* pair_j, int P,
+                  const double* world, int M, double margin, int mode, const double* x, const int64_t* start,
+                  int64_t B, const double* lower, const double* upper, const double* w, double* comps,
+                  double* grad, int threads) {
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  traj_model_t md = {&t, link_parent_joint, L, spheres, S, pair_i, pair_j, P, world, M, margin, mode};
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t b = 0; b < B; ++b) {
+    const int T = (int)(start[b + 1] - start[b]);
+    traj_cost_one(&md, x + start[b] * A, T, lower, upper, w, comps + 3 * b, grad ? grad + start[b] * A : NULL);
+  }
+}
+
+/* Nocedal two-loop recursion with masked history (_sco_optimization.py:119-167). */
+static void two_loop(const double* g, const double* s_buf, const double* y_buf, const double* rho, int m_used,
+                     int newest, int m, int n, double* out) {
+  double* q = (double*)malloc(sizeof(double) * n);
+  double alpha[16];
+  memcpy(q, g, sizeof(double) * n);
+  for (int i = 0; i < m; ++i) alpha[i] = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const int idx = (newest - i + m) % m;
+    if (i >= m_used) continue;
+    double sq = 0;
+    for (int k = 0; k < n; ++k) sq += s_buf[idx * n + k] * q[k];
+    alpha[idx] = rho[idx] * sq;
+    for (int k = 0; k < n; ++k) q[k] -= alpha[idx] * y_buf[idx * n + k];
+  }
+  double sy = 0, yy = 0;
+  for (int k = 0; k < n; ++k) {
+    sy += s_buf[newest * n + k] * y_buf[newest * n + k];
+    yy += y_buf[newest * n + k] * y_buf[newest * n + k];
+  }
+  const double gamma = sy / (yy + 1e-18);
+  for (int k = 0; k < n; ++k) out[k] = gamma * q[k];
+  for (int st = 0; st < m; ++st) {
+    const int idx = (newest - m_used + 1 + st + 2 * m) % m;
+    if (st >= m_used) continue;
+    double yr = 0;
+    for (int k = 0; k < n; ++k) yr += y_buf[idx * n + k] * out[k];
+    const double beta = rho[idx] * yr;
+    for (int k = 0; k < n; ++k) out[k] += s_buf[idx * n + k] * (alpha[idx] - beta);
+  }
+  for (int k = 0; k < n; ++k) out[k] = -out[k];
+  free(q);
+}
+
+void or_two_loop(const double* g, const double* s_buf, const double* y_buf, const double* rho, int m_used,
+                 int newest, int m, int n, double* out) {
+  two_loop(g, s_buf, y_buf, rho, m_used, newest, m, n, out);
+}
+
+/* The direct L-BFGS solve of _lbfgs_trajopt_optimization.py:76-205 for each
+ * trajectory.  cfg = {n_iters, m, escalation_interval} (ints) and
+ * fcfg = {w_smooth, w_vel, w_acc, w_jerk, w_limits, w_collision, w_collision_max,
+ * penalty_scale}; endpoints stay where x_init has them.  out_best (W, A),
+ * out_best_cost (B) = the tracked best cost. */
+void or_traj_lbfgs(const float* twists, const float* parent_tf, const int32_t* parent_idx, const int32_t* act_idx,
+                   const float* mimic_mul, const float* mimic_off, const int32_t* mimic_act_idx,
+                   const int32_t* topo_inv, int J, int A, const int32_t* link_parent_joint, int L,
+                   const double* spheres, int S, const int32_t* pair_i, const int32_t* pair_j, int P,
+                   const double* world, int M, double margin, int mode, const double* x_init,
+                   const int64_t* start, int64_t B, const double* lower, const double* upper, const int32_t* cfg,
+                   const double* fcfg, double* out_best, double* out_best_cost, int threads) {
+  static const double alphas[5] = {1.0, 0.5, 0.25, 0.1, 0.025};
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  traj_model_t md = {&t, link_parent_joint, L, spheres, S, pair_i, pair_j, P, world, M, margin, mode};
+  const int n_iters = cfg[0], m = cfg[1], esc = cfg[2];
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t b = 0; b < B; ++b) {
+    const int T = (int)(start[b + 1] - start[b]);
+    const int n = T * A;
+    double* x = (double*)malloc(sizeof(double) * n);
+    double* xp = (double*)malloc(sizeof(double) * n);
+    double* gp = (double*)malloc(sizeof(double) * n);
+    double* g = (double*)malloc(sizeof(double) * n);
+    double* dir = (double*)malloc(sizeof(double) * n);
+    double* xt = (double*)malloc(sizeof(double) * n);
+    double* best = (double*)malloc(sizeof(double) * n);
+    double* sb = (double*)calloc((size_t)m * n, sizeof(double));
+    double* yb = (double*)calloc((size_t)m * n, sizeof(double));
+    double rho[16] = {0};
+    memcpy(x, x_init + start[b] * A, sizeof(double) * n);
+    double w[6] = {fcfg[0], fcfg[1], fcfg[2], fcfg[3], fcfg[4], fcfg[5]};
+    double comps[3];
+    traj_cost_one(&md, x, T, lower, upper, w, comps, g);
+    for (int a = 0; a < A; ++a) { g[a] = 0; g[n - A + a] = 0; }
+    double best_cost = fcfg[0] * comps[0] + fcfg[4] * comps[1] + w[5] * comps[2];
+    memcpy(best, x, sizeof(double) * n);
+    memcpy(xp, x, sizeof(double) * n);
+    memcpy(gp, g, sizeof(double) * n);
+    int m_used = 0, newest = 0;
+    for (int k = 0; k < n_iters; ++k) {
+      if (k > 0 && k % esc == 0) w[5] = fmin(w[5] * fcfg[7], fcfg[6]);
+      traj_cost_one(&md, x, T, lower, upper, w, comps, g);
+      for (int a = 0; a < A; ++a) { g[a] = 0; g[n - A + a] = 0; }
+      const double cost = w[0] * comps[0] + w[4] * comps[1] + w[5] * comps[2];
+      double sy = 0, yy = 0;
+      for (int i = 0; i < n; ++i) {
+        const double s = x[i] - xp[i], y = g[i] - gp[i];
+        sy += s * y;
+        yy += y * y;
+      }
+      if (sy > 1e-10 * yy + 1e-30 && k > 0) {
+        newest = (newest + 1) % m;
+        for (int i = 0; i < n; ++i) { sb[newest * n + i] = x[i] - xp[i]; yb[newest * n + i] = g[i] - gp[i]; }
+        rho[newest] = 1.0 / (sy + 1e-30);
+        if (m_used < m) ++m_used;
+      }
+      if (m_used > 0) {
+        two_loop(g, sb, yb, rho, m_used, newest, m, n, dir);
+      } else {
+        double gn = 0;
+        for (int i = 0; i < n; ++i) gn += g[i] * g[i];
+        gn = sqrt(gn) + 1e-18;
+        for (int i = 0; i < n; ++i) dir[i] = -g[i] / gn;
+      }
+      for (int a = 0; a < A; ++a) { dir[a] = 0; dir[n - A + a] = 0; }
+      const double thresh = cost * (1.0 - 1e-4);
+      double trial[5], tcomp[5][3];
+      for (int i = 0; i < 5; ++i) {
+        for (int k2 = 0; k2 < n; ++k2) xt[k2] = x[k2] + alphas[i] * dir[k2];
+        traj_cost_one(&md, xt, T, lower, upper, w, tcomp[i], NULL);
+        trial[i] = w[0] * tcomp[i][0] + w[4] * tcomp[i][1] + w[5] * tcomp[i][2];
+      }
+      int pick = -1;
+      for (int i = 0; i < 5; ++i)
+        if (trial[i] < thresh) { pick = i; break; }
+      if (pick < 0) {
+        pick = 0;
+        for (int i = 1; i < 5; ++i)
+          if (trial[i] < trial[pick]) pick = i;
+      }
+      memcpy(xp, x, sizeof(double) * n);
+      memcpy(gp, g, sizeof(double) * n);
+      for (int i = 0; i < n; ++i) x[i] += alphas[pick] * dir[i];
+      const double eval = w[0] * tcomp[pick][0] + w[4] * tcomp[pick][1] + fcfg[6] * tcomp[pick][2];
+      if (eval < best_cost) {
+        best_cost = eval;
+        memcpy(best, x, sizeof(double) * n);
+      }
+    }
+    memcpy(out_best + start[b] * A, best, sizeof(double) * n);
+    out_best_cost[b] = best_cost;
+    free(x); free(xp); free(gp); free(g); free(dir); free(xt); free(best); free(sb); free(yb);
+  }
+}

@@ -19,6 +19,7 @@ namespace pk {
 constexpr int kMaxJoints = PK_MAX_JOINTS;
 constexpr int kMaxAct = PK_MAX_ACT;
 constexpr int kMaxEE = PK_MAX_EE;
+constexpr int kMaxGrids = PK_MAX_GRIDS;
 
 // Lanes per wavefront on CDNA; every kernel here is written for wave64.
 constexpr int kWave = 64;

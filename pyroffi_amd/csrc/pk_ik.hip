@@ -47,12 +47,20 @@ struct NodeRec {
   int pad[2];
 };
 
-struct GridArgs {
-  const int* counts;   // (n_cells) or null: evaluate every world sphere
+struct GridLevel {
+  const int* counts;   // (n_cells)
   const int* items;    // (n_cells, cap)
   float lo_x, lo_y, lo_z, inv_cell;
   int nx, ny, nz, cap;
-  float r_max;         // robot spheres of larger radius test every world sphere
+  float r_max;         // largest robot sphere radius this level serves
+};
+
+// Up to two grids of increasing reach; a robot sphere uses the first whose
+// r_max covers its radius, or every world sphere when none does (n == 0: no
+// grid at all).
+struct GridArgs {
+  GridLevel lv[kMaxGrids];
+  int n;
 };
 
 struct IkArgs {
@@ -297,12 +305,10 @@ __device__ __forceinline__ float world_penalty_all(float cx, float cy, float cz,
 
 // Same sum over the grid cell holding c: exact, since every sphere that can
 // be within the margin of a robot sphere of radius <= r_max (= grid pad -
-// margin) is listed (pk_grid.hip); a larger robot sphere or an overflowing
-// cell falls back to the whole world and a centre outside the grid has no such
-// sphere.
-__device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz, float r,
-                                                    const float4* ws, int M, const GridArgs& g) {
-  if (r > g.r_max) return world_penalty_all(cx, cy, cz, r, ws, M);
+// margin) is listed (pk_grid.hip); an overflowing cell falls back to the whole
+// world and a centre outside the grid has no such sphere.
+__device__ __forceinline__ float world_penalty_level(float cx, float cy, float cz, float r,
+                                                     const float4* ws, int M, const GridLevel& g) {
   const int ix = static_cast<int>(floorf((cx - g.lo_x) * g.inv_cell));
   const int iy = static_cast<int>(floorf((cy - g.lo_y) * g.inv_cell));
   const int iz = static_cast<int>(floorf((cz - g.lo_z) * g.inv_cell));
@@ -323,6 +329,16 @@ __device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz
   return pen;
 }
 
+// The radius r is the same in every lane (one robot sphere at a time), so the
+// level choice is a uniform branch.
+__device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz, float r,
+                                                    const float4* ws, int M, const GridArgs& g) {
+#pragma unroll
+  for (int i = 0; i < kMaxGrids; ++i)
+    if (i < g.n && r <= g.lv[i].r_max) return world_penalty_level(cx, cy, cz, r, ws, M, g.lv[i]);
+  return world_penalty_all(cx, cy, cz, r, ws, M);
+}
+
 // Collision merit penalty (unweighted) over every robot sphere.
 template <int MAXA, int NEE>
 __device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, const Pose* P,
@@ -341,8 +357,8 @@ __device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, c
           const float cx = R.m00 * s.x + R.m01 * s.y + R.m02 * s.z + T.tx;
           const float cy = R.m10 * s.x + R.m11 * s.y + R.m12 * s.z + T.ty;
           const float cz = R.m20 * s.x + R.m21 * s.y + R.m22 * s.z + T.tz;
-          pen += (grid.counts != nullptr) ? world_penalty_grid(cx, cy, cz, s.w, ws, M, grid)
-                                          : world_penalty_all(cx, cy, cz, s.w, ws, M);
+          pen += (grid.n > 0) ? world_penalty_grid(cx, cy, cz, s.w, ws, M, grid)
+                              : world_penalty_all(cx, cy, cz, s.w, ws, M);
         }
       }
     }
@@ -965,9 +981,7 @@ extern "C" int pk_ls_ik_batched_f32(
     int32_t n_joints, const int32_t* target_jnts, const int32_t* ancestor_masks, int32_t n_ee,
     const float* target_T, const float* robot_spheres_local,
     const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres, const float* world_spheres,
-    int32_t n_world_spheres, const int32_t* grid_counts, const int32_t* grid_items,
-    float grid_lo_x, float grid_lo_y, float grid_lo_z, float grid_cell, int32_t grid_nx,
-    int32_t grid_ny, int32_t grid_nz, int32_t grid_cap, float grid_pad, const float* lower,
+    int32_t n_world_spheres, const pk_world_grid* grids, int32_t n_grids, const float* lower,
     const float* upper, const int32_t* fixed_mask, int32_t max_iter, int32_t iter_begin,
     int32_t iter_end, float* state, float pos_weight, float ori_weight, float lambda_init,
     float eps_pos, float eps_ori, int32_t enable_collision, float collision_weight,
@@ -994,12 +1008,22 @@ extern "C" int pk_ls_ik_batched_f32(
   a.state = state;
   a.iter_begin = iter_begin;
   a.iter_end = iter_end;
-  if (grid_counts != nullptr && a.enable_collision) {
-    PK_REQUIRE(grid_items && grid_cell > 0.f && grid_nx > 0 && grid_ny > 0 && grid_nz > 0 && grid_cap > 0,
-               "invalid world grid");
-    PK_REQUIRE(grid_pad >= 0.f, "negative grid pad");
-    a.grid = pk::GridArgs{grid_counts, grid_items, grid_lo_x, grid_lo_y, grid_lo_z, 1.f / grid_cell,
-                          grid_nx, grid_ny, grid_nz, grid_cap, grid_pad - collision_margin};
+  PK_REQUIRE(n_grids >= 0 && n_grids <= PK_MAX_GRIDS, "n_grids=%d outside [0, %d]", n_grids,
+             PK_MAX_GRIDS);
+  PK_REQUIRE(n_grids == 0 || grids != nullptr, "null grids");
+  a.grid.n = 0;
+  if (a.enable_collision) {
+    for (int i = 0; i < n_grids; ++i) {
+      const pk_world_grid& g = grids[i];
+      PK_REQUIRE(g.counts && g.items && g.cell > 0.f && g.dims[0] > 0 && g.dims[1] > 0 &&
+                     g.dims[2] > 0 && g.cap > 0 && g.pad >= 0.f,
+                 "invalid world grid %d", i);
+      PK_REQUIRE(i == 0 || g.pad >= grids[i - 1].pad, "grids must be ordered by increasing pad");
+      a.grid.lv[i] = pk::GridLevel{g.counts, g.items, g.lo[0], g.lo[1], g.lo[2], 1.f / g.cell,
+                                   g.dims[0], g.dims[1], g.dims[2], g.cap,
+                                   g.pad - collision_margin};
+    }
+    a.grid.n = n_grids;
   }
   return pk::dispatch_ik(stream, a, true, grid_waves);
 }

@@ -21,6 +21,24 @@ _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
 _f32 = ctypes.c_float
 
+
+
+class WorldGrid(ctypes.Structure):
+    """``pk_world_grid``: one world grid (device arrays + geometry), passed by host pointer."""
+
+    _fields_ = [
+        ("counts", ctypes.c_void_p),
+        ("items", ctypes.c_void_p),
+        ("lo", ctypes.c_float * 3),
+        ("cell", ctypes.c_float),
+        ("dims", ctypes.c_int32 * 3),
+        ("cap", ctypes.c_int32),
+        ("pad", ctypes.c_float),
+    ]
+
+
+MAX_GRIDS = 2
+
 _SIGNATURES = {
     "pk_last_error": (ctypes.c_char_p, []),
     "pk_abi_version": (_i32, []),
@@ -55,7 +73,7 @@ _SIGNATURES = {
         + [_p, _p, _i32, _p]  # targets
         + [_p, _p, _i32]  # robot spheres
         + [_p, _i32]  # world
-        + [_p, _p, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _i32, _f32]  # grid
+        + [_p, _i32]  # grids (pk_world_grid*), n_grids
         + [_p, _p, _p]  # limits
         + [_i32, _i32, _i32, _p]  # max_iter, iter range, state
         + [_f32, _f32, _f32, _f32, _f32, _i32, _f32, _f32, _i32]  # attrs, grid_waves

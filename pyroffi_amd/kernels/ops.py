@@ -21,7 +21,7 @@ import math
 
 import torch
 
-from ._lib import check, lib
+from ._lib import MAX_GRIDS, PkInvalidArgument, WorldGrid, check, lib
 
 
 def _stream() -> ctypes.c_void_p:
@@ -286,6 +286,23 @@ def world_grid(world_spheres, pad: float, cell: float | None = None, cap: int = 
             "dims": tuple(int(v) for v in dims), "cap": int(cap)}
 
 
+def _grid_array(grid):
+    """``pk_world_grid[]`` from None, one ``world_grid`` dict or a list of them (increasing pad)."""
+    levels = [] if grid is None else ([grid] if isinstance(grid, dict) else list(grid))
+    if len(levels) > MAX_GRIDS:
+        raise PkInvalidArgument(f"at most {MAX_GRIDS} grids, got {len(levels)}")
+    arr = (WorldGrid * max(1, len(levels)))()
+    for i, g in enumerate(levels):
+        arr[i].counts = g["counts"].data_ptr()
+        arr[i].items = g["items"].data_ptr()
+        arr[i].lo[:] = [float(v) for v in g["lo"]]
+        arr[i].cell = float(g["cell"])
+        arr[i].dims[:] = [int(v) for v in g["dims"]]
+        arr[i].cap = int(g["cap"])
+        arr[i].pad = float(g["pad"])
+    return (arr if levels else None), len(levels)
+
+
 def ls_ik_batched(seeds, problem_seed_start, tables, target_jnts, ancestor_masks, target_T,
                   robot_spheres_local, robot_sphere_joint_idx, world_spheres, lower, upper, fixed_mask, *,
                   max_iter: int, iter_begin: int = 0, iter_end: int | None = None, state=None,
@@ -318,15 +335,11 @@ def ls_ik_batched(seeds, problem_seed_start, tables, target_jnts, ancestor_masks
     if out_err is None:
         out_err = torch.empty((S_total,), dtype=torch.float32, device=dev)
     iter_end = max_iter if iter_end is None else iter_end
-    g = grid or {}
-    gl = g.get("lo", (0.0, 0.0, 0.0))
-    gd = g.get("dims", (0, 0, 0))
+    grids = _grid_array(grid)
     st = lib().pk_ls_ik_batched_f32(
         _stream(), _ptr(sd), _ptr(ss), P, S_total, A, *[_ptr(t) for t in tb], tb[0].shape[0],
         _ptr(tj), _ptr(am), n_ee, _ptr(tT), _ptr(rs), _ptr(rj), rs.shape[0], _ptr(ws), ws.shape[0],
-        _ptr(g.get("counts")), _ptr(g.get("items")), gl[0], gl[1], gl[2], float(g.get("cell", 1.0)),
-        gd[0], gd[1], gd[2], int(g.get("cap", 0)), float(g.get("pad", 0.0)), _ptr(lo), _ptr(hi), _ptr(fm),
-        int(max_iter),
+        grids[0], grids[1], _ptr(lo), _ptr(hi), _ptr(fm), int(max_iter),
         int(iter_begin), int(iter_end), _ptr(state), float(pos_weight), float(ori_weight), float(lambda_init),
         float(eps_pos), float(eps_ori), int(bool(enable_collision)), float(collision_weight),
         float(collision_margin), int(grid_waves), _ptr(out_cfgs), _ptr(out_err),

@@ -16,7 +16,8 @@ need one call each.  ``UnevenLsIkSolver`` takes any mix:
   (``pk_world_grid_f32``) instead of every world sphere;
 * all chunk launches and the per-problem winner selections of all groups are
   captured once into a graph (``torch.cuda.CUDAGraph`` drives HIP stream
-  capture) and replayed;
+  capture) and replayed; each group is a branch of its own, so the robots'
+  launches run concurrently;
 * an autotuner times candidate (workgroups, chunk length) pairs per group on
   the actual batch and caches the winners by batch signature.
 """
@@ -68,6 +69,7 @@ class UnevenLsIkSolver:
                        collision_weight=collision_weight, collision_margin=collision_margin)
         self.use_grid = use_grid
         self.graph = None
+        self._streams = None
         for g in groups:
             self._upload(g)
 
@@ -100,11 +102,15 @@ class UnevenLsIkSolver:
             d["rj"] = torch.as_tensor(rj, device=dev)
             d["ws"] = torch.as_tensor(ws, device=dev)
             if self.use_grid and len(rs) and len(ws):
-                # Reach of the grid: the 90th-percentile robot sphere radius + margin.
-                # The few larger robot spheres (a torso, a base plate) test every
-                # world sphere in the kernel, so any choice here is exact.
-                pad = float(np.quantile(rs[:, 3], 0.9)) + self.kw["collision_margin"]
-                d["grid"] = world_grid(d["ws"], pad)
+                # A fine grid reaching the 90th-percentile robot sphere radius and,
+                # when a few robot spheres are larger (a torso, a base plate), a
+                # coarse one reaching the largest.  The kernel picks per sphere;
+                # every choice gives the same penalty.
+                margin = self.kw["collision_margin"]
+                r_small, r_big = float(np.quantile(rs[:, 3], 0.9)), float(np.max(rs[:, 3]))
+                d["grid"] = [world_grid(d["ws"], r_small + margin)]
+                if r_big > r_small:
+                    d["grid"].append(world_grid(d["ws"], r_big + margin, cap=128, max_cells=24))
         else:
             d["rs"] = torch.zeros((0, 4), device=dev)
             d["rj"] = torch.zeros((0,), dtype=torch.int32, device=dev)
@@ -131,8 +137,19 @@ class UnevenLsIkSolver:
                               out_idx=d["best_idx"])
 
     def run_eager(self) -> None:
-        for g in self.groups:
-            self._run_group(g, *g.tune)
+        """Every group on a stream of its own (forked from and joined to the current one)."""
+        if len(self.groups) == 1:
+            self._run_group(self.groups[0], *self.groups[0].tune)
+            return
+        main = torch.cuda.current_stream()
+        if self._streams is None:
+            self._streams = [torch.cuda.Stream() for _ in self.groups]
+        for g, s in zip(self.groups, self._streams):
+            s.wait_stream(main)
+            with torch.cuda.stream(s):
+                self._run_group(g, *g.tune)
+        for s in self._streams:
+            main.wait_stream(s)
 
     def _signature(self, g: RobotGroup) -> str:
         n = g.robot.joints.num_actuated_joints
