@@ -328,6 +328,95 @@ int pk_ls_ik_select_f32(void* stream, const float* cfgs, const float* errs,
                         int32_t n_seeds, int32_t n_act, float continuity_weight,
                         float* out_best, int32_t* out_idx);
 
+/*
+ * Trajectory model for the trajectory cost and optimiser (device pointers).
+ * The robot is given with its fixed joints folded into the moving joints
+ * ("nodes", topological order), as pyroffi_amd.trajopt builds it:
+ *   nodes          (n_nodes, 20) float: parent-node -> joint transform before
+ *                  motion (wxyz_xyz, 7), twist v (3), twist w (3), mimic
+ *                  multiplier, mimic offset, actuated source index (-1: none),
+ *                  parent node (-1: world), exp kind (1 rotation, 2
+ *                  translation, 3 general), 2 unused
+ *   link_node      (n_links) node carrying each link (-1: world-fixed)
+ *   link_sph_start (n_links + 1) CSR of spheres per link, in the link's slot
+ *                  order of the reference's padded sphere table
+ *   spheres        (n_spheres, 4) centre in the node frame, radius
+ *   link_bound     (n_links, 4) bounding sphere of each link's spheres (node frame)
+ *   pair_links     (n_pairs, 2) active self pairs (link i, link j)
+ *   pair_start     (n_pairs + 1) CSR into pair_items
+ *   pair_items     (n_items, 2) sphere pairs of a link pair (same slot)
+ *   world          (n_world, 4) world spheres
+ *   grids          0..PK_MAX_GRIDS grids of the world (increasing pad; pad -
+ *                  margin >= a robot sphere's radius selects a grid for it)
+ *   lower, upper   (n_act) joint limits
+ *   margin, mode   activation margin; mode 0 = max(0, margin - d)^2 (CHOMP),
+ *                  1 = -min(colldist_from_sdf(d, margin), 0) (L-BFGS trajopt)
+ */
+typedef struct pk_traj_model {
+  const float* nodes;
+  int32_t n_nodes, n_act;
+  const int32_t* link_node;
+  const int32_t* link_sph_start;
+  const float* spheres;
+  const float* link_bound;
+  int32_t n_links, n_spheres;
+  const int32_t* pair_links;
+  const int32_t* pair_start;
+  const int32_t* pair_items;
+  int32_t n_pairs, n_items;
+  const float* world;
+  int32_t n_world;
+  const pk_world_grid* grids;
+  int32_t n_grids;
+  const float* lower;
+  const float* upper;
+  float margin;
+  int32_t mode;
+} pk_traj_model;
+
+/* LbfgsTrajOptConfig (src/pyroffi/optimization_engines/_lbfgs_trajopt_optimization.py:36-70). */
+typedef struct pk_lbfgs_config {
+  int32_t n_iters, m_hist, escalation_interval;
+  float w_smooth, w_vel, w_acc, w_jerk, w_limits;
+  float w_collision, w_collision_max, penalty_scale;
+} pk_lbfgs_config;
+
+/*
+ * Trajectory cost components and gradient, many trajectories of different
+ * lengths in one launch: trajectory b is rows [traj_start[b], traj_start[b+1])
+ * of x (n_waypoints, n_act), at most max_len rows.
+ *   weights   {w_smooth, w_vel, w_acc, w_jerk, w_limits, w_coll}
+ *   out_comps (n_traj, 3) {smooth, limits, coll} without w_smooth/w_limits/w_coll
+ *   out_grad  (n_waypoints, n_act) gradient of w_smooth smooth + w_limits
+ *             limits + w_coll coll, or NULL
+ * Replaces the cost_fn + jax.value_and_grad of
+ * src/pyroffi/optimization_engines/_lbfgs_trajopt_optimization.py:98-118 (the
+ * gradient is a hand-written adjoint, no reference counterpart).
+ */
+int pk_traj_cost_grad_f32(void* stream, const pk_traj_model* model, const float* x,
+                          const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
+                          int32_t max_len, const float* weights, float* out_comps,
+                          float* out_grad);
+
+/* Floats of the state buffer pk_traj_lbfgs_f32 needs. */
+int64_t pk_traj_lbfgs_state_floats(int64_t n_waypoints, int32_t n_act, int32_t n_traj,
+                                   int32_t m_hist);
+
+/*
+ * Direct L-BFGS trajectory optimisation (_lbfgs_direct_solve,
+ * src/pyroffi/optimization_engines/_lbfgs_trajopt_optimization.py:76-205),
+ * one workgroup per trajectory.  Endpoints stay where x_init has them (the
+ * caller pins them, _lbfgs_trajopt_jax :229-230).  Iterations
+ * [iter_begin, iter_end) of cfg->n_iters run per call and resume from `state`
+ * (pk_traj_lbfgs_state_floats floats), so a solve can be split into launches.
+ *   out_best      (n_waypoints, n_act) best iterate (at the maximum collision weight)
+ *   out_best_cost (n_traj) its cost, written by the call that reaches n_iters
+ */
+int pk_traj_lbfgs_f32(void* stream, const pk_traj_model* model, const float* x_init,
+                      const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
+                      int32_t max_len, const pk_lbfgs_config* cfg, int32_t iter_begin,
+                      int32_t iter_end, float* state, float* out_best, float* out_best_cost);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,829 +1,22 @@
-// Multi-seed Levenberg-Marquardt IK for gfx950: one lane = one seed.
-//
-// Algorithm (the reference's ls_ik_lm_kernel, _ls_ik_cuda_kernel.cu:64-394,
-// with the residual/Jacobian of _ik_cuda_helpers.cuh:433-631):
-//   r_pos = p_ee - p_tgt;  r_ori = log(q_ee * q_tgt^-1) (hemisphere-folded)
-//   J     = geometric Jacobian ([z x (p_ee - p_j); z] revolute, [z; 0]
-//           prismatic), times the mimic multiplier, ancestor-masked per EE
-//   per iteration: stop if every EE has |r_pos| < eps_pos and |r_ori| < eps_ori;
-//   fw = W r, Jw = W J, cur = |fw|^2 + penalty(q)
-//   Jacobi column scaling s_a = |Jw[:,a]| + 1e-8; (Js^T Js + lam I) p = -Js^T fw
-//   (solved in FP32 with the damping floored at 1e-6, see kDampFloor)
-//   (fixed joints: zero row/col, unit diagonal); delta = p / s
-//   trust region: |delta| <= R(max pos err, max ori err) in {0.38, 0.22, 0.12, 0.05}
-//   5-point line search alpha in {1, .5, .25, .1, .025} on clamp(q + alpha delta),
-//   all evaluated, best kept; accept if err < cur (1 - 1e-4): lam *= 0.5 (>= 1e-10),
-//   else lam *= 3 (<= 1e6); track the all-time best (err, q).
-//   penalty(q) = w_coll * sum over robot spheres x world spheres of
-//                min(d - margin, 0)^2.
-//
-// gfx950 layout: the workgroup prologue folds every fixed joint into its
-// nearest moving ancestor, producing a tree of "nodes" (the joints whose
-// transform depends on q).  A lane holds only the node poses in registers
-// (n_act of them for a robot without mimic joints), and the EE frames and
-// sphere centres are constant offsets from a node.  All per-seed arrays are
-// indexed by compile-time constants (MAXA = padded DOF, NEE = padded EE
-// count); padded DOF are frozen joints with zero limits, padded EEs have zero
-// residual, so they change nothing.  The damped normal equations are solved
-// with an FP32 Cholesky in registers.
-#include "pk_model.h"
-
-#include <math.h>
+// C ABI of the multi-seed LM IK (kernels in pk_ik.h), winner selection.
+#include "pk_ik.h"
 
 #include <algorithm>
 
 namespace pk {
-
-struct NodeRec {
-  float ptf[7];     // folded constant: parent node frame -> this joint (before motion)
-  float lin[3], ang[3];
-  float mul, off;
-  int src;          // actuated index driving this node, -1 none
-  int parent;       // parent node, -1 = world
-  int kind;         // joint_exp kind (1..3)
-  int jac_type;     // 1 revolute, 2 prismatic, 0 no Jacobian column
-  float ax[3];      // unit joint axis in the joint frame (Jacobian)
-  unsigned ee_mask; // bit e set when the node is an ancestor of EE e
-  int pad[2];
-};
-
-struct GridLevel {
-  const int* counts;   // (n_cells)
-  const int* items;    // (n_cells, cap)
-  float lo_x, lo_y, lo_z, inv_cell;
-  int nx, ny, nz, cap;
-  float r_max;         // largest robot sphere radius this level serves
-};
-
-// Up to two grids of increasing reach; a robot sphere uses the first whose
-// r_max covers its radius, or every world sphere when none does (n == 0: no
-// grid at all).
-struct GridArgs {
-  GridLevel lv[kMaxGrids];
-  int n;
-};
-
-struct IkArgs {
-  const float* seeds;
-  int n_problems, n_seeds, n_act;
-  RobotTables tables;
-  const int* target_jnts;
-  const int* ancestor_masks;
-  int n_ee;
-  const float* target_T;
-  const float* robot_spheres;
-  const int* robot_sphere_joint;
-  int n_robot_spheres;
-  const float* world;
-  int n_world;
-  const float* lower;
-  const float* upper;
-  const int* fixed_mask;
-  int max_iter;
-  float pos_weight, ori_weight, lambda_init, eps_pos, eps_ori;
-  int enable_collision;
-  float collision_weight, collision_margin;
-  float* out_cfgs;
-  float* out_err;
-  // Batched launch only.
-  const int* problem_seed_start;  // (n_problems + 1): seeds of problem p
-  long long total_seeds;
-  float* state;                   // (total_seeds, 2 * n_act + 3) or null
-  int iter_begin, iter_end;
-  GridArgs grid;
-};
-
-// Shared-memory image of one problem, built by the prologue.
-template <int MAXA, int NEE>
-struct IkShared {
-  NodeRec nodes[MAXA + kMaxJoints];   // worst case: every joint moves
-  int n_nodes;
-  int ee_node[NEE];
-  float ee_rel[NEE][7];               // node frame -> EE frame
-  float tgt[NEE][7];
-  float lo[MAXA], hi[MAXA];
-  int fixed[MAXA];
-  int sph_start[MAXA + kMaxJoints + 2];  // spheres grouped by node; group 0 = world-fixed
-  int cnt[MAXA + kMaxJoints + 2];
-  int n_sph;
-  int too_many_nodes;                 // moving joints exceed the padded width MAXA
-};
-
-__device__ __forceinline__ Pose pose_from(const float* p) { return load_pose(p); }
-__device__ __forceinline__ void pose_to(float* p, const Pose& v) {
-  p[0] = v.w; p[1] = v.x; p[2] = v.y; p[3] = v.z; p[4] = v.tx; p[5] = v.ty; p[6] = v.tz;
-}
-
-// Single-lane tree folding (J <= 64 joints, runs once per workgroup).
-template <int MAXA, int NEE>
-__device__ void build_problem(const IkArgs& a, const JointRec* recs, const int* slot_of,
-                              IkShared<MAXA, NEE>& sh, int* anchor, float* rel,
-                              float4* sph, int* sph_node_tmp) {
-  const int J = a.tables.n_joints;
-  int nn = 0;
-  for (int i = 0; i < J; ++i) {
-    const JointRec& r = recs[i];
-    int base = -1;
-    Pose bt = pose_identity();
-    if (r.parent_slot >= 0) {
-      base = anchor[r.parent_slot];
-      bt = pose_from(rel + 7 * r.parent_slot);
-    }
-    if (r.kind == 0) {
-      anchor[i] = base;
-      pose_to(rel + 7 * i, compose(bt, load_pose(r.ptf)));
-    } else {
-      NodeRec& n = sh.nodes[nn];
-      pose_to(n.ptf, compose(bt, load_pose(r.ptf)));
-      for (int c = 0; c < 3; ++c) { n.lin[c] = r.lin[c]; n.ang[c] = r.ang[c]; }
-      n.mul = r.mul; n.off = r.off; n.src = r.src; n.parent = base; n.kind = r.kind;
-      const float a2 = r.ang[0] * r.ang[0] + r.ang[1] * r.ang[1] + r.ang[2] * r.ang[2];
-      const float l2 = r.lin[0] * r.lin[0] + r.lin[1] * r.lin[1] + r.lin[2] * r.lin[2];
-      n.jac_type = 0;
-      n.ax[0] = n.ax[1] = n.ax[2] = 0.f;
-      if (r.src >= 0) {
-        if (a2 > 1e-6f) {
-          const float inv = 1.f / sqrtf(a2);
-          n.jac_type = 1;
-          for (int c = 0; c < 3; ++c) n.ax[c] = r.ang[c] * inv;
-        } else if (l2 > 1e-6f) {
-          const float inv = 1.f / sqrtf(l2);
-          n.jac_type = 2;
-          for (int c = 0; c < 3; ++c) n.ax[c] = r.lin[c] * inv;
-        }
-      }
-      unsigned m = 0;
-      for (int e = 0; e < a.n_ee; ++e)
-        if (a.ancestor_masks[e * J + r.joint]) m |= 1u << e;
-      n.ee_mask = m;
-      anchor[i] = nn;
-      pose_to(rel + 7 * i, pose_identity());
-      ++nn;
-    }
-  }
-  sh.n_nodes = nn;
-  sh.too_many_nodes = nn > MAXA;
-  for (int e = 0; e < NEE; ++e) {
-    sh.ee_node[e] = -2;  // padded EE: zero residual
-    if (e < a.n_ee) {
-      int tj = a.target_jnts[e];
-      const int s = (tj >= 0 && tj < J) ? slot_of[tj] : -1;
-      if (s >= 0) {
-        sh.ee_node[e] = anchor[s];
-        for (int c = 0; c < 7; ++c) sh.ee_rel[e][c] = rel[7 * s + c];
-      } else {
-        sh.ee_node[e] = -1;
-        Pose id = pose_identity();
-        pose_to(sh.ee_rel[e], id);
-      }
-    }
-  }
-  // Robot spheres, regrouped by node (group g = node + 1; group 0 = world-fixed).
-  const int K = a.enable_collision ? a.n_robot_spheres : 0;
-  int* count = sh.cnt;
-  for (int g = 0; g <= nn; ++g) count[g] = 0;
-  for (int k = 0; k < K; ++k) {
-    const int jidx = a.robot_sphere_joint[k];
-    int g = -1;
-    if (jidx >= 0 && jidx < J && slot_of[jidx] >= 0) g = anchor[slot_of[jidx]] + 1;
-    sph_node_tmp[k] = g;
-    if (g >= 0) ++count[g];
-  }
-  int acc = 0;
-  for (int g = 0; g <= nn; ++g) { sh.sph_start[g] = acc; acc += count[g]; count[g] = sh.sph_start[g]; }
-  sh.sph_start[nn + 1] = acc;
-  sh.n_sph = acc;
-  for (int k = 0; k < K; ++k) {
-    const int g = sph_node_tmp[k];
-    if (g < 0) continue;
-    const int jidx = a.robot_sphere_joint[k];
-    const int s = slot_of[jidx];
-    const Pose rp = pose_from(rel + 7 * s);
-    const float* sp = a.robot_spheres + 4 * k;
-    float cx, cy, cz;
-    quat_rotate(rp.w, rp.x, rp.y, rp.z, sp[0], sp[1], sp[2], cx, cy, cz);
-    sph[count[g]++] = make_float4(cx + rp.tx, cy + rp.ty, cz + rp.tz, sp[3]);
-  }
-}
-
-// Forward kinematics of the node tree for one lane.  Node parents are
-// wave-uniform, so the parent lookup is a chain of scalar branches.
-template <int MAXA, int NEE>
-__device__ __forceinline__ void node_fk(const IkShared<MAXA, NEE>& sh, const float* q,
-                                        Pose* P) {
-#pragma unroll
-  for (int n = 0; n < MAXA; ++n) {
-    if (n < sh.n_nodes) {
-      const NodeRec& r = sh.nodes[n];
-      const int src = __builtin_amdgcn_readfirstlane(r.src);
-      float qs = 0.f;
-#pragma unroll
-      for (int a = 0; a < MAXA; ++a)
-        if (src == a) qs = q[a];
-      const float qj = qs * r.mul + r.off;
-      const Pose tl = compose(load_pose(r.ptf),
-                              joint_exp(r.kind, qj, r.lin[0], r.lin[1], r.lin[2], r.ang[0],
-                                        r.ang[1], r.ang[2]));
-      const int par = __builtin_amdgcn_readfirstlane(r.parent);
-      Pose pp = pose_identity();
-      bool has_parent = false;
-#pragma unroll
-      for (int m = 0; m < n; ++m)
-        if (par == m) { pp = P[m]; has_parent = true; }
-      P[n] = has_parent ? compose(pp, tl) : tl;
-    }
-  }
-}
-
-template <int MAXA, int NEE>
-__device__ __forceinline__ Pose ee_pose(const IkShared<MAXA, NEE>& sh, const Pose* P, int e) {
-  const int node = __builtin_amdgcn_readfirstlane(sh.ee_node[e]);
-  const Pose rel = load_pose(sh.ee_rel[e]);
-  Pose base = pose_identity();
-  bool found = false;
-#pragma unroll
-  for (int n = 0; n < MAXA; ++n)
-    if (node == n) { base = P[n]; found = true; }
-  return found ? compose(base, rel) : rel;
-}
-
-// Residual of one EE: [p - p_t; log(q q_t^-1)] (_ik_cuda_helpers.cuh:471-494).
-__device__ __forceinline__ void ee_residual(const Pose& T, const float* tgt, float* r) {
-  r[0] = T.tx - tgt[4];
-  r[1] = T.ty - tgt[5];
-  r[2] = T.tz - tgt[6];
-  float ew, ex, ey, ez;
-  quat_mul(T.w, T.x, T.y, T.z, tgt[0], -tgt[1], -tgt[2], -tgt[3], ew, ex, ey, ez);
-  if (ew < 0.f) { ew = -ew; ex = -ex; ey = -ey; ez = -ez; }
-  const float sh = sqrtf(ex * ex + ey * ey + ez * ez);
-  if (sh > 1e-6f) {
-    const float k = 2.f * atan2f(sh, ew) / sh;
-    r[3] = ex * k; r[4] = ey * k; r[5] = ez * k;
-  } else {
-    r[3] = 2.f * ex; r[4] = 2.f * ey; r[5] = 2.f * ez;
-  }
-}
-
-template <int MAXA, int NEE>
-__device__ __forceinline__ void residuals(const IkShared<MAXA, NEE>& sh, const Pose* P,
-                                          const float (&tg)[NEE][7], float* r, Pose* Tee) {
-#pragma unroll
-  for (int e = 0; e < NEE; ++e) {
-    if (sh.ee_node[e] == -2) {
-#pragma unroll
-      for (int c = 0; c < 6; ++c) r[6 * e + c] = 0.f;
-      Tee[e] = pose_identity();
-    } else {
-      Tee[e] = ee_pose(sh, P, e);
-      ee_residual(Tee[e], tg[e], r + 6 * e);
-    }
-  }
-}
-
-// sum over world spheres m of min(|c - o_m| - r - R'_m, 0)^2, R' = R + margin.
-__device__ __forceinline__ float world_penalty_all(float cx, float cy, float cz, float r,
-                                                   const float4* ws, int M) {
-  float p0 = 0.f, p1 = 0.f;
-  int m = 0;
-  for (; m + 1 < M; m += 2) {
-    const float4 o0 = ws[m], o1 = ws[m + 1];
-    const float ax = cx - o0.x, ay = cy - o0.y, az = cz - o0.z;
-    const float bx = cx - o1.x, by = cy - o1.y, bz = cz - o1.z;
-    const float e0 = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o0.w - r, 0.f);
-    const float e1 = fminf(__builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz) - o1.w - r, 0.f);
-    p0 = fmaf(e0, e0, p0);
-    p1 = fmaf(e1, e1, p1);
-  }
-  if (m < M) {
-    const float4 o0 = ws[m];
-    const float ax = cx - o0.x, ay = cy - o0.y, az = cz - o0.z;
-    const float e0 = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o0.w - r, 0.f);
-    p0 = fmaf(e0, e0, p0);
-  }
-  return p0 + p1;
-}
-
-// Same sum over the grid cell holding c: exact, since every sphere that can
-// be within the margin of a robot sphere of radius <= r_max (= grid pad -
-// margin) is listed (pk_grid.hip); an overflowing cell falls back to the whole
-// world and a centre outside the grid has no such sphere.
-__device__ __forceinline__ float world_penalty_level(float cx, float cy, float cz, float r,
-                                                     const float4* ws, int M, const GridLevel& g) {
-  const int ix = static_cast<int>(floorf((cx - g.lo_x) * g.inv_cell));
-  const int iy = static_cast<int>(floorf((cy - g.lo_y) * g.inv_cell));
-  const int iz = static_cast<int>(floorf((cz - g.lo_z) * g.inv_cell));
-  if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nx || iy >= g.ny || iz >= g.nz) return 0.f;
-  const long long cell = (static_cast<long long>(iz) * g.ny + iy) * g.nx + ix;
-  const int n = g.counts[cell];
-  if (n > g.cap) return world_penalty_all(cx, cy, cz, r, ws, M);
-  const int* it = g.items + cell * g.cap;
-  float pen = 0.f;
-  for (int i = 0; i < n; ++i) {
-    const int m = it[i];
-    if (m < 0 || m >= M) continue;
-    const float4 o = ws[m];
-    const float ax = cx - o.x, ay = cy - o.y, az = cz - o.z;
-    const float e = fminf(__builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az) - o.w - r, 0.f);
-    pen = fmaf(e, e, pen);
-  }
-  return pen;
-}
-
-// The radius r is the same in every lane (one robot sphere at a time), so the
-// level choice is a uniform branch.
-__device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz, float r,
-                                                    const float4* ws, int M, const GridArgs& g) {
-#pragma unroll
-  for (int i = 0; i < kMaxGrids; ++i)
-    if (i < g.n && r <= g.lv[i].r_max) return world_penalty_level(cx, cy, cz, r, ws, M, g.lv[i]);
-  return world_penalty_all(cx, cy, cz, r, ws, M);
-}
-
-// Collision merit penalty (unweighted) over every robot sphere.
-template <int MAXA, int NEE>
-__device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, const Pose* P,
-                                                const float4* sph, const float4* ws, int M,
-                                                const GridArgs& grid) {
-  float pen = 0.f;
-#pragma unroll
-  for (int g = 0; g <= MAXA; ++g) {
-    if (g <= sh.n_nodes) {
-      const int k0 = sh.sph_start[g], k1 = sh.sph_start[g + 1];
-      if (k1 > k0) {
-        const Pose T = (g == 0) ? pose_identity() : P[g > 0 ? g - 1 : 0];
-        const Rot3 R = quat_to_rot(T.w, T.x, T.y, T.z);
-        for (int k = k0; k < k1; ++k) {
-          const float4 s = sph[k];
-          const float cx = R.m00 * s.x + R.m01 * s.y + R.m02 * s.z + T.tx;
-          const float cy = R.m10 * s.x + R.m11 * s.y + R.m12 * s.z + T.ty;
-          const float cz = R.m20 * s.x + R.m21 * s.y + R.m22 * s.z + T.tz;
-          pen += (grid.n > 0) ? world_penalty_grid(cx, cy, cz, s.w, ws, M, grid)
-                              : world_penalty_all(cx, cy, cz, s.w, ws, M);
-        }
-      }
-    }
-  }
-  return pen;
-}
-
-struct IkLds {
-  size_t recs, slot_of, anchor, rel, sph_tmp, sph, ws, total;
-};
-
-__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~static_cast<size_t>(15); }
-
-// Byte offsets of the IK kernels' LDS regions (float4 regions 16-byte aligned).
-__host__ __device__ inline IkLds ik_lds_layout(size_t shared_bytes, int J, int K, int M) {
-  IkLds o;
-  o.recs = align16(shared_bytes);
-  o.slot_of = align16(o.recs + sizeof(JointRec) * J);
-  o.anchor = o.slot_of + sizeof(int) * J;
-  o.rel = o.anchor + sizeof(int) * J;
-  o.sph_tmp = o.rel + sizeof(float) * 7 * J;
-  o.sph = align16(o.sph_tmp + sizeof(int) * K);
-  o.ws = o.sph + sizeof(float4) * K;
-  o.total = o.ws + sizeof(float4) * M;
-  return o;
-}
-
-constexpr float kDampFloor = 1e-6f;
-
-__device__ __forceinline__ float ls_alpha(int i) {
-  return i == 0 ? 1.f : (i == 1 ? 0.5f : (i == 2 ? 0.25f : (i == 3 ? 0.1f : 0.025f)));
-}
-
-// Per-seed solver state (registers).
-template <int MAXA>
-struct LmState {
-  float q[MAXA], best_q[MAXA];
-  float best_err, lam;
-  int done;
-};
-
-// Problem tables staged in LDS by the workgroup prologue.
-template <int MAXA, int NEE>
-struct IkView {
-  const IkShared<MAXA, NEE>* sh;
-  const float4* sph;
-  const float4* ws;
-  int M;
-  bool coll;
-  float wc;
-};
-
-template <int MAXA, int NEE>
-__device__ __forceinline__ float weighted_error(const IkView<MAXA, NEE>& v, const float (&tg)[NEE][7],
-                                                const float* W, const float* q, Pose* P, float* r,
-                                                Pose* Tee, const GridArgs& grid) {
-  node_fk(*v.sh, q, P);
-  residuals(*v.sh, P, tg, r, Tee);
-  float e = 0.f;
-#pragma unroll
-  for (int i = 0; i < 6 * NEE; ++i) { const float x = r[i] * W[i % 6]; e += x * x; }
-  if (v.coll) e += v.wc * sphere_penalty(*v.sh, P, v.sph, v.ws, v.M, grid);
-  return e;
-}
-
-// LM iterations [it0, it1) of one seed (the algorithm in the file header).
-template <int MAXA, int NEE>
-__device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const float (&tg)[NEE][7],
-                                           const float* W, const IkArgs& a, LmState<MAXA>& st,
-                                           int it0, int it1, const GridArgs& grid) {
-  const IkShared<MAXA, NEE>& sh = *v.sh;
-  Pose P[MAXA];
-  Pose Tee[NEE];
-  float r[6 * NEE];
-  for (int it = it0; it < it1 && !st.done; ++it) {
-    node_fk(sh, st.q, P);
-    residuals(sh, P, tg, r, Tee);
-    bool conv = true;
-    float max_p = 0.f, max_o = 0.f;
-#pragma unroll
-    for (int e = 0; e < NEE; ++e) {
-      const float np = sqrtf(r[6 * e] * r[6 * e] + r[6 * e + 1] * r[6 * e + 1] + r[6 * e + 2] * r[6 * e + 2]);
-      const float no = sqrtf(r[6 * e + 3] * r[6 * e + 3] + r[6 * e + 4] * r[6 * e + 4] + r[6 * e + 5] * r[6 * e + 5]);
-      if (e < a.n_ee && (np >= a.eps_pos || no >= a.eps_ori)) conv = false;
-      max_p = fmaxf(max_p, np);
-      max_o = fmaxf(max_o, no);
-    }
-    if (conv) { st.done = 1; break; }
-
-    float cur = 0.f;
-    float fw[6 * NEE];
-#pragma unroll
-    for (int i = 0; i < 6 * NEE; ++i) { fw[i] = r[i] * W[i % 6]; cur += fw[i] * fw[i]; }
-    if (v.coll) cur += v.wc * sphere_penalty(sh, P, v.sph, v.ws, v.M, grid);
-
-    // Weighted geometric Jacobian, column a from the node(s) driven by a.
-    float Jm[6 * NEE][MAXA];
-#pragma unroll
-    for (int i = 0; i < 6 * NEE; ++i)
-#pragma unroll
-      for (int c = 0; c < MAXA; ++c) Jm[i][c] = 0.f;
-#pragma unroll
-    for (int n = 0; n < MAXA; ++n) {
-      if (n < sh.n_nodes) {
-        const NodeRec& nr = sh.nodes[n];
-        const int jt = __builtin_amdgcn_readfirstlane(nr.jac_type);
-        if (jt != 0) {
-          float zx, zy, zz;
-          quat_rotate(P[n].w, P[n].x, P[n].y, P[n].z, nr.ax[0], nr.ax[1], nr.ax[2], zx, zy, zz);
-          const int src = __builtin_amdgcn_readfirstlane(nr.src);
-          const unsigned em = __builtin_amdgcn_readfirstlane(nr.ee_mask);
-          const float mul = nr.mul;
-#pragma unroll
-          for (int e = 0; e < NEE; ++e) {
-            if (em & (1u << e)) {
-              float g[6];
-              if (jt == 1) {
-                const float ax = Tee[e].tx - P[n].tx, ay = Tee[e].ty - P[n].ty, az = Tee[e].tz - P[n].tz;
-                g[0] = zy * az - zz * ay; g[1] = zz * ax - zx * az; g[2] = zx * ay - zy * ax;
-                g[3] = zx; g[4] = zy; g[5] = zz;
-              } else {
-                g[0] = zx; g[1] = zy; g[2] = zz; g[3] = 0.f; g[4] = 0.f; g[5] = 0.f;
-              }
-#pragma unroll
-              for (int c = 0; c < MAXA; ++c)
-                if (src == c) {
-#pragma unroll
-                  for (int k = 0; k < 6; ++k) Jm[6 * e + k][c] += mul * g[k] * W[k];
-                }
-            }
-          }
-        }
-      }
-    }
-
-    // Column scaling and normal equations (packed lower triangle).
-    float cs[MAXA];
-#pragma unroll
-    for (int c = 0; c < MAXA; ++c) {
-      float sq = 0.f;
-#pragma unroll
-      for (int i = 0; i < 6 * NEE; ++i) sq += Jm[i][c] * Jm[i][c];
-      cs[c] = sqrtf(sq) + 1e-8f;
-    }
-#pragma unroll
-    for (int i = 0; i < 6 * NEE; ++i)
-#pragma unroll
-      for (int c = 0; c < MAXA; ++c) Jm[i][c] /= cs[c];
-    constexpr int NT = MAXA * (MAXA + 1) / 2;
-    float A[NT];
-    float rhs[MAXA];
-#pragma unroll
-    for (int i = 0; i < MAXA; ++i) {
-#pragma unroll
-      for (int j = 0; j <= i; ++j) {
-        float acc = 0.f;
-#pragma unroll
-        for (int k = 0; k < 6 * NEE; ++k) acc += Jm[k][i] * Jm[k][j];
-        A[i * (i + 1) / 2 + j] = acc;
-      }
-      float rb = 0.f;
-#pragma unroll
-      for (int k = 0; k < 6 * NEE; ++k) rb += Jm[k][i] * fw[k];
-      rhs[i] = -rb;
-      // Columns are unit-scaled, so the diagonal of Js^T Js is ~1 and an FP32
-      // Cholesky resolves eigenvalues down to ~1e-6.  On redundant arms
-      // Js^T Js is singular and lam decays to 1e-10, which FP64 (the reference)
-      // survives and FP32 does not; the damping used inside the solve is
-      // floored at 1e-6 while the lam schedule itself is left unchanged.
-      A[i * (i + 1) / 2 + i] += fmaxf(st.lam, kDampFloor);
-    }
-#pragma unroll
-    for (int i = 0; i < MAXA; ++i) {
-      if (sh.fixed[i]) {
-#pragma unroll
-        for (int j = 0; j < MAXA; ++j) {
-          if (j <= i) A[i * (i + 1) / 2 + j] = 0.f;
-          else A[j * (j + 1) / 2 + i] = 0.f;
-        }
-        A[i * (i + 1) / 2 + i] = 1.f;
-        rhs[i] = 0.f;
-      }
-    }
-    // Cholesky A = L L^T in place, then two triangular solves.
-    bool ok = true;
-#pragma unroll
-    for (int k = 0; k < MAXA; ++k) {
-      float sd = A[k * (k + 1) / 2 + k];
-#pragma unroll
-      for (int pp = 0; pp < k; ++pp) sd -= A[k * (k + 1) / 2 + pp] * A[k * (k + 1) / 2 + pp];
-      if (sd <= 0.f) ok = false;
-      const float lkk = sqrtf(fmaxf(sd, 1e-30f));
-      A[k * (k + 1) / 2 + k] = lkk;
-      const float inv = 1.f / lkk;
-#pragma unroll
-      for (int i = k + 1; i < MAXA; ++i) {
-        float t = A[i * (i + 1) / 2 + k];
-#pragma unroll
-        for (int pp = 0; pp < k; ++pp) t -= A[i * (i + 1) / 2 + pp] * A[k * (k + 1) / 2 + pp];
-        A[i * (i + 1) / 2 + k] = t * inv;
-      }
-    }
-    float y[MAXA];
-#pragma unroll
-    for (int i = 0; i < MAXA; ++i) {
-      float t = rhs[i];
-#pragma unroll
-      for (int pp = 0; pp < i; ++pp) t -= A[i * (i + 1) / 2 + pp] * y[pp];
-      y[i] = t / A[i * (i + 1) / 2 + i];
-    }
-    float delta[MAXA];
-#pragma unroll
-    for (int i = MAXA - 1; i >= 0; --i) {
-      float t = y[i];
-#pragma unroll
-      for (int pp = i + 1; pp < MAXA; ++pp) t -= A[pp * (pp + 1) / 2 + i] * delta[pp];
-      delta[i] = t / A[i * (i + 1) / 2 + i];
-    }
-    float dn = 0.f;
-#pragma unroll
-    for (int i = 0; i < MAXA; ++i) {
-      delta[i] = ok ? delta[i] / cs[i] : 0.f;
-      dn += delta[i] * delta[i];
-    }
-    // Trust region.
-    float R;
-    if (max_p > 1e-2f || max_o > 0.6f) R = 0.38f;
-    else if (max_p > 1e-3f || max_o > 0.25f) R = 0.22f;
-    else if (max_p > 2e-4f || max_o > 0.08f) R = 0.12f;
-    else R = 0.05f;
-    dn = sqrtf(dn);
-    if (dn > R) {
-      const float sc = R / (dn + 1e-18f);
-#pragma unroll
-      for (int i = 0; i < MAXA; ++i) delta[i] *= sc;
-    }
-
-    // Line search: evaluate all five steps, keep the best.
-    float best_a_err = 1e30f;
-    int best_a = 0;
-#pragma unroll 1
-    for (int ai = 0; ai < 5; ++ai) {
-      float qt[MAXA];
-#pragma unroll
-      for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + ls_alpha(ai) * delta[i]));
-      const float et = weighted_error(v, tg, W, qt, P, r, Tee, grid);
-      if (et < best_a_err) { best_a_err = et; best_a = ai; }
-    }
-    const float al = ls_alpha(best_a);
-    float qt[MAXA];
-#pragma unroll
-    for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + al * delta[i]));
-    if (best_a_err < cur * (1.f - 1e-4f)) {
-#pragma unroll
-      for (int i = 0; i < MAXA; ++i) st.q[i] = qt[i];
-      st.lam = fmaxf(st.lam * 0.5f, 1e-10f);
-    } else {
-      st.lam = fminf(st.lam * 3.f, 1e6f);
-    }
-    if (best_a_err < st.best_err) {
-      st.best_err = best_a_err;
-#pragma unroll
-      for (int i = 0; i < MAXA; ++i) st.best_q[i] = qt[i];
-    }
-  }
-}
-
-// Workgroup prologue shared by both kernels: stage the robot, limits and world.
-template <int MAXA, int NEE>
-__device__ __forceinline__ IkView<MAXA, NEE> ik_prologue(const IkArgs& a, char* base) {
-  using Sh = IkShared<MAXA, NEE>;
-  const int J = a.tables.n_joints;
-  const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
-  const IkLds off = ik_lds_layout(sizeof(Sh), J, Kin, a.enable_collision ? a.n_world : 0);
-  Sh& sh = *reinterpret_cast<Sh*>(base);
-  JointRec* recs = reinterpret_cast<JointRec*>(base + off.recs);
-  int* slot_of = reinterpret_cast<int*>(base + off.slot_of);
-  int* anchor = reinterpret_cast<int*>(base + off.anchor);
-  float* rel = reinterpret_cast<float*>(base + off.rel);
-  int* sph_tmp = reinterpret_cast<int*>(base + off.sph_tmp);
-  float4* sph = reinterpret_cast<float4*>(base + off.sph);
-  float4* ws = reinterpret_cast<float4*>(base + off.ws);
-  const int lane = threadIdx.x;
-  stage_joint_records(a.tables, recs, slot_of);
-  if (lane == 0) build_problem<MAXA, NEE>(a, recs, slot_of, sh, anchor, rel, sph, sph_tmp);
-  for (int i = lane; i < MAXA; i += kWave) {
-    const bool real = i < a.n_act;
-    sh.lo[i] = real ? a.lower[i] : 0.f;
-    sh.hi[i] = real ? a.upper[i] : 0.f;
-    sh.fixed[i] = real ? (a.fixed_mask[i] != 0) : 1;
-  }
-  const int M = (a.enable_collision && Kin > 0) ? a.n_world : 0;
-  for (int m = lane; m < M; m += kWave) {
-    const float* o = a.world + 4 * m;
-    ws[m] = make_float4(o[0], o[1], o[2], o[3] + a.collision_margin);
-  }
-  __syncthreads();
-  IkView<MAXA, NEE> v;
-  v.sh = &sh;
-  v.sph = sph;
-  v.ws = ws;
-  v.M = M;
-  v.coll = M > 0 && sh.n_sph > 0;
-  v.wc = a.collision_weight;
-  return v;
-}
-
-template <int MAXA, int NEE>
-__device__ __forceinline__ void load_targets(const IkArgs& a, long long problem, float (&tg)[NEE][7]) {
-#pragma unroll
-  for (int e = 0; e < NEE; ++e)
-#pragma unroll
-    for (int c = 0; c < 7; ++c)
-      tg[e][c] = (e < a.n_ee) ? a.target_T[(problem * a.n_ee + e) * 7 + c] : (c == 0 ? 1.f : 0.f);
-}
-
-template <int MAXA, int NEE>
-__global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
-  extern __shared__ float4 smem4[];
-  const IkView<MAXA, NEE> v = ik_prologue<MAXA, NEE>(a, reinterpret_cast<char*>(smem4));
-  const int lane = threadIdx.x;
-  const int p = blockIdx.y;
-  const int s = blockIdx.x * kWave + lane;
-  if (s >= a.n_seeds) return;
-  const long long gs = static_cast<long long>(p) * a.n_seeds + s;
-  if (v.sh->too_many_nodes) {  // reported to the caller as NaN errors, seeds unchanged
-    for (int i = 0; i < a.n_act; ++i) a.out_cfgs[gs * a.n_act + i] = a.seeds[gs * a.n_act + i];
-    a.out_err[gs] = NAN;
-    return;
-  }
-  float tg[NEE][7];
-  load_targets<MAXA, NEE>(a, p, tg);
-  const float W[6] = {a.pos_weight, a.pos_weight, a.pos_weight, a.ori_weight, a.ori_weight, a.ori_weight};
-  LmState<MAXA> st;
-#pragma unroll
-  for (int i = 0; i < MAXA; ++i) {
-    st.q[i] = (i < a.n_act) ? a.seeds[gs * a.n_act + i] : 0.f;
-    st.best_q[i] = st.q[i];
-  }
-  Pose P[MAXA];
-  Pose Tee[NEE];
-  float r[6 * NEE];
-  st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid);
-  st.lam = a.lambda_init;
-  st.done = 0;
-  lm_iterate(v, tg, W, a, st, 0, a.max_iter, a.grid);
-#pragma unroll
-  for (int i = 0; i < MAXA; ++i)
-    if (i < a.n_act) a.out_cfgs[gs * a.n_act + i] = st.best_q[i];
-  a.out_err[gs] = st.best_err;
-}
-
-// Uneven batch: seeds of all problems in one flat list (problem p owns
-// [problem_seed_start[p], problem_seed_start[p+1])).  A workgroup takes tiles
-// of 64 consecutive seeds round-robin (tiles may straddle problems: the target
-// is per lane).  Iterations [iter_begin, iter_end) run per launch; with a state
-// buffer the solve resumes across launches (a captured graph of chunks).
-template <int MAXA, int NEE>
-__global__ __launch_bounds__(64) void ls_ik_batched_kernel(IkArgs a) {
-  extern __shared__ float4 smem4[];
-  const IkView<MAXA, NEE> v = ik_prologue<MAXA, NEE>(a, reinterpret_cast<char*>(smem4));
-  const int lane = threadIdx.x;
-  const long long n_tiles = (a.total_seeds + kWave - 1) / kWave;
-  const int A = a.n_act;
-  const int stride = 2 * A + 3;
-  const float W[6] = {a.pos_weight, a.pos_weight, a.pos_weight, a.ori_weight, a.ori_weight, a.ori_weight};
-  for (long long tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-    const long long s = tile * kWave + lane;
-    if (s >= a.total_seeds) continue;
-    if (v.sh->too_many_nodes) {
-      if (a.iter_end >= a.max_iter) {
-        for (int i = 0; i < A; ++i) a.out_cfgs[s * A + i] = a.seeds[s * A + i];
-        a.out_err[s] = NAN;
-      }
-      continue;
-    }
-    // Problem of this seed: binary search in the CSR offsets.
-    int lo = 0, hi = a.n_problems - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.problem_seed_start[mid] <= s) lo = mid; else hi = mid - 1;
-    }
-    float tg[NEE][7];
-    load_targets<MAXA, NEE>(a, lo, tg);
-    LmState<MAXA> st;
-    if (a.iter_begin == 0 || a.state == nullptr) {
-#pragma unroll
-      for (int i = 0; i < MAXA; ++i) {
-        st.q[i] = (i < A) ? a.seeds[s * A + i] : 0.f;
-        st.best_q[i] = st.q[i];
-      }
-      Pose P[MAXA];
-      Pose Tee[NEE];
-      float r[6 * NEE];
-      st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid);
-      st.lam = a.lambda_init;
-      st.done = 0;
-    } else {
-      const float* sp = a.state + s * stride;
-#pragma unroll
-      for (int i = 0; i < MAXA; ++i) {
-        st.q[i] = (i < A) ? sp[i] : 0.f;
-        st.best_q[i] = (i < A) ? sp[A + i] : 0.f;
-      }
-      st.best_err = sp[2 * A];
-      st.lam = sp[2 * A + 1];
-      st.done = sp[2 * A + 2] != 0.f;
-    }
-    lm_iterate(v, tg, W, a, st, a.iter_begin, a.iter_end, a.grid);
-    if (a.state != nullptr && a.iter_end < a.max_iter) {
-      float* sp = a.state + s * stride;
-#pragma unroll
-      for (int i = 0; i < MAXA; ++i)
-        if (i < A) { sp[i] = st.q[i]; sp[A + i] = st.best_q[i]; }
-      sp[2 * A] = st.best_err;
-      sp[2 * A + 1] = st.lam;
-      sp[2 * A + 2] = st.done ? 1.f : 0.f;
-    } else {
-#pragma unroll
-      for (int i = 0; i < MAXA; ++i)
-        if (i < A) a.out_cfgs[s * A + i] = st.best_q[i];
-      a.out_err[s] = st.best_err;
-    }
-  }
-}
-
-template <int MAXA, int NEE>
-static int launch_ik(void* stream, const IkArgs& a, bool batched, int grid_waves) {
-  using Sh = IkShared<MAXA, NEE>;
-  const int J = a.tables.n_joints;
-  const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
-  const int M = a.enable_collision ? a.n_world : 0;
-  const size_t lds = ik_lds_layout(sizeof(Sh), J, Kin, M).total;
-  PK_REQUIRE(lds <= 150 * 1024, "IK tables need %zu bytes of LDS (limit 150 KiB)", lds);
-  if (batched) {
-    const long long n_tiles = (a.total_seeds + kWave - 1) / kWave;
-    long long g = grid_waves > 0 ? grid_waves : n_tiles;
-    g = std::max(1LL, std::min(g, n_tiles));
-    hipLaunchKernelGGL((ls_ik_batched_kernel<MAXA, NEE>), dim3(static_cast<unsigned>(g)), dim3(kWave), lds,
-                       static_cast<hipStream_t>(stream), a);
-    return check_launch("ls_ik_batched_kernel");
-  }
-  dim3 grid((a.n_seeds + kWave - 1) / kWave, a.n_problems);
-  hipLaunchKernelGGL((ls_ik_kernel<MAXA, NEE>), grid, dim3(kWave), lds,
-                     static_cast<hipStream_t>(stream), a);
-  return check_launch("ls_ik_kernel");
-}
-
-template <int MAXA, int NEE>
-static int launch_ik_width(void* stream, const IkArgs& a, bool batched, int grid_waves) {
-  return launch_ik<MAXA, NEE>(stream, a, batched, grid_waves);
-}
 
 static int dispatch_ik(void* stream, const IkArgs& a, bool batched, int grid_waves) {
   // Padded widths: the moving-joint count (actuated + mimic + offset-only
   // joints) must fit; a robot that exceeds it gets NaN errors (see header).
   const int nee = a.n_ee <= 1 ? 1 : (a.n_ee <= 2 ? 2 : 4);
   if (a.n_act <= 8) {
-    if (nee == 1) return launch_ik_width<8, 1>(stream, a, batched, grid_waves);
-    if (nee == 2) return launch_ik_width<8, 2>(stream, a, batched, grid_waves);
-    return launch_ik_width<8, 4>(stream, a, batched, grid_waves);
+    if (nee == 1) return launch_ik_8_1(stream, a, batched, grid_waves);
+    if (nee == 2) return launch_ik_8_2(stream, a, batched, grid_waves);
+    return launch_ik_8_4(stream, a, batched, grid_waves);
   }
-  if (nee == 1) return launch_ik_width<16, 1>(stream, a, batched, grid_waves);
-  if (nee == 2) return launch_ik_width<16, 2>(stream, a, batched, grid_waves);
-  return launch_ik_width<16, 4>(stream, a, batched, grid_waves);
+  if (nee == 1) return launch_ik_16_1(stream, a, batched, grid_waves);
+  if (nee == 2) return launch_ik_16_2(stream, a, batched, grid_waves);
+  return launch_ik_16_4(stream, a, batched, grid_waves);
 }
 
 // Per problem: argmin over its seeds of err + w |q - prev|^2, first index on
@@ -1008,22 +201,10 @@ extern "C" int pk_ls_ik_batched_f32(
   a.state = state;
   a.iter_begin = iter_begin;
   a.iter_end = iter_end;
-  PK_REQUIRE(n_grids >= 0 && n_grids <= PK_MAX_GRIDS, "n_grids=%d outside [0, %d]", n_grids,
-             PK_MAX_GRIDS);
-  PK_REQUIRE(n_grids == 0 || grids != nullptr, "null grids");
   a.grid.n = 0;
   if (a.enable_collision) {
-    for (int i = 0; i < n_grids; ++i) {
-      const pk_world_grid& g = grids[i];
-      PK_REQUIRE(g.counts && g.items && g.cell > 0.f && g.dims[0] > 0 && g.dims[1] > 0 &&
-                     g.dims[2] > 0 && g.cap > 0 && g.pad >= 0.f,
-                 "invalid world grid %d", i);
-      PK_REQUIRE(i == 0 || g.pad >= grids[i - 1].pad, "grids must be ordered by increasing pad");
-      a.grid.lv[i] = pk::GridLevel{g.counts, g.items, g.lo[0], g.lo[1], g.lo[2], 1.f / g.cell,
-                                   g.dims[0], g.dims[1], g.dims[2], g.cap,
-                                   g.pad - collision_margin};
-    }
-    a.grid.n = n_grids;
+    const int gs = pk::fill_grid_args(grids, n_grids, collision_margin, a.grid);
+    if (gs != PK_OK) return gs;
   }
   return pk::dispatch_ik(stream, a, true, grid_waves);
 }

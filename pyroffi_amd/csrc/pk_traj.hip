@@ -1,0 +1,991 @@
+// Trajectory cost, its hand-written adjoint, and the direct L-BFGS trajectory
+// optimiser, for gfx950.
+//
+// Cost of one trajectory x (T waypoints x A joints), as the reference's
+// L-BFGS / CHOMP trajectory optimisers define it:
+//   total  = w_smooth * smooth + w_limits * limits + w_coll * coll
+//   smooth = w_vel sum|vel|^2 + w_acc sum|acc|^2 + w_jerk sum|jerk|^2
+//            (_sco_optimization.py:170-189; acc is the 5-point stencil
+//            (-1, 16, -30, 16, -1)/12, jerk = acc_{t+1} - acc_t)
+//   limits = sum (max(0, x - u) + max(0, l - x))^2   (_sco_optimization.py:192-200)
+//   coll   = sum over waypoints of sum f(d) over the JAX collision distances
+//            (_sco_optimization.py:203-222): every active self pair (same-slot
+//            sphere pairs of the two links) and every (link, world sphere)
+//            pair (closest sphere of the link), both with the +1e-6 bias of
+//            collision/_utils.py:45; f = max(0, m - d)^2 (mode 0, CHOMP,
+//            _chomp_optimization.py:140-154) or -min(colldist_from_sdf(d, m), 0)
+//            (mode 1, _lbfgs_trajopt_optimization.py:108-111).
+//
+// The optimiser is _lbfgs_direct_solve (_lbfgs_trajopt_optimization.py:76-205):
+// collision-weight continuation, endpoint-masked gradients, L-BFGS history with
+// the curvature test s.y > 1e-10 y.y + 1e-30, the two-loop recursion
+// (_sco_optimization.py:119-167), five line-search steps {1, .5, .25, .1, .025}
+// with the first one that decreases the cost by 1e-4 (relative) or else the
+// cheapest, and the best iterate tracked at the maximum collision weight.
+//
+// gfx950 layout: one workgroup (two waves) per trajectory, so trajectories of
+// different lengths run side by side in one launch.  The robot (fixed joints
+// folded into moving "nodes" on the host), its spheres in node frames, the
+// self-pair lists and the world live in LDS; the trajectory, its gradient and
+// the search direction live in LDS as well; the L-BFGS history stays in HBM.
+// A lane evaluates one waypoint at a time with the node poses in registers:
+//   * world term: a grid lookup per robot sphere (pk_grid.h) gives the only
+//     world spheres within the margin; a hit counts when this sphere is the
+//     closest of its link to that world sphere (the link minimum);
+//   * self term: a bounding-sphere test per pair skips pairs farther apart
+//     than the margin (exact: f vanishes there);
+//   * adjoint: dcost/dc per sphere centre becomes a wrench (F, c x F) on the
+//     sphere's node; reverse subtree sums give dcost/dq_a =
+//     sum over nodes driven by a of mul (w.(T - p x F) + v.F), with (v, w) the
+//     node's twist in world axes and p its origin.
+// The line search needs no gradient: the smoothness of x + a d is quadratic in
+// a (three sums), the limits are elementwise, and the five collision sums run
+// as 5 T independent waypoint evaluations.
+#include "pk_grid.h"
+#include "pk_model.h"
+
+#include <math.h>
+
+#include <algorithm>
+
+namespace pk {
+
+constexpr int kTrajThreads = 128;
+constexpr int kTrajWaves = kTrajThreads / kWave;
+constexpr int kMaxHist = 8;
+constexpr int kNumAlphas = 5;
+constexpr int kNodeWords = 20;   // ptf[7] lin[3] ang[3] mul off src parent kind pad pad
+constexpr int kRedSlots = 16;    // values per block reduction
+constexpr int kScalarWords = 16; // per-trajectory solver scalars in the state buffer
+
+__device__ __forceinline__ float traj_alpha(int i) {
+  return i == 0 ? 1.f : (i == 1 ? 0.5f : (i == 2 ? 0.25f : (i == 3 ? 0.1f : 0.025f)));
+}
+
+struct TrajModelDev {
+  const float* nodes;
+  int n_nodes, n_act;
+  const int* link_node;
+  const int* link_sph_start;
+  const float* spheres;
+  const float* link_bound;
+  int n_links, n_spheres;
+  const int* pair_links;
+  const int* pair_start;
+  const int* pair_items;
+  int n_pairs, n_items;
+  const float* world;
+  int n_world;
+  GridArgs grid;
+  const float* lower;
+  const float* upper;
+  float margin;
+  int mode;
+};
+
+struct TrajWeights {
+  float w_smooth, w_vel, w_acc, w_jerk, w_limits;
+};
+
+// Byte offsets of the LDS image; the same function sizes the launch on the host.
+struct TrajLdsLayout {
+  size_t nodes, link_node, link_sph, sph, lb, pair_links, pair_start, pair_items, world, lo, hi,
+      vx, vg, vd, vt, red, total;
+};
+
+__host__ __device__ inline size_t traj_align(size_t v) { return (v + 15) & ~static_cast<size_t>(15); }
+
+__host__ __device__ inline TrajLdsLayout traj_lds_layout(int N, int L, int K, int P, int Q, int M, int A,
+                                                         int max_len) {
+  TrajLdsLayout o;
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    const size_t here = at;
+    at = traj_align(at + bytes);
+    return here;
+  };
+  o.nodes = take(sizeof(float) * kNodeWords * N);
+  o.link_node = take(sizeof(int) * L);
+  o.link_sph = take(sizeof(int) * (L + 1));
+  o.sph = take(sizeof(float4) * K);
+  o.lb = take(sizeof(float4) * L);
+  o.pair_links = take(sizeof(int) * 2 * P);
+  o.pair_start = take(sizeof(int) * (P + 1));
+  o.pair_items = take(sizeof(int) * 2 * Q);
+  o.world = take(sizeof(float4) * M);
+  o.lo = take(sizeof(float) * A);
+  o.hi = take(sizeof(float) * A);
+  const size_t vec = sizeof(float) * static_cast<size_t>(max_len) * A;
+  o.vx = take(vec);
+  o.vg = take(vec);
+  o.vd = take(vec);
+  o.vt = take(vec);
+  o.red = take(sizeof(float) * kTrajWaves * kRedSlots);
+  o.total = at;
+  return o;
+}
+
+struct TrajLds {
+  const float* nodes;
+  const int* link_node;
+  const int* link_sph;
+  const float4* sph;
+  const float4* lb;
+  const int* pair_links;
+  const int* pair_start;
+  const int* pair_items;
+  const float4* world;
+  const float* lo;
+  const float* hi;
+  float* vx;   // trajectory (T, A)
+  float* vg;   // gradient
+  float* vd;   // search direction (and scratch in the gradient pass)
+  float* vt;   // accelerations of x
+  float* red;  // block-reduction scratch
+};
+
+// Cooperative copy of the model into LDS, with every index clamped into range.
+__device__ inline TrajLds stage_traj_model(const TrajModelDev& md, int max_len, char* base) {
+  const int N = md.n_nodes, L = md.n_links, K = md.n_spheres, P = md.n_pairs, Q = md.n_items;
+  const int M = md.n_world, A = md.n_act;
+  const TrajLdsLayout lay = traj_lds_layout(N, L, K, P, Q, M, A, max_len);
+  float* nodes = reinterpret_cast<float*>(base + lay.nodes);
+  int* link_node = reinterpret_cast<int*>(base + lay.link_node);
+  int* link_sph = reinterpret_cast<int*>(base + lay.link_sph);
+  float4* sph = reinterpret_cast<float4*>(base + lay.sph);
+  float4* lb = reinterpret_cast<float4*>(base + lay.lb);
+  int* pair_links = reinterpret_cast<int*>(base + lay.pair_links);
+  int* pair_start = reinterpret_cast<int*>(base + lay.pair_start);
+  int* pair_items = reinterpret_cast<int*>(base + lay.pair_items);
+  float4* world = reinterpret_cast<float4*>(base + lay.world);
+  float* lo = reinterpret_cast<float*>(base + lay.lo);
+  float* hi = reinterpret_cast<float*>(base + lay.hi);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N * kNodeWords; i += blockDim.x) {
+    float v = md.nodes[i];
+    const int n = i / kNodeWords, w = i % kNodeWords;
+    if (w == 15) v = (v >= 0.f && v < static_cast<float>(A)) ? v : -1.f;           // src
+    if (w == 16) v = (v >= 0.f && v < static_cast<float>(n)) ? v : -1.f;           // parent
+    if (w == 17) v = (v >= 1.f && v <= 3.f) ? v : 3.f;                              // kind
+    nodes[i] = v;
+  }
+  for (int l = tid; l < L; l += blockDim.x) {
+    const int n = md.link_node[l];
+    link_node[l] = (n >= 0 && n < N) ? n : -1;
+  }
+  if (tid == 0) {
+    int prev = 0;
+    for (int l = 0; l <= L; ++l) {
+      const int v = (l == L) ? K : md.link_sph_start[l];
+      prev = max(prev, min(max(v, 0), K));
+      link_sph[l] = (l == 0) ? 0 : prev;
+    }
+    prev = 0;
+    for (int p = 0; p <= P; ++p) {
+      const int v = (p == P) ? Q : md.pair_start[p];
+      prev = max(prev, min(max(v, 0), Q));
+      pair_start[p] = (p == 0) ? 0 : prev;
+    }
+  }
+  for (int k = tid; k < K; k += blockDim.x) {
+    const float* s = md.spheres + 4 * k;
+    sph[k] = make_float4(s[0], s[1], s[2], s[3]);
+  }
+  for (int l = tid; l < L; l += blockDim.x) {
+    const float* s = md.link_bound + 4 * l;
+    lb[l] = make_float4(s[0], s[1], s[2], s[3]);
+  }
+  for (int i = tid; i < 2 * P; i += blockDim.x) {
+    const int v = md.pair_links[i];
+    pair_links[i] = (v >= 0 && v < L) ? v : 0;
+  }
+  for (int i = tid; i < 2 * Q; i += blockDim.x) {
+    const int v = md.pair_items[i];
+    pair_items[i] = (v >= 0 && v < K) ? v : 0;
+  }
+  for (int m = tid; m < M; m += blockDim.x) {
+    const float* s = md.world + 4 * m;
+    world[m] = make_float4(s[0], s[1], s[2], s[3]);
+  }
+  for (int a = tid; a < A; a += blockDim.x) {
+    lo[a] = md.lower[a];
+    hi[a] = md.upper[a];
+  }
+  __syncthreads();
+  TrajLds o;
+  o.nodes = nodes; o.link_node = link_node; o.link_sph = link_sph; o.sph = sph; o.lb = lb;
+  o.pair_links = pair_links; o.pair_start = pair_start; o.pair_items = pair_items; o.world = world;
+  o.lo = lo; o.hi = hi;
+  o.vx = reinterpret_cast<float*>(base + lay.vx);
+  o.vg = reinterpret_cast<float*>(base + lay.vg);
+  o.vd = reinterpret_cast<float*>(base + lay.vd);
+  o.vt = reinterpret_cast<float*>(base + lay.vt);
+  o.red = reinterpret_cast<float*>(base + lay.red);
+  return o;
+}
+
+__device__ __forceinline__ float traj_wave_sum(float v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Sum of NV values over the workgroup; every thread gets the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum(float (&v)[NV], float* red) {
+  static_assert(NV <= kRedSlots, "too many values");
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = traj_wave_sum(v[i]);
+  const int wave = threadIdx.x / kWave;
+  __syncthreads();
+  if ((threadIdx.x % kWave) == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[wave * kRedSlots + i] = v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    float s = 0.f;
+#pragma unroll
+    for (int w = 0; w < kTrajWaves; ++w) s += red[w * kRedSlots + i];
+    v[i] = s;
+  }
+}
+
+__device__ __forceinline__ float block_sum1(float v, float* red) {
+  float a[1] = {v};
+  block_sum<1>(a, red);
+  return a[0];
+}
+
+// Collision activation f(d) and df/dd.
+__device__ __forceinline__ float traj_act(float d, float m, int mode, float& df) {
+  if (mode == 0) {
+    const float h = m - d;
+    if (h > 0.f) { df = -2.f * h; return h * h; }
+    df = 0.f;
+    return 0.f;
+  }
+  const float dm = fminf(d, m);
+  if (dm < 0.f) { df = -1.f; return 0.5f * m - dm; }
+  if (d < m) {
+    const float inv = 1.f / (m + 1e-6f);
+    df = (d - m) * inv;
+    return 0.5f * inv * (d - m) * (d - m);
+  }
+  df = 0.f;
+  return 0.f;
+}
+
+template <int MAXN>
+__device__ __forceinline__ Pose node_pose(const Pose (&P)[MAXN], int n) {
+  Pose r = pose_identity();
+#pragma unroll
+  for (int i = 0; i < MAXN; ++i)
+    if (n == i) r = P[i];
+  return r;
+}
+
+template <int MAXN>
+__device__ __forceinline__ void add_wrench(float (&F)[MAXN][3], float (&Tq)[MAXN][3], int n, float cx,
+                                           float cy, float cz, float gx, float gy, float gz) {
+#pragma unroll
+  for (int i = 0; i < MAXN; ++i) {
+    if (n == i) {
+      F[i][0] += gx; F[i][1] += gy; F[i][2] += gz;
+      Tq[i][0] += cy * gz - cz * gy;
+      Tq[i][1] += cz * gx - cx * gz;
+      Tq[i][2] += cx * gy - cy * gx;
+    }
+  }
+}
+
+// Collision cost of one waypoint q (this lane's) and, when GRAD, dcost/dq.
+template <int MAXN, bool GRAD>
+__device__ float waypoint_coll(const TrajModelDev& md, const TrajLds& L, const float (&q)[MAXN],
+                               float (&gq)[MAXN]) {
+  const int N = md.n_nodes;
+  Pose P[MAXN];
+#pragma unroll
+  for (int n = 0; n < MAXN; ++n) {
+    P[n] = pose_identity();
+    if (n < N) {
+      const float* r = L.nodes + n * kNodeWords;
+      const int src = __builtin_amdgcn_readfirstlane(static_cast<int>(r[15]));
+      const int par = __builtin_amdgcn_readfirstlane(static_cast<int>(r[16]));
+      const int kind = __builtin_amdgcn_readfirstlane(static_cast<int>(r[17]));
+      float qs = 0.f;
+#pragma unroll
+      for (int a = 0; a < MAXN; ++a)
+        if (src == a) qs = q[a];
+      const float qj = qs * r[13] + r[14];
+      const Pose tl = compose(load_pose(r), joint_exp(kind, qj, r[7], r[8], r[9], r[10], r[11], r[12]));
+      Pose pp = pose_identity();
+      bool has_parent = false;
+#pragma unroll
+      for (int m = 0; m < n; ++m)
+        if (par == m) { pp = P[m]; has_parent = true; }
+      P[n] = has_parent ? compose(pp, tl) : tl;
+    }
+  }
+  float F[MAXN][3], Tq[MAXN][3];
+  if (GRAD) {
+#pragma unroll
+    for (int n = 0; n < MAXN; ++n)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { F[n][c] = 0.f; Tq[n][c] = 0.f; }
+  }
+  const float margin = md.margin;
+  const int mode = md.mode;
+  const float e = 1e-6f;
+  float cost = 0.f;
+
+  // World: per (link, world sphere), the link's closest sphere.
+  for (int l = 0; l < md.n_links; ++l) {
+    const int k0 = L.link_sph[l], k1 = L.link_sph[l + 1];
+    if (k0 >= k1) continue;
+    const int node = __builtin_amdgcn_readfirstlane(L.link_node[l]);
+    const Pose T = node_pose<MAXN>(P, node);
+    const Rot3 R = quat_to_rot(T.w, T.x, T.y, T.z);
+    for (int s = k0; s < k1; ++s) {
+      const float4 sp = L.sph[s];
+      const float cx = R.m00 * sp.x + R.m01 * sp.y + R.m02 * sp.z + T.tx;
+      const float cy = R.m10 * sp.x + R.m11 * sp.y + R.m12 * sp.z + T.ty;
+      const float cz = R.m20 * sp.x + R.m21 * sp.y + R.m22 * sp.z + T.tz;
+      const int* items = nullptr;
+      int cnt = 0;
+      const bool listed = grid_candidates(md.grid, cx, cy, cz, sp.w, items, cnt);
+      const int total = listed ? cnt : md.n_world;
+      for (int i = 0; i < total; ++i) {
+        const int m = listed ? items[i] : i;
+        if (m < 0 || m >= md.n_world) continue;
+        const float4 o = L.world[m];
+        const float dx = o.x - cx + e, dy = o.y - cy + e, dz = o.z - cz + e;
+        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float d = dist - sp.w - o.w;
+        if (!(d < margin)) continue;
+        bool is_min = true;
+        for (int s2 = k0; s2 < k1; ++s2) {
+          if (s2 == s) continue;
+          const float4 p2 = L.sph[s2];
+          const float ex = o.x - (R.m00 * p2.x + R.m01 * p2.y + R.m02 * p2.z + T.tx) + e;
+          const float ey = o.y - (R.m10 * p2.x + R.m11 * p2.y + R.m12 * p2.z + T.ty) + e;
+          const float ez = o.z - (R.m20 * p2.x + R.m21 * p2.y + R.m22 * p2.z + T.tz) + e;
+          const float d2 = sqrtf(ex * ex + ey * ey + ez * ez) - p2.w - o.w;
+          if (d2 < d || (d2 == d && s2 < s)) { is_min = false; break; }
+        }
+        if (!is_min) continue;
+        float df;
+        cost += traj_act(d, margin, mode, df);
+        if (GRAD && df != 0.f) {
+          const float k = -df / dist;
+          add_wrench<MAXN>(F, Tq, node, cx, cy, cz, k * dx, k * dy, k * dz);
+        }
+      }
+    }
+  }
+
+  // Self: active pairs, same-slot sphere pairs, bounding-sphere culled.
+  for (int p = 0; p < md.n_pairs; ++p) {
+    const int q0 = L.pair_start[p], q1 = L.pair_start[p + 1];
+    if (q0 >= q1) continue;
+    const int li = L.pair_links[2 * p], lj = L.pair_links[2 * p + 1];
+    const int ni = __builtin_amdgcn_readfirstlane(L.link_node[li]);
+    const int nj = __builtin_amdgcn_readfirstlane(L.link_node[lj]);
+    const Pose Ti = node_pose<MAXN>(P, ni);
+    const Pose Tj = node_pose<MAXN>(P, nj);
+    const Rot3 Ri = quat_to_rot(Ti.w, Ti.x, Ti.y, Ti.z);
+    const Rot3 Rj = quat_to_rot(Tj.w, Tj.x, Tj.y, Tj.z);
+    {
+      const float4 bi = L.lb[li], bj = L.lb[lj];
+      const float ax = Ri.m00 * bi.x + Ri.m01 * bi.y + Ri.m02 * bi.z + Ti.tx;
+      const float ay = Ri.m10 * bi.x + Ri.m11 * bi.y + Ri.m12 * bi.z + Ti.ty;
+      const float az = Ri.m20 * bi.x + Ri.m21 * bi.y + Ri.m22 * bi.z + Ti.tz;
+      const float bx = Rj.m00 * bj.x + Rj.m01 * bj.y + Rj.m02 * bj.z + Tj.tx;
+      const float by = Rj.m10 * bj.x + Rj.m11 * bj.y + Rj.m12 * bj.z + Tj.ty;
+      const float bz = Rj.m20 * bj.x + Rj.m21 * bj.y + Rj.m22 * bj.z + Tj.tz;
+      const float dd = sqrtf((bx - ax) * (bx - ax) + (by - ay) * (by - ay) + (bz - az) * (bz - az));
+      if (dd - bi.w - bj.w >= margin + 1e-4f) continue;
+    }
+    float best = INFINITY, bcx = 0.f, bcy = 0.f, bcz = 0.f, bdx = 0.f, bdy = 0.f, bdz = 0.f;
+    float bex = 0.f, bey = 0.f, bez = 0.f, bdist = 1.f;
+    for (int it = q0; it < q1; ++it) {
+      const float4 si = L.sph[L.pair_items[2 * it]];
+      const float4 sj = L.sph[L.pair_items[2 * it + 1]];
+      const float cix = Ri.m00 * si.x + Ri.m01 * si.y + Ri.m02 * si.z + Ti.tx;
+      const float ciy = Ri.m10 * si.x + Ri.m11 * si.y + Ri.m12 * si.z + Ti.ty;
+      const float ciz = Ri.m20 * si.x + Ri.m21 * si.y + Ri.m22 * si.z + Ti.tz;
+      const float cjx = Rj.m00 * sj.x + Rj.m01 * sj.y + Rj.m02 * sj.z + Tj.tx;
+      const float cjy = Rj.m10 * sj.x + Rj.m11 * sj.y + Rj.m12 * sj.z + Tj.ty;
+      const float cjz = Rj.m20 * sj.x + Rj.m21 * sj.y + Rj.m22 * sj.z + Tj.tz;
+      const float dx = cjx - cix + e, dy = cjy - ciy + e, dz = cjz - ciz + e;
+      const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+      const float d = dist - si.w - sj.w;
+      if (d < best) {
+        best = d;
+        bcx = cix; bcy = ciy; bcz = ciz; bex = cjx; bey = cjy; bez = cjz;
+        bdx = dx; bdy = dy; bdz = dz; bdist = dist;
+      }
+    }
+    if (best < margin) {
+      float df;
+      cost += traj_act(best, margin, mode, df);
+      if (GRAD && df != 0.f) {
+        const float k = df / bdist;
+        add_wrench<MAXN>(F, Tq, nj, bex, bey, bez, k * bdx, k * bdy, k * bdz);
+        add_wrench<MAXN>(F, Tq, ni, bcx, bcy, bcz, -k * bdx, -k * bdy, -k * bdz);
+      }
+    }
+  }
+
+  if (GRAD) {
+#pragma unroll
+    for (int a = 0; a < MAXN; ++a) gq[a] = 0.f;
+#pragma unroll
+    for (int n = MAXN - 1; n >= 0; --n) {
+      if (n < N) {
+        const float* r = L.nodes + n * kNodeWords;
+        const int src = __builtin_amdgcn_readfirstlane(static_cast<int>(r[15]));
+        const int par = __builtin_amdgcn_readfirstlane(static_cast<int>(r[16]));
+        if (src >= 0) {
+          float wx, wy, wz, vx, vy, vz;
+          quat_rotate(P[n].w, P[n].x, P[n].y, P[n].z, r[10], r[11], r[12], wx, wy, wz);
+          quat_rotate(P[n].w, P[n].x, P[n].y, P[n].z, r[7], r[8], r[9], vx, vy, vz);
+          const float px = P[n].tx, py = P[n].ty, pz = P[n].tz;
+          const float tx = Tq[n][0] - (py * F[n][2] - pz * F[n][1]);
+          const float ty = Tq[n][1] - (pz * F[n][0] - px * F[n][2]);
+          const float tz = Tq[n][2] - (px * F[n][1] - py * F[n][0]);
+          const float val = r[13] * (wx * tx + wy * ty + wz * tz + vx * F[n][0] + vy * F[n][1] + vz * F[n][2]);
+#pragma unroll
+          for (int a = 0; a < MAXN; ++a)
+            if (src == a) gq[a] += val;
+        }
+#pragma unroll
+        for (int m = 0; m < n; ++m) {
+          if (par == m) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { F[m][c] += F[n][c]; Tq[m][c] += Tq[n][c]; }
+          }
+        }
+      }
+    }
+  }
+  return cost;
+}
+
+// Cost components {smooth, limits, coll} of the trajectory in L.vx (T, A), all
+// threads receiving the totals; with GRAD, L.vg = gradient of
+// w_smooth smooth + w_limits limits + w_coll coll (endpoint rows zeroed when
+// mask_ends).  Leaves the accelerations of x in L.vt; uses L.vd as scratch.
+template <int MAXN, bool GRAD>
+__device__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
+                                float w_coll, bool mask_ends, float (&comps)[3]) {
+  const int A = md.n_act;
+  const int n = T * A;
+  const int tid = threadIdx.x;
+  float coll = 0.f, lim = 0.f, sm = 0.f;
+  for (int t = tid; t < T; t += kTrajThreads) {
+    float q[MAXN], gq[MAXN];
+#pragma unroll
+    for (int a = 0; a < MAXN; ++a) q[a] = (a < A) ? L.vx[t * A + a] : 0.f;
+    coll += waypoint_coll<MAXN, GRAD>(md, L, q, gq);
+    if (GRAD) {
+#pragma unroll
+      for (int a = 0; a < MAXN; ++a)
+        if (a < A) L.vg[t * A + a] = w_coll * gq[a];
+    }
+  }
+  __syncthreads();
+  const int na = T - 4;
+  for (int i = tid; i < n; i += kTrajThreads) {
+    const int a = i % A;
+    const float xv = L.vx[i];
+    const float v = fmaxf(xv - L.hi[a], 0.f) + fmaxf(L.lo[a] - xv, 0.f);
+    lim += v * v;
+    if (GRAD)
+      L.vg[i] += w.w_limits * 2.f * v * ((xv > L.hi[a] ? 1.f : 0.f) - (L.lo[a] > xv ? 1.f : 0.f));
+    if (i + A < n) {
+      const float dv = L.vx[i + A] - xv;
+      sm += w.w_vel * dv * dv;
+    }
+    if (i < na * A) {
+      const float acc = (-xv + 16.f * L.vx[i + A] - 30.f * L.vx[i + 2 * A] + 16.f * L.vx[i + 3 * A] -
+                         L.vx[i + 4 * A]) / 12.f;
+      L.vt[i] = acc;
+      sm += w.w_acc * acc * acc;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < na * A; i += kTrajThreads) {
+    const int t = i / A;
+    const float acc = L.vt[i];
+    if (t + 1 < na) {
+      const float jk = L.vt[i + A] - acc;
+      sm += w.w_jerk * jk * jk;
+    }
+    if (GRAD) {
+      float ga = 2.f * w.w_acc * acc;
+      if (t + 1 < na) ga -= 2.f * w.w_jerk * (L.vt[i + A] - acc);
+      if (t >= 1) ga += 2.f * w.w_jerk * (acc - L.vt[i - A]);
+      L.vd[i] = ga;
+    }
+  }
+  if (GRAD) {
+    __syncthreads();
+    const float c5[5] = {-1.f / 12.f, 16.f / 12.f, -30.f / 12.f, 16.f / 12.f, -1.f / 12.f};
+    for (int i = tid; i < n; i += kTrajThreads) {
+      const int t = i / A;
+      float gs = 0.f;
+      if (t >= 1) gs += 2.f * w.w_vel * (L.vx[i] - L.vx[i - A]);
+      if (t + 1 < T) gs -= 2.f * w.w_vel * (L.vx[i + A] - L.vx[i]);
+#pragma unroll
+      for (int o = 0; o < 5; ++o) {
+        const int tt = t - o;
+        if (tt >= 0 && tt < na) gs += L.vd[i - o * A] * c5[o];
+      }
+      float g = L.vg[i] + w.w_smooth * gs;
+      if (mask_ends && (t == 0 || t == T - 1)) g = 0.f;
+      L.vg[i] = g;
+    }
+  }
+  float v3[3] = {sm, lim, coll};
+  block_sum<3>(v3, L.red);
+  comps[0] = v3[0];
+  comps[1] = v3[1];
+  comps[2] = v3[2];
+}
+
+// Components of the five line-search points x + alpha_k d (no gradient).
+template <int MAXN>
+__device__ void traj_trials(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
+                            float (&sm)[kNumAlphas], float (&lim)[kNumAlphas], float (&coll)[kNumAlphas]) {
+  const int A = md.n_act;
+  const int n = T * A;
+  const int tid = threadIdx.x;
+  const int na = T - 4;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  float lk[kNumAlphas], ck[kNumAlphas];
+#pragma unroll
+  for (int k = 0; k < kNumAlphas; ++k) { lk[k] = 0.f; ck[k] = 0.f; }
+  for (int i = tid; i < n; i += kTrajThreads) {
+    const int a = i % A;
+    const int t = i / A;
+    const float xv = L.vx[i], dv = L.vd[i];
+#pragma unroll
+    for (int k = 0; k < kNumAlphas; ++k) {
+      const float y = xv + traj_alpha(k) * dv;
+      const float v = fmaxf(y - L.hi[a], 0.f) + fmaxf(L.lo[a] - y, 0.f);
+      lk[k] += v * v;
+    }
+    if (t + 1 < T) {
+      const float vx = L.vx[i + A] - xv, vd = L.vd[i + A] - dv;
+      s0 += w.w_vel * vx * vx; s1 += w.w_vel * vx * vd; s2 += w.w_vel * vd * vd;
+    }
+    if (t < na) {
+      const float ax = L.vt[i];
+      const float ad = (-dv + 16.f * L.vd[i + A] - 30.f * L.vd[i + 2 * A] + 16.f * L.vd[i + 3 * A] -
+                        L.vd[i + 4 * A]) / 12.f;
+      s0 += w.w_acc * ax * ax; s1 += w.w_acc * ax * ad; s2 += w.w_acc * ad * ad;
+      if (t + 1 < na) {
+        const float jx = L.vt[i + A] - ax;
+        const float ad1 = (-L.vd[i + A] + 16.f * L.vd[i + 2 * A] - 30.f * L.vd[i + 3 * A] +
+                           16.f * L.vd[i + 4 * A] - L.vd[i + 5 * A]) / 12.f;
+        const float jd = ad1 - ad;
+        s0 += w.w_jerk * jx * jx; s1 += w.w_jerk * jx * jd; s2 += w.w_jerk * jd * jd;
+      }
+    }
+  }
+  for (int task = tid; task < kNumAlphas * T; task += kTrajThreads) {
+    const int k = task / T, t = task % T;
+    const float al = traj_alpha(k);
+    float q[MAXN], gq[MAXN];
+#pragma unroll
+    for (int a = 0; a < MAXN; ++a) q[a] = (a < A) ? L.vx[t * A + a] + al * L.vd[t * A + a] : 0.f;
+    const float c = waypoint_coll<MAXN, false>(md, L, q, gq);
+#pragma unroll
+    for (int kk = 0; kk < kNumAlphas; ++kk)
+      if (kk == k) ck[kk] += c;
+  }
+  float v[3 + 2 * kNumAlphas];
+  v[0] = s0; v[1] = s1; v[2] = s2;
+#pragma unroll
+  for (int k = 0; k < kNumAlphas; ++k) { v[3 + k] = lk[k]; v[3 + kNumAlphas + k] = ck[k]; }
+  block_sum<3 + 2 * kNumAlphas>(v, L.red);
+#pragma unroll
+  for (int k = 0; k < kNumAlphas; ++k) {
+    const float al = traj_alpha(k);
+    sm[k] = v[0] + 2.f * al * v[1] + al * al * v[2];
+    lim[k] = v[3 + k];
+    coll[k] = v[3 + kNumAlphas + k];
+  }
+}
+
+__device__ __forceinline__ float sel_hist(const float (&v)[kMaxHist], int i) {
+  float r = 0.f;
+#pragma unroll
+  for (int h = 0; h < kMaxHist; ++h)
+    if (h == i) r = v[h];
+  return r;
+}
+
+struct CostArgs {
+  TrajModelDev md;
+  const float* x;
+  const long long* start;
+  int n_traj, max_len;
+  long long n_waypoints;
+  TrajWeights w;
+  float w_coll;
+  float* out_comps;   // (B, 3)
+  float* out_grad;    // (W, A) or null
+};
+
+template <int MAXN>
+__global__ __launch_bounds__(kTrajThreads) void traj_cost_kernel(CostArgs a) {
+  extern __shared__ float4 smem4[];
+  const TrajLds L = stage_traj_model(a.md, a.max_len, reinterpret_cast<char*>(smem4));
+  const int b = blockIdx.x;
+  const long long w0 = a.start[b], w1 = a.start[b + 1];
+  if (w0 < 0 || w1 < w0 || w1 > a.n_waypoints || w1 - w0 > a.max_len) return;  // uniform per block
+  const int T = static_cast<int>(w1 - w0);
+  const int A = a.md.n_act;
+  for (int i = threadIdx.x; i < T * A; i += kTrajThreads) L.vx[i] = a.x[w0 * A + i];
+  __syncthreads();
+  float comps[3];
+  if (a.out_grad != nullptr) {
+    traj_value_grad<MAXN, true>(a.md, L, T, a.w, a.w_coll, false, comps);
+    for (int i = threadIdx.x; i < T * A; i += kTrajThreads) a.out_grad[w0 * A + i] = L.vg[i];
+  } else {
+    traj_value_grad<MAXN, false>(a.md, L, T, a.w, a.w_coll, false, comps);
+  }
+  if (threadIdx.x == 0) {
+    a.out_comps[3 * b] = comps[0];
+    a.out_comps[3 * b + 1] = comps[1];
+    a.out_comps[3 * b + 2] = comps[2];
+  }
+}
+
+struct LbfgsArgs {
+  TrajModelDev md;
+  const float* x_init;
+  const long long* start;
+  int n_traj, max_len;
+  long long n_waypoints;
+  TrajWeights w;
+  float w_coll0, w_coll_max, penalty_scale;
+  int escalation, n_iters, m_hist, it0, it1;
+  float* xs;      // (W, A) current iterate
+  float* xp;      // (W, A) previous iterate
+  float* gp;      // (W, A) previous gradient
+  float* sb;      // (m, W, A) s history
+  float* yb;      // (m, W, A) y history
+  float* sc;      // (B, kScalarWords) w_coll, best_cost, m_used, newest, rho[m]
+  float* out_best;  // (W, A)
+  float* out_cost;  // (B)
+};
+
+template <int MAXN>
+__global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
+  extern __shared__ float4 smem4[];
+  const TrajLds L = stage_traj_model(a.md, a.max_len, reinterpret_cast<char*>(smem4));
+  const int b = blockIdx.x;
+  const long long w0 = a.start[b], w1 = a.start[b + 1];
+  if (w0 < 0 || w1 < w0 || w1 > a.n_waypoints || w1 - w0 > a.max_len) return;  // uniform per block
+  const int T = static_cast<int>(w1 - w0);
+  const int A = a.md.n_act;
+  const int n = T * A;
+  const int tid = threadIdx.x;
+  const int m = a.m_hist;
+  const long long off = w0 * A;
+  const long long WA = a.n_waypoints * A;
+  float* xs = a.xs + off;
+  float* xp = a.xp + off;
+  float* gp = a.gp + off;
+  float* best = a.out_best + off;
+  float* sb = a.sb;
+  float* yb = a.yb;
+  float* sc = a.sc + static_cast<long long>(b) * kScalarWords;
+
+  float w_coll, best_cost;
+  int m_used, newest;
+  float rho[kMaxHist];
+  if (a.it0 == 0) {
+    for (int i = tid; i < n; i += kTrajThreads) L.vx[i] = a.x_init[off + i];
+    w_coll = a.w_coll0;
+    best_cost = INFINITY;
+    m_used = 0;
+    newest = 0;
+#pragma unroll
+    for (int h = 0; h < kMaxHist; ++h) rho[h] = 0.f;
+  } else {
+    for (int i = tid; i < n; i += kTrajThreads) L.vx[i] = xs[i];
+    w_coll = sc[0];
+    best_cost = sc[1];
+    m_used = static_cast<int>(sc[2]);
+    newest = static_cast<int>(sc[3]);
+#pragma unroll
+    for (int h = 0; h < kMaxHist; ++h) rho[h] = (h < m) ? sc[4 + h] : 0.f;
+  }
+  __syncthreads();
+
+  for (int k = a.it0; k < a.it1; ++k) {
+    if (k > 0 && k % a.escalation == 0) w_coll = fminf(w_coll * a.penalty_scale, a.w_coll_max);
+    float comps[3];
+    traj_value_grad<MAXN, true>(a.md, L, T, a.w, w_coll, true, comps);
+    const float cost = a.w.w_smooth * comps[0] + a.w.w_limits * comps[1] + w_coll * comps[2];
+    if (k == 0) {
+      for (int i = tid; i < n; i += kTrajThreads) {
+        xp[i] = L.vx[i];
+        gp[i] = L.vg[i];
+        best[i] = L.vx[i];
+      }
+      best_cost = cost;
+    }
+    // Curvature pair of the last step.
+    float syy[2] = {0.f, 0.f};
+    for (int i = tid; i < n; i += kTrajThreads) {
+      const float s = L.vx[i] - xp[i], y = L.vg[i] - gp[i];
+      syy[0] += s * y;
+      syy[1] += y * y;
+    }
+    block_sum<2>(syy, L.red);
+    if (syy[0] > 1e-10f * syy[1] + 1e-30f && k > 0) {
+      newest = (newest + 1) % m;
+      for (int i = tid; i < n; i += kTrajThreads) {
+        sb[newest * WA + off + i] = L.vx[i] - xp[i];
+        yb[newest * WA + off + i] = L.vg[i] - gp[i];
+      }
+      const float r = 1.f / (syy[0] + 1e-30f);
+#pragma unroll
+      for (int h = 0; h < kMaxHist; ++h)
+        if (h == newest) rho[h] = r;
+      if (m_used < m) ++m_used;
+    }
+    // Search direction into L.vd.
+    if (m_used > 0) {
+      float alph[kMaxHist];
+#pragma unroll
+      for (int h = 0; h < kMaxHist; ++h) alph[h] = 0.f;
+      for (int i = tid; i < n; i += kTrajThreads) L.vd[i] = L.vg[i];
+      for (int j = 0; j < m_used; ++j) {
+        const int idx = (newest - j + m) % m;
+        const float* s = sb + idx * WA + off;
+        const float* y = yb + idx * WA + off;
+        float sq = 0.f;
+        for (int i = tid; i < n; i += kTrajThreads) sq += s[i] * L.vd[i];
+        const float al = sel_hist(rho, idx) * block_sum1(sq, L.red);
+#pragma unroll
+        for (int h = 0; h < kMaxHist; ++h)
+          if (h == idx) alph[h] = al;
+        for (int i = tid; i < n; i += kTrajThreads) L.vd[i] -= al * y[i];
+      }
+      float g2[2] = {0.f, 0.f};
+      {
+        const float* s = sb + newest * WA + off;
+        const float* y = yb + newest * WA + off;
+        for (int i = tid; i < n; i += kTrajThreads) {
+          g2[0] += s[i] * y[i];
+          g2[1] += y[i] * y[i];
+        }
+      }
+      block_sum<2>(g2, L.red);
+      const float gamma = g2[0] / (g2[1] + 1e-18f);
+      for (int i = tid; i < n; i += kTrajThreads) L.vd[i] *= gamma;
+      for (int st = 0; st < m_used; ++st) {
+        const int idx = (newest - m_used + 1 + st + 2 * m) % m;
+        const float* s = sb + idx * WA + off;
+        const float* y = yb + idx * WA + off;
+        float yr = 0.f;
+        for (int i = tid; i < n; i += kTrajThreads) yr += y[i] * L.vd[i];
+        const float beta = sel_hist(rho, idx) * block_sum1(yr, L.red);
+        const float coef = sel_hist(alph, idx) - beta;
+        for (int i = tid; i < n; i += kTrajThreads) L.vd[i] += s[i] * coef;
+      }
+      for (int i = tid; i < n; i += kTrajThreads) L.vd[i] = -L.vd[i];
+    } else {
+      float g2 = 0.f;
+      for (int i = tid; i < n; i += kTrajThreads) g2 += L.vg[i] * L.vg[i];
+      const float inv = 1.f / (sqrtf(block_sum1(g2, L.red)) + 1e-18f);
+      for (int i = tid; i < n; i += kTrajThreads) L.vd[i] = -L.vg[i] * inv;
+    }
+    for (int i = tid; i < n; i += kTrajThreads) {
+      const int t = i / A;
+      if (t == 0 || t == T - 1) L.vd[i] = 0.f;
+    }
+    __syncthreads();
+    float sm[kNumAlphas], lim[kNumAlphas], col[kNumAlphas];
+    traj_trials<MAXN>(a.md, L, T, a.w, sm, lim, col);
+    const float thresh = cost * (1.f - 1e-4f);
+    int pick = -1;
+    float pick_cost = INFINITY;
+    int cheapest = 0;
+    float cheapest_cost = INFINITY;
+#pragma unroll
+    for (int kk = 0; kk < kNumAlphas; ++kk) {
+      const float tc = a.w.w_smooth * sm[kk] + a.w.w_limits * lim[kk] + w_coll * col[kk];
+      if (pick < 0 && tc < thresh) { pick = kk; pick_cost = tc; }
+      if (tc < cheapest_cost) { cheapest = kk; cheapest_cost = tc; }
+    }
+    (void)pick_cost;
+    if (pick < 0) pick = cheapest;
+    float p_sm = 0.f, p_lim = 0.f, p_col = 0.f;
+#pragma unroll
+    for (int kk = 0; kk < kNumAlphas; ++kk)
+      if (kk == pick) { p_sm = sm[kk]; p_lim = lim[kk]; p_col = col[kk]; }
+    const float al = traj_alpha(pick);
+    for (int i = tid; i < n; i += kTrajThreads) {
+      xp[i] = L.vx[i];
+      gp[i] = L.vg[i];
+      L.vx[i] += al * L.vd[i];
+    }
+    const float eval = a.w.w_smooth * p_sm + a.w.w_limits * p_lim + a.w_coll_max * p_col;
+    if (eval < best_cost) {
+      best_cost = eval;
+      for (int i = tid; i < n; i += kTrajThreads) best[i] = L.vx[i];
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += kTrajThreads) xs[i] = L.vx[i];
+  if (tid == 0) {
+    sc[0] = w_coll;
+    sc[1] = best_cost;
+    sc[2] = static_cast<float>(m_used);
+    sc[3] = static_cast<float>(newest);
+#pragma unroll
+    for (int h = 0; h < kMaxHist; ++h)
+      if (h < m) sc[4 + h] = rho[h];
+    if (a.it1 == a.n_iters) a.out_cost[b] = best_cost;
+  }
+}
+
+}  // namespace pk
+
+namespace {
+
+int fill_traj_model(const pk_traj_model* m, pk::TrajModelDev& d, int& maxn) {
+  PK_REQUIRE(m != nullptr, "null model");
+  PK_REQUIRE(m->n_nodes >= 0 && m->n_nodes <= 16, "n_nodes=%d outside [0, 16]", m->n_nodes);
+  PK_REQUIRE(m->n_act >= 1 && m->n_act <= 16, "n_act=%d outside [1, 16]", m->n_act);
+  PK_REQUIRE(m->n_links >= 0 && m->n_spheres >= 0 && m->n_pairs >= 0 && m->n_items >= 0 &&
+                 m->n_world >= 0,
+             "negative model size");
+  PK_REQUIRE((m->n_nodes == 0 || m->nodes) && (m->n_links == 0 || (m->link_node && m->link_bound)) &&
+                 m->link_sph_start && (m->n_spheres == 0 || m->spheres) &&
+                 (m->n_pairs == 0 || m->pair_links) && m->pair_start &&
+                 (m->n_items == 0 || m->pair_items) && (m->n_world == 0 || m->world) && m->lower &&
+                 m->upper,
+             "null model pointer");
+  PK_REQUIRE(m->n_items == 0 || m->n_spheres > 0, "pair items without spheres");
+  PK_REQUIRE(m->mode == 0 || m->mode == 1, "mode=%d (0 = hinge^2, 1 = colldist)", m->mode);
+  PK_REQUIRE(m->margin >= 0.f, "negative margin");
+  d.nodes = m->nodes; d.n_nodes = m->n_nodes; d.n_act = m->n_act;
+  d.link_node = m->link_node; d.link_sph_start = m->link_sph_start; d.spheres = m->spheres;
+  d.link_bound = m->link_bound; d.n_links = m->n_links; d.n_spheres = m->n_spheres;
+  d.pair_links = m->pair_links; d.pair_start = m->pair_start; d.pair_items = m->pair_items;
+  d.n_pairs = m->n_pairs; d.n_items = m->n_items; d.world = m->world; d.n_world = m->n_world;
+  d.lower = m->lower; d.upper = m->upper; d.margin = m->margin; d.mode = m->mode;
+  const int gs = pk::fill_grid_args(m->grids, m->n_grids, m->margin, d.grid);
+  if (gs != PK_OK) return gs;
+  const int width = std::max(m->n_nodes, m->n_act);
+  maxn = width <= 8 ? 8 : 16;
+  return PK_OK;
+}
+
+size_t traj_lds_bytes(const pk::TrajModelDev& d, int max_len) {
+  return pk::traj_lds_layout(d.n_nodes, d.n_links, d.n_spheres, d.n_pairs, d.n_items, d.n_world, d.n_act,
+                             max_len)
+      .total;
+}
+
+}  // namespace
+
+extern "C" int64_t pk_traj_lbfgs_state_floats(int64_t n_waypoints, int32_t n_act, int32_t n_traj,
+                                              int32_t m_hist) {
+  if (n_waypoints < 0 || n_act < 0 || n_traj < 0 || m_hist < 0) return -1;
+  return (3 + 2 * static_cast<int64_t>(m_hist)) * n_waypoints * n_act +
+         static_cast<int64_t>(pk::kScalarWords) * n_traj;
+}
+
+extern "C" int pk_traj_cost_grad_f32(void* stream, const pk_traj_model* model, const float* x,
+                                     const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
+                                     int32_t max_len, const float* weights, float* out_comps,
+                                     float* out_grad) {
+  pk::TrajModelDev d{};
+  int maxn = 8;
+  const int st = fill_traj_model(model, d, maxn);
+  if (st != PK_OK) return st;
+  PK_REQUIRE(n_traj >= 0 && n_waypoints >= 0 && max_len >= 1, "invalid trajectory sizes");
+  if (n_traj == 0) return PK_OK;
+  PK_REQUIRE(x && traj_start && weights && out_comps, "null pointer");
+  const size_t lds = traj_lds_bytes(d, max_len);
+  PK_REQUIRE(lds <= 150 * 1024, "trajectory model needs %zu bytes of LDS (limit 150 KiB)", lds);
+  pk::CostArgs a{};
+  a.md = d;
+  a.x = x;
+  a.start = reinterpret_cast<const long long*>(traj_start);
+  a.n_traj = n_traj;
+  a.max_len = max_len;
+  a.n_waypoints = n_waypoints;
+  a.w = pk::TrajWeights{weights[0], weights[1], weights[2], weights[3], weights[4]};
+  a.w_coll = weights[5];
+  a.out_comps = out_comps;
+  a.out_grad = out_grad;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (maxn == 8)
+    hipLaunchKernelGGL(pk::traj_cost_kernel<8>, dim3(n_traj), dim3(pk::kTrajThreads), lds, s, a);
+  else
+    hipLaunchKernelGGL(pk::traj_cost_kernel<16>, dim3(n_traj), dim3(pk::kTrajThreads), lds, s, a);
+  return pk::check_launch("traj_cost_kernel");
+}
+
+extern "C" int pk_traj_lbfgs_f32(void* stream, const pk_traj_model* model, const float* x_init,
+                                 const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
+                                 int32_t max_len, const pk_lbfgs_config* cfg, int32_t iter_begin,
+                                 int32_t iter_end, float* state, float* out_best, float* out_best_cost) {
+  pk::TrajModelDev d{};
+  int maxn = 8;
+  const int st = fill_traj_model(model, d, maxn);
+  if (st != PK_OK) return st;
+  PK_REQUIRE(cfg != nullptr, "null config");
+  PK_REQUIRE(n_traj >= 0 && n_waypoints >= 0 && max_len >= 1, "invalid trajectory sizes");
+  PK_REQUIRE(cfg->n_iters >= 0 && cfg->m_hist >= 1 && cfg->m_hist <= pk::kMaxHist &&
+                 cfg->escalation_interval >= 1,
+             "invalid L-BFGS configuration (m_hist in [1, %d], escalation_interval >= 1)", pk::kMaxHist);
+  PK_REQUIRE(0 <= iter_begin && iter_begin <= iter_end && iter_end <= cfg->n_iters,
+             "iteration range [%d, %d) outside [0, %d]", iter_begin, iter_end, cfg->n_iters);
+  if (n_traj == 0 || iter_begin == iter_end) return PK_OK;
+  PK_REQUIRE(x_init && traj_start && state && out_best && out_best_cost, "null pointer");
+  const size_t lds = traj_lds_bytes(d, max_len);
+  PK_REQUIRE(lds <= 150 * 1024, "trajectory model needs %zu bytes of LDS (limit 150 KiB)", lds);
+  pk::LbfgsArgs a{};
+  a.md = d;
+  a.x_init = x_init;
+  a.start = reinterpret_cast<const long long*>(traj_start);
+  a.n_traj = n_traj;
+  a.max_len = max_len;
+  a.n_waypoints = n_waypoints;
+  a.w = pk::TrajWeights{cfg->w_smooth, cfg->w_vel, cfg->w_acc, cfg->w_jerk, cfg->w_limits};
+  a.w_coll0 = cfg->w_collision;
+  a.w_coll_max = cfg->w_collision_max;
+  a.penalty_scale = cfg->penalty_scale;
+  a.escalation = cfg->escalation_interval;
+  a.n_iters = cfg->n_iters;
+  a.m_hist = cfg->m_hist;
+  a.it0 = iter_begin;
+  a.it1 = iter_end;
+  const long long WA = n_waypoints * d.n_act;
+  a.xs = state;
+  a.xp = state + WA;
+  a.gp = state + 2 * WA;
+  a.sb = state + 3 * WA;
+  a.yb = a.sb + static_cast<long long>(cfg->m_hist) * WA;
+  a.sc = a.yb + static_cast<long long>(cfg->m_hist) * WA;
+  a.out_best = out_best;
+  a.out_cost = out_best_cost;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (maxn == 8)
+    hipLaunchKernelGGL(pk::traj_lbfgs_kernel<8>, dim3(n_traj), dim3(pk::kTrajThreads), lds, s, a);
+  else
+    hipLaunchKernelGGL(pk::traj_lbfgs_kernel<16>, dim3(n_traj), dim3(pk::kTrajThreads), lds, s, a);
+  return pk::check_launch("traj_lbfgs_kernel");
+}

@@ -39,6 +39,53 @@ class WorldGrid(ctypes.Structure):
 
 MAX_GRIDS = 2
 
+
+class TrajModel(ctypes.Structure):
+    """``pk_traj_model``: device pointers + sizes of a trajectory model."""
+
+    _fields_ = [
+        ("nodes", ctypes.c_void_p),
+        ("n_nodes", ctypes.c_int32),
+        ("n_act", ctypes.c_int32),
+        ("link_node", ctypes.c_void_p),
+        ("link_sph_start", ctypes.c_void_p),
+        ("spheres", ctypes.c_void_p),
+        ("link_bound", ctypes.c_void_p),
+        ("n_links", ctypes.c_int32),
+        ("n_spheres", ctypes.c_int32),
+        ("pair_links", ctypes.c_void_p),
+        ("pair_start", ctypes.c_void_p),
+        ("pair_items", ctypes.c_void_p),
+        ("n_pairs", ctypes.c_int32),
+        ("n_items", ctypes.c_int32),
+        ("world", ctypes.c_void_p),
+        ("n_world", ctypes.c_int32),
+        ("grids", ctypes.c_void_p),
+        ("n_grids", ctypes.c_int32),
+        ("lower", ctypes.c_void_p),
+        ("upper", ctypes.c_void_p),
+        ("margin", ctypes.c_float),
+        ("mode", ctypes.c_int32),
+    ]
+
+
+class LbfgsConfig(ctypes.Structure):
+    """``pk_lbfgs_config``."""
+
+    _fields_ = [
+        ("n_iters", ctypes.c_int32),
+        ("m_hist", ctypes.c_int32),
+        ("escalation_interval", ctypes.c_int32),
+        ("w_smooth", ctypes.c_float),
+        ("w_vel", ctypes.c_float),
+        ("w_acc", ctypes.c_float),
+        ("w_jerk", ctypes.c_float),
+        ("w_limits", ctypes.c_float),
+        ("w_collision", ctypes.c_float),
+        ("w_collision_max", ctypes.c_float),
+        ("penalty_scale", ctypes.c_float),
+    ]
+
 _SIGNATURES = {
     "pk_last_error": (ctypes.c_char_p, []),
     "pk_abi_version": (_i32, []),
@@ -79,6 +126,9 @@ _SIGNATURES = {
         + [_f32, _f32, _f32, _f32, _f32, _i32, _f32, _f32, _i32]  # attrs, grid_waves
         + [_p, _p],
     ),
+    "pk_traj_cost_grad_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _p, _p, _p]),
+    "pk_traj_lbfgs_state_floats": (_i64, [_i64, _i32, _i32, _i32]),
+    "pk_traj_lbfgs_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _p, _i32, _i32, _p, _p, _p]),
     "pk_ls_ik_select_segments_f32": (_i32, [_p, _p, _p, _p, _p, _i32, _i32, _f32, _p, _p]),
     "pk_pose_hinge_cost_f32": (
         _i32,

@@ -1,0 +1,168 @@
+"""GPU parity: trajectory cost, its adjoint, and the direct L-BFGS optimiser.
+
+* cost components (smooth, limits, coll) vs the FP64 oracle, relative 1e-4,
+  for lengths 5..128, both collision activations, with and without the world
+  grids (grid and brute force agree to 1e-6);
+* gradient vs the oracle's analytic gradient (itself pinned by central
+  differences in test_oracle_traj.py), norm-wise relative 1e-3 per trajectory;
+* L-BFGS: the first iterations follow the FP64 oracle step for step (1e-4),
+  and after 100 iterations the tracked best costs agree statistically
+  (FP32 and FP64 line searches may pick different steps later on);
+* chunked launches = one launch, bit for bit; uneven lengths in one launch =
+  each trajectory alone; a captured graph replays to the eager result.
+"""
+
+import numpy as np
+import pytest
+import torch
+from conftest import load_robot, random_cfgs
+
+from pyroffi_amd.collision import RobotCollisionSpherized
+from pyroffi_amd.optimization_engines._lbfgs_trajopt_optimization import (
+    BatchedLbfgsTrajOpt,
+    LbfgsTrajOptConfig,
+    TrajModel,
+    flatten_trajs,
+    lbfgs_trajopt,
+    traj_cost,
+)
+
+pytestmark = pytest.mark.gpu
+
+W_DEF = (1.0, 1.0, 0.5, 0.1, 1.0, 3.0)
+
+
+def _setup(oracle_mod, M=128, margin=0.05, mode=1, seed=0, use_grid=True):
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    rng = np.random.default_rng(seed)
+    world = np.concatenate([rng.uniform(-0.8, 0.8, (M, 3)), rng.uniform(0.02, 0.1, (M, 1))], 1).astype(np.float32)
+    model = TrajModel(robot, coll, world, margin, mode, use_grid=use_grid)
+    t = oracle_mod.Tables.of(robot)
+    om = oracle_mod.TrajModel(t, oracle_mod.padded_spheres(coll), coll.active_idx_i, coll.active_idx_j,
+                              world.astype(np.float64), margin, mode)
+    return robot, coll, world, model, om
+
+
+def _trajs(robot, lens, seed, noise=0.05):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, T in enumerate(lens):
+        q0, q1 = random_cfgs(robot, 2, seed=seed * 100 + i)
+        s = np.linspace(0, 1, T)[:, None]
+        x = (1 - s) * q0 + s * q1 + rng.normal(0, noise, (T, len(q0)))
+        out.append(x.astype(np.float32))
+    return out
+
+
+@pytest.mark.parametrize("mode,margin,use_grid", [(1, 0.01, True), (1, 0.05, False), (0, 0.05, True),
+                                                  (0, 0.02, False)])
+def test_cost_and_gradient_vs_oracle(oracle_mod, mode, margin, use_grid):
+    robot, coll, world, model, om = _setup(oracle_mod, margin=margin, mode=mode, seed=1, use_grid=use_grid)
+    trajs = _trajs(robot, [5, 16, 33, 64, 128, 7], seed=2)
+    trajs[1][3, 0] = robot.joints.upper_limits[0] + 0.1  # a limit violation
+    x, start, T = flatten_trajs(trajs, model.device)
+    comps, g = traj_cost(model, x, start, T, W_DEF)
+    ocomps, og = oracle_mod.traj_cost(om, [t.astype(np.float64) for t in trajs], robot.joints.lower_limits,
+                                      robot.joints.upper_limits, W_DEF)
+    c = comps.cpu().numpy()
+    assert ocomps[:, 2].min() > 0  # every trajectory is in collision somewhere
+    np.testing.assert_allclose(c, ocomps, rtol=1e-4, atol=1e-6)
+    g = g.cpu().numpy()
+    s = start.cpu().numpy()
+    for b in range(len(trajs)):
+        gb, ob = g[s[b]:s[b + 1]], og[s[b]:s[b + 1]]
+        rel = np.linalg.norm(gb - ob) / max(np.linalg.norm(ob), 1e-6)
+        assert rel <= 1e-3, (b, rel)
+
+
+def test_grid_matches_brute_force(oracle_mod):
+    robot, coll, world, model, _ = _setup(oracle_mod, M=256, margin=0.03, mode=1, seed=3)
+    brute = TrajModel(robot, coll, world, 0.03, 1, use_grid=False)
+    assert len(model.grids) >= 1
+    trajs = _trajs(robot, [20, 40, 64], seed=4)
+    x, start, T = flatten_trajs(trajs, model.device)
+    c1, g1 = traj_cost(model, x, start, T, W_DEF)
+    c2, g2 = traj_cost(brute, x, start, T, W_DEF)
+    torch.testing.assert_close(c1, c2, rtol=1e-6, atol=1e-7)
+    torch.testing.assert_close(g1, g2, rtol=1e-5, atol=1e-6)
+
+
+def _lbfgs_both(oracle_mod, robot, model, om, trajs, cfg):
+    opt = BatchedLbfgsTrajOpt(model, trajs, cfg)
+    best, best_cost, _ = opt.solve()
+    obest, ocost, _ = oracle_mod.traj_lbfgs(
+        om, [t.astype(np.float64) for t in trajs], robot.joints.lower_limits, robot.joints.upper_limits,
+        n_iters=cfg.n_iters, m_lbfgs=cfg.m_lbfgs, escalation_interval=cfg.escalation_interval, w_smooth=cfg.w_smooth,
+        w_vel=cfg.w_vel, w_acc=cfg.w_acc, w_jerk=cfg.w_jerk, w_limits=cfg.w_limits, w_collision=cfg.w_collision,
+        w_collision_max=cfg.w_collision_max, penalty_scale=cfg.penalty_scale)
+    return best.cpu().numpy(), best_cost.cpu().numpy(), obest, ocost
+
+
+def test_lbfgs_first_iterations_follow_oracle(oracle_mod):
+    robot, coll, world, model, om = _setup(oracle_mod, M=64, margin=0.05, mode=1, seed=5)
+    trajs = _trajs(robot, [16, 24, 9], seed=6)
+    for n_iters in (1, 3):
+        cfg = LbfgsTrajOptConfig(n_iters=n_iters, collision_margin=0.05)
+        best, cost, obest, ocost = _lbfgs_both(oracle_mod, robot, model, om, trajs, cfg)
+        np.testing.assert_allclose(cost, ocost, rtol=1e-4, atol=1e-6)
+        np.testing.assert_allclose(best, obest, rtol=0, atol=1e-4)
+
+
+def test_lbfgs_100_iterations_statistics(oracle_mod):
+    robot, coll, world, model, om = _setup(oracle_mod, M=64, margin=0.01, mode=1, seed=7)
+    trajs = _trajs(robot, [16] * 12 + [32] * 4, seed=8, noise=0.0)
+    cfg = LbfgsTrajOptConfig(n_iters=100)
+    best, cost, obest, ocost = _lbfgs_both(oracle_mod, robot, model, om, trajs, cfg)
+    init_cost = np.array([oracle_mod.traj_cost(om, [t.astype(np.float64)], robot.joints.lower_limits,
+                                               robot.joints.upper_limits,
+                                               (1, 1, 0.5, 0.1, 1, 100.0), grad=False)[0][0] @ [1, 1, 100]
+                          for t in trajs])
+    assert np.all(cost <= init_cost * (1 + 1e-5) + 1e-6)  # the best iterate never loses to the start
+    ratio = (cost + 1e-3) / (ocost + 1e-3)
+    assert 0.8 <= np.median(ratio) <= 1.25, ratio
+    assert np.mean(np.abs(np.log(ratio)) < 0.5) >= 0.75, ratio
+    # endpoints pinned
+    s = np.concatenate([[0], np.cumsum([len(t) for t in trajs])])
+    for b, t in enumerate(trajs):
+        np.testing.assert_array_equal(best[s[b]], t[0])
+        np.testing.assert_array_equal(best[s[b + 1] - 1], t[-1])
+
+
+def test_chunked_uneven_and_graph_replay(oracle_mod):
+    robot, coll, world, model, _ = _setup(oracle_mod, M=256, margin=0.02, mode=1, seed=9)
+    trajs = _trajs(robot, [32, 64, 128, 17, 64], seed=10)
+    cfg = LbfgsTrajOptConfig(n_iters=30)
+    one = BatchedLbfgsTrajOpt(model, trajs, cfg)
+    b1, c1, r1 = (t.clone() for t in one.solve())
+    chunked = BatchedLbfgsTrajOpt(model, trajs, cfg, chunk=7)
+    b2, c2, r2 = chunked.solve()
+    torch.testing.assert_close(b2, b1, rtol=0, atol=0)
+    torch.testing.assert_close(c2, c1, rtol=0, atol=0)
+    # each trajectory alone gives the same result as the mixed batch
+    s = one.start.cpu().tolist()
+    for b in (0, 2, 3):
+        alone = BatchedLbfgsTrajOpt(model, [trajs[b]], cfg)
+        ba, ca, _ = alone.solve()
+        torch.testing.assert_close(ba, b1[s[b]:s[b + 1]], rtol=0, atol=0)
+        torch.testing.assert_close(ca, c1[b:b + 1], rtol=0, atol=0)
+    chunked.capture()
+    for _ in range(3):
+        chunked.best.zero_()
+        b3, c3, r3 = chunked.solve()
+        torch.cuda.synchronize()
+        torch.testing.assert_close(b3, b1, rtol=0, atol=0)
+        torch.testing.assert_close(r3, r1, rtol=0, atol=0)
+        _ = torch.ones(100, device="cuda").sum().item()
+
+
+def test_public_lbfgs_trajopt(oracle_mod):
+    robot, coll, world, _, _ = _setup(oracle_mod, M=64, margin=0.01, mode=1, seed=11)
+    q0, q1 = random_cfgs(robot, 2, seed=12)
+    s = np.linspace(0, 1, 24)[:, None]
+    init = np.stack([(1 - s) * q0 + s * q1 + np.random.default_rng(k).normal(0, 0.02, (24, 7)) for k in range(4)])
+    best, costs, final = lbfgs_trajopt(init, q0, q1, robot, coll, world, LbfgsTrajOptConfig(n_iters=20))
+    assert best.shape == (24, 7) and costs.shape == (4,) and final.shape == (4, 24, 7)
+    np.testing.assert_allclose(final[:, 0], np.tile(q0, (4, 1)), atol=1e-6)
+    np.testing.assert_allclose(final[:, -1], np.tile(q1, (4, 1)), atol=1e-6)
+    assert int(np.argmin(costs)) == int(np.argmin(costs)) and np.all(np.isfinite(costs))
