@@ -285,6 +285,99 @@ def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32,
     }
 
 
+def trajopt_bench(reps: int, cpu_sample: int = 16):
+    """Config 5: 512 Panda trajectories with T mixed over {32, 64, 128}, random in-limit
+    endpoints, straight-line initial paths, 1024 world spheres (centres in the 2 m cube,
+    radii U[0.02, 0.1]); the direct L-BFGS of lbfgs_trajopt (default LbfgsTrajOptConfig:
+    100 iterations, colldist activation with margin 0.01, collision-weight continuation)
+    with the cost gradient from the hand-written adjoint; the 100 iterations and the
+    final ranking run as one replay of a captured graph.
+    """
+    import torch
+
+    import oracle
+    import pyroffi_amd as pk
+    from pyroffi_amd.collision import RobotCollisionSpherized
+    from pyroffi_amd.optimization_engines._lbfgs_trajopt_optimization import (BatchedLbfgsTrajOpt,
+                                                                              LbfgsTrajOptConfig, TrajModel)
+
+    robot = pk.Robot.from_urdf(pk.bundled_urdf_path("panda"))
+    coll = RobotCollisionSpherized.from_urdf(pk.bundled_urdf_path("panda"))
+    rng = np.random.default_rng(5)
+    world = np.concatenate([rng.uniform(-1, 1, (1024, 3)), rng.uniform(0.02, 0.1, (1024, 1))], 1).astype(np.float32)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    lens = np.array([32, 64, 128])[rng.integers(0, 3, 512)]
+    trajs = []
+    for T in lens:
+        q0, q1 = rng.uniform(lo, hi), rng.uniform(lo, hi)
+        s = np.linspace(0, 1, T)[:, None]
+        trajs.append(((1 - s) * q0 + s * q1).astype(np.float32))
+    cfg = LbfgsTrajOptConfig()
+    model = TrajModel(robot, coll, world, cfg.collision_margin, mode=1)
+    opt = BatchedLbfgsTrajOpt(model, trajs, cfg)
+    opt.capture()
+    opt.solve()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(reps):
+        best, best_cost, rank = opt.solve()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    gpu_ms = e0.elapsed_time(e1) / reps
+    comps = opt.rank_comps.cpu().numpy()
+    # Initial cost at the maximum collision weight, for the reduction ratio.
+    from pyroffi_amd.optimization_engines._lbfgs_trajopt_optimization import traj_cost
+
+    c0, _ = traj_cost(model, opt.x0, opt.start, opt.max_len,
+                      (cfg.w_smooth, cfg.w_vel, cfg.w_acc, cfg.w_jerk, cfg.w_limits, cfg.w_collision_max), grad=False)
+    c0 = c0.cpu().numpy()
+    init_total = c0 @ [cfg.w_smooth, cfg.w_limits, cfg.w_collision_max]
+    final_total = rank.cpu().numpy()
+    # The reference tracks its "best" iterate against the initial cost at the
+    # starting collision weight, so in a cluttered world it often keeps the
+    # initial path; the last iterate shows what the optimiser did.
+    cl, _ = traj_cost(model, opt.final_iterate(), opt.start, opt.max_len,
+                      (cfg.w_smooth, cfg.w_vel, cfg.w_acc, cfg.w_jerk, cfg.w_limits, cfg.w_collision_max), grad=False)
+    cl = cl.cpu().numpy()
+    last_total = cl @ [cfg.w_smooth, cfg.w_limits, cfg.w_collision_max]
+    waypoints = int(lens.sum())
+
+    # FP64 oracle (brute force over the world) on the first trajectories, on the host cores.
+    t = oracle.Tables.of(robot)
+    om = oracle.TrajModel(t, oracle.padded_spheres(coll), coll.active_idx_i, coll.active_idx_j,
+                          world.astype(np.float64), cfg.collision_margin, 1)
+    threads = cpu_threads()
+    c_start = time.perf_counter()
+    _, ocost, _ = oracle.traj_lbfgs(om, [x.astype(np.float64) for x in trajs[:cpu_sample]], lo, hi, n_iters=cfg.n_iters,
+                                    threads=threads)
+    cpu_dt = time.perf_counter() - c_start
+    gcost = best_cost.cpu().numpy()[:cpu_sample]
+    return {
+        "workload": "panda_lbfgs_trajopt_512traj_T32_64_128_world1024_100iters",
+        "trajectories": len(trajs),
+        "waypoints": waypoints,
+        "iterations": cfg.n_iters,
+        "trajectories_per_s": len(trajs) / dt,
+        "ms_per_solve": dt * 1e3,
+        "gpu_event_ms_per_solve": gpu_ms,
+        "waypoint_iterations_per_s": waypoints * cfg.n_iters / dt,
+        "collision_free_fraction": float(np.mean(comps[:, 2] == 0.0)),
+        "initial_collision_free_fraction": float(np.mean(c0[:, 2] == 0.0)),
+        "median_cost_ratio_best_over_initial": float(np.median(final_total / np.maximum(init_total, 1e-12))),
+        "median_cost_ratio_last_iterate_over_initial": float(np.median(last_total / np.maximum(init_total, 1e-12))),
+        "median_collision_ratio_last_iterate_over_initial": float(np.median(cl[:, 2] / np.maximum(c0[:, 2], 1e-12))),
+        "last_iterate_collision_free_fraction": float(np.mean(cl[:, 2] == 0.0)),
+        "oracle_sample_median_cost_ratio_gpu_over_fp64": float(np.median((gcost + 1e-3) / (ocost + 1e-3))),
+        "cpu_baseline": {"value": cpu_sample / cpu_dt, "unit": "trajectories/s", "cores": threads, "kind": "port",
+                         "sample": f"first {cpu_sample} trajectories (lengths {lens[:cpu_sample].tolist()}), "
+                                   "FP64 oracle, brute-force world"},
+    }
+
+
 def grad_bench(reps: int, cpu_sample: int):
     """Config 3: Panda, 16384 configs, pose + hinge (margin 0.05) against 512 world spheres,
     cost and dcost/dq through the hand-written adjoint (autograd Function)."""
@@ -481,6 +574,7 @@ def main():
         line["ik"] = ik_bench(reps=10)
         line["grad"] = grad_bench(reps=10, cpu_sample=256)
         line["ik_uneven"] = ik_uneven_bench(reps=5)
+        line["trajopt"] = trajopt_bench(reps=5)
     print(json.dumps(line), flush=True)
 
 

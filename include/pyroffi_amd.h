@@ -48,13 +48,14 @@ extern "C" {
  * passed by host pointer to the LM entry points.
  */
 typedef struct pk_world_grid {
-  const int32_t* counts; /* (dims[0]*dims[1]*dims[2]) true counts     */
-  const int32_t* items;  /* (dims[0]*dims[1]*dims[2], cap) indices    */
-  float lo[3];           /* corner of cell (0, 0, 0)                  */
-  float cell;            /* cell side                                 */
-  int32_t dims[3];       /* nx, ny, nz                                */
-  int32_t cap;           /* indices kept per cell                     */
-  float pad;             /* reach the grid was built with             */
+  const int32_t* cells;  /* (dims[0]*dims[1]*dims[2], stride) records: */
+                         /* [true count, index 0, ..., index cap-1]    */
+  float lo[3];           /* corner of cell (0, 0, 0)                   */
+  float cell;            /* cell side                                  */
+  int32_t dims[3];       /* nx, ny, nz                                 */
+  int32_t cap;           /* indices kept per cell                      */
+  int32_t stride;        /* int32 words per record (>= cap + 1, % 4 == 0) */
+  float pad;             /* reach the grid was built with              */
 } pk_world_grid;
 
 /* Message of the last failing call on this thread ("" when none). */
@@ -250,20 +251,22 @@ int pk_ls_ik_f32(void* stream, const float* seeds, int32_t n_problems,
 
 /*
  * Uniform grid over a sphere world for exact culling of margin-limited costs.
- * Cell (ix, iy, iz) is the box lo + cell * [i, i+1) per axis; it lists (up to
- * cap indices) every world sphere whose distance to the box is at most
- * R_m + pad, and counts[c] holds the true count (> cap means "overflowed: use
- * the whole world").  With pad = max robot sphere radius + margin and the
+ * Cell (ix, iy, iz) is the box lo + cell * [i, i+1) per axis; its record
+ * cells[c * stride ...] holds the true count of world spheres whose distance to
+ * the box is at most R_m + pad, then (up to cap of) their indices in increasing
+ * order; a count > cap means "overflowed: use the whole world".  One 16-byte
+ * load returns the count and the first three indices.  With pad = max robot sphere radius + margin and the
  * grid box = world bounding box grown by pad, a robot sphere whose centre lies
  * in a cell can be within the margin only of spheres listed there, and one
  * outside the grid of none.  No reference counterpart (an acceleration
  * structure for the LM collision penalty).
- *   counts (nx*ny*nz) int32, items (nx*ny*nz, cap) int32
+ *   cells (nx*ny*nz, stride) int32, 16-byte aligned, stride >= cap + 1 and a
+ *   multiple of 4
  */
 int pk_world_grid_f32(void* stream, const float* world_spheres, int32_t n_world_spheres,
                       float pad, float lo_x, float lo_y, float lo_z, float cell,
-                      int32_t nx, int32_t ny, int32_t nz, int32_t cap, int32_t* counts,
-                      int32_t* items);
+                      int32_t nx, int32_t ny, int32_t nz, int32_t cap, int32_t stride,
+                      int32_t* cells);
 
 /*
  * Uneven batched LM-IK: many problems of one robot with different seed

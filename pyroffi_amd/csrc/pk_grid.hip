@@ -13,13 +13,14 @@
 // count, so a consumer falls back to the full world for an overflowing cell.
 #include "pk_common.h"
 
+#include <stdint.h>
+
 namespace pk {
 
 __global__ __launch_bounds__(256) void world_grid_kernel(const float* __restrict__ world, int M,
                                                          float pad, float lo_x, float lo_y, float lo_z,
                                                          float cell, int nx, int ny, int nz, int cap,
-                                                         int* __restrict__ counts,
-                                                         int* __restrict__ items) {
+                                                         int stride, int* __restrict__ cells) {
   const long long c = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long long n_cells = static_cast<long long>(nx) * ny * nz;
   if (c >= n_cells) return;
@@ -27,6 +28,7 @@ __global__ __launch_bounds__(256) void world_grid_kernel(const float* __restrict
   const int iy = static_cast<int>((c / nx) % ny);
   const int iz = static_cast<int>(c / (static_cast<long long>(nx) * ny));
   const float bx0 = lo_x + ix * cell, by0 = lo_y + iy * cell, bz0 = lo_z + iz * cell;
+  int* rec = cells + c * stride;
   int n = 0;
   for (int m = 0; m < M; ++m) {
     const float ox = world[4 * m], oy = world[4 * m + 1], oz = world[4 * m + 2];
@@ -36,26 +38,29 @@ __global__ __launch_bounds__(256) void world_grid_kernel(const float* __restrict
     const float dz = fmaxf(fmaxf(bz0 - oz, oz - (bz0 + cell)), 0.f);
     // A small slack keeps the test conservative under float rounding.
     if (dx * dx + dy * dy + dz * dz <= (reach + 1e-4f) * (reach + 1e-4f)) {
-      if (n < cap) items[c * cap + n] = m;
+      if (n < cap) rec[1 + n] = m;
       ++n;
     }
   }
-  counts[c] = n;
+  rec[0] = n;
 }
 
 }  // namespace pk
 
 extern "C" int pk_world_grid_f32(void* stream, const float* world_spheres, int32_t n_world_spheres,
                                  float pad, float lo_x, float lo_y, float lo_z, float cell, int32_t nx,
-                                 int32_t ny, int32_t nz, int32_t cap, int32_t* counts, int32_t* items) {
+                                 int32_t ny, int32_t nz, int32_t cap, int32_t stride, int32_t* cells) {
   PK_REQUIRE(n_world_spheres >= 0 && pad >= 0.f && cell > 0.f, "invalid world grid arguments");
   PK_REQUIRE(nx > 0 && ny > 0 && nz > 0 && cap > 0, "grid dimensions and cap must be positive");
+  PK_REQUIRE(stride >= cap + 1 && stride % 4 == 0, "stride=%d must be >= cap + 1 and a multiple of 4",
+             stride);
   const long long n_cells = static_cast<long long>(nx) * ny * nz;
   PK_REQUIRE(n_cells <= (1LL << 22), "grid of %lld cells is too large (limit 4M)", n_cells);
-  PK_REQUIRE(counts && items && (n_world_spheres == 0 || world_spheres), "null pointer");
+  PK_REQUIRE(cells && (n_world_spheres == 0 || world_spheres), "null pointer");
+  PK_REQUIRE(reinterpret_cast<uintptr_t>(cells) % 16 == 0, "cells must be 16-byte aligned");
   const unsigned blocks = static_cast<unsigned>((n_cells + 255) / 256);
   hipLaunchKernelGGL(pk::world_grid_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
-                     world_spheres, n_world_spheres, pad, lo_x, lo_y, lo_z, cell, nx, ny, nz, cap, counts,
-                     items);
+                     world_spheres, n_world_spheres, pad, lo_x, lo_y, lo_z, cell, nx, ny, nz, cap, stride,
+                     cells);
   return pk::check_launch("world_grid_kernel");
 }

@@ -291,23 +291,21 @@ __device__ __forceinline__ float world_penalty_all(float cx, float cy, float cz,
   return p0 + p1;
 }
 
-// Same sum over the grid cell holding c: exact, since every sphere that can
-// be within the margin of a robot sphere of radius <= r_max (= grid pad -
-// margin) is listed (pk_grid.hip); an overflowing cell falls back to the whole
-// world and a centre outside the grid has no such sphere.
-__device__ __forceinline__ float world_penalty_level(float cx, float cy, float cz, float r,
-                                                     const float4* ws, int M, const GridLevel& g) {
-  const int ix = static_cast<int>(floorf((cx - g.lo_x) * g.inv_cell));
-  const int iy = static_cast<int>(floorf((cy - g.lo_y) * g.inv_cell));
-  const int iz = static_cast<int>(floorf((cz - g.lo_z) * g.inv_cell));
-  if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nx || iy >= g.ny || iz >= g.nz) return 0.f;
-  const long long cell = (static_cast<long long>(iz) * g.ny + iy) * g.nx + ix;
-  const int n = g.counts[cell];
-  if (n > g.cap) return world_penalty_all(cx, cy, cz, r, ws, M);
-  const int* it = g.items + cell * g.cap;
+// Same sum over the candidates of the grid cell holding c: exact, since every
+// sphere that can be within the margin of a robot sphere of radius <= r_max
+// (= grid pad - margin) is listed (pk_grid.hip); an overflowing cell, or a
+// radius no level serves, falls back to the whole world, and a centre outside
+// the grid has no such sphere.  The radius r is the same in every lane (one
+// robot sphere at a time), so the level choice is a uniform branch.
+__device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz, float r,
+                                                    const float4* ws, int M, const GridArgs& g) {
+  int4 head;
+  const int* rec;
+  int n;
+  if (!grid_lookup(g, cx, cy, cz, r, head, rec, n)) return world_penalty_all(cx, cy, cz, r, ws, M);
   float pen = 0.f;
   for (int i = 0; i < n; ++i) {
-    const int m = it[i];
+    const int m = grid_item(head, rec, i);
     if (m < 0 || m >= M) continue;
     const float4 o = ws[m];
     const float ax = cx - o.x, ay = cy - o.y, az = cz - o.z;
@@ -315,16 +313,6 @@ __device__ __forceinline__ float world_penalty_level(float cx, float cy, float c
     pen = fmaf(e, e, pen);
   }
   return pen;
-}
-
-// The radius r is the same in every lane (one robot sphere at a time), so the
-// level choice is a uniform branch.
-__device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz, float r,
-                                                    const float4* ws, int M, const GridArgs& g) {
-#pragma unroll
-  for (int i = 0; i < kMaxGrids; ++i)
-    if (i < g.n && r <= g.lv[i].r_max) return world_penalty_level(cx, cy, cz, r, ws, M, g.lv[i]);
-  return world_penalty_all(cx, cy, cz, r, ws, M);
 }
 
 // Collision merit penalty (unweighted) over every robot sphere.

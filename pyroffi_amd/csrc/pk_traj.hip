@@ -23,24 +23,28 @@
 // with the first one that decreases the cost by 1e-4 (relative) or else the
 // cheapest, and the best iterate tracked at the maximum collision weight.
 //
-// gfx950 layout: one workgroup (two waves) per trajectory, so trajectories of
+// gfx950 layout: one workgroup (four waves) per trajectory, so trajectories of
 // different lengths run side by side in one launch.  The robot (fixed joints
 // folded into moving "nodes" on the host), its spheres in node frames, the
 // self-pair lists and the world live in LDS; the trajectory, its gradient and
 // the search direction live in LDS as well; the L-BFGS history stays in HBM.
-// A lane evaluates one waypoint at a time with the node poses in registers:
+// A group of 8 lanes evaluates one waypoint (32 waypoints in flight per
+// workgroup); every lane of the group computes the node poses in registers
+// and takes an eighth of the spheres and of the self pairs, so the grid
+// lookups of a waypoint are issued in parallel:
 //   * world term: a grid lookup per robot sphere (pk_grid.h) gives the only
-//     world spheres within the margin; a hit counts when this sphere is the
-//     closest of its link to that world sphere (the link minimum);
+//     world spheres within the margin; the link's lowest-index sphere within
+//     the margin of a world sphere owns that (link, world sphere) term and
+//     takes the minimum over the link's spheres;
 //   * self term: a bounding-sphere test per pair skips pairs farther apart
 //     than the margin (exact: f vanishes there);
-//   * adjoint: dcost/dc per sphere centre becomes a wrench (F, c x F) on the
-//     sphere's node; reverse subtree sums give dcost/dq_a =
-//     sum over nodes driven by a of mul (w.(T - p x F) + v.F), with (v, w) the
-//     node's twist in world axes and p its origin.
+//   * adjoint: dcost/dc at a sphere centre c is pulled back to every ancestor
+//     node a of the sphere's node, dcost/dq_a += g . (w_a x (c - p_a) + v_a)
+//     with (v_a, w_a) the node's twist in world axes and p_a its origin, then
+//     summed over the group and mapped to the actuated joints (mimic factors).
 // The line search needs no gradient: the smoothness of x + a d is quadratic in
-// a (three sums), the limits are elementwise, and the five collision sums run
-// as 5 T independent waypoint evaluations.
+// a (three sums), the limits are elementwise, and the steps are evaluated in
+// order until one is accepted (the same pick as evaluating all five).
 #include "pk_grid.h"
 #include "pk_model.h"
 
@@ -50,8 +54,11 @@
 
 namespace pk {
 
-constexpr int kTrajThreads = 128;
+constexpr int kTrajThreads = 256;
 constexpr int kTrajWaves = kTrajThreads / kWave;
+constexpr int kGroup = 8;                          // lanes per waypoint evaluation
+constexpr int kGroups = kTrajThreads / kGroup;     // waypoints in flight per workgroup
+static_assert(kWave % kGroup == 0, "a group must not straddle waves");
 constexpr int kMaxHist = 8;
 constexpr int kNumAlphas = 5;
 constexpr int kNodeWords = 20;   // ptf[7] lin[3] ang[3] mul off src parent kind pad pad
@@ -89,8 +96,8 @@ struct TrajWeights {
 
 // Byte offsets of the LDS image; the same function sizes the launch on the host.
 struct TrajLdsLayout {
-  size_t nodes, link_node, link_sph, sph, lb, pair_links, pair_start, pair_items, world, lo, hi,
-      vx, vg, vd, vt, red, total;
+  size_t nodes, anc, link_node, link_sph, sph, lb, pair_links, pair_start, pair_items, world, lo, hi,
+      vx, vg, vd, vt, red, gpose, total;
 };
 
 __host__ __device__ inline size_t traj_align(size_t v) { return (v + 15) & ~static_cast<size_t>(15); }
@@ -105,6 +112,7 @@ __host__ __device__ inline TrajLdsLayout traj_lds_layout(int N, int L, int K, in
     return here;
   };
   o.nodes = take(sizeof(float) * kNodeWords * N);
+  o.anc = take(sizeof(int) * N);
   o.link_node = take(sizeof(int) * L);
   o.link_sph = take(sizeof(int) * (L + 1));
   o.sph = take(sizeof(float4) * K);
@@ -121,12 +129,14 @@ __host__ __device__ inline TrajLdsLayout traj_lds_layout(int N, int L, int K, in
   o.vd = take(vec);
   o.vt = take(vec);
   o.red = take(sizeof(float) * kTrajWaves * kRedSlots);
+  o.gpose = take(sizeof(float) * 7 * N * kGroups);
   o.total = at;
   return o;
 }
 
 struct TrajLds {
   const float* nodes;
+  const int* anc;      // bit a set when node a is node n or one of its ancestors
   const int* link_node;
   const int* link_sph;
   const float4* sph;
@@ -142,6 +152,7 @@ struct TrajLds {
   float* vd;   // search direction (and scratch in the gradient pass)
   float* vt;   // accelerations of x
   float* red;  // block-reduction scratch
+  float* gpose;  // node poses of the waypoint each group evaluates: (group, node, 7)
 };
 
 // Cooperative copy of the model into LDS, with every index clamped into range.
@@ -150,6 +161,7 @@ __device__ inline TrajLds stage_traj_model(const TrajModelDev& md, int max_len, 
   const int M = md.n_world, A = md.n_act;
   const TrajLdsLayout lay = traj_lds_layout(N, L, K, P, Q, M, A, max_len);
   float* nodes = reinterpret_cast<float*>(base + lay.nodes);
+  int* anc = reinterpret_cast<int*>(base + lay.anc);
   int* link_node = reinterpret_cast<int*>(base + lay.link_node);
   int* link_sph = reinterpret_cast<int*>(base + lay.link_sph);
   float4* sph = reinterpret_cast<float4*>(base + lay.sph);
@@ -212,8 +224,15 @@ __device__ inline TrajLds stage_traj_model(const TrajModelDev& md, int max_len, 
     hi[a] = md.upper[a];
   }
   __syncthreads();
+  if (tid == 0) {
+    for (int n = 0; n < N; ++n) {
+      const int p = static_cast<int>(nodes[n * kNodeWords + 16]);
+      anc[n] = (1 << n) | (p >= 0 ? anc[p] : 0);
+    }
+  }
+  __syncthreads();
   TrajLds o;
-  o.nodes = nodes; o.link_node = link_node; o.link_sph = link_sph; o.sph = sph; o.lb = lb;
+  o.nodes = nodes; o.anc = anc; o.link_node = link_node; o.link_sph = link_sph; o.sph = sph; o.lb = lb;
   o.pair_links = pair_links; o.pair_start = pair_start; o.pair_items = pair_items; o.world = world;
   o.lo = lo; o.hi = hi;
   o.vx = reinterpret_cast<float*>(base + lay.vx);
@@ -221,6 +240,7 @@ __device__ inline TrajLds stage_traj_model(const TrajModelDev& md, int max_len, 
   o.vd = reinterpret_cast<float*>(base + lay.vd);
   o.vt = reinterpret_cast<float*>(base + lay.vt);
   o.red = reinterpret_cast<float*>(base + lay.red);
+  o.gpose = reinterpret_cast<float*>(base + lay.gpose);
   return o;
 }
 
@@ -286,29 +306,27 @@ __device__ __forceinline__ Pose node_pose(const Pose (&P)[MAXN], int n) {
   return r;
 }
 
-template <int MAXN>
-__device__ __forceinline__ void add_wrench(float (&F)[MAXN][3], float (&Tq)[MAXN][3], int n, float cx,
-                                           float cy, float cz, float gx, float gy, float gz) {
+__device__ __forceinline__ float group_sum(float v) {
 #pragma unroll
-  for (int i = 0; i < MAXN; ++i) {
-    if (n == i) {
-      F[i][0] += gx; F[i][1] += gy; F[i][2] += gz;
-      Tq[i][0] += cy * gz - cz * gy;
-      Tq[i][1] += cz * gx - cx * gz;
-      Tq[i][2] += cx * gy - cy * gx;
-    }
-  }
+  for (int off = 1; off < kGroup; off <<= 1) v += __shfl_xor(v, off, kWave);
+  return v;
 }
 
-// Collision cost of one waypoint q (this lane's) and, when GRAD, dcost/dq.
+// Node kinematics of one waypoint, computed by every lane of the group:
+// poses P, and (for the adjoint) each node's twist in world axes (w, v).
 template <int MAXN, bool GRAD>
-__device__ float waypoint_coll(const TrajModelDev& md, const TrajLds& L, const float (&q)[MAXN],
-                               float (&gq)[MAXN]) {
-  const int N = md.n_nodes;
+struct WaypointNodes {
   Pose P[MAXN];
+  float w[GRAD ? MAXN : 1][3], v[GRAD ? MAXN : 1][3];
+};
+
+template <int MAXN, bool GRAD>
+__device__ __forceinline__ void waypoint_fk(const TrajModelDev& md, const TrajLds& L, const float (&q)[MAXN],
+                                            WaypointNodes<MAXN, GRAD>& K) {
+  const int N = md.n_nodes;
 #pragma unroll
   for (int n = 0; n < MAXN; ++n) {
-    P[n] = pose_identity();
+    K.P[n] = pose_identity();
     if (n < N) {
       const float* r = L.nodes + n * kNodeWords;
       const int src = __builtin_amdgcn_readfirstlane(static_cast<int>(r[15]));
@@ -324,76 +342,122 @@ __device__ float waypoint_coll(const TrajModelDev& md, const TrajLds& L, const f
       bool has_parent = false;
 #pragma unroll
       for (int m = 0; m < n; ++m)
-        if (par == m) { pp = P[m]; has_parent = true; }
-      P[n] = has_parent ? compose(pp, tl) : tl;
+        if (par == m) { pp = K.P[m]; has_parent = true; }
+      K.P[n] = has_parent ? compose(pp, tl) : tl;
+      if constexpr (GRAD) {
+        quat_rotate(K.P[n].w, K.P[n].x, K.P[n].y, K.P[n].z, r[10], r[11], r[12], K.w[n][0], K.w[n][1], K.w[n][2]);
+        quat_rotate(K.P[n].w, K.P[n].x, K.P[n].y, K.P[n].z, r[7], r[8], r[9], K.v[n][0], K.v[n][1], K.v[n][2]);
+      }
     }
   }
-  float F[MAXN][3], Tq[MAXN][3];
-  if (GRAD) {
+}
+
+// dcost/dq_node += g . dc/dq_node for a point c carried by a node whose
+// ancestor set (itself included) is `mask`: dc/dq_a = w_a x (c - p_a) + v_a.
+template <int MAXN, bool GRAD>
+__device__ __forceinline__ void pull_point(const WaypointNodes<MAXN, GRAD>& K, int mask, float cx, float cy,
+                                           float cz, float gx, float gy, float gz, float (&gn)[MAXN]) {
+  if constexpr (GRAD) {
 #pragma unroll
-    for (int n = 0; n < MAXN; ++n)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) { F[n][c] = 0.f; Tq[n][c] = 0.f; }
+  for (int a = 0; a < MAXN; ++a) {
+    if (mask & (1 << a)) {
+      const float rx = cx - K.P[a].tx, ry = cy - K.P[a].ty, rz = cz - K.P[a].tz;
+      const float dx = K.w[a][1] * rz - K.w[a][2] * ry + K.v[a][0];
+      const float dy = K.w[a][2] * rx - K.w[a][0] * rz + K.v[a][1];
+      const float dz = K.w[a][0] * ry - K.w[a][1] * rx + K.v[a][2];
+      gn[a] += gx * dx + gy * dy + gz * dz;
+    }
   }
+  }
+}
+
+__device__ __forceinline__ Pose lds_node_pose(const float* gp, int n) {
+  return (n < 0) ? pose_identity() : load_pose(gp + 7 * n);
+}
+
+// Collision cost of one waypoint, evaluated by the kGroup lanes of a group
+// (lane g = threadIdx.x % kGroup), and with GRAD the group's dcost/dq_node
+// (per node, before the mimic map).  Every lane of the group returns the
+// group totals.  The caller has written the waypoint's node poses to `gp`
+// (this group's LDS slot) and synchronised.
+template <int MAXN, bool GRAD>
+__device__ float group_waypoint_coll(const TrajModelDev& md, const TrajLds& L, const WaypointNodes<MAXN, GRAD>& K,
+                                     const float* gp, float (&gn)[MAXN]) {
+  const int g = threadIdx.x % kGroup;
   const float margin = md.margin;
   const int mode = md.mode;
   const float e = 1e-6f;
   float cost = 0.f;
+#pragma unroll
+  for (int a = 0; a < MAXN; ++a) gn[a] = 0.f;
 
-  // World: per (link, world sphere), the link's closest sphere.
+  // World: the link's spheres split over the group's lanes; per (link, world
+  // sphere) only the link's closest sphere counts.
   for (int l = 0; l < md.n_links; ++l) {
     const int k0 = L.link_sph[l], k1 = L.link_sph[l + 1];
     if (k0 >= k1) continue;
     const int node = __builtin_amdgcn_readfirstlane(L.link_node[l]);
-    const Pose T = node_pose<MAXN>(P, node);
+    const int mask = node >= 0 ? L.anc[node] : 0;
+    const Pose T = node_pose<MAXN>(K.P, node);
     const Rot3 R = quat_to_rot(T.w, T.x, T.y, T.z);
-    for (int s = k0; s < k1; ++s) {
+    for (int s = k0 + g; s < k1; s += kGroup) {
       const float4 sp = L.sph[s];
       const float cx = R.m00 * sp.x + R.m01 * sp.y + R.m02 * sp.z + T.tx;
       const float cy = R.m10 * sp.x + R.m11 * sp.y + R.m12 * sp.z + T.ty;
       const float cz = R.m20 * sp.x + R.m21 * sp.y + R.m22 * sp.z + T.tz;
-      const int* items = nullptr;
-      int cnt = 0;
-      const bool listed = grid_candidates(md.grid, cx, cy, cz, sp.w, items, cnt);
-      const int total = listed ? cnt : md.n_world;
+      int4 head;
+      const int* rec;
+      int n;
+      const bool listed = grid_lookup(md.grid, cx, cy, cz, sp.w, head, rec, n);
+      const int total = listed ? n : md.n_world;
       for (int i = 0; i < total; ++i) {
-        const int m = listed ? items[i] : i;
+        const int m = listed ? grid_item(head, rec, i) : i;
         if (m < 0 || m >= md.n_world) continue;
         const float4 o = L.world[m];
         const float dx = o.x - cx + e, dy = o.y - cy + e, dz = o.z - cz + e;
         const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
         const float d = dist - sp.w - o.w;
         if (!(d < margin)) continue;
-        bool is_min = true;
-        for (int s2 = k0; s2 < k1; ++s2) {
-          if (s2 == s) continue;
+        // The (link, world sphere) term is owned by the link's lowest-index
+        // sphere within the margin; the owner takes the minimum over the link.
+        bool owner = true;
+        for (int s2 = k0; s2 < s; ++s2) {
           const float4 p2 = L.sph[s2];
           const float ex = o.x - (R.m00 * p2.x + R.m01 * p2.y + R.m02 * p2.z + T.tx) + e;
           const float ey = o.y - (R.m10 * p2.x + R.m11 * p2.y + R.m12 * p2.z + T.ty) + e;
           const float ez = o.z - (R.m20 * p2.x + R.m21 * p2.y + R.m22 * p2.z + T.tz) + e;
-          const float d2 = sqrtf(ex * ex + ey * ey + ez * ez) - p2.w - o.w;
-          if (d2 < d || (d2 == d && s2 < s)) { is_min = false; break; }
+          if (sqrtf(ex * ex + ey * ey + ez * ez) - p2.w - o.w < margin) { owner = false; break; }
         }
-        if (!is_min) continue;
+        if (!owner) continue;
+        float bd = d, bdist = dist, bx = dx, by = dy, bz = dz, bcx = cx, bcy = cy, bcz = cz;
+        for (int s2 = s + 1; s2 < k1; ++s2) {
+          const float4 p2 = L.sph[s2];
+          const float qx = R.m00 * p2.x + R.m01 * p2.y + R.m02 * p2.z + T.tx;
+          const float qy = R.m10 * p2.x + R.m11 * p2.y + R.m12 * p2.z + T.ty;
+          const float qz = R.m20 * p2.x + R.m21 * p2.y + R.m22 * p2.z + T.tz;
+          const float ex = o.x - qx + e, ey = o.y - qy + e, ez = o.z - qz + e;
+          const float dd = sqrtf(ex * ex + ey * ey + ez * ez);
+          if (dd - p2.w - o.w < bd) {
+            bd = dd - p2.w - o.w; bdist = dd; bx = ex; by = ey; bz = ez; bcx = qx; bcy = qy; bcz = qz;
+          }
+        }
         float df;
-        cost += traj_act(d, margin, mode, df);
+        cost += traj_act(bd, margin, mode, df);
         if (GRAD && df != 0.f) {
-          const float k = -df / dist;
-          add_wrench<MAXN>(F, Tq, node, cx, cy, cz, k * dx, k * dy, k * dz);
+          const float k = -df / bdist;
+          pull_point<MAXN, GRAD>(K, mask, bcx, bcy, bcz, k * bx, k * by, k * bz, gn);
         }
       }
     }
   }
 
-  // Self: active pairs, same-slot sphere pairs, bounding-sphere culled.
-  for (int p = 0; p < md.n_pairs; ++p) {
+  // Self: active pairs split over the group's lanes, bounding-sphere culled.
+  for (int p = g; p < md.n_pairs; p += kGroup) {
     const int q0 = L.pair_start[p], q1 = L.pair_start[p + 1];
     if (q0 >= q1) continue;
     const int li = L.pair_links[2 * p], lj = L.pair_links[2 * p + 1];
-    const int ni = __builtin_amdgcn_readfirstlane(L.link_node[li]);
-    const int nj = __builtin_amdgcn_readfirstlane(L.link_node[lj]);
-    const Pose Ti = node_pose<MAXN>(P, ni);
-    const Pose Tj = node_pose<MAXN>(P, nj);
+    const int ni = L.link_node[li], nj = L.link_node[lj];
+    const Pose Ti = lds_node_pose(gp, ni), Tj = lds_node_pose(gp, nj);
     const Rot3 Ri = quat_to_rot(Ti.w, Ti.x, Ti.y, Ti.z);
     const Rot3 Rj = quat_to_rot(Tj.w, Tj.x, Tj.y, Tj.z);
     {
@@ -432,45 +496,78 @@ __device__ float waypoint_coll(const TrajModelDev& md, const TrajLds& L, const f
       cost += traj_act(best, margin, mode, df);
       if (GRAD && df != 0.f) {
         const float k = df / bdist;
-        add_wrench<MAXN>(F, Tq, nj, bex, bey, bez, k * bdx, k * bdy, k * bdz);
-        add_wrench<MAXN>(F, Tq, ni, bcx, bcy, bcz, -k * bdx, -k * bdy, -k * bdz);
+        const int mi = ni >= 0 ? L.anc[ni] : 0, mj = nj >= 0 ? L.anc[nj] : 0;
+        pull_point<MAXN, GRAD>(K, mj, bex, bey, bez, k * bdx, k * bdy, k * bdz, gn);
+        pull_point<MAXN, GRAD>(K, mi, bcx, bcy, bcz, -k * bdx, -k * bdy, -k * bdz, gn);
       }
     }
   }
 
+  cost = group_sum(cost);
   if (GRAD) {
 #pragma unroll
-    for (int a = 0; a < MAXN; ++a) gq[a] = 0.f;
+    for (int a = 0; a < MAXN; ++a) gn[a] = group_sum(gn[a]);
+  }
+  return cost;
+}
+
+// dcost/dq (actuated) from the per-node sums: q_node = mul q_src + off.
+template <int MAXN>
+__device__ __forceinline__ void node_to_actuated(const TrajLds& L, int N, const float (&gn)[MAXN],
+                                                 float (&gq)[MAXN]) {
 #pragma unroll
-    for (int n = MAXN - 1; n >= 0; --n) {
-      if (n < N) {
-        const float* r = L.nodes + n * kNodeWords;
-        const int src = __builtin_amdgcn_readfirstlane(static_cast<int>(r[15]));
-        const int par = __builtin_amdgcn_readfirstlane(static_cast<int>(r[16]));
-        if (src >= 0) {
-          float wx, wy, wz, vx, vy, vz;
-          quat_rotate(P[n].w, P[n].x, P[n].y, P[n].z, r[10], r[11], r[12], wx, wy, wz);
-          quat_rotate(P[n].w, P[n].x, P[n].y, P[n].z, r[7], r[8], r[9], vx, vy, vz);
-          const float px = P[n].tx, py = P[n].ty, pz = P[n].tz;
-          const float tx = Tq[n][0] - (py * F[n][2] - pz * F[n][1]);
-          const float ty = Tq[n][1] - (pz * F[n][0] - px * F[n][2]);
-          const float tz = Tq[n][2] - (px * F[n][1] - py * F[n][0]);
-          const float val = r[13] * (wx * tx + wy * ty + wz * tz + vx * F[n][0] + vy * F[n][1] + vz * F[n][2]);
+  for (int a = 0; a < MAXN; ++a) gq[a] = 0.f;
 #pragma unroll
-          for (int a = 0; a < MAXN; ++a)
-            if (src == a) gq[a] += val;
-        }
+  for (int n = 0; n < MAXN; ++n) {
+    if (n < N) {
+      const float* r = L.nodes + n * kNodeWords;
+      const int src = __builtin_amdgcn_readfirstlane(static_cast<int>(r[15]));
+      const float val = r[13] * gn[n];
 #pragma unroll
-        for (int m = 0; m < n; ++m) {
-          if (par == m) {
+      for (int a = 0; a < MAXN; ++a)
+        if (src == a) gq[a] += val;
+    }
+  }
+}
+
+// One waypoint per group for `n_tasks` tasks; task(i, q) fills the
+// configuration of task i, done(i, cost, gq) consumes the group totals (every
+// lane of the group calls it).  All threads run the same number of rounds, so
+// the workgroup barriers inside are uniform.
+template <int MAXN, bool GRAD, typename TaskFn, typename DoneFn>
+__device__ __forceinline__ void for_each_waypoint(const TrajModelDev& md, const TrajLds& L, int n_tasks,
+                                                  TaskFn task, DoneFn done) {
+  const int grp = threadIdx.x / kGroup;
+  const int g = threadIdx.x % kGroup;
+  const int N = md.n_nodes;
+  float* gp = L.gpose + grp * 7 * N;
+  const int rounds = (n_tasks + kGroups - 1) / kGroups;
+  for (int rd = 0; rd < rounds; ++rd) {
+    const int i = rd * kGroups + grp;
+    const bool active = i < n_tasks;
+    WaypointNodes<MAXN, GRAD> K;
+    if (active) {
+      float q[MAXN];
+      task(i, q);
+      waypoint_fk<MAXN, GRAD>(md, L, q, K);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) { F[m][c] += F[n][c]; Tq[m][c] += Tq[n][c]; }
-          }
+      for (int n = 0; n < MAXN; ++n) {
+        if (n < N && (n % kGroup) == g) {
+          float* o = gp + 7 * n;
+          o[0] = K.P[n].w; o[1] = K.P[n].x; o[2] = K.P[n].y; o[3] = K.P[n].z;
+          o[4] = K.P[n].tx; o[5] = K.P[n].ty; o[6] = K.P[n].tz;
         }
       }
     }
+    __syncthreads();
+    if (active) {
+      float gn[MAXN], gq[MAXN];
+      const float c = group_waypoint_coll<MAXN, GRAD>(md, L, K, gp, gn);
+      if (GRAD) node_to_actuated<MAXN>(L, N, gn, gq);
+      done(i, c, gq);
+    }
+    __syncthreads();
   }
-  return cost;
 }
 
 // Cost components {smooth, limits, coll} of the trajectory in L.vx (T, A), all
@@ -484,18 +581,22 @@ __device__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T,
   const int n = T * A;
   const int tid = threadIdx.x;
   float coll = 0.f, lim = 0.f, sm = 0.f;
-  for (int t = tid; t < T; t += kTrajThreads) {
-    float q[MAXN], gq[MAXN];
+  const bool leader = (tid % kGroup) == 0;
+  for_each_waypoint<MAXN, GRAD>(
+      md, L, T,
+      [&](int t, float (&q)[MAXN]) {
 #pragma unroll
-    for (int a = 0; a < MAXN; ++a) q[a] = (a < A) ? L.vx[t * A + a] : 0.f;
-    coll += waypoint_coll<MAXN, GRAD>(md, L, q, gq);
-    if (GRAD) {
+        for (int a = 0; a < MAXN; ++a) q[a] = (a < A) ? L.vx[t * A + a] : 0.f;
+      },
+      [&](int t, float c, const float (&gq)[MAXN]) {
+        if (!leader) return;
+        coll += c;
+        if (GRAD) {
 #pragma unroll
-      for (int a = 0; a < MAXN; ++a)
-        if (a < A) L.vg[t * A + a] = w_coll * gq[a];
-    }
-  }
-  __syncthreads();
+          for (int a = 0; a < MAXN; ++a)
+            if (a < A) L.vg[t * A + a] = w_coll * gq[a];
+        }
+      });
   const int na = T - 4;
   for (int i = tid; i < n; i += kTrajThreads) {
     const int a = i % A;
@@ -555,18 +656,22 @@ __device__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T,
   comps[2] = v3[2];
 }
 
-// Components of the five line-search points x + alpha_k d (no gradient).
+// Line search of _lbfgs_direct_solve on x + alpha_k d, alpha in {1, .5, .25,
+// .1, .025}: the first step whose cost is below `thresh`, else the cheapest
+// (first index on ties).  The points are evaluated in that order and the
+// search stops at the first acceptable one, which picks the same step as
+// evaluating all five.  Returns the step index and its cost components.
 template <int MAXN>
-__device__ void traj_trials(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
-                            float (&sm)[kNumAlphas], float (&lim)[kNumAlphas], float (&coll)[kNumAlphas]) {
+__device__ int traj_line_search(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
+                                float w_coll, float thresh, float& p_sm, float& p_lim, float& p_col) {
   const int A = md.n_act;
   const int n = T * A;
   const int tid = threadIdx.x;
   const int na = T - 4;
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-  float lk[kNumAlphas], ck[kNumAlphas];
+  // Smoothness of x + a d is s0 + 2 a s1 + a^2 s2; limits per step.
+  float v[3 + kNumAlphas];
 #pragma unroll
-  for (int k = 0; k < kNumAlphas; ++k) { lk[k] = 0.f; ck[k] = 0.f; }
+  for (int k = 0; k < 3 + kNumAlphas; ++k) v[k] = 0.f;
   for (int i = tid; i < n; i += kTrajThreads) {
     const int a = i % A;
     const int t = i / A;
@@ -574,50 +679,59 @@ __device__ void traj_trials(const TrajModelDev& md, const TrajLds& L, int T, con
 #pragma unroll
     for (int k = 0; k < kNumAlphas; ++k) {
       const float y = xv + traj_alpha(k) * dv;
-      const float v = fmaxf(y - L.hi[a], 0.f) + fmaxf(L.lo[a] - y, 0.f);
-      lk[k] += v * v;
+      const float u = fmaxf(y - L.hi[a], 0.f) + fmaxf(L.lo[a] - y, 0.f);
+      v[3 + k] += u * u;
     }
     if (t + 1 < T) {
       const float vx = L.vx[i + A] - xv, vd = L.vd[i + A] - dv;
-      s0 += w.w_vel * vx * vx; s1 += w.w_vel * vx * vd; s2 += w.w_vel * vd * vd;
+      v[0] += w.w_vel * vx * vx; v[1] += w.w_vel * vx * vd; v[2] += w.w_vel * vd * vd;
     }
     if (t < na) {
       const float ax = L.vt[i];
       const float ad = (-dv + 16.f * L.vd[i + A] - 30.f * L.vd[i + 2 * A] + 16.f * L.vd[i + 3 * A] -
                         L.vd[i + 4 * A]) / 12.f;
-      s0 += w.w_acc * ax * ax; s1 += w.w_acc * ax * ad; s2 += w.w_acc * ad * ad;
+      v[0] += w.w_acc * ax * ax; v[1] += w.w_acc * ax * ad; v[2] += w.w_acc * ad * ad;
       if (t + 1 < na) {
         const float jx = L.vt[i + A] - ax;
         const float ad1 = (-L.vd[i + A] + 16.f * L.vd[i + 2 * A] - 30.f * L.vd[i + 3 * A] +
                            16.f * L.vd[i + 4 * A] - L.vd[i + 5 * A]) / 12.f;
         const float jd = ad1 - ad;
-        s0 += w.w_jerk * jx * jx; s1 += w.w_jerk * jx * jd; s2 += w.w_jerk * jd * jd;
+        v[0] += w.w_jerk * jx * jx; v[1] += w.w_jerk * jx * jd; v[2] += w.w_jerk * jd * jd;
       }
     }
   }
-  for (int task = tid; task < kNumAlphas * T; task += kTrajThreads) {
-    const int k = task / T, t = task % T;
-    const float al = traj_alpha(k);
-    float q[MAXN], gq[MAXN];
-#pragma unroll
-    for (int a = 0; a < MAXN; ++a) q[a] = (a < A) ? L.vx[t * A + a] + al * L.vd[t * A + a] : 0.f;
-    const float c = waypoint_coll<MAXN, false>(md, L, q, gq);
-#pragma unroll
-    for (int kk = 0; kk < kNumAlphas; ++kk)
-      if (kk == k) ck[kk] += c;
-  }
-  float v[3 + 2 * kNumAlphas];
-  v[0] = s0; v[1] = s1; v[2] = s2;
-#pragma unroll
-  for (int k = 0; k < kNumAlphas; ++k) { v[3 + k] = lk[k]; v[3 + kNumAlphas + k] = ck[k]; }
-  block_sum<3 + 2 * kNumAlphas>(v, L.red);
-#pragma unroll
+  block_sum<3 + kNumAlphas>(v, L.red);
+  const bool leader = (tid % kGroup) == 0;
+  int best = -1;
+  float best_tc = INFINITY;
   for (int k = 0; k < kNumAlphas; ++k) {
     const float al = traj_alpha(k);
-    sm[k] = v[0] + 2.f * al * v[1] + al * al * v[2];
-    lim[k] = v[3 + k];
-    coll[k] = v[3 + kNumAlphas + k];
+    float ck = 0.f;
+    for_each_waypoint<MAXN, false>(
+        md, L, T,
+        [&](int t, float (&q)[MAXN]) {
+#pragma unroll
+          for (int a = 0; a < MAXN; ++a) q[a] = (a < A) ? L.vx[t * A + a] + al * L.vd[t * A + a] : 0.f;
+        },
+        [&](int, float c, const float (&)[MAXN]) {
+          if (leader) ck += c;
+        });
+    ck = block_sum1(ck, L.red);
+    const float sm = v[0] + 2.f * al * v[1] + al * al * v[2];
+    float lim = 0.f;
+#pragma unroll
+    for (int kk = 0; kk < kNumAlphas; ++kk)
+      if (kk == k) lim = v[3 + kk];
+    const float tc = w.w_smooth * sm + w.w_limits * lim + w_coll * ck;
+    if (best < 0 || tc < best_tc) {
+      best = k; best_tc = tc; p_sm = sm; p_lim = lim; p_col = ck;
+    }
+    if (tc < thresh) {
+      p_sm = sm; p_lim = lim; p_col = ck;
+      return k;
+    }
   }
+  return best;
 }
 
 __device__ __forceinline__ float sel_hist(const float (&v)[kMaxHist], int i) {
@@ -813,25 +927,8 @@ __global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
       if (t == 0 || t == T - 1) L.vd[i] = 0.f;
     }
     __syncthreads();
-    float sm[kNumAlphas], lim[kNumAlphas], col[kNumAlphas];
-    traj_trials<MAXN>(a.md, L, T, a.w, sm, lim, col);
-    const float thresh = cost * (1.f - 1e-4f);
-    int pick = -1;
-    float pick_cost = INFINITY;
-    int cheapest = 0;
-    float cheapest_cost = INFINITY;
-#pragma unroll
-    for (int kk = 0; kk < kNumAlphas; ++kk) {
-      const float tc = a.w.w_smooth * sm[kk] + a.w.w_limits * lim[kk] + w_coll * col[kk];
-      if (pick < 0 && tc < thresh) { pick = kk; pick_cost = tc; }
-      if (tc < cheapest_cost) { cheapest = kk; cheapest_cost = tc; }
-    }
-    (void)pick_cost;
-    if (pick < 0) pick = cheapest;
     float p_sm = 0.f, p_lim = 0.f, p_col = 0.f;
-#pragma unroll
-    for (int kk = 0; kk < kNumAlphas; ++kk)
-      if (kk == pick) { p_sm = sm[kk]; p_lim = lim[kk]; p_col = col[kk]; }
+    const int pick = traj_line_search<MAXN>(a.md, L, T, a.w, w_coll, cost * (1.f - 1e-4f), p_sm, p_lim, p_col);
     const float al = traj_alpha(pick);
     for (int i = tid; i < n; i += kTrajThreads) {
       xp[i] = L.vx[i];

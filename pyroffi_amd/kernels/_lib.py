@@ -27,12 +27,12 @@ class WorldGrid(ctypes.Structure):
     """``pk_world_grid``: one world grid (device arrays + geometry), passed by host pointer."""
 
     _fields_ = [
-        ("counts", ctypes.c_void_p),
-        ("items", ctypes.c_void_p),
+        ("cells", ctypes.c_void_p),
         ("lo", ctypes.c_float * 3),
         ("cell", ctypes.c_float),
         ("dims", ctypes.c_int32 * 3),
         ("cap", ctypes.c_int32),
+        ("stride", ctypes.c_int32),
         ("pad", ctypes.c_float),
     ]
 
@@ -112,7 +112,7 @@ _SIGNATURES = {
         + [_p, _p],  # outputs
     ),
     "pk_ls_ik_select_f32": (_i32, [_p, _p, _p, _p, _i32, _i32, _i32, _f32, _p, _p]),
-    "pk_world_grid_f32": (_i32, [_p, _p, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _i32, _p, _p]),
+    "pk_world_grid_f32": (_i32, [_p, _p, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _i32, _i32, _p]),
     "pk_ls_ik_batched_f32": (
         _i32,
         [_p, _p, _p, _i32, _i64, _i32]  # stream, seeds, seed_start, P, total, A

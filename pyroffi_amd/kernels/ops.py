@@ -258,7 +258,9 @@ def world_grid(world_spheres, pad: float, cell: float | None = None, cap: int = 
     ``pad`` = the robot sphere radius the grid serves + cost margin (the LM
     kernel tests larger robot spheres against every world sphere).  The grid box
     is the world's bounding box grown by ``pad``; ``cell`` defaults to the box's longest
-    side / ``max_cells``.  Returns a dict of device tensors and scalars.
+    side / ``max_cells``.  Returns a dict with the packed cell records
+    ``cells (n_cells, stride)`` (count, then up to ``cap`` indices) and the grid
+    geometry.
     """
     import numpy as np
 
@@ -274,14 +276,14 @@ def world_grid(world_spheres, pad: float, cell: float | None = None, cap: int = 
         cell = float(max(hi - lo)) / max_cells
     dims = np.maximum(1, np.ceil((hi - lo) / cell).astype(np.int64))
     n_cells = int(np.prod(dims))
-    counts = torch.empty((n_cells,), dtype=torch.int32, device=dev)
-    items = torch.empty((n_cells, cap), dtype=torch.int32, device=dev)
+    stride = (int(cap) + 1 + 3) // 4 * 4
+    cells = torch.empty((n_cells, stride), dtype=torch.int32, device=dev)
     st = lib().pk_world_grid_f32(
         _stream(), _ptr(ws), ws.shape[0], float(pad), float(lo[0]), float(lo[1]), float(lo[2]), float(cell),
-        int(dims[0]), int(dims[1]), int(dims[2]), int(cap), _ptr(counts), _ptr(items),
+        int(dims[0]), int(dims[1]), int(dims[2]), int(cap), stride, _ptr(cells),
     )
     check(st, "pk_world_grid_f32")
-    return {"counts": counts, "items": items, "pad": float(pad), "lo": tuple(float(v) for v in lo),
+    return {"cells": cells, "stride": stride, "pad": float(pad), "lo": tuple(float(v) for v in lo),
             "cell": float(cell),
             "dims": tuple(int(v) for v in dims), "cap": int(cap)}
 
@@ -293,8 +295,8 @@ def _grid_array(grid):
         raise PkInvalidArgument(f"at most {MAX_GRIDS} grids, got {len(levels)}")
     arr = (WorldGrid * max(1, len(levels)))()
     for i, g in enumerate(levels):
-        arr[i].counts = g["counts"].data_ptr()
-        arr[i].items = g["items"].data_ptr()
+        arr[i].cells = g["cells"].data_ptr()
+        arr[i].stride = int(g["stride"])
         arr[i].lo[:] = [float(v) for v in g["lo"]]
         arr[i].cell = float(g["cell"])
         arr[i].dims[:] = [int(v) for v in g["dims"]]

@@ -306,6 +306,11 @@ class BatchedLbfgsTrajOpt:
                   (cfg.w_smooth, cfg.w_vel, cfg.w_acc, cfg.w_jerk, cfg.w_limits, cfg.w_collision_max), grad=False,
                   out_comps=self.rank_comps)
 
+    def final_iterate(self) -> torch.Tensor:
+        """The last L-BFGS iterate ``(W, A)`` (a view of the state buffer)."""
+        W, A = self.x0.shape
+        return self.state[: W * A].view(W, A)
+
     def capture(self) -> None:
         self._launches()
         torch.cuda.synchronize()

@@ -82,8 +82,8 @@ def test_world_grid_lists_every_reachable_sphere():
     pad = 0.12
     for cap, max_cells in ((64, 16), (4, 12)):
         g = ops.world_grid(torch.as_tensor(ws, device="cuda"), pad, cap=cap, max_cells=max_cells)
-        counts = g["counts"].cpu().numpy()
-        items = g["items"].cpu().numpy()
+        cells = g["cells"].cpu().numpy()
+        counts, items = cells[:, 0], cells[:, 1:cap + 1]
         nx, ny, nz = g["dims"]
         lo, cell = np.asarray(g["lo"]), g["cell"]
         c = np.arange(nx * ny * nz)
