@@ -48,14 +48,12 @@ extern "C" {
  * passed by host pointer to the LM entry points.
  */
 typedef struct pk_world_grid {
-  const int32_t* cells;  /* (dims[0]*dims[1]*dims[2], stride) records: */
-                         /* [true count, index 0, ..., index cap-1]    */
-  float lo[3];           /* corner of cell (0, 0, 0)                   */
-  float cell;            /* cell side                                  */
-  int32_t dims[3];       /* nx, ny, nz                                 */
-  int32_t cap;           /* indices kept per cell                      */
-  int32_t stride;        /* int32 words per record (>= cap + 1, % 4 == 0) */
-  float pad;             /* reach the grid was built with              */
+  const int32_t* cells;  /* (dims[0]*dims[1]*dims[2], 2): start, count in items */
+  const int32_t* items;  /* world sphere indices, increasing within a cell      */
+  float lo[3];           /* corner of cell (0, 0, 0)                            */
+  float cell;            /* cell side                                           */
+  int32_t dims[3];       /* nx, ny, nz                                          */
+  float pad;             /* reach the grid was built with                       */
 } pk_world_grid;
 
 /* Message of the last failing call on this thread ("" when none). */
@@ -250,23 +248,28 @@ int pk_ls_ik_f32(void* stream, const float* seeds, int32_t n_problems,
                  float collision_margin, float* out_cfgs, float* out_err);
 
 /*
- * Uniform grid over a sphere world for exact culling of margin-limited costs.
- * Cell (ix, iy, iz) is the box lo + cell * [i, i+1) per axis; its record
- * cells[c * stride ...] holds the true count of world spheres whose distance to
- * the box is at most R_m + pad, then (up to cap of) their indices in increasing
- * order; a count > cap means "overflowed: use the whole world".  One 16-byte
- * load returns the count and the first three indices.  With pad = max robot sphere radius + margin and the
- * grid box = world bounding box grown by pad, a robot sphere whose centre lies
- * in a cell can be within the margin only of spheres listed there, and one
- * outside the grid of none.  No reference counterpart (an acceleration
- * structure for the LM collision penalty).
- *   cells (nx*ny*nz, stride) int32, 16-byte aligned, stride >= cap + 1 and a
- *   multiple of 4
+ * Uniform grid over a sphere world for exact culling of margin-limited costs,
+ * in CSR form.  Cell (ix, iy, iz) is the box lo + cell * [i, i+1) per axis and
+ * lists every world sphere whose distance to the box is at most R_m + pad.
+ * With pad = (robot sphere radius) + margin and the grid box = world bounding
+ * box grown by pad, a robot sphere whose centre lies in a cell can be within
+ * the margin only of spheres listed there, and one outside the grid of none.
+ * No reference counterpart (an acceleration structure for the collision costs).
+ * Two passes:
+ *   pk_world_grid_count_f32 writes cells[2c + 1] = count of cell c;
+ *   the caller writes cells[2c] = exclusive prefix sum of the counts and
+ *   allocates n_items = total count indices;
+ *   pk_world_grid_fill_f32 writes each cell's indices, in increasing order, to
+ *   items[cells[2c] ...] (never past n_items).
+ *   cells (nx*ny*nz, 2) int32, items (n_items) int32
  */
-int pk_world_grid_f32(void* stream, const float* world_spheres, int32_t n_world_spheres,
-                      float pad, float lo_x, float lo_y, float lo_z, float cell,
-                      int32_t nx, int32_t ny, int32_t nz, int32_t cap, int32_t stride,
-                      int32_t* cells);
+int pk_world_grid_count_f32(void* stream, const float* world_spheres, int32_t n_world_spheres,
+                            float pad, float lo_x, float lo_y, float lo_z, float cell, int32_t nx,
+                            int32_t ny, int32_t nz, int32_t* cells);
+int pk_world_grid_fill_f32(void* stream, const float* world_spheres, int32_t n_world_spheres,
+                           float pad, float lo_x, float lo_y, float lo_z, float cell, int32_t nx,
+                           int32_t ny, int32_t nz, const int32_t* cells, int32_t* items,
+                           int64_t n_items);
 
 /*
  * Uneven batched LM-IK: many problems of one robot with different seed
@@ -340,7 +343,8 @@ int pk_ls_ik_select_f32(void* stream, const float* cfgs, const float* errs,
  *                  multiplier, mimic offset, actuated source index (-1: none),
  *                  parent node (-1: world), exp kind (1 rotation, 2
  *                  translation, 3 general), 2 unused
- *   link_node      (n_links) node carrying each link (-1: world-fixed)
+ *   link_node      (n_links) node carrying each link (-1: world-fixed); links
+ *                  may be in any order (grouping them by node is faster)
  *   link_sph_start (n_links + 1) CSR of spheres per link, in the link's slot
  *                  order of the reference's padded sphere table
  *   spheres        (n_spheres, 4) centre in the node frame, radius
@@ -387,7 +391,10 @@ typedef struct pk_lbfgs_config {
 /*
  * Trajectory cost components and gradient, many trajectories of different
  * lengths in one launch: trajectory b is rows [traj_start[b], traj_start[b+1])
- * of x (n_waypoints, n_act), at most max_len rows.
+ * of x (n_waypoints, n_act), at most max_len rows.  One workgroup per
+ * trajectory, of block_threads threads (128, 256, 512; 0 = chosen from
+ * max_len); with traj_subset (device int32, n_subset entries) only those
+ * trajectories are processed, so length classes can run as separate launches.
  *   weights   {w_smooth, w_vel, w_acc, w_jerk, w_limits, w_coll}
  *   out_comps (n_traj, 3) {smooth, limits, coll} without w_smooth/w_limits/w_coll
  *   out_grad  (n_waypoints, n_act) gradient of w_smooth smooth + w_limits
@@ -398,7 +405,8 @@ typedef struct pk_lbfgs_config {
  */
 int pk_traj_cost_grad_f32(void* stream, const pk_traj_model* model, const float* x,
                           const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
-                          int32_t max_len, const float* weights, float* out_comps,
+                          int32_t max_len, int32_t block_threads, const int32_t* traj_subset,
+                          int32_t n_subset, const float* weights, float* out_comps,
                           float* out_grad);
 
 /* Floats of the state buffer pk_traj_lbfgs_f32 needs. */
@@ -408,7 +416,8 @@ int64_t pk_traj_lbfgs_state_floats(int64_t n_waypoints, int32_t n_act, int32_t n
 /*
  * Direct L-BFGS trajectory optimisation (_lbfgs_direct_solve,
  * src/pyroffi/optimization_engines/_lbfgs_trajopt_optimization.py:76-205),
- * one workgroup per trajectory.  Endpoints stay where x_init has them (the
+ * one workgroup per trajectory (block_threads / traj_subset as for
+ * pk_traj_cost_grad_f32).  Endpoints stay where x_init has them (the
  * caller pins them, _lbfgs_trajopt_jax :229-230).  Iterations
  * [iter_begin, iter_end) of cfg->n_iters run per call and resume from `state`
  * (pk_traj_lbfgs_state_floats floats), so a solve can be split into launches.
@@ -417,7 +426,8 @@ int64_t pk_traj_lbfgs_state_floats(int64_t n_waypoints, int32_t n_act, int32_t n
  */
 int pk_traj_lbfgs_f32(void* stream, const pk_traj_model* model, const float* x_init,
                       const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
-                      int32_t max_len, const pk_lbfgs_config* cfg, int32_t iter_begin,
+                      int32_t max_len, int32_t block_threads, const int32_t* traj_subset,
+                      int32_t n_subset, const pk_lbfgs_config* cfg, int32_t iter_begin,
                       int32_t iter_end, float* state, float* out_best, float* out_best_cost);
 
 #ifdef __cplusplus

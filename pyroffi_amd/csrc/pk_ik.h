@@ -293,19 +293,20 @@ __device__ __forceinline__ float world_penalty_all(float cx, float cy, float cz,
 
 // Same sum over the candidates of the grid cell holding c: exact, since every
 // sphere that can be within the margin of a robot sphere of radius <= r_max
-// (= grid pad - margin) is listed (pk_grid.hip); an overflowing cell, or a
-// radius no level serves, falls back to the whole world, and a centre outside
-// the grid has no such sphere.  The radius r is the same in every lane (one
+// (= grid pad - margin) is listed (pk_grid.hip); a radius no level serves
+// falls back to the whole world, and a centre outside the grid has no such
+// sphere.  The radius r is the same in every lane (one
 // robot sphere at a time), so the level choice is a uniform branch.
 __device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz, float r,
                                                     const float4* ws, int M, const GridArgs& g) {
-  int4 head;
-  const int* rec;
+  const int level = grid_level(g, r);
+  if (level < 0) return world_penalty_all(cx, cy, cz, r, ws, M);
+  const int* items;
   int n;
-  if (!grid_lookup(g, cx, cy, cz, r, head, rec, n)) return world_penalty_all(cx, cy, cz, r, ws, M);
+  grid_cell_list(g, level, cx, cy, cz, items, n);
   float pen = 0.f;
   for (int i = 0; i < n; ++i) {
-    const int m = grid_item(head, rec, i);
+    const int m = items[i];
     if (m < 0 || m >= M) continue;
     const float4 o = ws[m];
     const float ax = cx - o.x, ay = cy - o.y, az = cz - o.z;

@@ -28,14 +28,15 @@
 // folded into moving "nodes" on the host), its spheres in node frames, the
 // self-pair lists and the world live in LDS; the trajectory, its gradient and
 // the search direction live in LDS as well; the L-BFGS history stays in HBM.
-// A group of 8 lanes evaluates one waypoint (32 waypoints in flight per
-// workgroup); every lane of the group computes the node poses in registers
-// and takes an eighth of the spheres and of the self pairs, so the grid
-// lookups of a waypoint are issued in parallel:
-//   * world term: a grid lookup per robot sphere (pk_grid.h) gives the only
-//     world spheres within the margin; the link's lowest-index sphere within
-//     the margin of a world sphere owns that (link, world sphere) term and
-//     takes the minimum over the link's spheres;
+// A group of 8 lanes evaluates one waypoint (16 to 64 waypoints in flight per
+// workgroup); every lane of the group computes the node poses in registers,
+// the group writes the waypoint's world sphere centres and link bounding
+// spheres to its LDS slot, and each lane then takes an eighth of the spheres
+// and of the self pairs:
+//   * world term: a CSR grid (pk_grid.h, small enough to stay in L2) lists,
+//     per cell, the only world spheres within reach; the link's lowest-index
+//     sphere within the margin of a world sphere owns that (link, world
+//     sphere) term and takes the minimum over the link's spheres;
 //   * self term: a bounding-sphere test per pair skips pairs farther apart
 //     than the margin (exact: f vanishes there);
 //   * adjoint: dcost/dc at a sphere centre c is pulled back to every ancestor
@@ -54,10 +55,11 @@
 
 namespace pk {
 
-constexpr int kTrajThreads = 256;
-constexpr int kTrajWaves = kTrajThreads / kWave;
-constexpr int kGroup = 8;                          // lanes per waypoint evaluation
-constexpr int kGroups = kTrajThreads / kGroup;     // waypoints in flight per workgroup
+// A workgroup has 128, 256 or 512 threads (chosen per trajectory length on the
+// host); loops and reductions read the size from blockDim.x.
+constexpr int kMaxTrajThreads = 512;
+constexpr int kMaxTrajWaves = kMaxTrajThreads / kWave;
+constexpr int kGroup = 8;  // lanes per waypoint evaluation
 static_assert(kWave % kGroup == 0, "a group must not straddle waves");
 constexpr int kMaxHist = 8;
 constexpr int kNumAlphas = 5;
@@ -96,14 +98,14 @@ struct TrajWeights {
 
 // Byte offsets of the LDS image; the same function sizes the launch on the host.
 struct TrajLdsLayout {
-  size_t nodes, anc, link_node, link_sph, sph, lb, pair_links, pair_start, pair_items, world, lo, hi,
-      vx, vg, vd, vt, red, gpose, total;
+  size_t nodes, anc, link_node, link_sph, sph_link, sph, lb, pair_links, pair_start, pair_items, world, lo,
+      hi, vx, vg, vd, vt, red, gc, total;
 };
 
 __host__ __device__ inline size_t traj_align(size_t v) { return (v + 15) & ~static_cast<size_t>(15); }
 
 __host__ __device__ inline TrajLdsLayout traj_lds_layout(int N, int L, int K, int P, int Q, int M, int A,
-                                                         int max_len) {
+                                                         int max_len, int threads) {
   TrajLdsLayout o;
   size_t at = 0;
   auto take = [&](size_t bytes) {
@@ -115,6 +117,7 @@ __host__ __device__ inline TrajLdsLayout traj_lds_layout(int N, int L, int K, in
   o.anc = take(sizeof(int) * N);
   o.link_node = take(sizeof(int) * L);
   o.link_sph = take(sizeof(int) * (L + 1));
+  o.sph_link = take(sizeof(int) * K);
   o.sph = take(sizeof(float4) * K);
   o.lb = take(sizeof(float4) * L);
   o.pair_links = take(sizeof(int) * 2 * P);
@@ -128,8 +131,8 @@ __host__ __device__ inline TrajLdsLayout traj_lds_layout(int N, int L, int K, in
   o.vg = take(vec);
   o.vd = take(vec);
   o.vt = take(vec);
-  o.red = take(sizeof(float) * kTrajWaves * kRedSlots);
-  o.gpose = take(sizeof(float) * 7 * N * kGroups);
+  o.red = take(sizeof(float) * kMaxTrajWaves * kRedSlots);
+  o.gc = take(sizeof(float4) * (K + L) * (threads / kGroup));
   o.total = at;
   return o;
 }
@@ -139,6 +142,7 @@ struct TrajLds {
   const int* anc;      // bit a set when node a is node n or one of its ancestors
   const int* link_node;
   const int* link_sph;
+  const int* sph_link;  // link of each sphere
   const float4* sph;
   const float4* lb;
   const int* pair_links;
@@ -152,18 +156,19 @@ struct TrajLds {
   float* vd;   // search direction (and scratch in the gradient pass)
   float* vt;   // accelerations of x
   float* red;  // block-reduction scratch
-  float* gpose;  // node poses of the waypoint each group evaluates: (group, node, 7)
+  float4* gc;   // per group: world sphere centres + radius (K), link bounding spheres (L)
 };
 
 // Cooperative copy of the model into LDS, with every index clamped into range.
-__device__ inline TrajLds stage_traj_model(const TrajModelDev& md, int max_len, char* base) {
+__device__ __forceinline__ TrajLds stage_traj_model(const TrajModelDev& md, int max_len, char* base) {
   const int N = md.n_nodes, L = md.n_links, K = md.n_spheres, P = md.n_pairs, Q = md.n_items;
   const int M = md.n_world, A = md.n_act;
-  const TrajLdsLayout lay = traj_lds_layout(N, L, K, P, Q, M, A, max_len);
+  const TrajLdsLayout lay = traj_lds_layout(N, L, K, P, Q, M, A, max_len, blockDim.x);
   float* nodes = reinterpret_cast<float*>(base + lay.nodes);
   int* anc = reinterpret_cast<int*>(base + lay.anc);
   int* link_node = reinterpret_cast<int*>(base + lay.link_node);
   int* link_sph = reinterpret_cast<int*>(base + lay.link_sph);
+  int* sph_link = reinterpret_cast<int*>(base + lay.sph_link);
   float4* sph = reinterpret_cast<float4*>(base + lay.sph);
   float4* lb = reinterpret_cast<float4*>(base + lay.lb);
   int* pair_links = reinterpret_cast<int*>(base + lay.pair_links);
@@ -230,9 +235,12 @@ __device__ inline TrajLds stage_traj_model(const TrajModelDev& md, int max_len, 
       anc[n] = (1 << n) | (p >= 0 ? anc[p] : 0);
     }
   }
+  for (int l = tid; l < L; l += blockDim.x)
+    for (int k = link_sph[l]; k < link_sph[l + 1]; ++k) sph_link[k] = l;
   __syncthreads();
   TrajLds o;
-  o.nodes = nodes; o.anc = anc; o.link_node = link_node; o.link_sph = link_sph; o.sph = sph; o.lb = lb;
+  o.nodes = nodes; o.anc = anc; o.link_node = link_node; o.link_sph = link_sph; o.sph_link = sph_link;
+  o.sph = sph; o.lb = lb;
   o.pair_links = pair_links; o.pair_start = pair_start; o.pair_items = pair_items; o.world = world;
   o.lo = lo; o.hi = hi;
   o.vx = reinterpret_cast<float*>(base + lay.vx);
@@ -240,7 +248,7 @@ __device__ inline TrajLds stage_traj_model(const TrajModelDev& md, int max_len, 
   o.vd = reinterpret_cast<float*>(base + lay.vd);
   o.vt = reinterpret_cast<float*>(base + lay.vt);
   o.red = reinterpret_cast<float*>(base + lay.red);
-  o.gpose = reinterpret_cast<float*>(base + lay.gpose);
+  o.gc = reinterpret_cast<float4*>(base + lay.gc);
   return o;
 }
 
@@ -267,7 +275,8 @@ __device__ __forceinline__ void block_sum(float (&v)[NV], float* red) {
   for (int i = 0; i < NV; ++i) {
     float s = 0.f;
 #pragma unroll
-    for (int w = 0; w < kTrajWaves; ++w) s += red[w * kRedSlots + i];
+    for (int w = 0; w < kMaxTrajWaves; ++w)
+      if (w * kWave < static_cast<int>(blockDim.x)) s += red[w * kRedSlots + i];
     v[i] = s;
   }
 }
@@ -371,82 +380,103 @@ __device__ __forceinline__ void pull_point(const WaypointNodes<MAXN, GRAD>& K, i
   }
 }
 
-__device__ __forceinline__ Pose lds_node_pose(const float* gp, int n) {
-  return (n < 0) ? pose_identity() : load_pose(gp + 7 * n);
+// World-frame sphere centres (radius in w) and link bounding spheres of the
+// waypoint with node poses K.P, this lane's share, into the group's LDS slot
+// gc = [K spheres | L link bounds].  Links are visited in table order and a
+// node's rotation is rebuilt only when the node changes (the host sorts links
+// by node).
+template <int MAXN, bool GRAD>
+__device__ __forceinline__ void waypoint_centres(const TrajModelDev& md, const TrajLds& L,
+                                                 const WaypointNodes<MAXN, GRAD>& K, float4* gc) {
+  const int g = threadIdx.x % kGroup;
+  const int Ks = md.n_spheres;
+  int prev = -2;
+  Pose T = pose_identity();
+  Rot3 R = quat_to_rot(1.f, 0.f, 0.f, 0.f);
+  for (int l = 0; l < md.n_links; ++l) {
+    const int node = __builtin_amdgcn_readfirstlane(L.link_node[l]);
+    if (node != prev) {
+      T = node_pose<MAXN>(K.P, node);
+      R = quat_to_rot(T.w, T.x, T.y, T.z);
+      prev = node;
+    }
+    const int k0 = L.link_sph[l], k1 = L.link_sph[l + 1];
+    for (int s = k0 + g; s < k1; s += kGroup) {
+      const float4 sp = L.sph[s];
+      gc[s] = make_float4(R.m00 * sp.x + R.m01 * sp.y + R.m02 * sp.z + T.tx,
+                          R.m10 * sp.x + R.m11 * sp.y + R.m12 * sp.z + T.ty,
+                          R.m20 * sp.x + R.m21 * sp.y + R.m22 * sp.z + T.tz, sp.w);
+    }
+    if (g == l % kGroup) {
+      const float4 b = L.lb[l];
+      gc[Ks + l] = make_float4(R.m00 * b.x + R.m01 * b.y + R.m02 * b.z + T.tx,
+                               R.m10 * b.x + R.m11 * b.y + R.m12 * b.z + T.ty,
+                               R.m20 * b.x + R.m21 * b.y + R.m22 * b.z + T.tz, b.w);
+    }
+  }
 }
 
 // Collision cost of one waypoint, evaluated by the kGroup lanes of a group
-// (lane g = threadIdx.x % kGroup), and with GRAD the group's dcost/dq_node
-// (per node, before the mimic map).  Every lane of the group returns the
-// group totals.  The caller has written the waypoint's node poses to `gp`
-// (this group's LDS slot) and synchronised.
+// (lane g = threadIdx.x % kGroup) from the centres in gc, and with GRAD the
+// group's dcost/dq_node (per node, before the mimic map).  Every lane of the
+// group returns the group totals.
 template <int MAXN, bool GRAD>
-__device__ float group_waypoint_coll(const TrajModelDev& md, const TrajLds& L, const WaypointNodes<MAXN, GRAD>& K,
-                                     const float* gp, float (&gn)[MAXN]) {
+__device__ __forceinline__ float group_waypoint_coll(const TrajModelDev& md, const TrajLds& L, const WaypointNodes<MAXN, GRAD>& K,
+                                     const float4* gc, float (&gn)[MAXN]) {
   const int g = threadIdx.x % kGroup;
   const float margin = md.margin;
   const int mode = md.mode;
   const float e = 1e-6f;
+  const int Ks = md.n_spheres;
   float cost = 0.f;
 #pragma unroll
   for (int a = 0; a < MAXN; ++a) gn[a] = 0.f;
 
-  // World: the link's spheres split over the group's lanes; per (link, world
-  // sphere) only the link's closest sphere counts.
-  for (int l = 0; l < md.n_links; ++l) {
+  // World: spheres s = g, g + kGroup, ... of the waypoint.
+  for (int s = g; s < Ks; s += kGroup) {
+    const float4 c = gc[s];
+    const int level = grid_level(md.grid, c.w);
+    const int* items = nullptr;
+    int count = 0;
+    if (level >= 0) grid_cell_list(md.grid, level, c.x, c.y, c.z, items, count);
+    const bool brute = level < 0;  // no level serves this radius: every world sphere
+    const int total = brute ? md.n_world : count;
+    if (total == 0) continue;
+    const int l = L.sph_link[s];
     const int k0 = L.link_sph[l], k1 = L.link_sph[l + 1];
-    if (k0 >= k1) continue;
-    const int node = __builtin_amdgcn_readfirstlane(L.link_node[l]);
+    const int node = L.link_node[l];
     const int mask = node >= 0 ? L.anc[node] : 0;
-    const Pose T = node_pose<MAXN>(K.P, node);
-    const Rot3 R = quat_to_rot(T.w, T.x, T.y, T.z);
-    for (int s = k0 + g; s < k1; s += kGroup) {
-      const float4 sp = L.sph[s];
-      const float cx = R.m00 * sp.x + R.m01 * sp.y + R.m02 * sp.z + T.tx;
-      const float cy = R.m10 * sp.x + R.m11 * sp.y + R.m12 * sp.z + T.ty;
-      const float cz = R.m20 * sp.x + R.m21 * sp.y + R.m22 * sp.z + T.tz;
-      int4 head;
-      const int* rec;
-      int n;
-      const bool listed = grid_lookup(md.grid, cx, cy, cz, sp.w, head, rec, n);
-      const int total = listed ? n : md.n_world;
-      for (int i = 0; i < total; ++i) {
-        const int m = listed ? grid_item(head, rec, i) : i;
-        if (m < 0 || m >= md.n_world) continue;
-        const float4 o = L.world[m];
-        const float dx = o.x - cx + e, dy = o.y - cy + e, dz = o.z - cz + e;
-        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float d = dist - sp.w - o.w;
-        if (!(d < margin)) continue;
-        // The (link, world sphere) term is owned by the link's lowest-index
-        // sphere within the margin; the owner takes the minimum over the link.
-        bool owner = true;
-        for (int s2 = k0; s2 < s; ++s2) {
-          const float4 p2 = L.sph[s2];
-          const float ex = o.x - (R.m00 * p2.x + R.m01 * p2.y + R.m02 * p2.z + T.tx) + e;
-          const float ey = o.y - (R.m10 * p2.x + R.m11 * p2.y + R.m12 * p2.z + T.ty) + e;
-          const float ez = o.z - (R.m20 * p2.x + R.m21 * p2.y + R.m22 * p2.z + T.tz) + e;
-          if (sqrtf(ex * ex + ey * ey + ez * ez) - p2.w - o.w < margin) { owner = false; break; }
+    for (int i = 0; i < total; ++i) {
+      const int m = brute ? i : items[i];
+      if (m < 0 || m >= md.n_world) continue;
+      const float4 o = L.world[m];
+      const float dx = o.x - c.x + e, dy = o.y - c.y + e, dz = o.z - c.z + e;
+      const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+      const float d = dist - c.w - o.w;
+      if (!(d < margin)) continue;
+      // The (link, world sphere) term is owned by the link's lowest-index
+      // sphere within the margin; the owner takes the minimum over the link.
+      bool owner = true;
+      for (int s2 = k0; s2 < s; ++s2) {
+        const float4 c2 = gc[s2];
+        const float ex = o.x - c2.x + e, ey = o.y - c2.y + e, ez = o.z - c2.z + e;
+        if (sqrtf(ex * ex + ey * ey + ez * ez) - c2.w - o.w < margin) { owner = false; break; }
+      }
+      if (!owner) continue;
+      float bd = d, bdist = dist, bx = dx, by = dy, bz = dz, bcx = c.x, bcy = c.y, bcz = c.z;
+      for (int s2 = s + 1; s2 < k1; ++s2) {
+        const float4 c2 = gc[s2];
+        const float ex = o.x - c2.x + e, ey = o.y - c2.y + e, ez = o.z - c2.z + e;
+        const float dd = sqrtf(ex * ex + ey * ey + ez * ez);
+        if (dd - c2.w - o.w < bd) {
+          bd = dd - c2.w - o.w; bdist = dd; bx = ex; by = ey; bz = ez; bcx = c2.x; bcy = c2.y; bcz = c2.z;
         }
-        if (!owner) continue;
-        float bd = d, bdist = dist, bx = dx, by = dy, bz = dz, bcx = cx, bcy = cy, bcz = cz;
-        for (int s2 = s + 1; s2 < k1; ++s2) {
-          const float4 p2 = L.sph[s2];
-          const float qx = R.m00 * p2.x + R.m01 * p2.y + R.m02 * p2.z + T.tx;
-          const float qy = R.m10 * p2.x + R.m11 * p2.y + R.m12 * p2.z + T.ty;
-          const float qz = R.m20 * p2.x + R.m21 * p2.y + R.m22 * p2.z + T.tz;
-          const float ex = o.x - qx + e, ey = o.y - qy + e, ez = o.z - qz + e;
-          const float dd = sqrtf(ex * ex + ey * ey + ez * ez);
-          if (dd - p2.w - o.w < bd) {
-            bd = dd - p2.w - o.w; bdist = dd; bx = ex; by = ey; bz = ez; bcx = qx; bcy = qy; bcz = qz;
-          }
-        }
-        float df;
-        cost += traj_act(bd, margin, mode, df);
-        if (GRAD && df != 0.f) {
-          const float k = -df / bdist;
-          pull_point<MAXN, GRAD>(K, mask, bcx, bcy, bcz, k * bx, k * by, k * bz, gn);
-        }
+      }
+      float df;
+      cost += traj_act(bd, margin, mode, df);
+      if (GRAD && df != 0.f) {
+        const float k = -df / bdist;
+        pull_point<MAXN, GRAD>(K, mask, bcx, bcy, bcz, k * bx, k * by, k * bz, gn);
       }
     }
   }
@@ -456,49 +486,31 @@ __device__ float group_waypoint_coll(const TrajModelDev& md, const TrajLds& L, c
     const int q0 = L.pair_start[p], q1 = L.pair_start[p + 1];
     if (q0 >= q1) continue;
     const int li = L.pair_links[2 * p], lj = L.pair_links[2 * p + 1];
-    const int ni = L.link_node[li], nj = L.link_node[lj];
-    const Pose Ti = lds_node_pose(gp, ni), Tj = lds_node_pose(gp, nj);
-    const Rot3 Ri = quat_to_rot(Ti.w, Ti.x, Ti.y, Ti.z);
-    const Rot3 Rj = quat_to_rot(Tj.w, Tj.x, Tj.y, Tj.z);
     {
-      const float4 bi = L.lb[li], bj = L.lb[lj];
-      const float ax = Ri.m00 * bi.x + Ri.m01 * bi.y + Ri.m02 * bi.z + Ti.tx;
-      const float ay = Ri.m10 * bi.x + Ri.m11 * bi.y + Ri.m12 * bi.z + Ti.ty;
-      const float az = Ri.m20 * bi.x + Ri.m21 * bi.y + Ri.m22 * bi.z + Ti.tz;
-      const float bx = Rj.m00 * bj.x + Rj.m01 * bj.y + Rj.m02 * bj.z + Tj.tx;
-      const float by = Rj.m10 * bj.x + Rj.m11 * bj.y + Rj.m12 * bj.z + Tj.ty;
-      const float bz = Rj.m20 * bj.x + Rj.m21 * bj.y + Rj.m22 * bj.z + Tj.tz;
-      const float dd = sqrtf((bx - ax) * (bx - ax) + (by - ay) * (by - ay) + (bz - az) * (bz - az));
+      const float4 bi = gc[Ks + li], bj = gc[Ks + lj];
+      const float dd = sqrtf((bj.x - bi.x) * (bj.x - bi.x) + (bj.y - bi.y) * (bj.y - bi.y) +
+                             (bj.z - bi.z) * (bj.z - bi.z));
       if (dd - bi.w - bj.w >= margin + 1e-4f) continue;
     }
-    float best = INFINITY, bcx = 0.f, bcy = 0.f, bcz = 0.f, bdx = 0.f, bdy = 0.f, bdz = 0.f;
-    float bex = 0.f, bey = 0.f, bez = 0.f, bdist = 1.f;
+    float best = INFINITY, bdist = 1.f, bdx = 0.f, bdy = 0.f, bdz = 0.f;
+    float4 bci = make_float4(0.f, 0.f, 0.f, 0.f), bcj = bci;
     for (int it = q0; it < q1; ++it) {
-      const float4 si = L.sph[L.pair_items[2 * it]];
-      const float4 sj = L.sph[L.pair_items[2 * it + 1]];
-      const float cix = Ri.m00 * si.x + Ri.m01 * si.y + Ri.m02 * si.z + Ti.tx;
-      const float ciy = Ri.m10 * si.x + Ri.m11 * si.y + Ri.m12 * si.z + Ti.ty;
-      const float ciz = Ri.m20 * si.x + Ri.m21 * si.y + Ri.m22 * si.z + Ti.tz;
-      const float cjx = Rj.m00 * sj.x + Rj.m01 * sj.y + Rj.m02 * sj.z + Tj.tx;
-      const float cjy = Rj.m10 * sj.x + Rj.m11 * sj.y + Rj.m12 * sj.z + Tj.ty;
-      const float cjz = Rj.m20 * sj.x + Rj.m21 * sj.y + Rj.m22 * sj.z + Tj.tz;
-      const float dx = cjx - cix + e, dy = cjy - ciy + e, dz = cjz - ciz + e;
+      const float4 ci = gc[L.pair_items[2 * it]];
+      const float4 cj = gc[L.pair_items[2 * it + 1]];
+      const float dx = cj.x - ci.x + e, dy = cj.y - ci.y + e, dz = cj.z - ci.z + e;
       const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-      const float d = dist - si.w - sj.w;
-      if (d < best) {
-        best = d;
-        bcx = cix; bcy = ciy; bcz = ciz; bex = cjx; bey = cjy; bez = cjz;
-        bdx = dx; bdy = dy; bdz = dz; bdist = dist;
-      }
+      const float d = dist - ci.w - cj.w;
+      if (d < best) { best = d; bci = ci; bcj = cj; bdx = dx; bdy = dy; bdz = dz; bdist = dist; }
     }
     if (best < margin) {
       float df;
       cost += traj_act(best, margin, mode, df);
       if (GRAD && df != 0.f) {
         const float k = df / bdist;
+        const int ni = L.link_node[li], nj = L.link_node[lj];
         const int mi = ni >= 0 ? L.anc[ni] : 0, mj = nj >= 0 ? L.anc[nj] : 0;
-        pull_point<MAXN, GRAD>(K, mj, bex, bey, bez, k * bdx, k * bdy, k * bdz, gn);
-        pull_point<MAXN, GRAD>(K, mi, bcx, bcy, bcz, -k * bdx, -k * bdy, -k * bdz, gn);
+        pull_point<MAXN, GRAD>(K, mj, bcj.x, bcj.y, bcj.z, k * bdx, k * bdy, k * bdz, gn);
+        pull_point<MAXN, GRAD>(K, mi, bci.x, bci.y, bci.z, -k * bdx, -k * bdy, -k * bdz, gn);
       }
     }
   }
@@ -538,31 +550,24 @@ template <int MAXN, bool GRAD, typename TaskFn, typename DoneFn>
 __device__ __forceinline__ void for_each_waypoint(const TrajModelDev& md, const TrajLds& L, int n_tasks,
                                                   TaskFn task, DoneFn done) {
   const int grp = threadIdx.x / kGroup;
-  const int g = threadIdx.x % kGroup;
   const int N = md.n_nodes;
-  float* gp = L.gpose + grp * 7 * N;
-  const int rounds = (n_tasks + kGroups - 1) / kGroups;
+  float4* gc = L.gc + grp * (md.n_spheres + md.n_links);
+  const int n_groups = blockDim.x / kGroup;
+  const int rounds = (n_tasks + n_groups - 1) / n_groups;
   for (int rd = 0; rd < rounds; ++rd) {
-    const int i = rd * kGroups + grp;
+    const int i = rd * n_groups + grp;
     const bool active = i < n_tasks;
     WaypointNodes<MAXN, GRAD> K;
     if (active) {
       float q[MAXN];
       task(i, q);
       waypoint_fk<MAXN, GRAD>(md, L, q, K);
-#pragma unroll
-      for (int n = 0; n < MAXN; ++n) {
-        if (n < N && (n % kGroup) == g) {
-          float* o = gp + 7 * n;
-          o[0] = K.P[n].w; o[1] = K.P[n].x; o[2] = K.P[n].y; o[3] = K.P[n].z;
-          o[4] = K.P[n].tx; o[5] = K.P[n].ty; o[6] = K.P[n].tz;
-        }
-      }
+      waypoint_centres<MAXN, GRAD>(md, L, K, gc);
     }
     __syncthreads();
     if (active) {
       float gn[MAXN], gq[MAXN];
-      const float c = group_waypoint_coll<MAXN, GRAD>(md, L, K, gp, gn);
+      const float c = group_waypoint_coll<MAXN, GRAD>(md, L, K, gc, gn);
       if (GRAD) node_to_actuated<MAXN>(L, N, gn, gq);
       done(i, c, gq);
     }
@@ -575,7 +580,7 @@ __device__ __forceinline__ void for_each_waypoint(const TrajModelDev& md, const 
 // w_smooth smooth + w_limits limits + w_coll coll (endpoint rows zeroed when
 // mask_ends).  Leaves the accelerations of x in L.vt; uses L.vd as scratch.
 template <int MAXN, bool GRAD>
-__device__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
+__device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
                                 float w_coll, bool mask_ends, float (&comps)[3]) {
   const int A = md.n_act;
   const int n = T * A;
@@ -598,7 +603,7 @@ __device__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T,
         }
       });
   const int na = T - 4;
-  for (int i = tid; i < n; i += kTrajThreads) {
+  for (int i = tid; i < n; i += blockDim.x) {
     const int a = i % A;
     const float xv = L.vx[i];
     const float v = fmaxf(xv - L.hi[a], 0.f) + fmaxf(L.lo[a] - xv, 0.f);
@@ -617,7 +622,7 @@ __device__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T,
     }
   }
   __syncthreads();
-  for (int i = tid; i < na * A; i += kTrajThreads) {
+  for (int i = tid; i < na * A; i += blockDim.x) {
     const int t = i / A;
     const float acc = L.vt[i];
     if (t + 1 < na) {
@@ -634,7 +639,7 @@ __device__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T,
   if (GRAD) {
     __syncthreads();
     const float c5[5] = {-1.f / 12.f, 16.f / 12.f, -30.f / 12.f, 16.f / 12.f, -1.f / 12.f};
-    for (int i = tid; i < n; i += kTrajThreads) {
+    for (int i = tid; i < n; i += blockDim.x) {
       const int t = i / A;
       float gs = 0.f;
       if (t >= 1) gs += 2.f * w.w_vel * (L.vx[i] - L.vx[i - A]);
@@ -662,7 +667,7 @@ __device__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T,
 // search stops at the first acceptable one, which picks the same step as
 // evaluating all five.  Returns the step index and its cost components.
 template <int MAXN>
-__device__ int traj_line_search(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
+__device__ __forceinline__ int traj_line_search(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
                                 float w_coll, float thresh, float& p_sm, float& p_lim, float& p_col) {
   const int A = md.n_act;
   const int n = T * A;
@@ -672,7 +677,7 @@ __device__ int traj_line_search(const TrajModelDev& md, const TrajLds& L, int T,
   float v[3 + kNumAlphas];
 #pragma unroll
   for (int k = 0; k < 3 + kNumAlphas; ++k) v[k] = 0.f;
-  for (int i = tid; i < n; i += kTrajThreads) {
+  for (int i = tid; i < n; i += blockDim.x) {
     const int a = i % A;
     const int t = i / A;
     const float xv = L.vx[i], dv = L.vd[i];
@@ -746,6 +751,7 @@ struct CostArgs {
   TrajModelDev md;
   const float* x;
   const long long* start;
+  const int* subset;  // block b works on trajectory subset[b] (all trajectories when null)
   int n_traj, max_len;
   long long n_waypoints;
   TrajWeights w;
@@ -755,20 +761,21 @@ struct CostArgs {
 };
 
 template <int MAXN>
-__global__ __launch_bounds__(kTrajThreads) void traj_cost_kernel(CostArgs a) {
+__global__ __launch_bounds__(kMaxTrajThreads) void traj_cost_kernel(CostArgs a) {
   extern __shared__ float4 smem4[];
   const TrajLds L = stage_traj_model(a.md, a.max_len, reinterpret_cast<char*>(smem4));
-  const int b = blockIdx.x;
+  const int b = a.subset ? a.subset[blockIdx.x] : static_cast<int>(blockIdx.x);
+  if (b < 0 || b >= a.n_traj) return;  // uniform per block
   const long long w0 = a.start[b], w1 = a.start[b + 1];
   if (w0 < 0 || w1 < w0 || w1 > a.n_waypoints || w1 - w0 > a.max_len) return;  // uniform per block
   const int T = static_cast<int>(w1 - w0);
   const int A = a.md.n_act;
-  for (int i = threadIdx.x; i < T * A; i += kTrajThreads) L.vx[i] = a.x[w0 * A + i];
+  for (int i = threadIdx.x; i < T * A; i += blockDim.x) L.vx[i] = a.x[w0 * A + i];
   __syncthreads();
   float comps[3];
   if (a.out_grad != nullptr) {
     traj_value_grad<MAXN, true>(a.md, L, T, a.w, a.w_coll, false, comps);
-    for (int i = threadIdx.x; i < T * A; i += kTrajThreads) a.out_grad[w0 * A + i] = L.vg[i];
+    for (int i = threadIdx.x; i < T * A; i += blockDim.x) a.out_grad[w0 * A + i] = L.vg[i];
   } else {
     traj_value_grad<MAXN, false>(a.md, L, T, a.w, a.w_coll, false, comps);
   }
@@ -783,6 +790,7 @@ struct LbfgsArgs {
   TrajModelDev md;
   const float* x_init;
   const long long* start;
+  const int* subset;  // block b works on trajectory subset[b] (all trajectories when null)
   int n_traj, max_len;
   long long n_waypoints;
   TrajWeights w;
@@ -799,10 +807,11 @@ struct LbfgsArgs {
 };
 
 template <int MAXN>
-__global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
+__global__ __launch_bounds__(kMaxTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
   extern __shared__ float4 smem4[];
   const TrajLds L = stage_traj_model(a.md, a.max_len, reinterpret_cast<char*>(smem4));
-  const int b = blockIdx.x;
+  const int b = a.subset ? a.subset[blockIdx.x] : static_cast<int>(blockIdx.x);
+  if (b < 0 || b >= a.n_traj) return;  // uniform per block
   const long long w0 = a.start[b], w1 = a.start[b + 1];
   if (w0 < 0 || w1 < w0 || w1 > a.n_waypoints || w1 - w0 > a.max_len) return;  // uniform per block
   const int T = static_cast<int>(w1 - w0);
@@ -824,7 +833,7 @@ __global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
   int m_used, newest;
   float rho[kMaxHist];
   if (a.it0 == 0) {
-    for (int i = tid; i < n; i += kTrajThreads) L.vx[i] = a.x_init[off + i];
+    for (int i = tid; i < n; i += blockDim.x) L.vx[i] = a.x_init[off + i];
     w_coll = a.w_coll0;
     best_cost = INFINITY;
     m_used = 0;
@@ -832,7 +841,7 @@ __global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
 #pragma unroll
     for (int h = 0; h < kMaxHist; ++h) rho[h] = 0.f;
   } else {
-    for (int i = tid; i < n; i += kTrajThreads) L.vx[i] = xs[i];
+    for (int i = tid; i < n; i += blockDim.x) L.vx[i] = xs[i];
     w_coll = sc[0];
     best_cost = sc[1];
     m_used = static_cast<int>(sc[2]);
@@ -848,7 +857,7 @@ __global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
     traj_value_grad<MAXN, true>(a.md, L, T, a.w, w_coll, true, comps);
     const float cost = a.w.w_smooth * comps[0] + a.w.w_limits * comps[1] + w_coll * comps[2];
     if (k == 0) {
-      for (int i = tid; i < n; i += kTrajThreads) {
+      for (int i = tid; i < n; i += blockDim.x) {
         xp[i] = L.vx[i];
         gp[i] = L.vg[i];
         best[i] = L.vx[i];
@@ -857,7 +866,7 @@ __global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
     }
     // Curvature pair of the last step.
     float syy[2] = {0.f, 0.f};
-    for (int i = tid; i < n; i += kTrajThreads) {
+    for (int i = tid; i < n; i += blockDim.x) {
       const float s = L.vx[i] - xp[i], y = L.vg[i] - gp[i];
       syy[0] += s * y;
       syy[1] += y * y;
@@ -865,7 +874,7 @@ __global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
     block_sum<2>(syy, L.red);
     if (syy[0] > 1e-10f * syy[1] + 1e-30f && k > 0) {
       newest = (newest + 1) % m;
-      for (int i = tid; i < n; i += kTrajThreads) {
+      for (int i = tid; i < n; i += blockDim.x) {
         sb[newest * WA + off + i] = L.vx[i] - xp[i];
         yb[newest * WA + off + i] = L.vg[i] - gp[i];
       }
@@ -880,49 +889,49 @@ __global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
       float alph[kMaxHist];
 #pragma unroll
       for (int h = 0; h < kMaxHist; ++h) alph[h] = 0.f;
-      for (int i = tid; i < n; i += kTrajThreads) L.vd[i] = L.vg[i];
+      for (int i = tid; i < n; i += blockDim.x) L.vd[i] = L.vg[i];
       for (int j = 0; j < m_used; ++j) {
         const int idx = (newest - j + m) % m;
         const float* s = sb + idx * WA + off;
         const float* y = yb + idx * WA + off;
         float sq = 0.f;
-        for (int i = tid; i < n; i += kTrajThreads) sq += s[i] * L.vd[i];
+        for (int i = tid; i < n; i += blockDim.x) sq += s[i] * L.vd[i];
         const float al = sel_hist(rho, idx) * block_sum1(sq, L.red);
 #pragma unroll
         for (int h = 0; h < kMaxHist; ++h)
           if (h == idx) alph[h] = al;
-        for (int i = tid; i < n; i += kTrajThreads) L.vd[i] -= al * y[i];
+        for (int i = tid; i < n; i += blockDim.x) L.vd[i] -= al * y[i];
       }
       float g2[2] = {0.f, 0.f};
       {
         const float* s = sb + newest * WA + off;
         const float* y = yb + newest * WA + off;
-        for (int i = tid; i < n; i += kTrajThreads) {
+        for (int i = tid; i < n; i += blockDim.x) {
           g2[0] += s[i] * y[i];
           g2[1] += y[i] * y[i];
         }
       }
       block_sum<2>(g2, L.red);
       const float gamma = g2[0] / (g2[1] + 1e-18f);
-      for (int i = tid; i < n; i += kTrajThreads) L.vd[i] *= gamma;
+      for (int i = tid; i < n; i += blockDim.x) L.vd[i] *= gamma;
       for (int st = 0; st < m_used; ++st) {
         const int idx = (newest - m_used + 1 + st + 2 * m) % m;
         const float* s = sb + idx * WA + off;
         const float* y = yb + idx * WA + off;
         float yr = 0.f;
-        for (int i = tid; i < n; i += kTrajThreads) yr += y[i] * L.vd[i];
+        for (int i = tid; i < n; i += blockDim.x) yr += y[i] * L.vd[i];
         const float beta = sel_hist(rho, idx) * block_sum1(yr, L.red);
         const float coef = sel_hist(alph, idx) - beta;
-        for (int i = tid; i < n; i += kTrajThreads) L.vd[i] += s[i] * coef;
+        for (int i = tid; i < n; i += blockDim.x) L.vd[i] += s[i] * coef;
       }
-      for (int i = tid; i < n; i += kTrajThreads) L.vd[i] = -L.vd[i];
+      for (int i = tid; i < n; i += blockDim.x) L.vd[i] = -L.vd[i];
     } else {
       float g2 = 0.f;
-      for (int i = tid; i < n; i += kTrajThreads) g2 += L.vg[i] * L.vg[i];
+      for (int i = tid; i < n; i += blockDim.x) g2 += L.vg[i] * L.vg[i];
       const float inv = 1.f / (sqrtf(block_sum1(g2, L.red)) + 1e-18f);
-      for (int i = tid; i < n; i += kTrajThreads) L.vd[i] = -L.vg[i] * inv;
+      for (int i = tid; i < n; i += blockDim.x) L.vd[i] = -L.vg[i] * inv;
     }
-    for (int i = tid; i < n; i += kTrajThreads) {
+    for (int i = tid; i < n; i += blockDim.x) {
       const int t = i / A;
       if (t == 0 || t == T - 1) L.vd[i] = 0.f;
     }
@@ -930,7 +939,7 @@ __global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
     float p_sm = 0.f, p_lim = 0.f, p_col = 0.f;
     const int pick = traj_line_search<MAXN>(a.md, L, T, a.w, w_coll, cost * (1.f - 1e-4f), p_sm, p_lim, p_col);
     const float al = traj_alpha(pick);
-    for (int i = tid; i < n; i += kTrajThreads) {
+    for (int i = tid; i < n; i += blockDim.x) {
       xp[i] = L.vx[i];
       gp[i] = L.vg[i];
       L.vx[i] += al * L.vd[i];
@@ -938,11 +947,11 @@ __global__ __launch_bounds__(kTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a) {
     const float eval = a.w.w_smooth * p_sm + a.w.w_limits * p_lim + a.w_coll_max * p_col;
     if (eval < best_cost) {
       best_cost = eval;
-      for (int i = tid; i < n; i += kTrajThreads) best[i] = L.vx[i];
+      for (int i = tid; i < n; i += blockDim.x) best[i] = L.vx[i];
     }
     __syncthreads();
   }
-  for (int i = tid; i < n; i += kTrajThreads) xs[i] = L.vx[i];
+  for (int i = tid; i < n; i += blockDim.x) xs[i] = L.vx[i];
   if (tid == 0) {
     sc[0] = w_coll;
     sc[1] = best_cost;
@@ -988,10 +997,23 @@ int fill_traj_model(const pk_traj_model* m, pk::TrajModelDev& d, int& maxn) {
   return PK_OK;
 }
 
-size_t traj_lds_bytes(const pk::TrajModelDev& d, int max_len) {
+size_t traj_lds_bytes(const pk::TrajModelDev& d, int max_len, int threads) {
   return pk::traj_lds_layout(d.n_nodes, d.n_links, d.n_spheres, d.n_pairs, d.n_items, d.n_world, d.n_act,
-                             max_len)
+                             max_len, threads)
       .total;
+}
+
+// Threads per trajectory workgroup: the requested 128/256/512, or (0) by the
+// longest trajectory, so each of the 8-lane groups gets ~1-4 waypoints.
+int pick_threads(int requested, int max_len, int& threads) {
+  if (requested == 0) {
+    threads = max_len <= 32 ? 128 : (max_len <= 64 ? 256 : 512);
+    return PK_OK;
+  }
+  PK_REQUIRE(requested == 128 || requested == 256 || requested == 512,
+             "block_threads=%d (0, 128, 256 or 512)", requested);
+  threads = requested;
+  return PK_OK;
 }
 
 }  // namespace
@@ -1005,7 +1027,8 @@ extern "C" int64_t pk_traj_lbfgs_state_floats(int64_t n_waypoints, int32_t n_act
 
 extern "C" int pk_traj_cost_grad_f32(void* stream, const pk_traj_model* model, const float* x,
                                      const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
-                                     int32_t max_len, const float* weights, float* out_comps,
+                                     int32_t max_len, int32_t block_threads, const int32_t* traj_subset,
+                                     int32_t n_subset, const float* weights, float* out_comps,
                                      float* out_grad) {
   pk::TrajModelDev d{};
   int maxn = 8;
@@ -1014,12 +1037,19 @@ extern "C" int pk_traj_cost_grad_f32(void* stream, const pk_traj_model* model, c
   PK_REQUIRE(n_traj >= 0 && n_waypoints >= 0 && max_len >= 1, "invalid trajectory sizes");
   if (n_traj == 0) return PK_OK;
   PK_REQUIRE(x && traj_start && weights && out_comps, "null pointer");
-  const size_t lds = traj_lds_bytes(d, max_len);
+  PK_REQUIRE(traj_subset == nullptr || n_subset >= 0, "negative subset size");
+  int threads = 0;
+  const int ts = pick_threads(block_threads, max_len, threads);
+  if (ts != PK_OK) return ts;
+  const int blocks = traj_subset ? n_subset : n_traj;
+  if (blocks == 0) return PK_OK;
+  const size_t lds = traj_lds_bytes(d, max_len, threads);
   PK_REQUIRE(lds <= 150 * 1024, "trajectory model needs %zu bytes of LDS (limit 150 KiB)", lds);
   pk::CostArgs a{};
   a.md = d;
   a.x = x;
   a.start = reinterpret_cast<const long long*>(traj_start);
+  a.subset = traj_subset;
   a.n_traj = n_traj;
   a.max_len = max_len;
   a.n_waypoints = n_waypoints;
@@ -1029,15 +1059,16 @@ extern "C" int pk_traj_cost_grad_f32(void* stream, const pk_traj_model* model, c
   a.out_grad = out_grad;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (maxn == 8)
-    hipLaunchKernelGGL(pk::traj_cost_kernel<8>, dim3(n_traj), dim3(pk::kTrajThreads), lds, s, a);
+    hipLaunchKernelGGL(pk::traj_cost_kernel<8>, dim3(blocks), dim3(threads), lds, s, a);
   else
-    hipLaunchKernelGGL(pk::traj_cost_kernel<16>, dim3(n_traj), dim3(pk::kTrajThreads), lds, s, a);
+    hipLaunchKernelGGL(pk::traj_cost_kernel<16>, dim3(blocks), dim3(threads), lds, s, a);
   return pk::check_launch("traj_cost_kernel");
 }
 
 extern "C" int pk_traj_lbfgs_f32(void* stream, const pk_traj_model* model, const float* x_init,
                                  const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
-                                 int32_t max_len, const pk_lbfgs_config* cfg, int32_t iter_begin,
+                                 int32_t max_len, int32_t block_threads, const int32_t* traj_subset,
+                                 int32_t n_subset, const pk_lbfgs_config* cfg, int32_t iter_begin,
                                  int32_t iter_end, float* state, float* out_best, float* out_best_cost) {
   pk::TrajModelDev d{};
   int maxn = 8;
@@ -1052,12 +1083,19 @@ extern "C" int pk_traj_lbfgs_f32(void* stream, const pk_traj_model* model, const
              "iteration range [%d, %d) outside [0, %d]", iter_begin, iter_end, cfg->n_iters);
   if (n_traj == 0 || iter_begin == iter_end) return PK_OK;
   PK_REQUIRE(x_init && traj_start && state && out_best && out_best_cost, "null pointer");
-  const size_t lds = traj_lds_bytes(d, max_len);
+  PK_REQUIRE(traj_subset == nullptr || n_subset >= 0, "negative subset size");
+  int threads = 0;
+  const int ts = pick_threads(block_threads, max_len, threads);
+  if (ts != PK_OK) return ts;
+  const int blocks = traj_subset ? n_subset : n_traj;
+  if (blocks == 0) return PK_OK;
+  const size_t lds = traj_lds_bytes(d, max_len, threads);
   PK_REQUIRE(lds <= 150 * 1024, "trajectory model needs %zu bytes of LDS (limit 150 KiB)", lds);
   pk::LbfgsArgs a{};
   a.md = d;
   a.x_init = x_init;
   a.start = reinterpret_cast<const long long*>(traj_start);
+  a.subset = traj_subset;
   a.n_traj = n_traj;
   a.max_len = max_len;
   a.n_waypoints = n_waypoints;
@@ -1081,8 +1119,8 @@ extern "C" int pk_traj_lbfgs_f32(void* stream, const pk_traj_model* model, const
   a.out_cost = out_best_cost;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (maxn == 8)
-    hipLaunchKernelGGL(pk::traj_lbfgs_kernel<8>, dim3(n_traj), dim3(pk::kTrajThreads), lds, s, a);
+    hipLaunchKernelGGL(pk::traj_lbfgs_kernel<8>, dim3(blocks), dim3(threads), lds, s, a);
   else
-    hipLaunchKernelGGL(pk::traj_lbfgs_kernel<16>, dim3(n_traj), dim3(pk::kTrajThreads), lds, s, a);
+    hipLaunchKernelGGL(pk::traj_lbfgs_kernel<16>, dim3(blocks), dim3(threads), lds, s, a);
   return pk::check_launch("traj_lbfgs_kernel");
 }

@@ -28,11 +28,10 @@ class WorldGrid(ctypes.Structure):
 
     _fields_ = [
         ("cells", ctypes.c_void_p),
+        ("items", ctypes.c_void_p),
         ("lo", ctypes.c_float * 3),
         ("cell", ctypes.c_float),
         ("dims", ctypes.c_int32 * 3),
-        ("cap", ctypes.c_int32),
-        ("stride", ctypes.c_int32),
         ("pad", ctypes.c_float),
     ]
 
@@ -112,7 +111,8 @@ _SIGNATURES = {
         + [_p, _p],  # outputs
     ),
     "pk_ls_ik_select_f32": (_i32, [_p, _p, _p, _p, _i32, _i32, _i32, _f32, _p, _p]),
-    "pk_world_grid_f32": (_i32, [_p, _p, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _i32, _i32, _p]),
+    "pk_world_grid_count_f32": (_i32, [_p, _p, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _p]),
+    "pk_world_grid_fill_f32": (_i32, [_p, _p, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _p, _p, _i64]),
     "pk_ls_ik_batched_f32": (
         _i32,
         [_p, _p, _p, _i32, _i64, _i32]  # stream, seeds, seed_start, P, total, A
@@ -126,9 +126,9 @@ _SIGNATURES = {
         + [_f32, _f32, _f32, _f32, _f32, _i32, _f32, _f32, _i32]  # attrs, grid_waves
         + [_p, _p],
     ),
-    "pk_traj_cost_grad_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _p, _p, _p]),
+    "pk_traj_cost_grad_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _p, _p, _p]),
     "pk_traj_lbfgs_state_floats": (_i64, [_i64, _i32, _i32, _i32]),
-    "pk_traj_lbfgs_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _p, _i32, _i32, _p, _p, _p]),
+    "pk_traj_lbfgs_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _p, _i32, _i32, _p, _p, _p]),
     "pk_ls_ik_select_segments_f32": (_i32, [_p, _p, _p, _p, _p, _i32, _i32, _f32, _p, _p]),
     "pk_pose_hinge_cost_f32": (
         _i32,

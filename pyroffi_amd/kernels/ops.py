@@ -252,15 +252,14 @@ def pose_hinge_cost(link_poses, spheres_local, link_sphere_start, world_spheres,
     return cost, grad
 
 
-def world_grid(world_spheres, pad: float, cell: float | None = None, cap: int = 32, max_cells: int = 48):
-    """Build the exact-culling grid of a sphere world (see ``pk_world_grid_f32``).
+def world_grid(world_spheres, pad: float, cell: float | None = None, max_cells: int = 48):
+    """Build the exact-culling CSR grid of a sphere world (``pk_world_grid_count/fill_f32``).
 
-    ``pad`` = the robot sphere radius the grid serves + cost margin (the LM
-    kernel tests larger robot spheres against every world sphere).  The grid box
-    is the world's bounding box grown by ``pad``; ``cell`` defaults to the box's longest
-    side / ``max_cells``.  Returns a dict with the packed cell records
-    ``cells (n_cells, stride)`` (count, then up to ``cap`` indices) and the grid
-    geometry.
+    ``pad`` = the robot sphere radius the grid serves + cost margin (kernels
+    test larger robot spheres against every world sphere).  The grid box is
+    the world's bounding box grown by ``pad``; ``cell`` defaults to the box's
+    longest side / ``max_cells``.  Returns a dict with ``cells (n_cells, 2)``
+    (start, count), ``items`` (world sphere indices) and the grid geometry.
     """
     import numpy as np
 
@@ -276,16 +275,20 @@ def world_grid(world_spheres, pad: float, cell: float | None = None, cap: int = 
         cell = float(max(hi - lo)) / max_cells
     dims = np.maximum(1, np.ceil((hi - lo) / cell).astype(np.int64))
     n_cells = int(np.prod(dims))
-    stride = (int(cap) + 1 + 3) // 4 * 4
-    cells = torch.empty((n_cells, stride), dtype=torch.int32, device=dev)
-    st = lib().pk_world_grid_f32(
-        _stream(), _ptr(ws), ws.shape[0], float(pad), float(lo[0]), float(lo[1]), float(lo[2]), float(cell),
-        int(dims[0]), int(dims[1]), int(dims[2]), int(cap), stride, _ptr(cells),
-    )
-    check(st, "pk_world_grid_f32")
-    return {"cells": cells, "stride": stride, "pad": float(pad), "lo": tuple(float(v) for v in lo),
-            "cell": float(cell),
-            "dims": tuple(int(v) for v in dims), "cap": int(cap)}
+    geo = (float(pad), float(lo[0]), float(lo[1]), float(lo[2]), float(cell), int(dims[0]), int(dims[1]),
+           int(dims[2]))
+    cells = torch.zeros((n_cells, 2), dtype=torch.int32, device=dev)
+    check(lib().pk_world_grid_count_f32(_stream(), _ptr(ws), ws.shape[0], *geo, _ptr(cells)),
+          "pk_world_grid_count_f32")
+    counts = cells[:, 1].to(torch.int64)
+    starts = torch.cumsum(counts, 0) - counts
+    cells[:, 0] = starts.to(torch.int32)
+    n_items = int(counts.sum().item())
+    items = torch.empty((max(n_items, 1),), dtype=torch.int32, device=dev)
+    check(lib().pk_world_grid_fill_f32(_stream(), _ptr(ws), ws.shape[0], *geo, _ptr(cells), _ptr(items), n_items),
+          "pk_world_grid_fill_f32")
+    return {"cells": cells, "items": items, "pad": float(pad), "lo": tuple(float(v) for v in lo),
+            "cell": float(cell), "dims": tuple(int(v) for v in dims)}
 
 
 def _grid_array(grid):
@@ -296,11 +299,10 @@ def _grid_array(grid):
     arr = (WorldGrid * max(1, len(levels)))()
     for i, g in enumerate(levels):
         arr[i].cells = g["cells"].data_ptr()
-        arr[i].stride = int(g["stride"])
+        arr[i].items = g["items"].data_ptr()
         arr[i].lo[:] = [float(v) for v in g["lo"]]
         arr[i].cell = float(g["cell"])
         arr[i].dims[:] = [int(v) for v in g["dims"]]
-        arr[i].cap = int(g["cap"])
         arr[i].pad = float(g["pad"])
     return (arr if levels else None), len(levels)
 

@@ -109,13 +109,19 @@ def build_traj_tables(robot, robot_coll) -> dict:
     L = int(robot.links.num_links)
     centers = np.asarray(robot_coll.coll.pose.translation(), np.float64).reshape(L, -1, 3)
     radii = np.asarray(robot_coll.coll.radius, np.float64).reshape(L, -1)
-    link_node = np.full(L, -1, np.int32)
+    node_of = np.array([-1 if int(lpj[l]) < 0 else int(anchor[int(lpj[l])]) for l in range(L)])
+    # Links grouped by node (stable), so the kernel rebuilds a node's rotation
+    # once per waypoint; the tables below are in this order.
+    order = np.argsort(node_of, kind="stable")
+    new_index = np.empty(L, np.int64)
+    new_index[order] = np.arange(L)
+    link_node = node_of[order].astype(np.int32)
     sph, start, bound = [], [0], np.zeros((L, 4), np.float32)
     slot_index = {}
-    for l in range(L):
+    for nl, l in enumerate(order):
+        l = int(l)
         pj = int(lpj[l])
-        node, r0 = (-1, _IDENTITY) if pj < 0 else (int(anchor[pj]), rel[pj])
-        link_node[l] = node
+        r0 = _IDENTITY if pj < 0 else rel[pj]
         rows = []
         for s in range(radii.shape[1]):
             r = radii[l, s]
@@ -129,7 +135,7 @@ def build_traj_tables(robot, robot_coll) -> dict:
             lo, hi = rows[:, :3].min(0), rows[:, :3].max(0)
             cb = (lo + hi) / 2
             rb = np.max(np.linalg.norm(rows[:, :3] - cb, axis=1) + rows[:, 3])
-            bound[l] = [*cb, rb * (1 + 1e-6) + 1e-6]
+            bound[nl] = [*cb, rb * (1 + 1e-6) + 1e-6]
         sph.extend(rows.tolist())
         start.append(len(sph))
     pi = np.asarray(robot_coll.active_idx_i)
@@ -140,10 +146,11 @@ def build_traj_tables(robot, robot_coll) -> dict:
         while (int(i), s) in slot_index and (int(j), s) in slot_index:
             items.append([slot_index[(int(i), s)], slot_index[(int(j), s)]])
             s += 1
-        pair_links.append([int(i), int(j)])
+        pair_links.append([int(new_index[int(i)]), int(new_index[int(j)])])
         pair_start.append(len(items))
     return {
         "nodes": fold["nodes"],
+        "link_order": order.astype(np.int32),  # table link k is robot link link_order[k]
         "n_act": int(robot.joints.num_actuated_joints),
         "link_node": link_node,
         "link_sph_start": np.asarray(start, np.int32),
@@ -179,7 +186,7 @@ class TrajModel:
             r_small, r_big = float(np.quantile(r, 0.9)), float(np.max(r))
             self.grids.append(world_grid(self.t["world"], r_small + self.margin))
             if r_big > r_small:
-                self.grids.append(world_grid(self.t["world"], r_big + self.margin, cap=128, max_cells=24))
+                self.grids.append(world_grid(self.t["world"], r_big + self.margin, max_cells=24))
         self.device = dev
         self._struct = self._make_struct()
 
@@ -236,8 +243,12 @@ def flatten_trajs(trajs, device) -> tuple[torch.Tensor, torch.Tensor, int]:
 
 
 def traj_cost(model: TrajModel, x: torch.Tensor, start: torch.Tensor, max_len: int, weights, grad: bool = True,
-              out_comps=None, out_grad=None):
-    """Components ``(B, 3)`` = (smooth, limits, coll) and the gradient ``(W, A)`` of the weighted total."""
+              out_comps=None, out_grad=None, block_threads: int = 0, subset: torch.Tensor | None = None):
+    """Components ``(B, 3)`` = (smooth, limits, coll) and the gradient ``(W, A)`` of the weighted total.
+
+    ``subset`` (int32 device tensor) restricts the launch to those trajectories;
+    ``block_threads`` (128/256/512, 0 = from ``max_len``) sets the workgroup size.
+    """
     from ..kernels._lib import check, lib
     from ..kernels.ops import _ptr, _stream
 
@@ -249,6 +260,7 @@ def traj_cost(model: TrajModel, x: torch.Tensor, start: torch.Tensor, max_len: i
     if grad and out_grad is None:
         out_grad = torch.empty_like(x)
     st = lib().pk_traj_cost_grad_f32(_stream(), model.struct_ref(), _ptr(x), _ptr(start), B, W, int(max_len),
+                                     int(block_threads), _ptr(subset), 0 if subset is None else subset.shape[0],
                                      w.ctypes.data_as(ctypes.c_void_p), _ptr(out_comps),
                                      _ptr(out_grad if grad else None))
     check(st, "pk_traj_cost_grad_f32")
@@ -265,14 +277,24 @@ def _config_struct(cfg: LbfgsTrajOptConfig):
     return c
 
 
+# Workgroup size per trajectory length: each 8-lane group gets one to four
+# waypoints, so long and short trajectories need about the same number of
+# rounds per iteration.
+_LENGTH_CLASSES = ((32, 128), (64, 256), (1 << 30, 512))
+
+
 class BatchedLbfgsTrajOpt:
     """Many trajectories (any lengths) through the direct L-BFGS in chunked launches.
 
-    ``chunk`` iterations run per launch (0: all in one); ``capture()`` records
-    every launch of a solve into one graph and ``solve()`` replays it.
+    Trajectories are split into length classes (up to 32, 64, longer), each
+    launched with its own workgroup size on a stream of its own, so the classes
+    run concurrently.  ``chunk`` iterations run per launch (0: all in one);
+    ``capture()`` records every launch of a solve into one graph and
+    ``solve()`` replays it.
     """
 
-    def __init__(self, model: TrajModel, trajs, cfg: LbfgsTrajOptConfig = LbfgsTrajOptConfig(), chunk: int = 0):
+    def __init__(self, model: TrajModel, trajs, cfg: LbfgsTrajOptConfig = LbfgsTrajOptConfig(), chunk: int = 0,
+                 split_classes: bool = True):
         from ..kernels._lib import lib
 
         self.model, self.cfg = model, cfg
@@ -287,6 +309,19 @@ class BatchedLbfgsTrajOpt:
         self.chunk = int(chunk)
         self._cfg = _config_struct(cfg)
         self.graph = None
+        lens = np.diff(self.start.cpu().numpy())
+        self.classes = []  # (subset tensor or None, n, max_len, threads)
+        if split_classes:
+            lo = 0
+            for hi, threads in _LENGTH_CLASSES:
+                idx = np.nonzero((lens > lo) & (lens <= hi))[0].astype(np.int32)
+                if len(idx):
+                    self.classes.append((torch.as_tensor(idx, device=model.device), len(idx),
+                                         int(lens[idx].max()), threads))
+                lo = hi
+        else:
+            self.classes.append((None, self.B, self.max_len, 0))
+        self._streams = None
 
     def _launches(self):
         from ..kernels._lib import check, lib
@@ -295,16 +330,32 @@ class BatchedLbfgsTrajOpt:
         n = self.cfg.n_iters
         step = n if self.chunk <= 0 else self.chunk
         W = self.x0.shape[0]
-        for b0 in range(0, max(n, 1), max(step, 1)):
-            b1 = min(b0 + step, n)
-            st = lib().pk_traj_lbfgs_f32(_stream(), self.model.struct_ref(), _ptr(self.x0), _ptr(self.start), self.B,
-                                         W, self.max_len, ctypes.byref(self._cfg), b0, b1, _ptr(self.state),
-                                         _ptr(self.best), _ptr(self.best_cost))
-            check(st, "pk_traj_lbfgs_f32")
         cfg = self.cfg
-        traj_cost(self.model, self.best, self.start, self.max_len,
-                  (cfg.w_smooth, cfg.w_vel, cfg.w_acc, cfg.w_jerk, cfg.w_limits, cfg.w_collision_max), grad=False,
-                  out_comps=self.rank_comps)
+        rank_w = (cfg.w_smooth, cfg.w_vel, cfg.w_acc, cfg.w_jerk, cfg.w_limits, cfg.w_collision_max)
+
+        def run_class(subset, count, max_len, threads):
+            for b0 in range(0, max(n, 1), max(step, 1)):
+                b1 = min(b0 + step, n)
+                st = lib().pk_traj_lbfgs_f32(_stream(), self.model.struct_ref(), _ptr(self.x0), _ptr(self.start),
+                                             self.B, W, max_len, threads, _ptr(subset),
+                                             0 if subset is None else count, ctypes.byref(self._cfg), b0, b1,
+                                             _ptr(self.state), _ptr(self.best), _ptr(self.best_cost))
+                check(st, "pk_traj_lbfgs_f32")
+            traj_cost(self.model, self.best, self.start, max_len, rank_w, grad=False, out_comps=self.rank_comps,
+                      block_threads=threads, subset=subset)
+
+        if len(self.classes) == 1:
+            run_class(*self.classes[0])
+            return
+        main = torch.cuda.current_stream()
+        if self._streams is None:
+            self._streams = [torch.cuda.Stream() for _ in self.classes]
+        for c, s in zip(self.classes, self._streams):
+            s.wait_stream(main)
+            with torch.cuda.stream(s):
+                run_class(*c)
+        for s in self._streams:
+            main.wait_stream(s)
 
     def final_iterate(self) -> torch.Tensor:
         """The last L-BFGS iterate ``(W, A)`` (a view of the state buffer)."""

@@ -110,7 +110,7 @@ class UnevenLsIkSolver:
                 r_small, r_big = float(np.quantile(rs[:, 3], 0.9)), float(np.max(rs[:, 3]))
                 d["grid"] = [world_grid(d["ws"], r_small + margin)]
                 if r_big > r_small:
-                    d["grid"].append(world_grid(d["ws"], r_big + margin, cap=128, max_cells=24))
+                    d["grid"].append(world_grid(d["ws"], r_big + margin, max_cells=24))
         else:
             d["rs"] = torch.zeros((0, 4), device=dev)
             d["rj"] = torch.zeros((0,), dtype=torch.int32, device=dev)

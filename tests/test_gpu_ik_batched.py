@@ -80,10 +80,12 @@ def _run(solver, g, **over):
 def test_world_grid_lists_every_reachable_sphere():
     ws = _world(256, 3)
     pad = 0.12
-    for cap, max_cells in ((64, 16), (4, 12)):
-        g = ops.world_grid(torch.as_tensor(ws, device="cuda"), pad, cap=cap, max_cells=max_cells)
+    for max_cells in (16, 7):
+        g = ops.world_grid(torch.as_tensor(ws, device="cuda"), pad, max_cells=max_cells)
         cells = g["cells"].cpu().numpy()
-        counts, items = cells[:, 0], cells[:, 1:cap + 1]
+        items = g["items"].cpu().numpy()
+        starts, counts = cells[:, 0], cells[:, 1]
+        np.testing.assert_array_equal(starts[1:], np.cumsum(counts)[:-1])
         nx, ny, nz = g["dims"]
         lo, cell = np.asarray(g["lo"]), g["cell"]
         c = np.arange(nx * ny * nz)
@@ -95,12 +97,10 @@ def test_world_grid_lists_every_reachable_sphere():
         must = dist <= reach[None] - 1e-5
         may = dist <= reach[None] + 1e-3
         assert np.all(counts >= must.sum(1)) and np.all(counts <= may.sum(1))
-        for k in np.nonzero(counts <= cap)[0]:
-            listed = items[k, :counts[k]]
+        for k in range(len(counts)):
+            listed = items[starts[k]:starts[k] + counts[k]]
             assert np.all(np.diff(listed) > 0)
             assert set(np.nonzero(must[k])[0]) <= set(listed.tolist()) <= set(np.nonzero(may[k])[0])
-        if cap == 4:
-            assert (counts > cap).any()  # the overflow path is exercised
 
 
 def test_initial_error_grid_vs_brute_force_vs_oracle(oracle_mod):

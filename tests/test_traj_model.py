@@ -55,14 +55,17 @@ def test_node_frame_spheres_and_pairs(oracle_mod, name):
     P = _node_poses(h["nodes"], q)
     lp = oracle_mod.fk_links(t, q[None])[0]
     start = h["link_sph_start"]
+    order = h["link_order"]
+    assert np.all(np.diff(h["link_node"]) >= 0)  # grouped by node
     for l in range(robot.links.num_links):
         n = h["link_node"][l]
-        real = sp[l][sp[l, :, 3] > 0]
+        rl = order[l]
+        real = sp[rl][sp[rl, :, 3] > 0]
         rows = h["spheres"][start[l]:start[l + 1]].astype(np.float64)
         assert len(rows) == len(real)
         for s in range(len(rows)):
             c = rows[s, :3] if n < 0 else quat_rotate(P[n][:4], rows[s, :3]) + P[n][4:]
-            ref = quat_rotate(lp[l, :4], real[s, :3]) + lp[l, 4:]
+            ref = quat_rotate(lp[rl, :4], real[s, :3]) + lp[rl, 4:]
             np.testing.assert_allclose(c, ref, atol=2e-6)
             assert rows[s, 3] == pytest.approx(real[s, 3])
             # the bounding sphere covers the sphere
@@ -74,5 +77,5 @@ def test_node_frame_spheres_and_pairs(oracle_mod, name):
         assert k1 - k0 == min(counts[i], counts[j])
         for s, (a, b) in enumerate(h["pair_items"][k0:k1]):
             assert a == start[i] + s and b == start[j] + s
-    np.testing.assert_array_equal(h["pair_links"][:, 0], coll.active_idx_i)
-    np.testing.assert_array_equal(h["pair_links"][:, 1], coll.active_idx_j)
+    np.testing.assert_array_equal(order[h["pair_links"][:, 0]], coll.active_idx_i)
+    np.testing.assert_array_equal(order[h["pair_links"][:, 1]], coll.active_idx_j)
