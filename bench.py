@@ -11,8 +11,14 @@ inputs already resident in HBM.
 Printed as one JSON line: whole-job configs/s, the roofline of the dominant
 kernel (the sphere sweep, FP32 VALU-bound; timed with HIP events on the stream
 it runs on), the FP64 CPU oracle timed on this host's cores for a bounded
-sample of the same workload, and the LM-IK line (16 targets x 64 seeds, 30
-iterations, lambda = 1e-2; solves/s and success at 1 mm / 0.01 rad).
+sample of the same workload, and one object per further workload:
+  ik        LM-IK, Panda, 16 targets x 64 seeds, 30 iterations, lambda = 1e-2
+  grad      pose + hinge cost and its adjoint, 16384 Panda configurations
+  ik_uneven 4096 targets over Panda/UR5/Fetch/Baxter, 16..256 seeds each, 50
+            iterations, 256 world spheres: fused graph-captured solves/s vs
+            one plain launch per target, at equal success
+  trajopt   512 Panda trajectories (T in {32, 64, 128}), 1024 world spheres,
+            100 L-BFGS iterations in one captured graph
 """
 
 from __future__ import annotations
@@ -187,13 +193,30 @@ def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32,
     """
     import torch
 
-    import oracle
+    from pyroffi_amd.optimization_engines._ls_ik_batched import UnevenLsIkSolver
+
+    groups, samples = uneven_workload(per_robot, plain_per_robot)
+    n_seeds = int(sum(g.seeds.shape[0] for g in groups))
+    solver = UnevenLsIkSolver(groups, max_iter=50)
+    report = solver.autotune(reps=1, cache=False)
+    solver.capture()
+    solver.solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = solver.solve()
+    torch.cuda.synchronize()
+    fused_dt = (time.perf_counter() - t0) / reps
+    winners = [w.cpu().numpy() for w, _ in out]
+    return _uneven_report(groups, samples, winners, fused_dt, n_seeds, report, per_robot, plain_per_robot,
+                          cpu_per_robot)
+
+
+def uneven_workload(per_robot: int = 1024, plain_per_robot: int = 32):
+    """Config 4 problems: per robot, FK-generated targets, log-uniform seed counts, 256 world spheres."""
     import pyroffi_amd as pk
-    from pyroffi_amd._se3 import quat_conj, quat_mul, so3_log
     from pyroffi_amd.collision import RobotCollisionSpherized
-    from pyroffi_amd.optimization_engines._ls_ik import ee_tables
-    from pyroffi_amd.optimization_engines._ls_ik_batched import (RobotGroup, UnevenLsIkSolver,
-                                                                 log_uniform_seed_counts, make_group)
+    from pyroffi_amd.optimization_engines._ls_ik_batched import RobotGroup, log_uniform_seed_counts, make_group
 
     rng = np.random.default_rng(4)
     world = np.concatenate([rng.uniform(-1, 1, (M_WORLD, 3)), rng.uniform(0.02, 0.15, (M_WORLD, 1))], 1)
@@ -215,18 +238,16 @@ def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32,
         samples.append(RobotGroup(robot=robot, target_link_indices=g.target_link_indices, targets=g.targets[:n],
                                   seeds=g.seeds[:g.seed_start[n]], seed_start=g.seed_start[:n + 1],
                                   collision=g.collision))
-    n_seeds = int(sum(g.seeds.shape[0] for g in groups))
-    solver = UnevenLsIkSolver(groups, max_iter=50)
-    report = solver.autotune(reps=1, cache=False)
-    solver.capture()
-    solver.solve()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        out = solver.solve()
-    torch.cuda.synchronize()
-    fused_dt = (time.perf_counter() - t0) / reps
-    winners = [w.cpu().numpy() for w, _ in out]
+    return groups, samples
+
+
+def _uneven_report(groups, samples, winners, fused_dt, n_seeds, report, per_robot, plain_per_robot, cpu_per_robot):
+    import torch
+
+    import oracle
+    from pyroffi_amd._se3 import quat_conj, quat_mul, so3_log
+    from pyroffi_amd.optimization_engines._ls_ik import ee_tables
+    from pyroffi_amd.optimization_engines._ls_ik_batched import UnevenLsIkSolver
 
     plain = UnevenLsIkSolver(samples, max_iter=50)
     plain.solve_per_call()
@@ -296,6 +317,28 @@ def trajopt_bench(reps: int, cpu_sample: int = 16):
     import torch
 
     import oracle
+
+    robot, coll, world, trajs, lens, cfg, model, opt = trajopt_workload()
+    opt.capture()
+    opt.solve()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(reps):
+        best, best_cost, rank = opt.solve()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    gpu_ms = e0.elapsed_time(e1) / reps
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    return _trajopt_report(robot, coll, world, trajs, lens, cfg, model, opt, best_cost, rank, dt, gpu_ms, lo, hi,
+                           cpu_sample)
+
+
+def trajopt_workload():
+    """Config 5 inputs: 512 Panda straight-line trajectories (T in {32, 64, 128}), 1024 world spheres."""
     import pyroffi_amd as pk
     from pyroffi_amd.collision import RobotCollisionSpherized
     from pyroffi_amd.optimization_engines._lbfgs_trajopt_optimization import (BatchedLbfgsTrajOpt,
@@ -315,19 +358,13 @@ def trajopt_bench(reps: int, cpu_sample: int = 16):
     cfg = LbfgsTrajOptConfig()
     model = TrajModel(robot, coll, world, cfg.collision_margin, mode=1)
     opt = BatchedLbfgsTrajOpt(model, trajs, cfg)
-    opt.capture()
-    opt.solve()
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(reps):
-        best, best_cost, rank = opt.solve()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
-    gpu_ms = e0.elapsed_time(e1) / reps
+    return robot, coll, world, trajs, lens, cfg, model, opt
+
+
+def _trajopt_report(robot, coll, world, trajs, lens, cfg, model, opt, best_cost, rank, dt, gpu_ms, lo, hi,
+                    cpu_sample):
+    import oracle
+
     comps = opt.rank_comps.cpu().numpy()
     # Initial cost at the maximum collision weight, for the reduction ratio.
     from pyroffi_amd.optimization_engines._lbfgs_trajopt_optimization import traj_cost
@@ -490,6 +527,27 @@ def load_traffic():
     return None
 
 
+def profile_workload(name: str):
+    """Short runs for rocprofv3: a few solves of the config-4 or config-5 workload."""
+    import torch
+
+    if name == "ik_uneven":
+        from pyroffi_amd.optimization_engines._ls_ik_batched import UnevenLsIkSolver
+
+        groups, _ = uneven_workload()
+        solver = UnevenLsIkSolver(groups, max_iter=50)
+        for g, tune in zip(groups, ((4096, 0), (2048, 0), (2048, 0), (1024, 0))):
+            g.tune = tune
+        for _ in range(3):
+            solver.solve()
+    else:
+        *_, opt = trajopt_workload()
+        for _ in range(3):
+            opt.solve()
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": name}), flush=True)
+
+
 def profile_run():
     import torch
 
@@ -511,12 +569,16 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=B)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ik", action="store_true")
-    ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: 5 steps of the workload plus a 256 MiB device copy "
-                         "(the byte-count calibration for the HBM counters)")
+    ap.add_argument("--profile", nargs="?", const="sweep", choices=("sweep", "ik_uneven", "trajopt"),
+                    help="short run for rocprofv3: 'sweep' = 5 steps of the headline workload plus a 256 MiB "
+                         "device copy (the byte-count calibration for the HBM counters); 'ik_uneven' / "
+                         "'trajopt' = 3 solves of the uneven LM-IK / trajectory workload")
     args = ap.parse_args()
-    if args.profile:
+    if args.profile == "sweep":
         profile_run()
+        return
+    if args.profile:
+        profile_workload(args.profile)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the runs here have one "

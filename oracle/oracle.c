@@ -22,6 +22,12 @@
  *   or_ls_ik         _ls_ik_cuda_kernel.cu:64-394 with the residual/Jacobian
  *                    of _ik_cuda_helpers.cuh:433-631 and the Cholesky of
  *                    _ik_cuda_helpers.cuh:207-243
+ *   or_pose_hinge_cost  pose residual + CHOMP hinge^2 (_chomp_optimization.py:140-154)
+ *   or_traj_cost     trajectory smoothness / limits / collision costs and
+ *                    their analytic gradient (_sco_optimization.py:170-222,
+ *                    collision/_collision.py:136-167)
+ *   or_two_loop      _sco_optimization.py:119-167
+ *   or_traj_lbfgs    _lbfgs_trajopt_optimization.py:76-205
  *
  * Every function here is pinned by tests/test_oracle.py against checks that
  * share no code with it (known answers, scipy, finite differences).

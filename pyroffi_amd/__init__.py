@@ -5,7 +5,11 @@ Public API (same names and arguments as pyroffi):
 * ``Robot.from_urdf`` / ``Robot.forward_kinematics``
 * ``collision.RobotCollisionSpherized``, ``collision.CUDARobotCollisionChecker``,
   ``collision.colldist_from_sdf``
-* ``ls_ik_solve_cuda`` / ``ls_ik_solve_cuda_batch``
+* ``ls_ik_solve_cuda`` / ``ls_ik_solve_cuda_batch``; ``UnevenLsIkSolver`` for
+  uneven multi-robot batches in one graph
+* ``costs.pose_hinge_cost`` (differentiable, hand-written adjoint)
+* ``lbfgs_trajopt`` / ``LbfgsTrajOptConfig`` (direct L-BFGS trajectory
+  optimisation) and ``lbfgs_trajopt_uneven``
 
 All computation runs in the HIP library ``libpyroffi_amd.so`` (C ABI in
 ``include/pyroffi_amd.h``); torch supplies device memory and the stream.
@@ -16,9 +20,20 @@ from ._robot import Robot
 from ._robot_urdf_parser import JointInfo, LinkInfo, RobotURDFParser
 from ._se3 import SE3, SO3
 from ._urdf import URDF, bundled_robot_names, bundled_srdf_path, bundled_urdf_path, load_bundled_urdf
-from .optimization_engines import ls_ik_solve_cuda, ls_ik_solve_cuda_batch
+from .optimization_engines import (
+    LbfgsTrajOptConfig,
+    UnevenLsIkSolver,
+    lbfgs_trajopt,
+    lbfgs_trajopt_uneven,
+    ls_ik_solve_cuda,
+    ls_ik_solve_cuda_batch,
+)
 
 __all__ = [
+    "LbfgsTrajOptConfig",
+    "UnevenLsIkSolver",
+    "lbfgs_trajopt",
+    "lbfgs_trajopt_uneven",
     "SE3",
     "SO3",
     "URDF",

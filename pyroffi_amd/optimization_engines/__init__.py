@@ -1,5 +1,14 @@
-"""IK solvers."""
+"""IK and trajectory solvers."""
 
+from ._lbfgs_trajopt_optimization import LbfgsTrajOptConfig, lbfgs_trajopt, lbfgs_trajopt_uneven
 from ._ls_ik import ls_ik_solve_cuda, ls_ik_solve_cuda_batch
+from ._ls_ik_batched import UnevenLsIkSolver
 
-__all__ = ["ls_ik_solve_cuda", "ls_ik_solve_cuda_batch"]
+__all__ = [
+    "LbfgsTrajOptConfig",
+    "UnevenLsIkSolver",
+    "lbfgs_trajopt",
+    "lbfgs_trajopt_uneven",
+    "ls_ik_solve_cuda",
+    "ls_ik_solve_cuda_batch",
+]
