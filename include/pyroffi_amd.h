@@ -160,11 +160,14 @@ int pk_collision_self_sphere_f32(void* stream, const float* centers,
  *                     [link_sphere_start[n], link_sphere_start[n+1])
  *   world_spheres     (Ms, 4) float32
  *   self_entries      (n_entries) uint32: one per sphere pair to test,
- *                     (a | b << 10 | p << 20): sphere indices a, b < 1024
- *                     into spheres_local and active pair p < 4096
+ *                     (a | b << 10 | p << 20): sphere indices a, b into
+ *                     spheres_local and active pair p < 4096, entries of one
+ *                     pair adjacent; n_spheres <= 1023 when n_entries > 0;
+ *                     entries out of range are skipped
  *   outputs (any may be NULL):
- *     out_world       (batch, n_links, Ms)  per-link min over its spheres
- *                     (1e9 for links without spheres)  == compute_world_collision_distance
+ *     out_world       (batch, n_links, Ms)  per-link min over its spheres,
+ *                     starting from 1e9 and skipping NaN distances as the
+ *                     reference does  == compute_world_collision_distance
  *     out_world_min   (batch)               min over all links and obstacles
  *     out_self        (batch, n_pairs)      == compute_self_collision_distance
  *                     (full S_i x S_j minimum, the reference CUDA semantics)

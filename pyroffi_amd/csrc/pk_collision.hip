@@ -54,34 +54,158 @@ struct SweepArgs {
   float* out_self;           // (B, P)
 };
 
+// Self entry i | j << 10 | pair << 20 with the sphere indices checked
+// against K and the pair against P; entries out of range use the blank slot K
+// (K < 1024) and a pair inside [0, P).
+__device__ __forceinline__ unsigned self_entry(unsigned en, int K, int P) {
+  const int i = static_cast<int>(en & 1023u), j = static_cast<int>((en >> 10) & 1023u);
+  const int p = static_cast<int>(en >> 20);
+  const bool ok = i < K && j < K && p < P;
+  const unsigned bi = static_cast<unsigned>(ok ? i : K), bj = static_cast<unsigned>(ok ? j : K);
+  return bi | (bj << 10) | (static_cast<unsigned>(max(min(p, P - 1), 0)) << 20);
+}
+
+// Expanded distance test of one link against a lane's T obstacles.
+//
+// With the link origin p, the rotated sphere offset w = R s and the obstacle
+// offset q = o - p, |c - o|^2 = |w|^2 - 2 w.q + |q|^2: one float4 (-2w, |w|^2)
+// per sphere turns a pair into three FMAs and a minimum, and |q|^2 is added
+// once per group.  Rounding of that form is at most 12 u (r^2 + |q|^2), with
+// u = 2^-24 and r^2 >= |w|^2 over the link; a squared distance a with
+// sqrt(a) >= 0.4 (r^2 + |q|^2) + 1e-4 therefore has |sqrt(a) - |c - o|| <=
+// 1e-6 m.  Groups below that bound (deep contacts) and NaN results are
+// recomputed with the subtraction form, which keeps the skip-NaN minimum of
+// the reference.
+constexpr float kTauScale = 0.4f;
+constexpr float kTauFloor = 1e-4f;
+
+template <int T>
+__device__ __forceinline__ void link_sweep(int n, int g0, int g1, const float4* sc, const float4* se,
+                                           const int* gstart, const float* ptile, const float* lrad2,
+                                           const float (&ox)[T], const float (&oy)[T], const float (&oz)[T],
+                                           float (&acc)[T]) {
+  const float px = ptile[7 * n + 4], py = ptile[7 * n + 5], pz = ptile[7 * n + 6];
+  const float c0 = fmaf(kTauScale, lrad2[n], kTauFloor);
+  float qx[T], qy[T], qz[T], qq[T], rt[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    qx[t] = ox[t] - px;
+    qy[t] = oy[t] - py;
+    qz[t] = oz[t] - pz;
+    qq[t] = qx[t] * qx[t] + qy[t] * qy[t] + qz[t] * qz[t];
+    rt[t] = fmaf(kTauScale, qq[t], c0);
+  }
+  for (int g = g0; g < g1; ++g) {
+    const int k0 = gstart[g], k1 = gstart[g + 1];
+    const float rg = sc[k0].w;
+    float m[T];
+    {
+      const float4 e = se[k0];
+#pragma unroll
+      for (int t = 0; t < T; ++t) m[t] = fmaf(e.x, qx[t], fmaf(e.y, qy[t], fmaf(e.z, qz[t], e.w)));
+    }
+    int k = k0 + 1;
+    for (; k + 1 < k1; k += 2) {  // two spheres per three-operand minimum
+      const float4 e = se[k], f = se[k + 1];
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+        m[t] = __builtin_elementwise_minimum(
+            m[t], __builtin_elementwise_minimum(fmaf(e.x, qx[t], fmaf(e.y, qy[t], fmaf(e.z, qz[t], e.w))),
+                                                fmaf(f.x, qx[t], fmaf(f.y, qy[t], fmaf(f.z, qz[t], f.w)))));
+    }
+    if (k < k1) {
+      const float4 e = se[k];
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+        m[t] = __builtin_elementwise_minimum(m[t], fmaf(e.x, qx[t], fmaf(e.y, qy[t], fmaf(e.z, qz[t], e.w))));
+    }
+    float v[T];
+    bool bad[T];
+    bool exact = false;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const float d = fast_sqrt(m[t] + qq[t]);
+      bad[t] = !(d >= rt[t]);
+      exact = exact || bad[t];
+      v[t] = d - rg;
+    }
+    if (__any(exact)) {  // rare: some lane is in deep contact or saw a NaN
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        if (bad[t]) {
+          float e2 = INFINITY;
+          for (int j = k0; j < k1; ++j) {
+            const float4 s = sc[j];
+            const float dx = s.x - ox[t], dy = s.y - oy[t], dz = s.z - oz[t];
+            e2 = fminf(e2, dx * dx + dy * dy + dz * dz);
+          }
+          const float x = fast_sqrt(e2) - rg;
+          v[t] = (x == x) ? x : INFINITY;  // inf - inf is skipped like a NaN
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) acc[t] = __builtin_elementwise_minimum(acc[t], v[t]);
+  }
+}
+
+// Waves per sweep workgroup.  They share the per-launch tables (local
+// spheres, self entries, groups) in LDS and each works through its own block
+// of configurations with private sphere, pose and pair-minimum buffers, so a
+// CU holds several times more waves than with one table copy per wave.
+constexpr int kSweepWaves = 4;
+
+// Orders one wave's LDS writes before its own later LDS reads from other
+// lanes.  LDS instructions of a wave execute in order, so a wavefront-scope
+// barrier only has to stop the compiler from moving accesses across it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // SRC = 0: link poses + local spheres (fused path); SRC = 1: precomputed
 // world-frame sphere slots (the reference FFI layout).  T = obstacles per lane.
+// Six waves per SIMD (at most 80 VGPRs) for up to four obstacles per lane,
+// three with eight, which need the registers.
 template <int SRC, int T>
-__global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
+__global__ __launch_bounds__(kWave * kSweepWaves, T == 8 ? 3 : 6) void sweep_kernel(SweepArgs a) {
   extern __shared__ float smem[];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
+  // The wave index is uniform across the wave: in a scalar register it keeps
+  // the wave's buffer addresses and configuration range out of VGPRs.
+  const int lane = tid % kWave, wave = __builtin_amdgcn_readfirstlane(tid / kWave);
   const int L = a.n_links;
   const int K = a.n_spheres;
   const int M = a.n_world;
-  // LDS carve-up.
-  float4* sc = reinterpret_cast<float4*>(smem);               // K world spheres
-  float4* sl = sc + K;                                        // K local spheres (SRC 0)
-  int* lstart = reinterpret_cast<int*>(sl + (SRC == 0 ? K : 0));  // L + 1
-  int* slink = lstart + (L + 1);                              // K: link of sphere (SRC 0)
-  int* gstart = slink + (SRC == 0 ? K : 0);                   // K + 1: sphere groups
-  int* lgroup = gstart + (K + 1);                             // L + 1: groups of each link
-  float* ptile = reinterpret_cast<float*>(lgroup + (L + 1));  // L * 7 poses (SRC 0)
-  unsigned* pmin = reinterpret_cast<unsigned*>(ptile + (SRC == 0 ? L * 7 : 0));  // P
-  unsigned* ents = pmin + a.n_pairs;                          // n_entries
+  const int E = a.n_entries;
+  const int P = a.n_pairs;
+  // LDS carve-up: shared tables, then one float4 block and one word block per wave.
+  const int f4_per_wave = (K + 1) + (SRC == 0 ? K : 0);
+  const int words_per_wave = (SRC == 0 ? 7 * L : 0) + P;
+  float4* sl = reinterpret_cast<float4*>(smem);                          // K local spheres (SRC 0)
+  float4* sc = sl + (SRC == 0 ? K : 0) + wave * f4_per_wave;            // K world spheres + 1 blank
+  float4* se = sc + (K + 1);                                             // K expanded entries (SRC 0)
+  unsigned* ents = reinterpret_cast<unsigned*>(sl + (SRC == 0 ? K : 0) + kSweepWaves * f4_per_wave);  // E
+  int* lstart = reinterpret_cast<int*>(ents + E);                        // L + 1
+  int* slink = lstart + (L + 1);                                         // K: link of sphere (SRC 0)
+  int* gstart = slink + (SRC == 0 ? K : 0);                              // K + 1: sphere groups
+  int* lgroup = gstart + (K + 1);                                        // L + 1: groups of each link
+  float* lrad2 = reinterpret_cast<float*>(lgroup + (L + 1));             // L: bound of |offset|^2 (SRC 0)
+  float* ptile = lrad2 + (SRC == 0 ? L : 0) + wave * words_per_wave;     // L * 7 poses (SRC 0)
+  unsigned* pmin = reinterpret_cast<unsigned*>(ptile + (SRC == 0 ? 7 * L : 0));  // P
 
-  // Per-launch tables.
+  // Per-launch tables, staged by the whole workgroup.
+  const int nt = kWave * kSweepWaves;
   if (SRC == 0) {
-    for (int k = lane; k < K; k += kWave) {
+    // A NaN radius makes every distance of its sphere NaN, which the minima
+    // skip; a radius of -inf makes them +inf, which the minima skip as well.
+    for (int k = tid; k < K; k += nt) {
       const float* s = a.spheres_local + 4 * k;
-      sl[k] = make_float4(s[0], s[1], s[2], s[3]);
+      sl[k] = make_float4(s[0], s[1], s[2], s[3] != s[3] ? -INFINITY : s[3]);
     }
     // Clamp the CSR offsets into a monotone [0, K] sequence.
-    if (lane == 0) {
+    if (tid == 0) {
       int prev = 0;
       for (int n = 0; n <= L; ++n) {
         const int v = (n == L) ? K : a.link_sphere_start[n];
@@ -89,27 +213,32 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
         lstart[n] = (n == 0) ? 0 : prev;
       }
     }
-    for (int k = lane; k < K; k += kWave) slink[k] = 0;
+    for (int k = tid; k < K; k += nt) slink[k] = 0;
   } else {
-    for (int n = lane; n <= L; n += kWave) lstart[n] = n * a.s_per_link;
+    for (int n = tid; n <= L; n += nt) lstart[n] = n * a.s_per_link;
   }
-  // Self entries that point outside the tables are dropped.
-  for (int e = lane; e < a.n_entries; e += kWave) {
-    const unsigned en = a.self_entries[e];
-    const bool ok = static_cast<int>(en & 1023u) < K && static_cast<int>((en >> 10) & 1023u) < K &&
-                    static_cast<int>(en >> 20) < a.n_pairs;
-    ents[e] = ok ? en : 0xffffffffu;
-  }
+  // The blank slot K is infinitely far from everything: distances to it are
+  // +inf, which no minimum takes.
+  if (lane == 0) sc[K] = make_float4(0.f, 0.f, 0.f, -INFINITY);
+  for (int e = tid; e < E; e += nt) ents[e] = self_entry(a.self_entries[e], K, P);
   __syncthreads();
   if (SRC == 0) {
-    for (int n = lane; n < L; n += kWave)
-      for (int k = lstart[n]; k < lstart[n + 1]; ++k) slink[k] = n;
+    for (int n = tid; n < L; n += nt) {
+      float r2 = 0.f;
+      for (int k = lstart[n]; k < lstart[n + 1]; ++k) {
+        slink[k] = n;
+        const float4 s = sl[k];
+        r2 = fmaxf(r2, s.x * s.x + s.y * s.y + s.z * s.z);
+      }
+      // Slack for the rounding of the rotated offsets.
+      lrad2[n] = r2 * 1.001f + 1e-12f;
+    }
   }
   // Sphere groups: runs of equal radius inside a link (SRC 0), or single
   // slots (SRC 1, whose radii are per configuration).  For one obstacle,
   // min_k(|c_k - o|) - r = sqrt(min_k |c_k - o|^2) - r inside a group, so a
   // group costs one square root per obstacle.
-  if (lane == 0) {
+  if (tid == 0) {
     int g = 0;
     for (int n = 0; n < L; ++n) {
       lgroup[n] = g;
@@ -120,7 +249,8 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
     gstart[g] = K;
   }
 
-  // This lane's obstacles (first chunk stays resident when M <= 64 * T).
+  // This lane's obstacles.  The launcher picks T with M <= 64 T for T < 8,
+  // so only T = 8 sweeps the world in several chunks and reloads them.
   const int chunk = kWave * T;
   const int n_chunks = (M + chunk - 1) / chunk;
   float ox[T], oy[T], oz[T], orad[T];
@@ -139,23 +269,26 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
   if (n_chunks > 0) load_obstacles(0);
   __syncthreads();
 
-  // Each workgroup owns a contiguous block of configurations, so the pose
-  // rows it reads and the distance rows it writes share cache lines.
-  const long long per = (a.batch + gridDim.x - 1) / gridDim.x;
-  const long long b_begin = static_cast<long long>(blockIdx.x) * per;
+  // Each wave owns a contiguous block of configurations, so the pose rows it
+  // reads and the distance rows it writes share cache lines.  From here on
+  // the waves of a workgroup run independently.
+  const long long n_waves = static_cast<long long>(gridDim.x) * kSweepWaves;
+  const long long per = (a.batch + n_waves - 1) / n_waves;
+  const long long b_begin = (static_cast<long long>(blockIdx.x) * kSweepWaves + wave) * per;
   const long long b_end = min(a.batch, b_begin + per);
   for (long long b = b_begin; b < b_end; ++b) {
     // 1. Robot spheres of configuration b in the world frame -> LDS.
     if (SRC == 0) {
       const float* pb = a.link_poses + b * L * 7;
       for (int f = lane; f < L * 7; f += kWave) ptile[f] = pb[f];
-      __syncthreads();
+      wave_sync();
       for (int k = lane; k < K; k += kWave) {
         const float* p = ptile + 7 * slink[k];
         const float4 s = sl[k];
         float wx, wy, wz;
         quat_rotate(p[0], p[1], p[2], p[3], s.x, s.y, s.z, wx, wy, wz);
         sc[k] = make_float4(wx + p[4], wy + p[5], wz + p[6], s.w);
+        se[k] = make_float4(-2.f * wx, -2.f * wy, -2.f * wz, wx * wx + wy * wy + wz * wz);
       }
     } else {
       const long long BK = a.batch * K;
@@ -169,20 +302,22 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
                         r < 0.f ? -INFINITY : r);
       }
     }
-    for (int p = lane; p < a.n_pairs; p += kWave) pmin[p] = float_to_ordered(kBig);
-    __syncthreads();
+    for (int p = lane; p < P; p += kWave) pmin[p] = float_to_ordered(kBig);
+    wave_sync();
 
     // 2. World sweep.
     float gmin = INFINITY;
     if (a.out_world != nullptr || a.out_min != nullptr) {
       for (int c = 0; c < n_chunks; ++c) {
-        if (c > 0) load_obstacles(c);
+        if (T == 8 && c > 0) load_obstacles(c);
         for (int n = 0; n < L; ++n) {
           float acc[T];
 #pragma unroll
           for (int t = 0; t < T; ++t) acc[t] = INFINITY;
           const int g1 = lgroup[n + 1];
-          for (int g = lgroup[n]; g < g1; ++g) {
+          if (SRC == 0) {
+            if (lgroup[n] < g1) link_sweep<T>(n, lgroup[n], g1, sc, se, gstart, ptile, lrad2, ox, oy, oz, acc);
+          } else for (int g = lgroup[n]; g < g1; ++g) {
             const int k0 = gstart[g], k1 = gstart[g + 1];
             const float rg = sc[k0].w;
             if (SRC == 1 && rg == -INFINITY) continue;  // padding slot (wave-uniform)
@@ -200,52 +335,60 @@ __global__ __launch_bounds__(64) void sweep_kernel(SweepArgs a) {
 #pragma unroll
             for (int t = 0; t < T; ++t) acc[t] = fminf(acc[t], fast_sqrt(m2[t]) - rg);
           }
+          // The reference's per-link minimum starts at kBig and skips NaN
+          // (_collision_cuda_kernel.cu:186-197): v is never NaN, so the
+          // per-configuration minimum needs no NaN handling.  Padding lanes
+          // (orad = -inf) give kBig.
+          float v[T];
 #pragma unroll
-          for (int t = 0; t < T; ++t) {
-            const float v = (acc[t] == INFINITY) ? kBig : acc[t] - orad[t];
-            const int m = c * chunk + t * kWave + lane;
-            if (m < M) {
-              if (a.out_world != nullptr) a.out_world[(b * L + n) * M + m] = v;
-              gmin = fminf(gmin, v);
+          for (int t = 0; t < T; ++t) v[t] = fminf(kBig, acc[t] - orad[t]);
+          if (a.out_world != nullptr) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+              const int m = c * chunk + t * kWave + lane;
+              if (m < M) a.out_world[(b * L + n) * M + m] = v[t];
             }
           }
+#pragma unroll
+          for (int t = 0; t < T; ++t) gmin = __builtin_elementwise_minimum(gmin, v[t]);
         }
       }
       if (a.out_min != nullptr) {
         const float w = wave_min(gmin);
-        if (lane == 0) a.out_min[b] = (w == INFINITY) ? kBig : w;
+        if (lane == 0) a.out_min[b] = fminf(kBig, w);
       }
-      if (n_chunks > 1) load_obstacles(0);
+      if (T == 8 && n_chunks > 1) load_obstacles(0);
     }
 
     // 3. Self pairs.  Entries are grouped by pair; lane l reduces the
-    // contiguous run [l E / 64, (l + 1) E / 64) in registers and publishes one
-    // atomic min per pair it touches.
+    // contiguous run [l E / 64, (l + 1) E / 64) in registers and publishes an
+    // atomic min where the pair changes and at the end of its run.
     if (a.out_self != nullptr) {
-      const int E = a.n_entries;
       const int e0 = static_cast<int>((static_cast<long long>(E) * lane) / kWave);
       const int e1 = static_cast<int>((static_cast<long long>(E) * (lane + 1)) / kWave);
-      unsigned cur_p = 0xffffffffu;
+      const int e_last = max(E - 1, 0);  // with E = 0 the read stays inside the LDS block and is unused
+      unsigned w = ents[min(e0, e_last)];
+      unsigned p = 0;
       float cur = INFINITY;
       for (int e = e0; e < e1; ++e) {
-        const unsigned en = ents[e];
-        if (en == 0xffffffffu) continue;
-        const unsigned p = en >> 20;
-        if (p != cur_p) {
-          if (cur_p != 0xffffffffu) atomicMin(&pmin[cur_p], float_to_ordered(cur));
-          cur_p = p;
+        const unsigned wn = ents[min(e + 1, e_last)];  // next entry's load overlaps this one
+        const float4 s1 = sc[w & 1023u];
+        const float4 s2 = sc[(w >> 10) & 1023u];
+        const float dx = s1.x - s2.x, dy = s1.y - s2.y, dz = s1.z - s2.z;
+        const float d = fast_sqrt(dx * dx + dy * dy + dz * dz) - s1.w - s2.w;
+        cur = (d < cur) ? d : cur;  // a NaN distance is skipped
+        p = w >> 20;
+        if ((wn >> 20) != p) {  // last entry of its pair
+          atomicMin(&pmin[p], float_to_ordered(cur));
           cur = INFINITY;
         }
-        const float4 s1 = sc[en & 1023u];
-        const float4 s2 = sc[(en >> 10) & 1023u];
-        const float dx = s1.x - s2.x, dy = s1.y - s2.y, dz = s1.z - s2.z;
-        cur = fminf(cur, fast_sqrt(dx * dx + dy * dy + dz * dz) - s1.w - s2.w);
+        w = wn;
       }
-      if (cur_p != 0xffffffffu) atomicMin(&pmin[cur_p], float_to_ordered(cur));
-      __syncthreads();
-      for (int p = lane; p < a.n_pairs; p += kWave) a.out_self[b * a.n_pairs + p] = ordered_to_float(pmin[p]);
+      if (e1 > e0) atomicMin(&pmin[p], float_to_ordered(cur));  // +inf changes nothing
+      wave_sync();
+      for (int q = lane; q < P; q += kWave) a.out_self[b * P + q] = ordered_to_float(pmin[q]);
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
@@ -290,19 +433,23 @@ static int pick_t(int M) {
 template <int SRC>
 static int launch_sweep(void* stream, const SweepArgs& a) {
   const int K = a.n_spheres, L = a.n_links;
-  size_t lds = sizeof(float4) * K + sizeof(int) * (L + 1) + sizeof(int) * (K + 1) + sizeof(int) * (L + 1) +
-               sizeof(unsigned) * (a.n_pairs + a.n_entries);
-  if (SRC == 0) lds += sizeof(float4) * K + sizeof(int) * K + sizeof(float) * L * 7;
+  const size_t per_wave = sizeof(float4) * ((K + 1) + (SRC == 0 ? K : 0)) +
+                          sizeof(float) * ((SRC == 0 ? 7 * L : 0) + a.n_pairs);
+  size_t lds = sizeof(float4) * (SRC == 0 ? K : 0) + sizeof(unsigned) * a.n_entries +
+               sizeof(int) * ((L + 1) + (SRC == 0 ? K : 0) + (K + 1) + (L + 1)) +
+               sizeof(float) * (SRC == 0 ? L : 0) + per_wave * kSweepWaves;
+  PK_REQUIRE(a.n_entries == 0 || K < 1024, "self entries address at most 1023 spheres (got %d)", K);
   PK_REQUIRE(lds <= 150 * 1024, "sphere tables need %zu bytes of LDS (limit 150 KiB)", lds);
-  // Enough waves to cover every CU several times; each loops over configs.
-  const long long want = (a.batch + 7) / 8;
-  const unsigned grid = static_cast<unsigned>(std::max(1LL, std::min(want, 4096LL)));
+  // About eight configurations per wave and up to 8192 waves (32 per CU);
+  // each wave loops over its block of configurations.
+  const long long want = (a.batch + 8 * kSweepWaves - 1) / (8 * kSweepWaves);
+  const unsigned grid = static_cast<unsigned>(std::max(1LL, std::min(want, 8192LL / kSweepWaves)));
   hipStream_t s = static_cast<hipStream_t>(stream);
   switch (pick_t(a.n_world)) {
-    case 1: hipLaunchKernelGGL((sweep_kernel<SRC, 1>), dim3(grid), dim3(kWave), lds, s, a); break;
-    case 2: hipLaunchKernelGGL((sweep_kernel<SRC, 2>), dim3(grid), dim3(kWave), lds, s, a); break;
-    case 4: hipLaunchKernelGGL((sweep_kernel<SRC, 4>), dim3(grid), dim3(kWave), lds, s, a); break;
-    default: hipLaunchKernelGGL((sweep_kernel<SRC, 8>), dim3(grid), dim3(kWave), lds, s, a); break;
+    case 1: hipLaunchKernelGGL((sweep_kernel<SRC, 1>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a); break;
+    case 2: hipLaunchKernelGGL((sweep_kernel<SRC, 2>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a); break;
+    case 4: hipLaunchKernelGGL((sweep_kernel<SRC, 4>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a); break;
+    default: hipLaunchKernelGGL((sweep_kernel<SRC, 8>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a); break;
   }
   return check_launch("sweep_kernel");
 }

@@ -99,6 +99,33 @@ def test_world_and_self_distances(oracle_mod, name, m):
     np.testing.assert_allclose(got_self, selfd, atol=TOL, rtol=0)
 
 
+@pytest.mark.parametrize("name", ["panda", "fetch"])
+def test_world_distances_deep_contact(oracle_mod, name):
+    """Obstacles on and next to the robot's sphere centres: squared distances
+    from 0 to a few cm, where the kernel switches from the expanded form to the
+    subtraction form."""
+    robot, path = load_robot(name)
+    coll = RobotCollisionSpherized.from_urdf(path)
+    q = random_cfgs(robot, 8, seed=41).astype(np.float32)
+    lp = oracle_mod.fk_links(oracle_mod.Tables.of(robot), q.astype(np.float64))  # (8, N, 7)
+    sp = oracle_mod.padded_spheres(coll)  # (N, S, 4)
+    link, slot = np.nonzero(sp[:, :, 3] > 0)
+    rng = np.random.default_rng(42)
+    b, k = rng.integers(0, 8, 300), rng.integers(0, len(link), 300)
+    qw, qv = lp[b, link[k], :1], lp[b, link[k], 1:4]
+    v = sp[link[k], slot[k], :3]
+    t = 2.0 * np.cross(qv, v)
+    pick = v + qw * t + np.cross(qv, t) + lp[b, link[k], 4:]
+    offs = rng.normal(size=(300, 3))
+    offs *= (np.geomspace(1e-6, 0.08, 300) / np.linalg.norm(offs, axis=1))[:, None]
+    world = np.concatenate([pick + offs, rng.uniform(0.01, 0.1, (300, 1))], 1).astype(np.float32)
+    full, mins, _ = _oracle_distances(oracle_mod, robot, coll, q, world)
+    chk = CUDARobotCollisionChecker(coll)
+    wg = Sphere.from_center_and_radius(world[:, :3], world[:, 3])
+    np.testing.assert_allclose(chk.compute_world_collision_distance(robot, q, wg), full, atol=TOL, rtol=0)
+    np.testing.assert_allclose(chk.compute_min_world_distance(robot, q, wg), mins, atol=TOL, rtol=0)
+
+
 def test_compute_all_config2_shape(oracle_mod):
     """The fused two-launch path on 4096 configs x 256 world spheres."""
     robot, path = load_robot("panda")
