@@ -67,14 +67,18 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Rounding allowance of the expanded reach test: 32 unit roundoffs (2^-24).
+constexpr float kHingeTol = 2e-6f;
+
 template <int T>
-__global__ __launch_bounds__(64) void pose_hinge_kernel(HingeArgs a) {
+__global__ __launch_bounds__(64, 4) void pose_hinge_kernel(HingeArgs a) {
   extern __shared__ float4 smem4[];
   const int lane = threadIdx.x;
   const int L = a.n_links, K = a.n_spheres, M = a.n_world, J = a.n_joints;
   float4* sl = smem4;                                        // K local spheres
   float4* sc = sl + K;                                       // K world spheres
-  float* wl = reinterpret_cast<float*>(sc + K);              // L * 6 link wrenches
+  float4* se = sc + K;                                       // K expanded entries
+  float* wl = reinterpret_cast<float*>(se + K);              // L * 6 link wrenches
   float* ptile = wl + L * 6;                                 // L * 7 poses
   float* gacc = ptile + L * 7;                               // n_act
   float* wsub = gacc + a.n_act;                              // J * 6 subtree wrenches
@@ -82,6 +86,7 @@ __global__ __launch_bounds__(64) void pose_hinge_kernel(HingeArgs a) {
   int* slink = lstart + (L + 1);                             // K
   int* topo = slink + K;                                     // J
   JointAdj* jt = reinterpret_cast<JointAdj*>(topo + J);      // J
+  float* lbnd = reinterpret_cast<float*>(jt + J);            // L: r_link^2 + r_max^2
 
   if (lane == 0) {
     int prev = 0;
@@ -112,8 +117,16 @@ __global__ __launch_bounds__(64) void pose_hinge_kernel(HingeArgs a) {
     topo[j] = (tj >= 0 && tj < J) ? tj : j;
   }
   __syncthreads();
-  for (int n = lane; n < L; n += kWave)
-    for (int k = lstart[n]; k < lstart[n + 1]; ++k) slink[k] = n;
+  for (int n = lane; n < L; n += kWave) {
+    float r2 = 0.f, rm = 0.f;
+    for (int k = lstart[n]; k < lstart[n + 1]; ++k) {
+      slink[k] = n;
+      const float4 q = sl[k];
+      r2 = fmaxf(r2, q.x * q.x + q.y * q.y + q.z * q.z);
+      rm = fmaxf(rm, fabsf(q.w));
+    }
+    lbnd[n] = r2 * 1.001f + rm * rm;  // slack for the rounding of the rotated offsets
+  }
 
   const int chunk = kWave * T;
   const int n_chunks = (M + chunk - 1) / chunk;
@@ -125,8 +138,8 @@ __global__ __launch_bounds__(64) void pose_hinge_kernel(HingeArgs a) {
       if (m < M) {
         ox[t] = a.world[4 * m]; oy[t] = a.world[4 * m + 1]; oz[t] = a.world[4 * m + 2];
         orm[t] = a.world[4 * m + 3] + a.margin;
-      } else {  // contributes nothing: far away with an infinitely negative reach
-        ox[t] = 1e6f; oy[t] = 1e6f; oz[t] = 1e6f; orm[t] = -INFINITY;
+      } else {  // contributes nothing: far away, no reach
+        ox[t] = 1e6f; oy[t] = 1e6f; oz[t] = 1e6f; orm[t] = 0.f;
       }
     }
   };
@@ -148,6 +161,7 @@ __global__ __launch_bounds__(64) void pose_hinge_kernel(HingeArgs a) {
       float wx, wy, wz;
       quat_rotate(p[0], p[1], p[2], p[3], s.x, s.y, s.z, wx, wy, wz);
       sc[k] = make_float4(wx + p[4], wy + p[5], wz + p[6], s.w);
+      se[k] = make_float4(-2.f * wx, -2.f * wy, -2.f * wz, wx * wx + wy * wy + wz * wz - s.w * s.w);
     }
     __syncthreads();
 
@@ -157,21 +171,43 @@ __global__ __launch_bounds__(64) void pose_hinge_kernel(HingeArgs a) {
       for (int n = 0; n < L; ++n) {
         const int k0 = lstart[n], k1 = lstart[n + 1];
         if (k0 == k1) continue;
+        // Reach test in the link frame: with w = R s, q = o - p and
+        // R' = R_o + margin, v = |w|^2 - r^2 - 2 r R' - 2 w.q + |q|^2 - R'^2
+        // = d^2 - (r + R')^2, three FMAs and one more for the radius term.
+        // Its rounding is below tol = kHingeTol (r_link^2 + r_max^2 + |q|^2 +
+        // R'^2), so v >= tol means the hinge is zero; the rest take the exact
+        // form.
+        const float px = ptile[7 * n + 4], py = ptile[7 * n + 5], pz = ptile[7 * n + 6];
+        const float tb = kHingeTol * lbnd[n];
+        float qx[T], qy[T], qz[T], ct[T];  // ct = |q|^2 - R'^2 - tol
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          qx[t] = ox[t] - px;
+          qy[t] = oy[t] - py;
+          qz[t] = oz[t] - pz;
+          const float qq = qx[t] * qx[t] + qy[t] * qy[t] + qz[t] * qz[t];
+          ct[t] = fmaf(-orm[t], orm[t], qq) - fmaf(kHingeTol, fmaf(orm[t], orm[t], qq), tb);
+        }
         float F0 = 0.f, F1 = 0.f, F2 = 0.f, T0 = 0.f, T1 = 0.f, T2 = 0.f;
         for (int k = k0; k < k1; ++k) {
           const float4 s = sc[k];
+          const float4 e = se[k];
+          const float m2r = -2.f * s.w;
           float fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll
           for (int t = 0; t < T; ++t) {
-            const float dx = s.x - ox[t], dy = s.y - oy[t], dz = s.z - oz[t];
-            const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
-            const float rs = __builtin_amdgcn_rsqf(d2);
-            const float h = fmaxf((s.w + orm[t]) - d2 * rs, 0.f);  // margin - d
-            cost = fmaf(h, h, cost);
-            const float g = -2.f * h * rs;
-            fx = fmaf(g, dx, fx);
-            fy = fmaf(g, dy, fy);
-            fz = fmaf(g, dz, fz);
+            const float v = fmaf(e.x, qx[t], fmaf(e.y, qy[t], fmaf(e.z, qz[t], fmaf(m2r, orm[t], e.w)))) + ct[t];
+            if (v < 0.f) {
+              const float dx = s.x - ox[t], dy = s.y - oy[t], dz = s.z - oz[t];
+              const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
+              const float rs = __builtin_amdgcn_rsqf(d2);
+              const float h = fmaxf((s.w + orm[t]) - d2 * rs, 0.f);  // margin - d
+              cost = fmaf(h, h, cost);
+              const float g = -2.f * h * rs;
+              fx = fmaf(g, dx, fx);
+              fy = fmaf(g, dy, fy);
+              fz = fmaf(g, dz, fz);
+            }
           }
           if (want_grad) {
             F0 += fx; F1 += fy; F2 += fz;
@@ -287,8 +323,8 @@ extern "C" int pk_pose_hinge_cost_f32(
                   ori_weight, twists, parent_idx, act_idx, mimic_act_idx, mimic_mul,
                   topo_inv, link_parent_joint, out_grad ? n_joints : 0, n_act, out_cost, out_grad};
   const int J = a.n_joints;
-  const size_t lds = sizeof(float4) * 2 * n_spheres +
-                     sizeof(float) * (n_links * 13 + n_act + J * 6) +
+  const size_t lds = sizeof(float4) * 3 * n_spheres +
+                     sizeof(float) * (n_links * 14 + n_act + J * 6) +
                      sizeof(int) * (n_links + 1 + n_spheres + J) + sizeof(pk::JointAdj) * J;
   PK_REQUIRE(lds <= 150 * 1024, "tables need %zu bytes of LDS", lds);
   const long long want = (batch + 3) / 4;

@@ -48,6 +48,40 @@ def test_cost_and_gradient_vs_fd(oracle_mod, name, M):
     assert rel.max() <= 1e-4, (rel.max(), np.argmax(rel))
 
 
+def test_reach_boundary_vs_fd(oracle_mod):
+    """Obstacles 1-10 mm inside and outside the hinge reach r + R + margin of a
+    robot sphere: the kernel's expanded reach test must hand every active pair
+    to the exact hinge and skip only inactive ones."""
+    robot, coll, t, link, _, q, tgt = _setup(oracle_mod, "panda", 16, 1, seed=77)
+    margin = 0.05
+    sp = oracle_mod.padded_spheres(coll)  # (N, S, 4)
+    lp = oracle_mod.fk_links(t, q.astype(np.float64))
+    li, si = np.nonzero(sp[:, :, 3] > 0)
+    rng = np.random.default_rng(78)
+    M = 256
+    b, k = rng.integers(0, 16, M), rng.integers(0, len(li), M)
+    qw, qv = lp[b, li[k], :1], lp[b, li[k], 1:4]
+    v = sp[li[k], si[k], :3]
+    tt = 2.0 * np.cross(qv, v)
+    centre = v + qw * tt + np.cross(qv, tt) + lp[b, li[k], 4:]
+    R = rng.uniform(0.02, 0.1, M)
+    delta = np.geomspace(1e-3, 1e-2, M) * np.where(np.arange(M) % 2 == 0, 1.0, -1.0)
+    u = rng.normal(size=(M, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    world = np.concatenate([centre + u * (sp[li[k], si[k], 3] + R + margin + delta)[:, None], R[:, None]], 1)
+    world = world.astype(np.float32)
+    qt = torch.as_tensor(q, device="cuda").requires_grad_(True)
+    cost = pose_hinge_cost(robot, coll, qt, tgt.astype(np.float32), world, margin=margin, ee_link=-1)
+    (grad,) = torch.autograd.grad(cost.sum(), qt)
+    ocost = oracle_mod.pose_hinge_cost(t, q.astype(np.float64), sp, world.astype(np.float64), margin, -1, tgt)
+    assert np.all(ocost > 0)
+    np.testing.assert_allclose(cost.detach().cpu().numpy(), ocost, rtol=1e-5, atol=1e-9)
+    fd = oracle_mod.pose_hinge_fd_grad(t, q.astype(np.float64), sp, world.astype(np.float64), margin, -1, tgt)
+    g = grad.cpu().numpy()
+    rel = np.linalg.norm(g - fd, axis=1) / np.maximum(np.linalg.norm(fd, axis=1), 1e-6)
+    assert rel.max() <= 1e-4, (rel.max(), np.argmax(rel))
+
+
 def test_cotangent_scaling_and_no_grad_path(oracle_mod):
     robot, coll, t, link, world, q, tgt = _setup(oracle_mod, "panda", 33, 512, seed=5)
     qt = torch.as_tensor(q, device="cuda").requires_grad_(True)
