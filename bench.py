@@ -415,12 +415,13 @@ def _trajopt_report(robot, coll, world, trajs, lens, cfg, model, opt, best_cost,
     }
 
 
-def grad_bench(reps: int, cpu_sample: int):
-    """Config 3: Panda, 16384 configs, pose + hinge (margin 0.05) against 512 world spheres,
-    cost and dcost/dq through the hand-written adjoint (autograd Function)."""
+def grad_workload():
+    """Config 3 inputs: Panda, 16384 configs, FK-generated targets, 512 world spheres.
+    Returns (robot, coll, link, q_np, world_np, q, tgt, world, step, run): ``step`` is one
+    cost + dcost/dq through the autograd Function, ``run`` one launch of the adjoint kernel
+    on precomputed link poses."""
     import torch
 
-    import oracle
     import pyroffi_amd as pk
     from pyroffi_amd.collision import RobotCollisionSpherized
     from pyroffi_amd.costs import pose_hinge_cost
@@ -445,6 +446,23 @@ def grad_bench(reps: int, cpu_sample: int):
         (g,) = torch.autograd.grad(c.sum(), qq)
         return c, g
 
+    t = robot.device_tables()
+    ct = coll.device_tables(q.device)
+    poses = fk_links(q, *t.robot_buffers(), t.link_parent_joint)
+    run = lambda: raw_cost(poses, ct["spheres_local"], ct["link_sphere_start"], world, 0.05, link, tgt, t.twists,
+                           t.parent_idx, t.act_idx, t.mimic_act_idx, t.mimic_mul, t.topo_inv, t.link_parent_joint, 7)
+    return robot, coll, link, q_np, world_np, q, tgt, world, step, run
+
+
+def grad_bench(reps: int, cpu_sample: int):
+    """Config 3: Panda, 16384 configs, pose + hinge (margin 0.05) against 512 world spheres,
+    cost and dcost/dq through the hand-written adjoint (autograd Function)."""
+    import torch
+
+    import oracle
+
+    robot, coll, link, q_np, world_np, q, tgt, world, step, run = grad_workload()
+    nb, m = q_np.shape[0], world_np.shape[0]
     step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -453,13 +471,8 @@ def grad_bench(reps: int, cpu_sample: int):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
     # The adjoint kernel alone (HIP events on the launch stream).
-    t = robot.device_tables()
-    ct = coll.device_tables(q.device)
-    poses = fk_links(q, *t.robot_buffers(), t.link_parent_joint)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    run = lambda: raw_cost(poses, ct["spheres_local"], ct["link_sphere_start"], world, 0.05, link, tgt, t.twists,
-                           t.parent_idx, t.act_idx, t.mimic_act_idx, t.mimic_mul, t.topo_inv, t.link_parent_joint, 7)
     run()
     e0.record(stream)
     for _ in range(reps):
@@ -528,10 +541,19 @@ def load_traffic():
 
 
 def profile_workload(name: str):
-    """Short runs for rocprofv3: a few solves of the config-4 or config-5 workload."""
+    """Short runs for rocprofv3: 20 adjoint launches of the config-3 workload (plus the copy
+    calibration), or a few solves of the config-4 or config-5 workload."""
     import torch
 
-    if name == "ik_uneven":
+    if name == "grad":
+        *_, run = grad_workload()
+        for _ in range(20):
+            run()
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)  # reads 256 MiB, writes 256 MiB
+    elif name == "ik_uneven":
         from pyroffi_amd.optimization_engines._ls_ik_batched import UnevenLsIkSolver
 
         groups, _ = uneven_workload()
@@ -569,10 +591,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=B)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ik", action="store_true")
-    ap.add_argument("--profile", nargs="?", const="sweep", choices=("sweep", "ik_uneven", "trajopt"),
+    ap.add_argument("--profile", nargs="?", const="sweep", choices=("sweep", "grad", "ik_uneven", "trajopt"),
                     help="short run for rocprofv3: 'sweep' = 5 steps of the headline workload plus a 256 MiB "
-                         "device copy (the byte-count calibration for the HBM counters); 'ik_uneven' / "
-                         "'trajopt' = 3 solves of the uneven LM-IK / trajectory workload")
+                         "device copy (the byte-count calibration for the HBM counters); 'grad' = 20 launches "
+                         "of the config-3 adjoint kernel plus the same copy; 'ik_uneven' / 'trajopt' = 3 solves "
+                         "of the uneven LM-IK / trajectory workload")
     args = ap.parse_args()
     if args.profile == "sweep":
         profile_run()

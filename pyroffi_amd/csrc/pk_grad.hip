@@ -61,10 +61,22 @@ struct JointAdj {
   int src, parent, child_link;
 };
 
+// Sum over the wave with DPP moves (no LDS): quad swaps, the half-row and
+// row mirrors leave each 16-lane row's sum in every lane of the row; the
+// row_bcast15 / row_bcast31 steps fold rows 0-2 into row 3.  The total is
+// read from lane 63 (a wave-uniform value).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+  v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_add<0x140, 0xf>(v);  // row_mirror
+  v = dpp_add<0x142, 0xa>(v);  // row_bcast15 into rows 1, 3
+  v = dpp_add<0x143, 0xc>(v);  // row_bcast31 into rows 2, 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // Rounding allowance of the expanded reach test: 32 unit roundoffs (2^-24).
@@ -179,14 +191,14 @@ __global__ __launch_bounds__(64, 4) void pose_hinge_kernel(HingeArgs a) {
         // form.
         const float px = ptile[7 * n + 4], py = ptile[7 * n + 5], pz = ptile[7 * n + 6];
         const float tb = kHingeTol * lbnd[n];
-        float qx[T], qy[T], qz[T], ct[T];  // ct = |q|^2 - R'^2 - tol
+        float qx[T], qy[T], qz[T], nct[T];  // nct = tol - (|q|^2 - R'^2)
 #pragma unroll
         for (int t = 0; t < T; ++t) {
           qx[t] = ox[t] - px;
           qy[t] = oy[t] - py;
           qz[t] = oz[t] - pz;
           const float qq = qx[t] * qx[t] + qy[t] * qy[t] + qz[t] * qz[t];
-          ct[t] = fmaf(-orm[t], orm[t], qq) - fmaf(kHingeTol, fmaf(orm[t], orm[t], qq), tb);
+          nct[t] = fmaf(kHingeTol, fmaf(orm[t], orm[t], qq), tb) - fmaf(-orm[t], orm[t], qq);
         }
         float F0 = 0.f, F1 = 0.f, F2 = 0.f, T0 = 0.f, T1 = 0.f, T2 = 0.f;
         for (int k = k0; k < k1; ++k) {
@@ -196,8 +208,8 @@ __global__ __launch_bounds__(64, 4) void pose_hinge_kernel(HingeArgs a) {
           float fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll
           for (int t = 0; t < T; ++t) {
-            const float v = fmaf(e.x, qx[t], fmaf(e.y, qy[t], fmaf(e.z, qz[t], fmaf(m2r, orm[t], e.w)))) + ct[t];
-            if (v < 0.f) {
+            const float v = fmaf(e.x, qx[t], fmaf(e.y, qy[t], fmaf(e.z, qz[t], fmaf(m2r, orm[t], e.w))));
+            if (v < nct[t]) {
               const float dx = s.x - ox[t], dy = s.y - oy[t], dz = s.z - oz[t];
               const float d2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f);
               const float rs = __builtin_amdgcn_rsqf(d2);
