@@ -61,23 +61,7 @@ struct JointAdj {
   int src, parent, child_link;
 };
 
-// Sum over the wave with DPP moves (no LDS): quad swaps, the half-row and
-// row mirrors leave each 16-lane row's sum in every lane of the row; the
-// row_bcast15 / row_bcast31 steps fold rows 0-2 into row 3.  The total is
-// read from lane 63 (a wave-uniform value).
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_add(float v) {
-  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
-}
-__device__ __forceinline__ float wave_sum(float v) {
-  v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
-  v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
-  v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
-  v = dpp_add<0x140, 0xf>(v);  // row_mirror
-  v = dpp_add<0x142, 0xa>(v);  // row_bcast15 into rows 1, 3
-  v = dpp_add<0x143, 0xc>(v);  // row_bcast31 into rows 2, 3
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-}
+__device__ __forceinline__ float wave_sum(float v) { return wave_sum_dpp(v); }
 
 // Rounding allowance of the expanded reach test: 32 unit roundoffs (2^-24).
 constexpr float kHingeTol = 2e-6f;

@@ -145,4 +145,30 @@ __device__ __forceinline__ void fk_walk(const JointRec* recs, int J, CfgFn cfg_l
   }
 }
 
+// Reductions with DPP lane moves (no LDS traffic).  Quad swaps and the
+// half-row / row mirrors combine lanes inside 16-lane rows; row_bcast15 and
+// row_bcast31 fold rows 0-2 into row 3.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+
+// Sum over the 64 lanes; the result is wave-uniform (read from lane 63).
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_add<0x140, 0xf>(v);  // row_mirror
+  v = dpp_add<0x142, 0xa>(v);  // row_bcast15 into rows 1, 3
+  v = dpp_add<0x143, 0xc>(v);  // row_bcast31 into rows 2, 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Sum over each aligned group of 8 lanes, returned in every lane of the group.
+__device__ __forceinline__ float group8_sum_dpp(float v) {
+  v = dpp_add<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  return dpp_add<0x141, 0xf>(v);  // row_half_mirror: lane i + lane 7 - i
+}
+
 }  // namespace pk

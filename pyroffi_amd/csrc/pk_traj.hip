@@ -252,11 +252,7 @@ __device__ __forceinline__ TrajLds stage_traj_model(const TrajModelDev& md, int 
   return o;
 }
 
-__device__ __forceinline__ float traj_wave_sum(float v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
-}
+__device__ __forceinline__ float traj_wave_sum(float v) { return wave_sum_dpp(v); }
 
 // Sum of NV values over the workgroup; every thread gets the totals.
 template <int NV>
@@ -315,11 +311,8 @@ __device__ __forceinline__ Pose node_pose(const Pose (&P)[MAXN], int n) {
   return r;
 }
 
-__device__ __forceinline__ float group_sum(float v) {
-#pragma unroll
-  for (int off = 1; off < kGroup; off <<= 1) v += __shfl_xor(v, off, kWave);
-  return v;
-}
+static_assert(kGroup == 8, "group sums are 8-lane DPP reductions");
+__device__ __forceinline__ float group_sum(float v) { return group8_sum_dpp(v); }
 
 // Node kinematics of one waypoint, computed by every lane of the group:
 // poses P, and (for the adjoint) each node's twist in world axes (w, v).
