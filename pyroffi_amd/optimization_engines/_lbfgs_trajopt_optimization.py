@@ -280,7 +280,9 @@ def _config_struct(cfg: LbfgsTrajOptConfig):
 # Workgroup size per trajectory length: each 8-lane group gets one to four
 # waypoints, so long and short trajectories need about the same number of
 # rounds per iteration.
-_LENGTH_CLASSES = ((32, 128), (64, 256), (1 << 30, 512))
+# (longest length, workgroup threads): 8 lanes per waypoint, so trajectories
+# up to 32 and 64 waypoints evaluate in one round; longer ones take several.
+_LENGTH_CLASSES = ((32, 256), (64, 512), (1 << 30, 512))
 
 
 class BatchedLbfgsTrajOpt:
@@ -288,7 +290,9 @@ class BatchedLbfgsTrajOpt:
 
     Trajectories are split into length classes (up to 32, 64, longer), each
     launched with its own workgroup size on a stream of its own, so the classes
-    run concurrently.  ``chunk`` iterations run per launch (0: all in one);
+    run concurrently.  The longest class sets the solve time and is launched
+    first, so its workgroups are placed before the shorter classes fill the
+    remaining CUs.  ``chunk`` iterations run per launch (0: all in one);
     ``capture()`` records every launch of a solve into one graph and
     ``solve()`` replays it.
     """
@@ -319,6 +323,7 @@ class BatchedLbfgsTrajOpt:
                     self.classes.append((torch.as_tensor(idx, device=model.device), len(idx),
                                          int(lens[idx].max()), threads))
                 lo = hi
+            self.classes.reverse()  # longest first
         else:
             self.classes.append((None, self.B, self.max_len, 0))
         self._streams = None
