@@ -54,6 +54,7 @@ class RobotGroup:
     collision: tuple | None = None  # (robot spheres (K,4), joint idx (K,), world (M,4))
     device: dict = field(default_factory=dict)
     tune: tuple = (0, 0)
+    tune_ms: float = 0.0  # autotuned time of the group alone (launch order)
 
 
 class UnevenLsIkSolver:
@@ -137,14 +138,20 @@ class UnevenLsIkSolver:
                               out_idx=d["best_idx"])
 
     def run_eager(self) -> None:
-        """Every group on a stream of its own (forked from and joined to the current one)."""
+        """Every group on a stream of its own (forked from and joined to the current one).
+
+        Groups are launched longest first (by their autotuned time): the longest
+        sets the batch time, and its workgroups take the CUs before the others
+        fill the remaining ones.
+        """
         if len(self.groups) == 1:
             self._run_group(self.groups[0], *self.groups[0].tune)
             return
         main = torch.cuda.current_stream()
         if self._streams is None:
             self._streams = [torch.cuda.Stream() for _ in self.groups]
-        for g, s in zip(self.groups, self._streams):
+        order = sorted(self.groups, key=lambda g: -g.tune_ms)
+        for g, s in zip(order, self._streams):
             s.wait_stream(main)
             with torch.cuda.stream(s):
                 self._run_group(g, *g.tune)
@@ -170,7 +177,8 @@ class UnevenLsIkSolver:
         for g in self.groups:
             key = self._signature(g)
             if key in saved:
-                g.tune = tuple(saved[key])
+                g.tune = tuple(saved[key][:2])
+                g.tune_ms = float(saved[key][2]) if len(saved[key]) > 2 else 0.0
                 report[key] = {"choice": g.tune, "cached": True}
                 continue
             timings = {}
@@ -184,7 +192,8 @@ class UnevenLsIkSolver:
                     torch.cuda.synchronize()
                     timings[(gw, ch)] = (time.perf_counter() - t0) / reps
             g.tune = min(timings, key=timings.get)
-            saved[key] = list(g.tune)
+            g.tune_ms = timings[g.tune] * 1e3
+            saved[key] = [*g.tune, g.tune_ms]
             report[key] = {"choice": g.tune, "ms": {f"{k[0]}/{k[1]}": v * 1e3 for k, v in timings.items()}}
         if cache:
             try:
