@@ -365,8 +365,22 @@ __host__ __device__ inline IkLds ik_lds_layout(size_t shared_bytes, int J, int K
 
 constexpr float kDampFloor = 1e-6f;
 
+// Lanes below which the single-launch kernel gives every seed 8 lanes:
+// 256 CUs x 4 SIMDs x 2 waves x 64 lanes.
+constexpr long long kSmallBatchLanes = 256LL * 4 * 2 * 64;
+
 __device__ __forceinline__ float ls_alpha(int i) {
   return i == 0 ? 1.f : (i == 1 ? 0.5f : (i == 2 ? 0.25f : (i == 3 ? 0.1f : 0.025f)));
+}
+
+// One step of the line-search argmin over an 8-lane group: take the partner
+// lane's (error, step index) through a DPP move and keep the smaller error,
+// the lower index on ties.
+template <int CTRL>
+__device__ __forceinline__ void ls_argmin_step(float& e, int& i) {
+  const float pe = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(e), CTRL, 0xf, 0xf, false));
+  const int pi = __builtin_amdgcn_mov_dpp(i, CTRL, 0xf, 0xf, false);
+  if (pe < e || (pe == e && pi < i)) { e = pe; i = pi; }
 }
 
 // Per-seed solver state (registers).
@@ -402,7 +416,10 @@ __device__ __forceinline__ float weighted_error(const IkView<MAXA, NEE>& v, cons
 }
 
 // LM iterations [it0, it1) of one seed (the algorithm in the file header).
-template <int MAXA, int NEE>
+// G = 1: one lane per seed.  G = 8: the 8 lanes of an aligned group hold the
+// same seed, compute the same step, and lane k evaluates line-search step k
+// (k < 5); an argmin over the group gives every lane the sequential pick.
+template <int MAXA, int NEE, int G = 1>
 __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const float (&tg)[NEE][7],
                                            const float* W, const IkArgs& a, LmState<MAXA>& st,
                                            int it0, int it1, const GridArgs& grid) {
@@ -573,16 +590,33 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
       for (int i = 0; i < MAXA; ++i) delta[i] *= sc;
     }
 
-    // Line search: evaluate all five steps, keep the best.
+    // Line search: evaluate all five steps, keep the best (the first on ties).
     float best_a_err = 1e30f;
     int best_a = 0;
+    if (G == 1) {
 #pragma unroll 1
-    for (int ai = 0; ai < 5; ++ai) {
+      for (int ai = 0; ai < 5; ++ai) {
+        float qt[MAXA];
+#pragma unroll
+        for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + ls_alpha(ai) * delta[i]));
+        const float et = weighted_error(v, tg, W, qt, P, r, Tee, grid);
+        if (et < best_a_err) { best_a_err = et; best_a = ai; }
+      }
+    } else {
+      static_assert(G == 1 || G == 8, "line-search groups are 1 or 8 lanes");
+      const int ai = min(static_cast<int>(threadIdx.x % G), 4);
       float qt[MAXA];
 #pragma unroll
       for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + ls_alpha(ai) * delta[i]));
       const float et = weighted_error(v, tg, W, qt, P, r, Tee, grid);
-      if (et < best_a_err) { best_a_err = et; best_a = ai; }
+      // Sequential semantics: only errors below 1e30 replace the start value
+      // (NaN never does); the first index wins ties.
+      float e = (et < 1e30f) ? et : INFINITY;
+      int idx = ai;
+      ls_argmin_step<0xb1>(e, idx);   // quad_perm [1,0,3,2]
+      ls_argmin_step<0x4e>(e, idx);   // quad_perm [2,3,0,1]
+      ls_argmin_step<0x141>(e, idx);  // row_half_mirror
+      if (e != INFINITY) { best_a_err = e; best_a = idx; }
     }
     const float al = ls_alpha(best_a);
     float qt[MAXA];
@@ -652,18 +686,22 @@ __device__ __forceinline__ void load_targets(const IkArgs& a, long long problem,
       tg[e][c] = (e < a.n_ee) ? a.target_T[(problem * a.n_ee + e) * 7 + c] : (c == 0 ? 1.f : 0.f);
 }
 
-template <int MAXA, int NEE>
+// G lanes per seed (see lm_iterate); the group's first lane writes the result.
+template <int MAXA, int NEE, int G>
 __global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
   extern __shared__ float4 smem4[];
   const IkView<MAXA, NEE> v = ik_prologue<MAXA, NEE>(a, reinterpret_cast<char*>(smem4));
   const int lane = threadIdx.x;
   const int p = blockIdx.y;
-  const int s = blockIdx.x * kWave + lane;
+  const int s = blockIdx.x * (kWave / G) + lane / G;
+  const bool writer = (lane % G) == 0;
   if (s >= a.n_seeds) return;
   const long long gs = static_cast<long long>(p) * a.n_seeds + s;
   if (v.sh->too_many_nodes) {  // reported to the caller as NaN errors, seeds unchanged
-    for (int i = 0; i < a.n_act; ++i) a.out_cfgs[gs * a.n_act + i] = a.seeds[gs * a.n_act + i];
-    a.out_err[gs] = NAN;
+    if (writer) {
+      for (int i = 0; i < a.n_act; ++i) a.out_cfgs[gs * a.n_act + i] = a.seeds[gs * a.n_act + i];
+      a.out_err[gs] = NAN;
+    }
     return;
   }
   float tg[NEE][7];
@@ -681,11 +719,13 @@ __global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
   st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid);
   st.lam = a.lambda_init;
   st.done = 0;
-  lm_iterate(v, tg, W, a, st, 0, a.max_iter, a.grid);
+  lm_iterate<MAXA, NEE, G>(v, tg, W, a, st, 0, a.max_iter, a.grid);
+  if (writer) {
 #pragma unroll
-  for (int i = 0; i < MAXA; ++i)
-    if (i < a.n_act) a.out_cfgs[gs * a.n_act + i] = st.best_q[i];
-  a.out_err[gs] = st.best_err;
+    for (int i = 0; i < MAXA; ++i)
+      if (i < a.n_act) a.out_cfgs[gs * a.n_act + i] = st.best_q[i];
+    a.out_err[gs] = st.best_err;
+  }
 }
 
 // Uneven batch: seeds of all problems in one flat list (problem p owns
@@ -778,9 +818,20 @@ inline int launch_ik(void* stream, const IkArgs& a, bool batched, int grid_waves
                        static_cast<hipStream_t>(stream), a);
     return check_launch("ls_ik_batched_kernel");
   }
-  dim3 grid((a.n_seeds + kWave - 1) / kWave, a.n_problems);
-  hipLaunchKernelGGL((ls_ik_kernel<MAXA, NEE>), grid, dim3(kWave), lds,
-                     static_cast<hipStream_t>(stream), a);
+  // Small batches leave most of the GPU idle and run at the latency of one
+  // seed's iterations: give each seed 8 lanes (parallel line search) while
+  // the lanes still fit one pass of the CUs at two waves per SIMD.
+  const long long lanes8 = static_cast<long long>(a.n_problems) * a.n_seeds * 8;
+  if (lanes8 <= kSmallBatchLanes) {
+    constexpr int G = 8;
+    dim3 grid((a.n_seeds + kWave / G - 1) / (kWave / G), a.n_problems);
+    hipLaunchKernelGGL((ls_ik_kernel<MAXA, NEE, G>), grid, dim3(kWave), lds,
+                       static_cast<hipStream_t>(stream), a);
+  } else {
+    dim3 grid((a.n_seeds + kWave - 1) / kWave, a.n_problems);
+    hipLaunchKernelGGL((ls_ik_kernel<MAXA, NEE, 1>), grid, dim3(kWave), lds,
+                       static_cast<hipStream_t>(stream), a);
+  }
   return check_launch("ls_ik_kernel");
 }
 

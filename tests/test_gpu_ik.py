@@ -171,3 +171,26 @@ def test_public_api_solves(oracle_mod):
     sol_c = pk.ls_ik_solve_cuda_batch(robot, link, pk.SE3(tgt), 0, prev, num_seeds=64, max_iter=40,
                                       collision_free=True, collision_checker=chk, collision_world=world)
     assert sol_c.shape == (8, 7)
+
+
+@pytest.mark.parametrize("name", ["panda", "baxter"])
+def test_parallel_line_search_matches_one_lane_per_seed(oracle_mod, name):
+    """Small launches give each seed 8 lanes (one line-search step per lane);
+    large ones one lane.  The same seeds give bit-identical results both ways."""
+    robot, path = load_robot(name)
+    ee = {"panda": ["panda_hand"], "baxter": ["left_gripper_base", "right_gripper_base"]}[name]
+    links = [robot.links.names.index(n) for n in ee]
+    t = oracle_mod.Tables.of(robot)
+    rng = np.random.default_rng(61)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    tgt = oracle_mod.fk_links(t, random_cfgs(robot, 2, seed=62))[:, links]
+    big = rng.uniform(lo, hi, (2, 8200, len(lo))).astype(np.float32)  # 16400 seeds: one lane each
+    small = np.ascontiguousarray(big[:, :32])                         # 64 seeds: eight lanes each
+    coll = RobotCollisionSpherized.from_urdf(path)
+    world = np.concatenate([rng.uniform(-1, 1, (64, 3)), rng.uniform(0.02, 0.15, (64, 1))], 1).astype(np.float32)
+    rs, rj, ws, _ = prepare_ls_collision_buffers(robot, coll, world)
+    kw = dict(max_iter=20, lambda_init=1e-2, collision=(rs, rj, ws, 1e4, 0.02))
+    cb, eb = ls_ik_run(robot, big, tgt, links, **kw)
+    cs, es = ls_ik_run(robot, small, tgt, links, **kw)
+    torch.testing.assert_close(cs, cb[:, :32], rtol=0, atol=0)
+    torch.testing.assert_close(es, eb[:, :32], rtol=0, atol=0)
