@@ -191,3 +191,34 @@ def test_colldist_from_sdf_on_device():
     m = 0.05
     ref = [-0.1 - 0.5 * m, -0.5 / (m + 1e-6) * m * m, -0.5 / (m + 1e-6) * 0.04 ** 2, 0.0, 0.0]
     np.testing.assert_allclose(got, ref, atol=1e-7)
+
+
+def test_sweep_skips_nan_poses_like_the_reference():
+    """A NaN link pose makes its spheres' distances NaN; the reference's minima
+    start at 1e9 and skip NaN (fminf), so that link reads 1e9 against every
+    obstacle and in every self pair it belongs to, and nothing else changes."""
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    q = random_cfgs(robot, 6, seed=91).astype(np.float32)
+    poses = robot.forward_kinematics(torch.as_tensor(q, device="cuda")).contiguous()
+    world = torch.as_tensor(_world(256, seed=92), device="cuda")
+    t = coll.device_tables("cuda")
+    run = lambda lp: ops.sphere_sweep(lp, t["spheres_local"], t["link_sphere_start"], world, t["self_entries"],
+                                      coll.num_pairs, world=True, world_min=True, self_pairs=True)
+    ow, om, osf = (x.cpu().numpy() for x in run(poses))
+    link = robot.links.names.index("panda_link5")
+    bad = poses.clone()
+    bad[2, link] = float("nan")
+    bw, bm, bsf = (x.cpu().numpy() for x in run(bad))
+    keep = np.ones(bw.shape[:2], bool)
+    keep[2, link] = False
+    np.testing.assert_array_equal(bw[keep], ow[keep])
+    assert np.all(bw[2, link] == np.float32(1e9))
+    others = np.delete(ow[2], link, axis=0)
+    assert bm[2] == np.float32(min(others.min(), 1e9))
+    np.testing.assert_array_equal(np.delete(bm, 2), np.delete(om, 2))
+    touch = (coll.active_idx_i == link) | (coll.active_idx_j == link)
+    assert touch.any()
+    assert np.all(bsf[2, touch] == np.float32(1e9))
+    np.testing.assert_array_equal(bsf[2, ~touch], osf[2, ~touch])
+    np.testing.assert_array_equal(np.delete(bsf, 2, axis=0), np.delete(osf, 2, axis=0))
