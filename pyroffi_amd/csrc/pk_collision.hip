@@ -1,9 +1,11 @@
 // Sphere-robot collision distances for gfx950.
 //
-// Work decomposition (one 64-lane workgroup = one wave):
-//   * the wave loops over configurations (grid-stride);
+// Work decomposition (the fused sweep_kernel; workgroups of kSweepWaves waves
+// share the per-launch tables, each wave has its own buffers):
+//   * each wave loops over its own block of configurations;
 //   * for one configuration it places every robot sphere in the world frame
-//     into LDS (one float4 per sphere: centre, radius);
+//     into LDS (one float4 per sphere: centre, radius, plus the expanded
+//     entry of the link-frame distance test);
 //   * lanes run across world obstacles -- each lane keeps T obstacles in
 //     registers for the whole launch -- and every robot sphere is read once
 //     from LDS as a broadcast and tested against the lane's T obstacles;
