@@ -38,6 +38,9 @@ METRIC = "LM-IK solves/s @≤1mm/0.01rad fused-batched vs plain; FK+coll configs
 B = 65536
 M_WORLD = 256
 PEAK_FP32_TFLOPS = 157.3
+# The same peak without packed FP32 (one FMA per lane per cycle): this build
+# compiles with packed FP32 instructions disabled.
+PEAK_FP32_UNPACKED_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # Algorithmic work per configuration (flop convention: sphere-sphere distance
 # including the min = 12; SE(3) sphere placement ~ 33 per sphere).
@@ -643,6 +646,8 @@ def main():
             "traffic": load_traffic(),
             "flop_per_config": FLOP_SWEEP,
             "avg_launch_ms": g["sweep_ms"],
+            "peak_without_packed_fp32": PEAK_FP32_UNPACKED_TFLOPS,
+            "frac_without_packed_fp32": achieved / PEAK_FP32_UNPACKED_TFLOPS,
         },
         "fk_kernel": {
             "bound": "hbm",
