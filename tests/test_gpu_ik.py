@@ -185,12 +185,12 @@ def test_parallel_line_search_matches_one_lane_per_seed(oracle_mod, name):
     lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
     tgt = oracle_mod.fk_links(t, random_cfgs(robot, 2, seed=62))[:, links]
     big = rng.uniform(lo, hi, (2, 8200, len(lo))).astype(np.float32)  # 16400 seeds: one lane each
-    small = np.ascontiguousarray(big[:, :32])                         # 64 seeds: eight lanes each
+    small = np.ascontiguousarray(big[:, :29])                         # 58 seeds: eight lanes each
     coll = RobotCollisionSpherized.from_urdf(path)
     world = np.concatenate([rng.uniform(-1, 1, (64, 3)), rng.uniform(0.02, 0.15, (64, 1))], 1).astype(np.float32)
     rs, rj, ws, _ = prepare_ls_collision_buffers(robot, coll, world)
     kw = dict(max_iter=20, lambda_init=1e-2, collision=(rs, rj, ws, 1e4, 0.02))
     cb, eb = ls_ik_run(robot, big, tgt, links, **kw)
     cs, es = ls_ik_run(robot, small, tgt, links, **kw)
-    torch.testing.assert_close(cs, cb[:, :32], rtol=0, atol=0)
-    torch.testing.assert_close(es, eb[:, :32], rtol=0, atol=0)
+    torch.testing.assert_close(cs, cb[:, :29], rtol=0, atol=0)
+    torch.testing.assert_close(es, eb[:, :29], rtol=0, atol=0)
