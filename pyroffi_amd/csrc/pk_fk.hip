@@ -23,9 +23,11 @@ __global__ __launch_bounds__(64) void fk_kernel(
   const int J = tables.n_joints;
   JointRec* recs = reinterpret_cast<JointRec*>(smem);
   int* slot_of = reinterpret_cast<int*>(recs + J);
-  int* out_slot = slot_of + J;                    // per output row: slot, -1 = identity
   const int n_out = out_mode ? n_links : J;
-  float* cfg_tile = reinterpret_cast<float*>(out_slot + n_out);   // [a][lane], stride 65
+  const int row = n_out * 7;
+  int* src_off = slot_of + J;   // per output float of a row: LDS offset of the pose component,
+                                // -1 for an identity 1, -2 for an identity 0
+  float* cfg_tile = reinterpret_cast<float*>(src_off + row);      // [a][lane], stride 65
   float* poses = cfg_tile + n_act * kLaneStride;                  // [slot*7+c][lane]
 
   const int lane = threadIdx.x;
@@ -33,15 +35,16 @@ __global__ __launch_bounds__(64) void fk_kernel(
   const int n_valid = static_cast<int>(min(static_cast<long long>(kWave), batch - b0));
 
   stage_joint_records(tables, recs, slot_of);
-  for (int r = lane; r < n_out; r += kWave) {
+  for (int r = lane; r < row; r += kWave) {
+    const int o = r / 7, c = r - 7 * (r / 7);
     int s;
     if (out_mode) {
-      const int pj = link_parent_joint[r];
+      const int pj = link_parent_joint[o];
       s = (pj >= 0 && pj < J) ? slot_of[pj] : -1;
     } else {
-      s = slot_of[r];
+      s = slot_of[o];
     }
-    out_slot[r] = s;
+    src_off[r] = (s >= 0) ? (s * 7 + c) * kLaneStride : (c == 0 ? -1 : -2);
   }
   // Configuration tile: element f of the contiguous (n_valid x n_act) block.
   const float* cfg_blk = cfg + b0 * n_act;
@@ -57,26 +60,36 @@ __global__ __launch_bounds__(64) void fk_kernel(
   __syncthreads();
 
   // Copy-out: the block's rows are contiguous, n_valid * n_out * 7 floats.
-  const int row = n_out * 7;
+  // Element f = lane + 64 i is float rem of configuration l; four elements
+  // per pass keep their LDS reads in flight together.
   float* out_blk = out + b0 * row;
   const int total = n_valid * row;
   int l = lane / row;
   int rem = lane - l * row;
-  for (int f = lane; f < total; f += kWave) {
-    const int o = rem / 7;
-    const int c = rem - o * 7;
-    const int s = out_slot[o];
-    float v;
-    if (s >= 0) v = poses[(s * 7 + c) * kLaneStride + l];
-    else v = (c == 0) ? 1.f : 0.f;
-    out_blk[f] = v;
-    rem += kWave;
-    while (rem >= row) { rem -= row; ++l; }
+  constexpr int U = 4;
+  for (int f0 = lane; f0 < total; f0 += U * kWave) {
+    int ls[U], rs[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ls[u] = l;
+      rs[u] = rem;
+      rem += kWave;
+      while (rem >= row) { rem -= row; ++l; }
+    }
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int so = (f0 + u * kWave < total) ? src_off[rs[u]] : -2;
+      v[u] = (so >= 0) ? poses[so + ls[u]] : (so == -1 ? 1.f : 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (f0 + u * kWave < total) out_blk[f0 + u * kWave] = v[u];
   }
 }
 
 static size_t fk_lds_bytes(int J, int n_act, int n_out) {
-  return sizeof(JointRec) * J + sizeof(int) * (J + n_out) +
+  return sizeof(JointRec) * J + sizeof(int) * (J + 7 * n_out) +
          sizeof(float) * (static_cast<size_t>(n_act) + 7 * J) * kLaneStride;
 }
 
