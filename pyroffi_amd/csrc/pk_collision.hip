@@ -312,6 +312,21 @@ __global__ __launch_bounds__(kWave * kSweepWaves, T == 8 ? 3 : 6) void sweep_ker
     if (a.out_world != nullptr || a.out_min != nullptr) {
       for (int c = 0; c < n_chunks; ++c) {
         if (T == 8 && c > 0) load_obstacles(c);
+        if (SRC == 0 && a.out_world == nullptr) {
+          // Only the configuration minimum is wanted.  Subtracting r_o is
+          // monotone, so min_n fminf(kBig, acc_n - r_o) equals
+          // fminf(kBig, min_n acc_n - r_o) (NaN and infinite cases included):
+          // one accumulator runs through all links.
+          float acc[T];
+#pragma unroll
+          for (int t = 0; t < T; ++t) acc[t] = INFINITY;
+          for (int n = 0; n < L; ++n)
+            if (lgroup[n] < lgroup[n + 1])
+              link_sweep<T>(n, lgroup[n], lgroup[n + 1], sc, se, gstart, ptile, lrad2, ox, oy, oz, acc);
+#pragma unroll
+          for (int t = 0; t < T; ++t) gmin = __builtin_elementwise_minimum(gmin, fminf(kBig, acc[t] - orad[t]));
+          continue;
+        }
         for (int n = 0; n < L; ++n) {
           float acc[T];
 #pragma unroll
