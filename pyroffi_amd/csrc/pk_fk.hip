@@ -4,8 +4,11 @@
 // records and the 64 x n_act configuration tile in LDS (coalesced loads), every
 // lane walks the joint tree for its configuration with all poses held in an
 // LDS tile, and the finished tile -- 64 x (n_out x 7) floats, contiguous in
-// HBM -- is copied out with unit-stride stores.  The kernel is bound by the
-// output write: 28 B per joint (or link) per configuration.
+// HBM -- is copied out with 16-byte stores.  The output is 28 B per joint (or
+// link) per configuration; with one wave per SIMD (a 64-lane workgroup holds
+// about 21 KB of LDS for Panda) the three phases -- staging, walk, copy-out --
+// run back to back, so the kernel is latency-bound rather than at the write
+// bandwidth.
 //
 // Reference behaviour: FkCudaImpl / fk_kernel_warp
 // (src/pyroffi/cuda_kernels/_fk_cuda_kernel.cu:39-137, 236-454) and the link
@@ -13,6 +16,8 @@
 #include "pk_model.h"
 
 namespace pk {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 // out_mode 0: joint poses (B, J, 7); 1: link poses (B, L, 7).
 __global__ __launch_bounds__(64) void fk_kernel(
@@ -34,23 +39,44 @@ __global__ __launch_bounds__(64) void fk_kernel(
   const long long b0 = static_cast<long long>(blockIdx.x) * kWave;
   const int n_valid = static_cast<int>(min(static_cast<long long>(kWave), batch - b0));
 
-  stage_joint_records(tables, recs, slot_of);
-  for (int r = lane; r < row; r += kWave) {
-    const int o = r / 7, c = r - 7 * (r / 7);
-    int s;
-    if (out_mode) {
-      const int pj = link_parent_joint[o];
-      s = (pj >= 0 && pj < J) ? slot_of[pj] : -1;
-    } else {
-      s = slot_of[o];
-    }
-    src_off[r] = (s >= 0) ? (s * 7 + c) * kLaneStride : (c == 0 ? -1 : -2);
-  }
-  // Configuration tile: element f of the contiguous (n_valid x n_act) block.
+  // The configuration block and the link -> joint map are loaded into
+  // registers first, so their latency overlaps the table staging.
+  // Configuration tile: element f = lane + 64 i of the contiguous
+  // (n_valid x n_act) block, at most kMaxAct elements per lane.
   const float* cfg_blk = cfg + b0 * n_act;
-  for (int f = lane; f < n_valid * n_act; f += kWave) {
-    const int l = f / n_act, a = f - l * n_act;
-    cfg_tile[a * kLaneStride + l] = cfg_blk[f];
+  const int n_cfg = n_valid * n_act;
+  float cv[kMaxAct];
+#pragma unroll
+  for (int i = 0; i < kMaxAct; ++i) {
+    const int f = lane + i * kWave;
+    cv[i] = (f < n_cfg) ? cfg_blk[f] : 0.f;
+  }
+  constexpr int kRowIters = (7 * (kMaxJoints + 1) + kWave - 1) / kWave;
+  int pjv[kRowIters];
+#pragma unroll
+  for (int i = 0; i < kRowIters; ++i) {
+    const int r = lane + i * kWave;
+    pjv[i] = (out_mode && r < row) ? link_parent_joint[r / 7] : -1;
+  }
+
+  stage_joint_records(tables, recs, slot_of);
+#pragma unroll
+  for (int i = 0; i < kRowIters; ++i) {
+    const int r = lane + i * kWave;
+    if (r < row) {
+      const int o = r / 7, c = r - 7 * o;
+      const int pj = out_mode ? pjv[i] : o;
+      const int s = (pj >= 0 && pj < J) ? slot_of[pj] : -1;
+      src_off[r] = (s >= 0) ? (s * 7 + c) * kLaneStride : (c == 0 ? -1 : -2);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxAct; ++i) {
+    const int f = lane + i * kWave;
+    if (f < n_cfg) {
+      const int l = f / n_act, a = f - l * n_act;
+      cfg_tile[a * kLaneStride + l] = cv[i];
+    }
   }
   __syncthreads();
 
@@ -59,32 +85,37 @@ __global__ __launch_bounds__(64) void fk_kernel(
   }
   __syncthreads();
 
-  // Copy-out: the block's rows are contiguous, n_valid * n_out * 7 floats.
-  // Element f = lane + 64 i is float rem of configuration l; four elements
-  // per pass keep their LDS reads in flight together.
+  // Copy-out: the block's rows are contiguous, n_valid * n_out * 7 floats,
+  // written as 16-byte non-temporal stores (the block starts 16-byte
+  // aligned: 64 rows of 7 n_out floats).  Lane f writes floats 4f .. 4f + 3;
+  // float rem of configuration l comes from LDS offset src_off[rem] + l.
   float* out_blk = out + b0 * row;
   const int total = n_valid * row;
-  int l = lane / row;
-  int rem = lane - l * row;
-  constexpr int U = 4;
-  for (int f0 = lane; f0 < total; f0 += U * kWave) {
-    int ls[U], rs[U];
+  const int n4 = total / 4;
+  int l = (4 * lane) / row;
+  int rem = 4 * lane - l * row;
+  int adv_l = (4 * kWave) / row;            // per pass: 256 floats = adv_l rows + adv_r
+  const int adv_r = 4 * kWave - adv_l * row;
+  for (int f4 = lane; f4 < n4; f4 += kWave) {
+    float v[4];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      ls[u] = l;
-      rs[u] = rem;
-      rem += kWave;
-      while (rem >= row) { rem -= row; ++l; }
+    for (int k = 0; k < 4; ++k) {
+      int r = rem + k, lk = l;
+      if (r >= row) { r -= row; ++lk; }      // row >= 7 > 3: at most one wrap
+      const int so = src_off[r];
+      v[k] = (so >= 0) ? poses[so + lk] : (so == -1 ? 1.f : 0.f);
     }
-    float v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int so = (f0 + u * kWave < total) ? src_off[rs[u]] : -2;
-      v[u] = (so >= 0) ? poses[so + ls[u]] : (so == -1 ? 1.f : 0.f);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (f0 + u * kWave < total) out_blk[f0 + u * kWave] = v[u];
+    const v4f w = {v[0], v[1], v[2], v[3]};
+    __builtin_nontemporal_store(w, reinterpret_cast<v4f*>(out_blk) + f4);  // streamed past L2
+    rem += adv_r;
+    l += adv_l;
+    if (rem >= row) { rem -= row; ++l; }
+  }
+  // Tail of a partial last block (total not a multiple of 4).
+  for (int f = 4 * n4 + lane; f < total; f += kWave) {
+    const int lk = f / row, r = f - lk * row;
+    const int so = src_off[r];
+    out_blk[f] = (so >= 0) ? poses[so + lk] : (so == -1 ? 1.f : 0.f);
   }
 }
 
