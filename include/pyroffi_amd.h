@@ -161,9 +161,10 @@ int pk_collision_self_sphere_f32(void* stream, const float* centers,
  *   world_spheres     (Ms, 4) float32
  *   self_entries      (n_entries) uint32: one per sphere pair to test,
  *                     (a | b << 10 | p << 20): sphere indices a, b into
- *                     spheres_local and active pair p < 4096, entries of one
- *                     pair adjacent; n_spheres <= 1023 when n_entries > 0;
- *                     entries out of range are skipped
+ *                     spheres_local and active pair p, entries of one
+ *                     pair adjacent; n_spheres <= 1023 and n_pairs <= 2047
+ *                     when n_entries > 0 (else PK_ERR_INVALID); entries out
+ *                     of range are skipped
  *   outputs (any may be NULL):
  *     out_world       (batch, n_links, Ms)  per-link min over its spheres,
  *                     starting from 1e9 and skipping NaN distances as the

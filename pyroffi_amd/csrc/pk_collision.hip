@@ -11,7 +11,8 @@
 //     from LDS as a broadcast and tested against the lane's T obstacles;
 //   * per-link minima are written as coalesced rows (lanes = obstacles), the
 //     per-configuration minimum is a wavefront min-reduction, and self pairs
-//     are reduced with LDS atomic-min over an order-preserving integer key.
+//     are reduced in registers along each lane's run of the entry list and
+//     published with LDS float atomic-min.
 //
 // Distance of two spheres: |c1 - c2| - r1 - r2 (positive = separated), the
 // reference's sphere_sphere_dist (_collision_cuda_helpers.cuh:14-19).
@@ -58,7 +59,7 @@ struct SweepArgs {
 
 // Self entry i | j << 10 | pair << 20 with the sphere indices checked
 // against K and the pair against P; entries out of range use the blank slot K
-// (K < 1024) and a pair inside [0, P).
+// (K < 1024) and a pair inside [0, P) (P < 2048: bit 31 is the run flag).
 __device__ __forceinline__ unsigned self_entry(unsigned en, int K, int P) {
   const int i = static_cast<int>(en & 1023u), j = static_cast<int>((en >> 10) & 1023u);
   const int p = static_cast<int>(en >> 20);
@@ -188,14 +189,15 @@ __global__ __launch_bounds__(kWave * kSweepWaves, T == 8 ? 3 : 6) void sweep_ker
   float4* sl = reinterpret_cast<float4*>(smem);                          // K local spheres (SRC 0)
   float4* sc = sl + (SRC == 0 ? K : 0) + wave * f4_per_wave;            // K world spheres + 1 blank
   float4* se = sc + (K + 1);                                             // K expanded entries (SRC 0)
-  unsigned* ents = reinterpret_cast<unsigned*>(sl + (SRC == 0 ? K : 0) + kSweepWaves * f4_per_wave);  // E
-  int* lstart = reinterpret_cast<int*>(ents + E);                        // L + 1
+  const int n_it = (E + kWave - 1) / kWave;                             // self steps per lane
+  unsigned* sent = reinterpret_cast<unsigned*>(sl + (SRC == 0 ? K : 0) + kSweepWaves * f4_per_wave);  // n_it * 64
+  int* lstart = reinterpret_cast<int*>(sent + n_it * kWave);             // L + 1
   int* slink = lstart + (L + 1);                                         // K: link of sphere (SRC 0)
   int* gstart = slink + (SRC == 0 ? K : 0);                              // K + 1: sphere groups
   int* lgroup = gstart + (K + 1);                                        // L + 1: groups of each link
   float* lrad2 = reinterpret_cast<float*>(lgroup + (L + 1));             // L: bound of |offset|^2 (SRC 0)
   float* ptile = lrad2 + (SRC == 0 ? L : 0) + wave * words_per_wave;     // L * 7 poses (SRC 0)
-  unsigned* pmin = reinterpret_cast<unsigned*>(ptile + (SRC == 0 ? 7 * L : 0));  // P
+  float* pmin = ptile + (SRC == 0 ? 7 * L : 0);                          // P
 
   // Per-launch tables, staged by the whole workgroup.
   const int nt = kWave * kSweepWaves;
@@ -222,7 +224,25 @@ __global__ __launch_bounds__(kWave * kSweepWaves, T == 8 ? 3 : 6) void sweep_ker
   // The blank slot K is infinitely far from everything: distances to it are
   // +inf, which no minimum takes.
   if (lane == 0) sc[K] = make_float4(0.f, 0.f, 0.f, -INFINITY);
-  for (int e = tid; e < E; e += nt) ents[e] = self_entry(a.self_entries[e], K, P);
+  // Self entries, transposed: lane l owns the contiguous run
+  // [l E / 64, (l + 1) E / 64) of the entry list (grouped by pair), and step
+  // `it` of all 64 runs sits at sent[it * 64 + l].  Bits: sphere i (0-9),
+  // sphere j (10-19), pair (20-30), and bit 31 on the last entry of a pair
+  // within a run, where the lane publishes its minimum.  Slots past the end
+  // of a shorter run hold the blank sphere pair (distance +inf, no flag).
+  for (int s = tid; s < n_it * kWave; s += nt) {
+    const int it = s / kWave, l = s - it * kWave;
+    const int e0 = static_cast<int>((static_cast<long long>(E) * l) / kWave);
+    const int e1 = static_cast<int>((static_cast<long long>(E) * (l + 1)) / kWave);
+    const int e = e0 + it;
+    unsigned v = static_cast<unsigned>(K) | (static_cast<unsigned>(K) << 10);
+    if (e < e1) {
+      v = self_entry(a.self_entries[e], K, P);
+      const bool last = (e + 1 == e1) || ((self_entry(a.self_entries[e + 1], K, P) >> 20) != (v >> 20));
+      if (last) v |= 0x80000000u;
+    }
+    sent[s] = v;
+  }
   __syncthreads();
   if (SRC == 0) {
     for (int n = tid; n < L; n += nt) {
@@ -304,7 +324,7 @@ __global__ __launch_bounds__(kWave * kSweepWaves, T == 8 ? 3 : 6) void sweep_ker
                         r < 0.f ? -INFINITY : r);
       }
     }
-    for (int p = lane; p < P; p += kWave) pmin[p] = float_to_ordered(kBig);
+    for (int p = lane; p < P; p += kWave) pmin[p] = kBig;
     wave_sync();
 
     // 2. World sweep.
@@ -377,33 +397,26 @@ __global__ __launch_bounds__(kWave * kSweepWaves, T == 8 ? 3 : 6) void sweep_ker
       if (T == 8 && n_chunks > 1) load_obstacles(0);
     }
 
-    // 3. Self pairs.  Entries are grouped by pair; lane l reduces the
-    // contiguous run [l E / 64, (l + 1) E / 64) in registers and publishes an
-    // atomic min where the pair changes and at the end of its run.
+    // 3. Self pairs.  Every lane takes n_it steps through its run of the
+    // transposed entry table (a scalar loop), keeps the running minimum of
+    // its current pair in a register and publishes it with an LDS float
+    // atomic-min at the pair's last entry.  The running minimum skips NaN
+    // distances; published values are never NaN.
     if (a.out_self != nullptr) {
-      const int e0 = static_cast<int>((static_cast<long long>(E) * lane) / kWave);
-      const int e1 = static_cast<int>((static_cast<long long>(E) * (lane + 1)) / kWave);
-      const int e_last = max(E - 1, 0);  // with E = 0 the read stays inside the LDS block and is unused
-      unsigned w = ents[min(e0, e_last)];
-      unsigned p = 0;
       float cur = INFINITY;
-      for (int e = e0; e < e1; ++e) {
-        const unsigned wn = ents[min(e + 1, e_last)];  // next entry's load overlaps this one
+      for (int it = 0; it < n_it; ++it) {
+        const unsigned w = sent[it * kWave + lane];
         const float4 s1 = sc[w & 1023u];
-        const float4 s2 = sc[(w >> 10) & 1023u];
+        const float4 s2 = sc[__builtin_amdgcn_ubfe(w, 10, 10)];
         const float dx = s1.x - s2.x, dy = s1.y - s2.y, dz = s1.z - s2.z;
-        const float d = fast_sqrt(dx * dx + dy * dy + dz * dz) - s1.w - s2.w;
-        cur = (d < cur) ? d : cur;  // a NaN distance is skipped
-        p = w >> 20;
-        if ((wn >> 20) != p) {  // last entry of its pair
-          atomicMin(&pmin[p], float_to_ordered(cur));
+        cur = fminf(cur, fast_sqrt(dx * dx + dy * dy + dz * dz) - s1.w - s2.w);
+        if (static_cast<int>(w) < 0) {
+          atomicMin(&pmin[(w >> 20) & 2047u], cur);
           cur = INFINITY;
         }
-        w = wn;
       }
-      if (e1 > e0) atomicMin(&pmin[p], float_to_ordered(cur));  // +inf changes nothing
       wave_sync();
-      for (int q = lane; q < P; q += kWave) a.out_self[b * P + q] = ordered_to_float(pmin[q]);
+      for (int q = lane; q < P; q += kWave) a.out_self[b * P + q] = pmin[q];
     }
     wave_sync();
   }
@@ -452,10 +465,12 @@ static int launch_sweep(void* stream, const SweepArgs& a) {
   const int K = a.n_spheres, L = a.n_links;
   const size_t per_wave = sizeof(float4) * ((K + 1) + (SRC == 0 ? K : 0)) +
                           sizeof(float) * ((SRC == 0 ? 7 * L : 0) + a.n_pairs);
-  size_t lds = sizeof(float4) * (SRC == 0 ? K : 0) + sizeof(unsigned) * a.n_entries +
+  const size_t n_slots = static_cast<size_t>((a.n_entries + kWave - 1) / kWave) * kWave;
+  size_t lds = sizeof(float4) * (SRC == 0 ? K : 0) + sizeof(unsigned) * n_slots +
                sizeof(int) * ((L + 1) + (SRC == 0 ? K : 0) + (K + 1) + (L + 1)) +
                sizeof(float) * (SRC == 0 ? L : 0) + per_wave * kSweepWaves;
   PK_REQUIRE(a.n_entries == 0 || K < 1024, "self entries address at most 1023 spheres (got %d)", K);
+  PK_REQUIRE(a.n_entries == 0 || a.n_pairs < 2048, "self entries address at most 2047 pairs (got %d)", a.n_pairs);
   PK_REQUIRE(lds <= 150 * 1024, "sphere tables need %zu bytes of LDS (limit 150 KiB)", lds);
   // About eight configurations per wave and up to 8192 waves (32 per CU);
   // each wave loops over its block of configurations.
@@ -527,8 +542,8 @@ extern "C" int pk_sphere_sweep_f32(void* stream, const float* link_poses, int64_
   PK_REQUIRE(batch >= 0 && n_links >= 0 && n_spheres >= 0 && n_world_spheres >= 0 &&
                  n_entries >= 0 && n_pairs >= 0,
              "negative size");
-  PK_REQUIRE(n_spheres <= 1024 && n_pairs <= 4096,
-             "self-entry encoding needs n_spheres <= 1024 and n_pairs <= 4096");
+  PK_REQUIRE(n_spheres <= 1024 && n_pairs <= 4096 && (n_entries == 0 || n_pairs <= 2047),
+             "self-entry encoding needs n_spheres <= 1024 and n_pairs <= 2047 (4096 without entries)");
   PK_REQUIRE(n_links <= pk::kMaxJoints + 1, "n_links=%d too large", n_links);
   if (batch == 0) return PK_OK;
   PK_REQUIRE(link_poses && link_sphere_start && (n_spheres == 0 || spheres_local),

@@ -60,6 +60,8 @@ def test_invalid_arguments_are_rejected_on_host():
     # self-entry encoding limits
     st = lib.pk_sphere_sweep_f32(n, n, 2, 13, n, n, 2000, n, 0, n, 0, 0, n, n, n)
     assert st == 1
+    st = lib.pk_sphere_sweep_f32(n, n, 2, 13, n, n, 59, n, 0, n, 1, 2048, n, n, n)
+    assert st == 1 and b"2047" in lib.pk_last_error()
     # null output pointer with work to do
     st = lib.pk_fk_links_f32(n, n, 4, 7, *([n] * 8), 12, n, 13, n)
     assert st == 1
