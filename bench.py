@@ -576,7 +576,7 @@ def profile_workload(name: str):
 def profile_run():
     import torch
 
-    g = gpu_bench(steps=5, warmup=2)
+    g = gpu_bench(steps=200, warmup=20)  # the default run's step counts
     src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
@@ -595,8 +595,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ik", action="store_true")
     ap.add_argument("--profile", nargs="?", const="sweep", choices=("sweep", "grad", "ik_uneven", "trajopt"),
-                    help="short run for rocprofv3: 'sweep' = 5 steps of the headline workload plus a 256 MiB "
-                         "device copy (the byte-count calibration for the HBM counters); 'grad' = 20 launches "
+                    help="run for rocprofv3: 'sweep' = the headline workload's 20 + 200 steps and 200 timed "
+                         "launches of each kernel, plus a 256 MiB device copy (the byte-count calibration for the HBM counters); 'grad' = 20 launches "
                          "of the config-3 adjoint kernel plus the same copy; 'ik_uneven' / 'trajopt' = 3 solves "
                          "of the uneven LM-IK / trajectory workload")
     args = ap.parse_args()
