@@ -21,6 +21,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 namespace pk {
 
@@ -460,6 +462,46 @@ static int pick_t(int M) {
   return 8;
 }
 
+// Workgroups of `fn` resident at once on the current device (per-CU
+// occupancy for `block` threads and `lds` bytes of LDS, times the CU count),
+// or 0 if the runtime cannot say.  Cached per (device, kernel, LDS size).
+static long long resident_groups(const void* fn, int block, size_t lds) {
+  struct Entry { int dev; const void* fn; size_t lds; long long groups; };
+  static std::mutex mu;
+  static std::vector<Entry> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  for (const Entry& e : cache)
+    if (e.dev == dev && e.fn == fn && e.lds == lds) return e.groups;
+  int n_cu = 0, per_cu = 0;
+  long long groups = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds) == hipSuccess)
+    groups = static_cast<long long>(n_cu) * per_cu;
+  else
+    (void)hipGetLastError();
+  if (cache.size() < 256) cache.push_back({dev, fn, lds, groups});
+  return groups;
+}
+
+// One launch of sweep_kernel<SRC, T>: at least eight configurations per wave,
+// and no more workgroups than are resident at once (each wave loops over its
+// block of configurations, so a second, partial round would only run at low
+// occupancy).
+template <int SRC, int T>
+static void launch_sweep_t(hipStream_t s, const SweepArgs& a, size_t lds) {
+  const void* fn = reinterpret_cast<const void*>(&sweep_kernel<SRC, T>);
+  const long long want = (a.batch + 8 * kSweepWaves - 1) / (8 * kSweepWaves);
+  long long cap = resident_groups(fn, kWave * kSweepWaves, lds);
+  if (cap <= 0) cap = 8192 / kSweepWaves;
+  const unsigned grid = static_cast<unsigned>(std::max(1LL, std::min(want, cap)));
+  hipLaunchKernelGGL((sweep_kernel<SRC, T>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a);
+}
+
 template <int SRC>
 static int launch_sweep(void* stream, const SweepArgs& a) {
   const int K = a.n_spheres, L = a.n_links;
@@ -472,16 +514,12 @@ static int launch_sweep(void* stream, const SweepArgs& a) {
   PK_REQUIRE(a.n_entries == 0 || K < 1024, "self entries address at most 1023 spheres (got %d)", K);
   PK_REQUIRE(a.n_entries == 0 || a.n_pairs < 2048, "self entries address at most 2047 pairs (got %d)", a.n_pairs);
   PK_REQUIRE(lds <= 150 * 1024, "sphere tables need %zu bytes of LDS (limit 150 KiB)", lds);
-  // About eight configurations per wave and up to 8192 waves (32 per CU);
-  // each wave loops over its block of configurations.
-  const long long want = (a.batch + 8 * kSweepWaves - 1) / (8 * kSweepWaves);
-  const unsigned grid = static_cast<unsigned>(std::max(1LL, std::min(want, 8192LL / kSweepWaves)));
   hipStream_t s = static_cast<hipStream_t>(stream);
   switch (pick_t(a.n_world)) {
-    case 1: hipLaunchKernelGGL((sweep_kernel<SRC, 1>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a); break;
-    case 2: hipLaunchKernelGGL((sweep_kernel<SRC, 2>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a); break;
-    case 4: hipLaunchKernelGGL((sweep_kernel<SRC, 4>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a); break;
-    default: hipLaunchKernelGGL((sweep_kernel<SRC, 8>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a); break;
+    case 1: launch_sweep_t<SRC, 1>(s, a, lds); break;
+    case 2: launch_sweep_t<SRC, 2>(s, a, lds); break;
+    case 4: launch_sweep_t<SRC, 4>(s, a, lds); break;
+    default: launch_sweep_t<SRC, 8>(s, a, lds); break;
   }
   return check_launch("sweep_kernel");
 }
