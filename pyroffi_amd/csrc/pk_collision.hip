@@ -30,10 +30,24 @@ constexpr float kBig = 1e9f;   // the reference's "no sphere" sentinel
 
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_min(float v) {
+  // Lanes outside ROW_MASK receive +inf, the identity of the minimum.
+  return __builtin_elementwise_minimum(
+      v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(INFINITY), __float_as_int(v),
+                                                    CTRL, ROW_MASK, 0xf, false)));
+}
+
+// Minimum over the 64 lanes of a value that is never NaN, wave-uniform (read
+// from lane 63), with the DPP steps of wave_sum_dpp.
 __device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
-  return v;
+  v = dpp_min<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v = dpp_min<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v = dpp_min<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_min<0x140, 0xf>(v);  // row_mirror
+  v = dpp_min<0x142, 0xa>(v);  // row_bcast15 into rows 1, 3
+  v = dpp_min<0x143, 0xc>(v);  // row_bcast31 into rows 2, 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 struct SweepArgs {
@@ -463,9 +477,10 @@ static int pick_t(int M) {
 }
 
 // Workgroups of `fn` resident at once on the current device (per-CU
-// occupancy for `block` threads and `lds` bytes of LDS, times the CU count),
-// or 0 if the runtime cannot say.  Cached per (device, kernel, LDS size).
-static long long resident_groups(const void* fn, int block, size_t lds) {
+// occupancy for `block` threads and `lds` bytes of LDS, at most `max_per_cu`,
+// times the CU count), or 0 if the runtime cannot say.  Cached per (device,
+// kernel, LDS size).
+static long long resident_groups(const void* fn, int block, size_t lds, int max_per_cu) {
   struct Entry { int dev; const void* fn; size_t lds; long long groups; };
   static std::mutex mu;
   static std::vector<Entry> cache;
@@ -481,7 +496,7 @@ static long long resident_groups(const void* fn, int block, size_t lds) {
   long long groups = 0;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds) == hipSuccess)
-    groups = static_cast<long long>(n_cu) * per_cu;
+    groups = static_cast<long long>(n_cu) * std::min(per_cu, max_per_cu);
   else
     (void)hipGetLastError();
   if (cache.size() < 256) cache.push_back({dev, fn, lds, groups});
@@ -491,12 +506,16 @@ static long long resident_groups(const void* fn, int block, size_t lds) {
 // One launch of sweep_kernel<SRC, T>: at least eight configurations per wave,
 // and no more workgroups than are resident at once (each wave loops over its
 // block of configurations, so a second, partial round would only run at low
-// occupancy).
+// occupancy).  Residency is capped at six workgroups per CU (six waves per
+// SIMD): measured on Panda, seven -- possible when the kernel needs at most
+// 72 VGPRs -- made the launch 12% slower, the self phase included.
+constexpr int kSweepGroupsPerCu = 6;
+
 template <int SRC, int T>
 static void launch_sweep_t(hipStream_t s, const SweepArgs& a, size_t lds) {
   const void* fn = reinterpret_cast<const void*>(&sweep_kernel<SRC, T>);
   const long long want = (a.batch + 8 * kSweepWaves - 1) / (8 * kSweepWaves);
-  long long cap = resident_groups(fn, kWave * kSweepWaves, lds);
+  long long cap = resident_groups(fn, kWave * kSweepWaves, lds, kSweepGroupsPerCu);
   if (cap <= 0) cap = 8192 / kSweepWaves;
   const unsigned grid = static_cast<unsigned>(std::max(1LL, std::min(want, cap)));
   hipLaunchKernelGGL((sweep_kernel<SRC, T>), dim3(grid), dim3(kWave * kSweepWaves), lds, s, a);
