@@ -222,3 +222,46 @@ def test_sweep_skips_nan_poses_like_the_reference():
     assert np.all(bsf[2, touch] == np.float32(1e9))
     np.testing.assert_array_equal(bsf[2, ~touch], osf[2, ~touch])
     np.testing.assert_array_equal(np.delete(bsf, 2, axis=0), np.delete(osf, 2, axis=0))
+
+
+def _quat_rotate(q, v):
+    """Rotate v by unit quaternions q = (w, x, y, z): v + 2w (u x v) + 2 u x (u x v)."""
+    w, u = q[..., :1], q[..., 1:4]
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+@pytest.mark.parametrize("n_entries,n_pairs", [(0, 3), (1, 1), (5, 2), (63, 7), (64, 7), (65, 7),
+                                               (200, 40), (700, 2047)])
+def test_self_entry_table_runs(oracle_mod, n_entries, n_pairs):
+    """Synthetic self-entry lists (arbitrary sphere pairs per pair id, grouped
+    by pair, some entries out of range) against a float64 brute force: entry
+    counts below, at and above one per lane, runs that end mid-pair, pairs
+    without entries (1e9) and the largest pair count the encoding allows."""
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    K = coll.num_spheres
+    q = random_cfgs(robot, 37, seed=n_entries + 1).astype(np.float32)
+    rng = np.random.default_rng(n_entries * 7 + n_pairs)
+    p = np.sort(rng.integers(0, n_pairs, n_entries))
+    i, j = rng.integers(0, K, n_entries), rng.integers(0, K, n_entries)
+    bad = rng.random(n_entries) < 0.05  # out-of-range sphere index: skipped
+    i = np.where(bad, K + rng.integers(0, 100, n_entries), i)
+    ents = (i | (j << 10) | (p << 20)).astype(np.int32)
+    # Expected: float64 centres from the oracle's link poses.
+    lp = oracle_mod.fk_links(oracle_mod.Tables.of(robot), q.astype(np.float64))  # (B, L, 7)
+    sl = np.asarray(coll.spheres_local, np.float64)
+    start = np.asarray(coll.link_sphere_start)
+    link_of = np.repeat(np.arange(len(start) - 1), np.diff(start))
+    c = _quat_rotate(lp[:, link_of, :4], sl[None, :, :3]) + lp[:, link_of, 4:]  # (B, K, 3)
+    want = np.full((len(q), n_pairs), 1e9)
+    for e in range(n_entries):
+        if bad[e]:
+            continue
+        d = np.linalg.norm(c[:, i[e]] - c[:, j[e]], axis=1) - sl[i[e], 3] - sl[j[e], 3]
+        want[:, p[e]] = np.minimum(want[:, p[e]], d)
+    t = robot.device_tables()
+    poses = ops.fk_links(torch.as_tensor(q, device="cuda"), *t.robot_buffers(), t.link_parent_joint)
+    _, _, got = ops.sphere_sweep(poses, coll.spheres_local, coll.link_sphere_start, np.zeros((0, 4), np.float32),
+                                 ents, n_pairs, self_pairs=True)
+    np.testing.assert_allclose(got.cpu().numpy(), want, atol=TOL, rtol=0)
