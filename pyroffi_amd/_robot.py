@@ -42,6 +42,22 @@ class _DeviceTables:
         self.link_parent_joint = i(links.parent_joint_indices)
         self.lower = f(joints.lower_limits)
         self.upper = f(joints.upper_limits)
+        self._derived: dict = {}
+
+    def derived(self, key, make):
+        """Device tensors derived from host data, built once per ``key``.
+
+        Solver calls reuse them instead of uploading small tables every call:
+        an upload from pageable memory waits for the stream's queued work,
+        which would keep the host from preparing the next call while the GPU
+        runs the current one.
+        """
+        v = self._derived.get(key)
+        if v is None:
+            if len(self._derived) >= 256:
+                self._derived.clear()
+            v = self._derived[key] = make()
+        return v
 
     def robot_buffers(self) -> tuple[torch.Tensor, ...]:
         """The 8 robot buffers of the reference FFI calls, in order."""

@@ -20,6 +20,8 @@ supported.
 
 from __future__ import annotations
 
+import hashlib
+
 from typing import Any
 
 import numpy as np
@@ -137,22 +139,33 @@ def ls_ik_run(robot, seeds, target_T, target_link_indices, *, max_iter=60, pos_w
     t = robot.device_tables()
     dev = t.twists.device
     n_act = robot.joints.num_actuated_joints
-    tj, masks = ee_tables(robot, target_link_indices)
+    links = tuple(int(i) for i in target_link_indices)
     fm = np.zeros(n_act, np.int32) if fixed_joint_mask is None else np.asarray(fixed_joint_mask).astype(np.int32)
+    # Device copies of the small per-call tables, built once per content.
+    tj_d, masks_d, fm_d = t.derived(
+        ("ls_ik", links, fm.tobytes()),
+        lambda: (*(torch.as_tensor(x, device=dev) for x in ee_tables(robot, links)), torch.as_tensor(fm, device=dev)))
     if collision is not None:
         rs, rj, ws, w, m = collision
         enable = True
     else:
         rs, rj, ws, w, m = np.zeros((0, 4), np.float32), np.zeros(0, np.int32), np.zeros((0, 4), np.float32), 0.0, 0.02
         enable = False
+    rs = np.ascontiguousarray(rs, np.float32)
+    rj = np.ascontiguousarray(rj, np.int32)
+    ws = np.ascontiguousarray(ws, np.float32)
+    ckey = ("ls_ik_coll", hashlib.sha1(rs.tobytes() + b"|" + rj.tobytes() + b"|" + ws.tobytes()).digest(),
+            rs.shape, rj.shape, ws.shape)
+    rs_d, rj_d, ws_d, wc_d, wb_d, wh_d = t.derived(
+        ckey, lambda: (torch.as_tensor(rs, device=dev), torch.as_tensor(rj, device=dev),
+                       torch.as_tensor(ws, device=dev),
+                       *(torch.zeros((0, n), dtype=torch.float32, device=dev) for n in (7, 15, 6))))
     seeds_t = seeds if isinstance(seeds, torch.Tensor) else torch.as_tensor(np.asarray(seeds, np.float32))
     seeds_t = seeds_t.to(dev, torch.float32).contiguous()
     tT = target_T if isinstance(target_T, torch.Tensor) else torch.as_tensor(np.asarray(target_T, np.float32))
-    z = lambda n: torch.zeros((0, n), dtype=torch.float32, device=dev)
     return ls_ik_hip(
-        seeds_t, *t.robot_buffers(), torch.as_tensor(tj, device=dev), torch.as_tensor(masks, device=dev),
-        tT.to(dev, torch.float32), torch.as_tensor(rs, device=dev), torch.as_tensor(rj, device=dev),
-        torch.as_tensor(ws, device=dev), z(7), z(15), z(6), t.lower, t.upper, torch.as_tensor(fm, device=dev),
+        seeds_t, *t.robot_buffers(), tj_d, masks_d,
+        tT.to(dev, torch.float32), rs_d, rj_d, ws_d, wc_d, wb_d, wh_d, t.lower, t.upper, fm_d,
         max_iter=max_iter, pos_weight=pos_weight, ori_weight=ori_weight, lambda_init=lambda_init,
         eps_pos=eps_pos, eps_ori=eps_ori, enable_collision=enable, collision_weight=w, collision_margin=m,
     )
