@@ -488,19 +488,21 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
       }
     }
 
-    // Column scaling and normal equations (packed lower triangle).
-    float cs[MAXA];
+    // Column scaling and normal equations (packed lower triangle).  One
+    // division per column (an IEEE FP32 division is ~10 instructions); the
+    // 6 NEE entries of the column are multiplied by its reciprocal.
+    float ics[MAXA];
 #pragma unroll
     for (int c = 0; c < MAXA; ++c) {
       float sq = 0.f;
 #pragma unroll
       for (int i = 0; i < 6 * NEE; ++i) sq += Jm[i][c] * Jm[i][c];
-      cs[c] = sqrtf(sq) + 1e-8f;
+      ics[c] = 1.f / (sqrtf(sq) + 1e-8f);
     }
 #pragma unroll
     for (int i = 0; i < 6 * NEE; ++i)
 #pragma unroll
-      for (int c = 0; c < MAXA; ++c) Jm[i][c] /= cs[c];
+      for (int c = 0; c < MAXA; ++c) Jm[i][c] *= ics[c];
     constexpr int NT = MAXA * (MAXA + 1) / 2;
     float A[NT];
     float rhs[MAXA];
@@ -536,8 +538,10 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
         rhs[i] = 0.f;
       }
     }
-    // Cholesky A = L L^T in place, then two triangular solves.
+    // Cholesky A = L L^T in place, then two triangular solves that multiply
+    // by the reciprocals of the diagonal.
     bool ok = true;
+    float idg[MAXA];
 #pragma unroll
     for (int k = 0; k < MAXA; ++k) {
       float sd = A[k * (k + 1) / 2 + k];
@@ -547,6 +551,7 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
       const float lkk = sqrtf(fmaxf(sd, 1e-30f));
       A[k * (k + 1) / 2 + k] = lkk;
       const float inv = 1.f / lkk;
+      idg[k] = inv;
 #pragma unroll
       for (int i = k + 1; i < MAXA; ++i) {
         float t = A[i * (i + 1) / 2 + k];
@@ -561,7 +566,7 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
       float t = rhs[i];
 #pragma unroll
       for (int pp = 0; pp < i; ++pp) t -= A[i * (i + 1) / 2 + pp] * y[pp];
-      y[i] = t / A[i * (i + 1) / 2 + i];
+      y[i] = t * idg[i];
     }
     float delta[MAXA];
 #pragma unroll
@@ -569,12 +574,12 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
       float t = y[i];
 #pragma unroll
       for (int pp = i + 1; pp < MAXA; ++pp) t -= A[pp * (pp + 1) / 2 + i] * delta[pp];
-      delta[i] = t / A[i * (i + 1) / 2 + i];
+      delta[i] = t * idg[i];
     }
     float dn = 0.f;
 #pragma unroll
     for (int i = 0; i < MAXA; ++i) {
-      delta[i] = ok ? delta[i] / cs[i] : 0.f;
+      delta[i] = ok ? delta[i] * ics[i] : 0.f;
       dn += delta[i] * delta[i];
     }
     // Trust region.
