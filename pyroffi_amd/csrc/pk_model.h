@@ -28,7 +28,8 @@ struct JointRec {
   int src;           // actuated source index, -1 when none
   int parent_slot;   // slot of the parent joint, -1 for roots
   int joint;         // original joint index
-  int kind;          // 0 constant (T_pc = ptf), 1 rotation, 2 translation, 3 general
+  int kind;          // 0 constant (T_pc = ptf), 2 translation, 3 general, 4 rotation
+                     // (ang = unit axis, lin[0] = its original norm)
   int pad;
 };
 static_assert(sizeof(JointRec) == 20 * 4, "JointRec layout");
@@ -86,8 +87,17 @@ __device__ inline void stage_joint_records(const RobotTables& t, JointRec* recs,
     int kind;
     if (src == -1 && r.off == 0.f) kind = 0;
     else if (a2 == 0.f) kind = 2;
-    else if (l2 == 0.f) kind = 1;
+    else if (l2 == 0.f) kind = 4;
     else kind = 3;
+    if (kind == 4) {
+      // Pure rotation, staged as the unit axis (in ang) and its norm (in the
+      // unused lin[0]), which joint_exp kind 4 evaluates without a square
+      // root or a division.
+      const float na = sqrtf(a2);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) r.ang[c] /= na;
+      r.lin[0] = na;
+    }
     r.kind = kind;
     r.pad = 0;
     recs[i] = r;

@@ -82,11 +82,22 @@ __device__ __forceinline__ Rot3 quat_to_rot(float w, float x, float y, float z) 
 // `kind` selects a cheaper exact form for the two joint types a URDF produces:
 //   1 = pure rotation (v = 0): t = 0
 //   2 = pure translation (omega = 0): R = I, t = v q
-//   3 = general twist.
+//   3 = general twist
+//   4 = pure rotation staged as a unit axis (ox, oy, oz) and the norm |omega|
+//       in vx: theta = |omega| q, q_rot = (cos(theta/2), sin(theta/2) axis),
+//       the same rotation as kind 1 without its square root and division.
 __device__ __forceinline__ Pose joint_exp(int kind, float q,
                                           float vx, float vy, float vz,
                                           float ox, float oy, float oz) {
   Pose p = pose_identity();
+  if (kind == 4) {
+    const float th = vx * q;
+    if (fabsf(th) < 1e-6f) return p;  // th^2 < 1e-12: the identity, as for kind 1
+    float s, c;
+    sincosf(0.5f * th, &s, &c);
+    p.w = c; p.x = s * ox; p.y = s * oy; p.z = s * oz;
+    return p;
+  }
   if (kind == 2) {
     p.tx = vx * q; p.ty = vy * q; p.tz = vz * q;
     return p;
