@@ -44,7 +44,7 @@ struct NodeRec {
   float mul, off;
   int src;          // actuated index driving this node, -1 none
   int parent;       // parent node, -1 = world
-  int kind;         // joint_exp kind (1..3)
+  int kind;         // joint_exp kind (1..4)
   int jac_type;     // 1 revolute, 2 prismatic, 0 no Jacobian column
   float ax[3];      // unit joint axis in the joint frame (Jacobian)
   unsigned ee_mask; // bit e set when the node is an ancestor of EE e
@@ -209,9 +209,10 @@ __device__ __forceinline__ void node_fk(const IkShared<MAXA, NEE>& sh, const flo
       for (int a = 0; a < MAXA; ++a)
         if (src == a) qs = q[a];
       const float qj = qs * r.mul + r.off;
+      const int kind = __builtin_amdgcn_readfirstlane(r.kind);
       const Pose tl = compose(load_pose(r.ptf),
-                              joint_exp(r.kind, qj, r.lin[0], r.lin[1], r.lin[2], r.ang[0],
-                                        r.ang[1], r.ang[2]));
+                              joint_exp<true>(kind, qj, r.lin[0], r.lin[1], r.lin[2], r.ang[0],
+                                              r.ang[1], r.ang[2]));
       const int par = __builtin_amdgcn_readfirstlane(r.parent);
       Pose pp = pose_identity();
       bool has_parent = false;

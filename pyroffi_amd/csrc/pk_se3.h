@@ -86,6 +86,12 @@ __device__ __forceinline__ Rot3 quat_to_rot(float w, float x, float y, float z) 
 //   4 = pure rotation staged as a unit axis (ox, oy, oz) and the norm |omega|
 //       in vx: theta = |omega| q, q_rot = (cos(theta/2), sin(theta/2) axis),
 //       the same rotation as kind 1 without its square root and division.
+//
+// kNativeTrig evaluates the kind-4 half-angle sine and cosine with the
+// hardware v_sin/v_cos (absolute error ~1e-6) instead of the correctly
+// rounded sincosf: for solvers' internal kinematics only, never for poses
+// that are returned.
+template <bool kNativeTrig = false>
 __device__ __forceinline__ Pose joint_exp(int kind, float q,
                                           float vx, float vy, float vz,
                                           float ox, float oy, float oz) {
@@ -94,7 +100,12 @@ __device__ __forceinline__ Pose joint_exp(int kind, float q,
     const float th = vx * q;
     if (fabsf(th) < 1e-6f) return p;  // th^2 < 1e-12: the identity, as for kind 1
     float s, c;
-    sincosf(0.5f * th, &s, &c);
+    if (kNativeTrig) {
+      s = __sinf(0.5f * th);
+      c = __cosf(0.5f * th);
+    } else {
+      sincosf(0.5f * th, &s, &c);
+    }
     p.w = c; p.x = s * ox; p.y = s * oy; p.z = s * oz;
     return p;
   }
