@@ -63,7 +63,7 @@ constexpr int kGroup = 8;  // lanes per waypoint evaluation
 static_assert(kWave % kGroup == 0, "a group must not straddle waves");
 constexpr int kMaxHist = 8;
 constexpr int kNumAlphas = 5;
-constexpr int kNodeWords = 20;   // ptf[7] lin[3] ang[3] mul off src parent kind pad pad
+constexpr int kNodeWords = 20;   // ptf[7] lin[3] ang[3] mul off src parent kind |ang| 1/|ang|
 constexpr int kRedSlots = 16;    // values per block reduction
 constexpr int kScalarWords = 16; // per-trajectory solver scalars in the state buffer
 
@@ -184,6 +184,11 @@ __device__ __forceinline__ TrajLds stage_traj_model(const TrajModelDev& md, int 
     if (w == 15) v = (v >= 0.f && v < static_cast<float>(A)) ? v : -1.f;           // src
     if (w == 16) v = (v >= 0.f && v < static_cast<float>(n)) ? v : -1.f;           // parent
     if (w == 17) v = (v >= 1.f && v <= 3.f) ? v : 3.f;                              // kind
+    if (w == 18 || w == 19) {  // axis norm and its inverse, for pure rotations (kind 1)
+      const float* r = md.nodes + n * kNodeWords;
+      const float na = sqrtf(r[10] * r[10] + r[11] * r[11] + r[12] * r[12]);
+      v = (r[17] == 1.f && na > 0.f) ? (w == 18 ? na : 1.f / na) : 0.f;
+    }
     nodes[i] = v;
   }
   for (int l = tid; l < L; l += blockDim.x) {
@@ -339,7 +344,12 @@ __device__ __forceinline__ void waypoint_fk(const TrajModelDev& md, const TrajLd
       for (int a = 0; a < MAXN; ++a)
         if (src == a) qs = q[a];
       const float qj = qs * r[13] + r[14];
-      const Pose tl = compose(load_pose(r), joint_exp(kind, qj, r[7], r[8], r[9], r[10], r[11], r[12]));
+      // A pure rotation (kind 1, non-zero axis) takes the closed form with
+      // the unit axis r[10..12] / |ang| and the hardware sine and cosine.
+      const float na = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r[18])));
+      const Pose tl = compose(load_pose(r), (kind == 1 && na > 0.f)
+          ? joint_exp<true>(4, qj, na, 0.f, 0.f, r[10] * r[19], r[11] * r[19], r[12] * r[19])
+          : joint_exp(kind, qj, r[7], r[8], r[9], r[10], r[11], r[12]));
       Pose pp = pose_identity();
       bool has_parent = false;
 #pragma unroll
