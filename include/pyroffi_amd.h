@@ -102,21 +102,70 @@ int pk_fk_links_f32(void* stream, const float* cfg, int64_t batch, int32_t n_act
                     int32_t n_links, float* out_links);
 
 /*
- * World collision, sphere robot vs sphere world, reduced over each link's S
- * spheres: out[b, n, m] = min_s (|c[b, s*N+n] - o_m| - r[b, s*N+n] - R_m).
- * Sphere slots with r < 0 are padding and skipped; a link with no real sphere
- * reports 1e9.
+ * Adjoint of pk_fk_links_f32: the vector-Jacobian product of the link poses.
+ * Given a cotangent on every wxyz_xyz link pose, returns dq = J^T cot, the
+ * gradient reverse-mode AD of Robot.forward_kinematics
+ * (src/pyroffi/_robot.py:198-263, through _link_poses_from_joint_poses
+ * :180-196) returns for the actuated configuration; mimic joints add their
+ * multiplier times their own term to their source.  The reference has no
+ * native kernel for it (JAX differentiates its FK); this is the backward of a
+ * jax.custom_vjp over FkCudaFfi.  FK is recomputed from cfg, so the forward
+ * needs to keep no residual but the configuration.
+ *
+ *   cfg        (batch, n_act)        float32
+ *   cot_links  (batch, n_links, 7)   float32, cotangent of out_links
+ *   out_dq     (batch, n_act)        float32
+ */
+int pk_fk_vjp_f32(void* stream, const float* cfg, int64_t batch, int32_t n_act,
+                  const float* twists, const float* parent_tf,
+                  const int32_t* parent_idx, const int32_t* act_idx,
+                  const float* mimic_mul, const float* mimic_off,
+                  const int32_t* mimic_act_idx, const int32_t* topo_inv,
+                  int32_t n_joints, const int32_t* link_parent_joint,
+                  int32_t n_links, const float* cot_links, float* out_dq);
+
+/*
+ * World collision per robot sphere: out[b, k, m] = signed distance of sphere k
+ * (centre c[:, b, k], radius r[b, k]) to obstacle m of the packed world, 1e9
+ * for a padding slot (r < 0).  Obstacles are indexed spheres first, then
+ * capsules, boxes and half-spaces (M = Ms + Mc + Mb + Mh), as in the
+ * reference.
+ *
+ * Replaces CollisionWorldSphereFfi
+ * (src/pyroffi/cuda_kernels/_collision_cuda_kernel.cu:659-669, kernels
+ * :64-161, bound in src/pyroffi/cuda_kernels/_collision_cuda_ffi.py:70-107).
+ *
+ *   sphere_centers   (3, batch, k) float32 SoA
+ *   sphere_radii     (batch, k)    float32
+ *   world_spheres    (Ms, 4)   [x, y, z, r]
+ *   world_capsules   (Mc, 7)   [a_xyz, b_xyz, r]
+ *   world_boxes      (Mb, 15)  [centre, axis1, axis2, axis3, half lengths]
+ *   world_halfspaces (Mh, 6)   [unit normal, point]
+ *   out              (batch, k, M) float32
+ */
+int pk_collision_world_sphere_f32(void* stream, const float* sphere_centers,
+                                  const float* sphere_radii, int64_t batch, int32_t k,
+                                  const float* world_spheres, int32_t n_world_spheres,
+                                  const float* world_capsules, int32_t n_world_capsules,
+                                  const float* world_boxes, int32_t n_world_boxes,
+                                  const float* world_halfspaces, int32_t n_world_halfspaces,
+                                  float* out);
+
+/*
+ * World collision, sphere robot, reduced over each link's S spheres:
+ * out[b, n, m] = min_s dist(sphere (b, s*N+n), obstacle m).  Sphere slots with
+ * r < 0 are padding and skipped; a link with no real sphere reports 1e9.
+ * Obstacle kinds and their order as for pk_collision_world_sphere_f32; a
+ * sphere-only world runs the fused sweep kernel, any other world the
+ * mixed-kind kernel.
  *
  * Replaces CollisionWorldSphereReducedFfi
  * (src/pyroffi/cuda_kernels/_collision_cuda_kernel.cu:673-839, bound in
- * src/pyroffi/cuda_kernels/_collision_cuda_ffi.py:109-151).  Only sphere
- * worlds are supported: n_world_capsules, n_world_boxes and
- * n_world_halfspaces must be 0 (PK_ERR_INVALID otherwise).
+ * src/pyroffi/cuda_kernels/_collision_cuda_ffi.py:109-151).
  *
  *   sphere_centers (3, batch, k) float32 SoA,  k = S * n_links
  *   sphere_radii   (batch, k)    float32
- *   world_spheres  (Ms, 4)       float32 [x, y, z, r]
- *   out            (batch, n_links, Ms) float32
+ *   out            (batch, n_links, M) float32
  */
 int pk_collision_world_sphere_reduced_f32(
     void* stream, const float* sphere_centers, const float* sphere_radii,
@@ -143,6 +192,31 @@ int pk_collision_self_sphere_f32(void* stream, const float* centers,
                                  int32_t n_links, const int32_t* pair_i,
                                  const int32_t* pair_j, int32_t n_pairs,
                                  float* out);
+
+/*
+ * The JAX path's distance conventions for a spherized robot, from link poses:
+ *   out_world[b, l, m] = min over slots s of |(o_m - c_ls) + 1e-6| - (r_ls + R_m)
+ *     (RobotCollisionSpherized.compute_world_collision_distance,
+ *     src/pyroffi/collision/_robot_collision.py:830-895)
+ *   out_self[b, p]     = min over slots s of |(c_js - c_is) + 1e-6| - (r_is + r_js)
+ *     for the pair (i, j) = (pair_i[p], pair_j[p]): same-slot ("diagonal")
+ *     sphere pairs only (compute_self_collision_distance, :796-828)
+ * with the 1e-6 added to every component inside the norm
+ * (src/pyroffi/collision/_utils.py:45).  Padding slots (r = -1e9) take part
+ * as in JAX and read about +1e9.  Either output may be NULL.  The reference
+ * computes these with JAX only; there is no FFI target to replace.
+ *
+ *   link_poses      (batch, n_links, 7)        float32
+ *   spheres_padded  (n_links, s_per_link, 4)   link-frame [x, y, z, r], r = -1e9 padding
+ *   world_spheres   (M, 4)
+ *   out_world       (batch, n_links, M),  out_self (batch, n_pairs)
+ */
+int pk_sphere_distances_jax_f32(void* stream, const float* link_poses, int64_t batch,
+                                int32_t n_links, const float* spheres_padded,
+                                int32_t s_per_link, const float* world_spheres,
+                                int32_t n_world_spheres, const int32_t* pair_i,
+                                const int32_t* pair_j, int32_t n_pairs, float* out_world,
+                                float* out_self);
 
 /*
  * Fused sphere sweep from link poses (no reference counterpart as one call:
@@ -337,6 +411,69 @@ int pk_ls_ik_select_f32(void* stream, const float* cfgs, const float* errs,
                         const float* previous, int32_t n_problems,
                         int32_t n_seeds, int32_t n_act, float continuity_weight,
                         float* out_best, int32_t* out_idx);
+
+/*
+ * The reference's pure-JAX LM solvers, per seed, with its residual
+ * f = log(T_ee^-1 T_tgt) (src/pyroffi/optimization_engines/_ik_primitives.py:93-103,
+ * jaxlie SE3.log) and the exact Jacobian of it:
+ *   adaptive_weights = 0: _ls_ik_single (src/pyroffi/optimization_engines/_ls_ik.py:192-350),
+ *     weights W = (pos_weight x3, ori_weight x3); limit_prior_weight and
+ *     kick_scale must be 0 to match it;
+ *   adaptive_weights = 1: _lm_refine_single (src/pyroffi/optimization_engines/_hjcd_ik.py:140-365),
+ *     per-EE adaptive weights, the soft joint-limit prior of weight
+ *     limit_prior_weight, and after six non-improving steps a kick of
+ *     kick_scale * N(0, 1) per free joint (counter-based normal keyed by
+ *     kick_key and the seed's flat index p * n_seeds + s) with lambda reset.
+ * The reference evaluates both inside jax.vmap over seeds; there is no FFI
+ * target to replace.  Seeds, targets and the robot buffers are those of
+ * pk_ls_ik_f32 (n_ee in [1, PK_MAX_EE]).
+ *   out_cfgs (n_problems, n_seeds, n_act) the all-time best configuration per seed
+ *   out_err  (n_problems, n_seeds) the winner metric at it: the sum over EEs of
+ *            |W f|^2 (adaptive_weights = 0, ls_ik_solve :503-516) or |f|^2
+ *            (adaptive_weights = 1, hjcd_solve _hjcd_ik.py:616-630)
+ * A robot whose moving joints exceed the padded width gets NaN errors.
+ */
+int pk_ik_jax_lm_f32(void* stream, const float* seeds, int32_t n_problems, int32_t n_seeds,
+                     int32_t n_act, const float* twists, const float* parent_tf,
+                     const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul,
+                     const float* mimic_off, const int32_t* mimic_act_idx, const int32_t* topo_inv,
+                     int32_t n_joints, const int32_t* target_jnts, const int32_t* ancestor_masks,
+                     int32_t n_ee, const float* target_T, const float* lower, const float* upper,
+                     const int32_t* fixed_mask, int32_t max_iter, float pos_weight, float ori_weight,
+                     float lambda_init, int32_t adaptive_weights, float limit_prior_weight,
+                     float kick_scale, uint64_t kick_key, float* out_cfgs, float* out_err);
+
+/*
+ * HJCD-IK coarse phase: _coarse_search_single
+ * (src/pyroffi/optimization_engines/_hjcd_ik.py:55-137) for every seed, k_max
+ * greedy Hamiltonian coordinate-descent steps on end effector ee_of_seed[s]
+ * (hjcd_solve splits the seeds equally over the EEs, :570-589), skipped once
+ * |f_pos| < epsilon and |f_rot| < nu.
+ *   out_cfgs (n_problems, n_seeds, n_act)
+ *   out_err  (n_problems, n_seeds) the sum over all EEs of |f|^2 (:591-597)
+ */
+int pk_hjcd_coarse_f32(void* stream, const float* seeds, int32_t n_problems, int32_t n_seeds,
+                       int32_t n_act, const float* twists, const float* parent_tf,
+                       const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul,
+                       const float* mimic_off, const int32_t* mimic_act_idx, const int32_t* topo_inv,
+                       int32_t n_joints, const int32_t* target_jnts, const int32_t* ancestor_masks,
+                       int32_t n_ee, const float* target_T, const int32_t* ee_of_seed,
+                       const float* lower, const float* upper, const int32_t* fixed_mask,
+                       int32_t k_max, float epsilon, float nu, float* out_cfgs, float* out_err);
+
+/*
+ * HJCD-IK refinement seeds (src/pyroffi/optimization_engines/_hjcd_ik.py:599-611):
+ * per problem the top_k coarse results by error (stable: ties by index, NaN
+ * last), tiled repeats times; copy r >= top_k adds
+ * noise[p, r] * 0.15 * (upper - lower) (zero on fixed joints) and is clipped.
+ *   coarse_cfgs (n_problems, n_seeds, n_act), coarse_err (n_problems, n_seeds)
+ *   noise       (n_problems, top_k * repeats, n_act) standard normal draws
+ *   out_seeds   (n_problems, top_k * repeats, n_act)
+ */
+int pk_hjcd_refine_seeds_f32(void* stream, const float* coarse_cfgs, const float* coarse_err,
+                             int32_t n_problems, int32_t n_seeds, int32_t n_act, int32_t top_k,
+                             int32_t repeats, const float* noise, const float* lower,
+                             const float* upper, const int32_t* fixed_mask, float* out_seeds);
 
 /*
  * Trajectory model for the trajectory cost and optimiser (device pointers).

@@ -60,8 +60,18 @@ def lib() -> ctypes.CDLL:
         traj_model = [_f, _f, _i, _i, _f, _f, _i, _i, _I, _I, _i, _I, _d, _I, _i, _i, _I, _d, _I, _D, _I]
         h.or_traj_cost.argtypes = traj_model + [_d, ctypes.POINTER(ctypes.c_int64), _L, _d, _d, _d, _d, _d, _I]
         h.or_traj_lbfgs.argtypes = traj_model + [_d, ctypes.POINTER(ctypes.c_int64), _L, _d, _d, _i, _d, _d, _d,
-                                                 _I]
+                                                 _d, _d, _I]
         h.or_two_loop.argtypes = [_d, _d, _d, _d, _I, _I, _I, _I, _d]
+        h.or_obstacle_dist.argtypes = [_d, _d, _L, _d, _I, _d, _I, _d, _I, _d, _I, _d, _I]
+        h.or_obstacle_dist.restype = None
+        robot8 = [_f, _f, _i, _i, _f, _f, _i, _i, _I]
+        h.or_jres.argtypes = [_d, _L, _I] + robot8 + [_i, _I, _d, _d, _d]
+        h.or_jres.restype = None
+        h.or_jlm.argtypes = [_d, _L, _I, _I] + robot8 + [_i, _I, _d, _d, _d, _i, _i, _d, ctypes.c_uint64, _d, _d,
+                                                         _I]
+        h.or_jlm.restype = None
+        h.or_hjcd_coarse.argtypes = [_d, _L, _I, _I] + robot8 + [_i, _I, _d, _i, _d, _d, _i, _I, _D, _D, _d, _d, _I]
+        h.or_hjcd_coarse.restype = None
         for fn in (h.or_fk, h.or_link_poses, h.or_world_dist, h.or_self_dist, h.or_ls_ik, h.or_res_jac,
                    h.or_traj_cost, h.or_traj_lbfgs, h.or_two_loop):
             fn.restype = None
@@ -289,17 +299,28 @@ def traj_cost(model: TrajModel, trajs, lower, upper, weights, grad: bool = True,
 
 def traj_lbfgs(model: TrajModel, trajs, lower, upper, *, n_iters=100, m_lbfgs=6, escalation_interval=20,
                w_smooth=1.0, w_vel=1.0, w_acc=0.5, w_jerk=0.1, w_limits=1.0, w_collision=1.0,
-               w_collision_max=100.0, penalty_scale=2.0, threads: int = 0):
-    """FP64 direct L-BFGS per trajectory: ``(best (W, A) flat, best tracked cost (B,), starts (B + 1))``."""
+               w_collision_max=100.0, penalty_scale=2.0, threads: int = 0, iterates: bool = False):
+    """FP64 direct L-BFGS per trajectory: ``(best (W, A) flat, best tracked cost (B,), starts (B + 1))``.
+
+    With ``iterates=True`` two more results follow: the last iterate ``(W, A)``
+    and the per-iteration trace ``(B, n_iters, 3)`` = (cost of the current
+    iterate at the current collision weight, accepted step index, accepted
+    point's cost at the maximum collision weight).
+    """
     x, start = _traj_flat(trajs)
     B = len(start) - 1
     best = np.empty_like(x)
     cost = np.empty(B)
+    last = np.empty_like(x) if iterates else None
+    trace = np.empty((B, max(n_iters, 0), 3)) if iterates else None
     cfg = _c([n_iters, m_lbfgs, escalation_interval], np.int32)
     fcfg = _c([w_smooth, w_vel, w_acc, w_jerk, w_limits, w_collision, w_collision_max, penalty_scale], np.float64)
     lo, hi = _c(lower, np.float64), _c(upper, np.float64)
     lib().or_traj_lbfgs(*model.args(), _p(x, _d), start.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), B,
-                        _p(lo, _d), _p(hi, _d), _p(cfg, _i), _p(fcfg, _d), _p(best, _d), _p(cost, _d), threads)
+                        _p(lo, _d), _p(hi, _d), _p(cfg, _i), _p(fcfg, _d), _p(best, _d), _p(cost, _d),
+                        None if last is None else _p(last, _d), None if trace is None else _p(trace, _d), threads)
+    if iterates:
+        return best, cost, start, last, trace
     return best, cost, start
 
 
@@ -311,3 +332,75 @@ def two_loop(g, s_buf, y_buf, rho, m_used: int, newest: int):
     out = np.empty(n)
     lib().or_two_loop(_p(gg, _d), _p(sb, _d), _p(yb, _d), _p(r, _d), int(m_used), int(newest), m, n, _p(out, _d))
     return out
+
+
+def obstacle_dist(centres, radii, spheres=None, capsules=None, boxes=None, halfspaces=None, threads: int = 0):
+    """``(N, M)`` FP64 signed distances of spheres ``(N, 3)`` / ``(N,)`` to the packed world
+    (spheres ``(Ms, 4)``, capsules ``(Mc, 7)``, boxes ``(Mb, 15)``, half-spaces ``(Mh, 6)``)."""
+    c = _c(centres, np.float64).reshape(-1, 3)
+    r = _c(radii, np.float64).reshape(-1)
+    parts = [_c(np.zeros((0, w)) if a is None else a, np.float64).reshape(-1, w)
+             for a, w in ((spheres, 4), (capsules, 7), (boxes, 15), (halfspaces, 6))]
+    M = sum(p.shape[0] for p in parts)
+    out = np.empty((c.shape[0], M))
+    args = []
+    for p in parts:
+        args += [_p(p, _d), p.shape[0]]
+    lib().or_obstacle_dist(_p(c, _d), _p(r, _d), c.shape[0], *args, _p(out, _d), threads)
+    return out
+
+
+def jres(tables: Tables, cfg, target_jnts, target_T, jac: bool = False):
+    """JAX residual ``log(T_ee^-1 T_tgt)`` per configuration: ``f (B, n_ee, 6)`` and,
+    with ``jac``, the exact Jacobian ``(B, 6 n_ee, A)``.  ``target_T`` is ``(B, n_ee, 7)``."""
+    q = _c(cfg, np.float64).reshape(-1, tables.A)
+    B = q.shape[0]
+    tj = _c(target_jnts, np.int32).reshape(-1)
+    n_ee = tj.shape[0]
+    tT = _c(np.broadcast_to(np.asarray(target_T, np.float64).reshape(-1, n_ee, 7), (B, n_ee, 7)), np.float64)
+    f = np.empty((B, n_ee, 6))
+    J = np.empty((B, 6 * n_ee, tables.A)) if jac else None
+    t = tables.ptrs()
+    lib().or_jres(_p(q, _d), B, tables.A, *t, tables.J, _p(tj, _i), n_ee, _p(tT, _d), _p(f, _d),
+                  None if J is None else _p(J, _d))
+    return (f, J) if jac else f
+
+
+def jlm(tables: Tables, seeds, target_jnts, target_T, lower, upper, fixed_mask=None, *, max_iter=60,
+        pos_weight=50.0, ori_weight=10.0, lambda_init=5e-3, adaptive=False, prior_weight=0.0, kick_scale=0.0,
+        kick_key=0, threads: int = 0):
+    """FP64 JAX-convention LM per seed: ``(cfgs (P, S, A), winner errors (P, S))``."""
+    sd = _c(seeds, np.float64)
+    P, S, A = sd.shape
+    tj = _c(target_jnts, np.int32).reshape(-1)
+    n_ee = tj.shape[0]
+    tT = _c(target_T, np.float64).reshape(P, n_ee, 7)
+    fm = _c(np.zeros(A) if fixed_mask is None else fixed_mask, np.int32)
+    iopt = _c([max_iter, int(bool(adaptive))], np.int32)
+    dopt = _c([pos_weight, ori_weight, lambda_init, prior_weight, kick_scale], np.float64)
+    out = np.empty((P, S, A))
+    err = np.empty((P, S))
+    lo, hi = _c(lower, np.float64), _c(upper, np.float64)
+    lib().or_jlm(_p(sd, _d), P, S, A, *tables.ptrs(), tables.J, _p(tj, _i), n_ee, _p(tT, _d), _p(lo, _d),
+                 _p(hi, _d), _p(fm, _i), _p(iopt, _i), _p(dopt, _d), ctypes.c_uint64(int(kick_key)), _p(out, _d),
+                 _p(err, _d), threads)
+    return out, err
+
+
+def hjcd_coarse(tables: Tables, seeds, target_jnts, target_T, ee_of_seed, lower, upper, fixed_mask=None, *,
+                k_max=20, epsilon=0.02, nu=np.pi / 2, threads: int = 0):
+    """FP64 HJCD coarse phase per seed: ``(cfgs (P, S, A), sum over EEs of |f|^2 (P, S))``."""
+    sd = _c(seeds, np.float64)
+    P, S, A = sd.shape
+    tj = _c(target_jnts, np.int32).reshape(-1)
+    n_ee = tj.shape[0]
+    tT = _c(target_T, np.float64).reshape(P, n_ee, 7)
+    es = _c(ee_of_seed, np.int32).reshape(S)
+    fm = _c(np.zeros(A) if fixed_mask is None else fixed_mask, np.int32)
+    out = np.empty((P, S, A))
+    err = np.empty((P, S))
+    lo, hi = _c(lower, np.float64), _c(upper, np.float64)
+    lib().or_hjcd_coarse(_p(sd, _d), P, S, A, *tables.ptrs(), tables.J, _p(tj, _i), n_ee, _p(tT, _d), _p(es, _i),
+                         _p(lo, _d), _p(hi, _d), _p(fm, _i), int(k_max), float(epsilon), float(nu), _p(out, _d),
+                         _p(err, _d), threads)
+    return out, err

@@ -28,6 +28,14 @@
  *                    collision/_collision.py:136-167)
  *   or_two_loop      _sco_optimization.py:119-167
  *   or_traj_lbfgs    _lbfgs_trajopt_optimization.py:76-205
+ *   or_jres          the JAX residual log(T_ee^-1 T_tgt) (_ik_primitives.py:93-103,
+ *                    jaxlie 1.5.0 SE3.log) and its exact Jacobian
+ *   or_jlm           _ls_ik_single (_ls_ik.py:192-350) and _lm_refine_single
+ *                    (_hjcd_ik.py:140-365) per seed
+ *   or_hjcd_coarse   _coarse_search_single (_hjcd_ik.py:55-137)
+ *   or_obstacle_dist sphere vs sphere / capsule / box / half-space
+ *                    (_collision_cuda_helpers.cuh:14-142, packed rows of
+ *                    collision/_cuda_collision.py:98-168)
  *
  * Every function here is pinned by tests/test_oracle.py against checks that
  * share no code with it (known answers, scipy, finite differences).
@@ -251,6 +259,65 @@ void or_self_dist(const double* link_poses, int64_t B, int L, const double* sphe
         }
       }
       out[b * P + p] = best;
+    }
+  }
+}
+
+/* Signed distance of sphere (p, r) to one packed obstacle of kind k
+ * (0 sphere [c, R], 1 capsule [a, b, R], 2 box [c, ax1, ax2, ax3, h],
+ * 3 half-space [n, point]). */
+static double obstacle_dist(const double* p, double r, int k, const double* o) {
+  if (k == 0) {
+    const double dx = p[0] - o[0], dy = p[1] - o[1], dz = p[2] - o[2];
+    return sqrt(dx * dx + dy * dy + dz * dz) - (r + o[3]);
+  }
+  if (k == 1) {
+    const double u[3] = {o[3] - o[0], o[4] - o[1], o[5] - o[2]};
+    const double w[3] = {p[0] - o[0], p[1] - o[1], p[2] - o[2]};
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double t = 0.0;
+    if (uu > 1e-12) {
+      t = (w[0] * u[0] + w[1] * u[1] + w[2] * u[2]) / uu;
+      t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    }
+    const double dx = w[0] - t * u[0], dy = w[1] - t * u[1], dz = w[2] - t * u[2];
+    return sqrt(dx * dx + dy * dy + dz * dz) - (r + o[6]);
+  }
+  if (k == 2) {
+    const double d[3] = {p[0] - o[0], p[1] - o[1], p[2] - o[2]};
+    double e[3], out = 0.0, in = -INFINITY;
+    for (int a = 0; a < 3; ++a) {
+      const double loc = d[0] * o[3 + 3 * a] + d[1] * o[4 + 3 * a] + d[2] * o[5 + 3 * a];
+      e[a] = fabs(loc) - o[12 + a];
+      if (e[a] > 0) out += e[a] * e[a];
+      if (e[a] > in) in = e[a];
+    }
+    return sqrt(out) + (in < 0 ? in : 0.0) - r;
+  }
+  return (p[0] - o[3]) * o[0] + (p[1] - o[4]) * o[1] + (p[2] - o[5]) * o[2] - r;
+}
+
+/* out (N, M) with M = Ms + Mc + Mb + Mh obstacles in that order; sphere n is
+ * centres[3n..3n+2], radius radii[n]; r < 0 is padding and reads 1e9. */
+void or_obstacle_dist(const double* centres, const double* radii, int64_t N, const double* ws, int ms,
+                      const double* wc, int mc, const double* wb, int mb, const double* wh, int mh, double* out,
+                      int threads) {
+  const int M = ms + mc + mb + mh;
+  set_threads(threads);
+#pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < N; ++n) {
+    for (int m = 0; m < M; ++m) {
+      double d = 1e9;
+      if (radii[n] >= 0) {
+        int k = 0, i = m;
+        const double* o;
+        if (i < ms) { o = ws + 4 * i; }
+        else if ((i -= ms) < mc) { k = 1; o = wc + 7 * i; }
+        else if ((i -= mc) < mb) { k = 2; o = wb + 15 * i; }
+        else { i -= mb; k = 3; o = wh + 6 * i; }
+        d = obstacle_dist(centres + 3 * n, radii[n], k, o);
+      }
+      out[n * M + m] = d;
     }
   }
 }
@@ -870,14 +937,19 @@ void or_two_loop(const double* g, const double* s_buf, const double* y_buf, cons
  * trajectory.  cfg = {n_iters, m, escalation_interval} (ints) and
  * fcfg = {w_smooth, w_vel, w_acc, w_jerk, w_limits, w_collision, w_collision_max,
  * penalty_scale}; endpoints stay where x_init has them.  out_best (W, A),
- * out_best_cost (B) = the tracked best cost. */
+ * out_best_cost (B) = the tracked best cost.  When non-NULL, out_last (W, A)
+ * receives the last iterate and out_trace (B, n_iters, 3) per iteration the
+ * cost of the current iterate at the current collision weight, the index of
+ * the accepted step in the step list, and the accepted point's cost at the
+ * maximum collision weight (the value the best tracking compares). */
 void or_traj_lbfgs(const float* twists, const float* parent_tf, const int32_t* parent_idx, const int32_t* act_idx,
                    const float* mimic_mul, const float* mimic_off, const int32_t* mimic_act_idx,
                    const int32_t* topo_inv, int J, int A, const int32_t* link_parent_joint, int L,
                    const double* spheres, int S, const int32_t* pair_i, const int32_t* pair_j, int P,
                    const double* world, int M, double margin, int mode, const double* x_init,
                    const int64_t* start, int64_t B, const double* lower, const double* upper, const int32_t* cfg,
-                   const double* fcfg, double* out_best, double* out_best_cost, int threads) {
+                   const double* fcfg, double* out_best, double* out_best_cost, double* out_last,
+                   double* out_trace, int threads) {
   static const double alphas[5] = {1.0, 0.5, 0.25, 0.1, 0.025};
   tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
   traj_model_t md = {&t, link_parent_joint, L, spheres, S, pair_i, pair_j, P, world, M, margin, mode};
@@ -952,13 +1024,401 @@ void or_traj_lbfgs(const float* twists, const float* parent_tf, const int32_t* p
       memcpy(gp, g, sizeof(double) * n);
       for (int i = 0; i < n; ++i) x[i] += alphas[pick] * dir[i];
       const double eval = w[0] * tcomp[pick][0] + w[4] * tcomp[pick][1] + fcfg[6] * tcomp[pick][2];
+      if (out_trace) {
+        double* tr = out_trace + ((size_t)b * n_iters + k) * 3;
+        tr[0] = cost;
+        tr[1] = pick;
+        tr[2] = eval;
+      }
       if (eval < best_cost) {
         best_cost = eval;
         memcpy(best, x, sizeof(double) * n);
       }
     }
     memcpy(out_best + start[b] * A, best, sizeof(double) * n);
+    if (out_last) memcpy(out_last + start[b] * A, x, sizeof(double) * n);
     out_best_cost[b] = best_cost;
     free(x); free(xp); free(gp); free(g); free(dir); free(xt); free(best); free(sb); free(yb);
+  }
+}
+/* ------------------------------------------------- JAX-convention LM / HJCD */
+
+/* jaxlie 1.5.0 (a third-party dependency, pinned in pyproject.toml:27, absent
+ * here): SO3.log, SE3.log, inverse and compose in float64 (epsilon 1e-10),
+ * as the reference's JAX residual _ik_primitives.py:93-103 evaluates them.
+ * jlog_dir returns the directional derivative of log(T) under the left
+ * perturbation T -> exp(d) T, branch by branch, i.e. what reverse-mode AD of
+ * that code computes. */
+#define JL_EPS 1e-10
+#define OR_PI 3.14159265358979323846
+
+typedef struct {
+  double q[4], t[3];     /* T_rel: quaternion (as composed) and translation */
+  double om[3], rho[3];  /* log = [rho; om] */
+  double F, Fw, Fn2;     /* om = q_v F, dF = Fw dq_w + Fn2 d|q_v|^2 */
+  double c, dc;          /* V^-1 = I - Om/2 + c Om^2, dc = dc * d(theta^2) */
+} jlog_t;
+
+static void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static void jlog_eval(const double* q, const double* t, jlog_t* L) {
+  for (int i = 0; i < 4; ++i) L->q[i] = q[i];
+  for (int i = 0; i < 3; ++i) L->t[i] = t[i];
+  const double w = q[0], n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  if (n2 < JL_EPS) {
+    L->F = 2.0 / w - 2.0 / 3.0 * n2 / (w * w * w);
+    L->Fw = -2.0 / (w * w) + 2.0 * n2 / (w * w * w * w);
+    L->Fn2 = -2.0 / 3.0 / (w * w * w);
+  } else {
+    const double n = sqrt(n2);
+    if (fabs(w) < JL_EPS) {
+      const double s = w > 0 ? 1.0 : -1.0;
+      L->F = s * OR_PI / n;
+      L->Fw = 0.0;
+      L->Fn2 = -s * OR_PI / (2.0 * n2 * n);
+    } else {
+      const double a = atan2(w < 0 ? -n : n, fabs(w));
+      L->F = 2.0 * a / n;
+      L->Fw = -2.0 / (n2 + w * w);
+      L->Fn2 = w / (n2 * (n2 + w * w)) - a / (n2 * n);
+    }
+  }
+  for (int i = 0; i < 3; ++i) L->om[i] = q[1 + i] * L->F;
+  const double th2 = L->om[0] * L->om[0] + L->om[1] * L->om[1] + L->om[2] * L->om[2];
+  if (th2 < JL_EPS) {
+    L->c = 1.0 / 12.0;
+    L->dc = 0.0;
+  } else {
+    const double th = sqrt(th2), h = 0.5 * th;
+    const double g = 1.0 - h * cos(h) / sin(h);
+    const double gp = -0.5 * cos(h) / sin(h) + 0.5 * h / (sin(h) * sin(h));
+    L->c = g / th2;
+    L->dc = (gp / th2 - 2.0 * g / (th2 * th)) / (2.0 * th);
+  }
+  double ot[3], oot[3];
+  cross3(L->om, t, ot);
+  cross3(L->om, ot, oot);
+  for (int i = 0; i < 3; ++i) L->rho[i] = t[i] - 0.5 * ot[i] + L->c * oot[i];
+}
+
+/* d log(T) for the left perturbation d = (dv, dw): out = [d rho; d om]. */
+static void jlog_dir(const jlog_t* L, const double* dv, const double* dw, double* out) {
+  const double* q = L->q;
+  const double dqw = -0.5 * (dw[0] * q[1] + dw[1] * q[2] + dw[2] * q[3]);
+  double dqv[3], wxq[3];
+  cross3(dw, q + 1, wxq);
+  for (int i = 0; i < 3; ++i) dqv[i] = 0.5 * (q[0] * dw[i] + wxq[i]);
+  const double dn2 = 2.0 * (q[1] * dqv[0] + q[2] * dqv[1] + q[3] * dqv[2]);
+  const double dF = L->Fw * dqw + L->Fn2 * dn2;
+  double dom[3];
+  for (int i = 0; i < 3; ++i) dom[i] = dqv[i] * L->F + q[1 + i] * dF;
+  double dt[3], wxt[3];
+  cross3(dw, L->t, wxt);
+  for (int i = 0; i < 3; ++i) dt[i] = wxt[i] + dv[i];
+  const double dth2 = 2.0 * (L->om[0] * dom[0] + L->om[1] * dom[1] + L->om[2] * dom[2]);
+  double a1[3], a2[3], ot[3], oot[3], dot_[3], odt[3], oodt[3], domot[3], odomt[3];
+  cross3(dom, L->t, a1);        /* dw x t */
+  cross3(L->om, dt, a2);        /* w x dt */
+  cross3(L->om, L->t, ot);
+  cross3(L->om, ot, oot);       /* w x (w x t) */
+  cross3(dom, ot, domot);       /* dw x (w x t) */
+  cross3(L->om, a1, odomt);     /* w x (dw x t) */
+  cross3(L->om, dt, odt);
+  cross3(L->om, odt, oodt);     /* w x (w x dt) */
+  (void)dot_;
+  for (int i = 0; i < 3; ++i)
+    out[i] = dt[i] - 0.5 * (a1[i] + a2[i]) + L->dc * dth2 * oot[i] + L->c * (domot[i] + odomt[i] + oodt[i]);
+  for (int i = 0; i < 3; ++i) out[3 + i] = dom[i];
+}
+
+/* T_rel = T_act^-1 T_tgt (jaxlie inverse, then compose). */
+static void jrel(const se3* A, const double* tgt, double* q, double* t) {
+  qmul(A->w, -A->x, -A->y, -A->z, tgt[0], tgt[1], tgt[2], tgt[3], q);
+  const double d[3] = {tgt[4] - A->tx, tgt[5] - A->ty, tgt[6] - A->tz};
+  se3 inv = {A->w, -A->x, -A->y, -A->z, 0, 0, 0};
+  qrot(&inv, d, t);
+}
+
+typedef struct {
+  const tables_t* t;
+  int n_ee;
+  const int32_t* tj;   /* (n_ee) target joints */
+  const double* tgt;   /* (n_ee, 7) */
+} jprob_t;
+
+/* Residuals f = log(T_ee^-1 T_tgt) of EEs [e0, e1) at q (f[6 (e - e0) ..]) and,
+ * when Jm is non-NULL, their exact Jacobian (6 (e1 - e0) x A, row-major):
+ * joint j of the EE's chain with spatial twist xi_s = (R_j v + t_j x w_s,
+ * w_s = R_j w) moves T_ee by the body twist xi_b = Ad(T_ee^-1) xi_s, which
+ * perturbs T_rel on the left by -xi_b; its column is mul_j * D log[-xi_b]
+ * added to the joint's actuated source. */
+static void jres_jac(const jprob_t* p, const double* q, int e0, int e1, double* f, double* Jm) {
+  se3 Tw[OR_MAXJ];
+  const tables_t* t = p->t;
+  const int A = t->A;
+  fk_one(t, q, Tw);
+  if (Jm) memset(Jm, 0, sizeof(double) * 6 * (e1 - e0) * A);
+  for (int e = e0; e < e1; ++e) {
+    const int r0 = 6 * (e - e0);
+    const se3* Te = &Tw[p->tj[e]];
+    double qr[4], tr[3];
+    jrel(Te, p->tgt + 7 * e, qr, tr);
+    jlog_t L;
+    jlog_eval(qr, tr, &L);
+    for (int c = 0; c < 3; ++c) { f[r0 + c] = L.rho[c]; f[r0 + 3 + c] = L.om[c]; }
+    if (!Jm) continue;
+    const se3 inv = {Te->w, -Te->x, -Te->y, -Te->z, 0, 0, 0};
+    for (int j = p->tj[e]; j >= 0; j = t->parent_idx[j]) {
+      const int m = t->mimic_act_idx[j];
+      const int src = (m != -1) ? m : t->act_idx[j];
+      if (src < 0 || src >= A) continue;
+      const float* tw = t->twists + 6 * j;
+      const double vl[3] = {tw[0], tw[1], tw[2]}, wl[3] = {tw[3], tw[4], tw[5]};
+      double ws[3], vs[3], arm[3], axw[3], vb[3], wb[3], col[6];
+      qrot(&Tw[j], wl, ws);
+      qrot(&Tw[j], vl, vs);
+      arm[0] = Tw[j].tx - Te->tx; arm[1] = Tw[j].ty - Te->ty; arm[2] = Tw[j].tz - Te->tz;
+      cross3(arm, ws, axw);
+      for (int c = 0; c < 3; ++c) vs[c] += axw[c];
+      qrot(&inv, vs, vb);
+      qrot(&inv, ws, wb);
+      jlog_dir(&L, vb, wb, col);
+      const double mul = t->mimic_mul[j];
+      for (int c = 0; c < 6; ++c) Jm[(r0 + c) * A + src] -= mul * col[c];
+    }
+  }
+}
+
+/* _adaptive_weights (_ik_primitives.py:106-116) of one 6-vector. */
+static double adaptive_scale(const double* f) {
+  const double pe = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) + 1e-8;
+  const double oe = sqrt(f[3] * f[3] + f[4] * f[4] + f[5] * f[5]) + 1e-8;
+  const double s = pe / oe;
+  return s < 0.05 ? 0.05 : (s > 1.0 ? 1.0 : s);
+}
+
+static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+/* Counter-based standard normal for the stall kicks: splitmix64 (Steele,
+ * Lea and Flood) of (key, seed, iteration, joint), Box-Muller of two 24-bit
+ * uniforms.  The HIP solver draws the same bits. */
+static uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static double kick_normal(uint64_t key, uint64_t gid, int it, int j) {
+  const uint64_t h = mix64(key ^ mix64((gid << 20) ^ ((uint64_t)it << 6) ^ (uint64_t)j));
+  const double u1 = (double)((h >> 40) + 1) * (1.0 / 16777216.0);
+  const double u2 = (double)((h >> 16) & 0xFFFFFFull) * (1.0 / 16777216.0);
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * OR_PI * u2);
+}
+
+typedef struct {
+  int max_iter;
+  double pos_weight, ori_weight, lambda_init;
+  int adaptive;          /* 1: per-EE adaptive weights (HJCD), 0: W = (pw, pw, pw, ow, ow, ow) */
+  double prior_weight;   /* soft joint-limit prior (HJCD), 0 off */
+  double kick_scale;     /* stall kicks (HJCD), 0 off */
+  uint64_t kick_key;
+} jlm_opts_t;
+
+static void jweights(const jlm_opts_t* o, const double* f, int n_ee, double* w) {
+  for (int e = 0; e < n_ee; ++e) {
+    const double s = o->adaptive ? adaptive_scale(f + 6 * e) : 0.0;
+    for (int c = 0; c < 6; ++c)
+      w[6 * e + c] = o->adaptive ? (c < 3 ? 1.0 : s) : (c < 3 ? o->pos_weight : o->ori_weight);
+  }
+}
+
+/* One seed of _ls_ik_single (_ls_ik.py:192-350; adaptive = 0, no prior, no
+ * kicks) or _lm_refine_single (_hjcd_ik.py:140-365).  Returns the all-time
+ * best configuration; *out_err = the winner metric at it, sum over EEs of
+ * |W f|^2 (ls_ik_solve, _ls_ik.py:503-516) or |f|^2 (hjcd_solve,
+ * _hjcd_ik.py:616-630). */
+static void jlm_seed(const jprob_t* p, const double* lo, const double* hi, const int32_t* fixed,
+                     const jlm_opts_t* o, const double* seed, uint64_t gid, double* out_q, double* out_err) {
+  static const double alphas[5] = {1.0, 0.5, 0.25, 0.1, 0.025};
+  const int A = p->t->A, n_ee = p->n_ee, R6 = 6 * n_ee;
+  double c[OR_MAXA], best[OR_MAXA], f[6 * OR_MAXEE], w[6 * OR_MAXEE], Jm[6 * OR_MAXEE * OR_MAXA];
+  double mid[OR_MAXA], dprior[OR_MAXA];
+  for (int a = 0; a < A; ++a) {
+    c[a] = best[a] = seed[a];
+    mid[a] = 0.5 * (lo[a] + hi[a]);
+    const double half = 0.5 * (hi[a] - lo[a]) + 1e-8;
+    dprior[a] = o->prior_weight / (half * half);
+  }
+  jres_jac(p, c, 0, n_ee, f, NULL);
+  jweights(o, f, n_ee, w);
+  double best_err = 0.0;
+  for (int i = 0; i < R6; ++i) best_err += (w[i] * f[i]) * (w[i] * f[i]);
+  double lam = o->lambda_init;
+  int stall = 0;
+  for (int it = 0; it < o->max_iter; ++it) {
+    jres_jac(p, c, 0, n_ee, f, Jm);
+    jweights(o, f, n_ee, w);
+    double fw[6 * OR_MAXEE], cur = 0.0, col[OR_MAXA];
+    for (int i = 0; i < R6; ++i) { fw[i] = w[i] * f[i]; cur += fw[i] * fw[i]; }
+    for (int a = 0; a < A; ++a) {
+      double s = 0.0;
+      for (int i = 0; i < R6; ++i) { Jm[i * A + a] *= w[i]; s += Jm[i * A + a] * Jm[i * A + a]; }
+      col[a] = sqrt(s) + 1e-8;
+    }
+    double M[OR_MAXA * OR_MAXA], rhs[OR_MAXA];
+    for (int a = 0; a < A; ++a) {
+      for (int b = 0; b < A; ++b) {
+        double s = 0.0;
+        for (int i = 0; i < R6; ++i) s += Jm[i * A + a] * Jm[i * A + b];
+        M[a * A + b] = s / (col[a] * col[b]);
+      }
+      M[a * A + a] += lam + dprior[a] / (col[a] * col[a]);
+      double s = 0.0;
+      for (int i = 0; i < R6; ++i) s += Jm[i * A + a] * fw[i];
+      rhs[a] = -(s / col[a] + dprior[a] * (c[a] - mid[a]) / col[a]);
+    }
+    chol_solve(M, rhs, A);
+    double delta[OR_MAXA], dn = 0.0;
+    for (int a = 0; a < A; ++a) {
+      delta[a] = fixed[a] ? 0.0 : rhs[a] / col[a];
+      dn += delta[a] * delta[a];
+    }
+    double pe = 0.0, oe = 0.0;
+    for (int e = 0; e < n_ee; ++e) {
+      const double* fe = f + 6 * e;
+      pe = fmax(pe, sqrt(fe[0] * fe[0] + fe[1] * fe[1] + fe[2] * fe[2]));
+      oe = fmax(oe, sqrt(fe[3] * fe[3] + fe[4] * fe[4] + fe[5] * fe[5]));
+    }
+    const double R = (pe > 1e-2 || oe > 0.6) ? 0.38 : (pe > 1e-3 || oe > 0.25) ? 0.22
+                   : (pe > 2e-4 || oe > 0.08) ? 0.12 : 0.05;
+    dn = sqrt(dn) + 1e-18;
+    if (dn > R)
+      for (int a = 0; a < A; ++a) delta[a] *= R / dn;
+    double best_a = INFINITY, qa[OR_MAXA], newc[OR_MAXA];
+    int pick = 0;
+    for (int k = 0; k < 5; ++k) {
+      for (int a = 0; a < A; ++a) qa[a] = clampd(c[a] + alphas[k] * delta[a], lo[a], hi[a]);
+      double fa[6 * OR_MAXEE], ea = 0.0;
+      jres_jac(p, qa, 0, n_ee, fa, NULL);
+      for (int i = 0; i < R6; ++i) ea += (w[i] * fa[i]) * (w[i] * fa[i]);
+      if (ea < best_a) { best_a = ea; pick = k; }
+    }
+    for (int a = 0; a < A; ++a) newc[a] = clampd(c[a] + alphas[pick] * delta[a], lo[a], hi[a]);
+    const int improved = best_a < cur * (1.0 - 1e-4);
+    lam = clampd(improved ? lam * 0.5 : lam * 3.0, 1e-10, 1e6);
+    if (best_a < best_err) {
+      best_err = best_a;
+      for (int a = 0; a < A; ++a) best[a] = newc[a];
+    }
+    if (improved)
+      for (int a = 0; a < A; ++a) c[a] = newc[a];
+    if (o->kick_scale > 0.0) {
+      stall = improved ? 0 : stall + 1;
+      if (stall >= 6) {
+        for (int a = 0; a < A; ++a) {
+          const double k = fixed[a] ? 0.0 : kick_normal(o->kick_key, gid, it, a) * o->kick_scale;
+          c[a] = clampd(c[a] + k, lo[a], hi[a]);
+        }
+        lam = o->lambda_init;
+        stall = 0;
+      }
+    }
+  }
+  jres_jac(p, best, 0, n_ee, f, NULL);
+  double e = 0.0;
+  for (int i = 0; i < R6; ++i) {
+    const double wi = o->adaptive ? 1.0 : ((i % 6) < 3 ? o->pos_weight : o->ori_weight);
+    e += (wi * f[i]) * (wi * f[i]);
+  }
+  for (int a = 0; a < A; ++a) out_q[a] = best[a];
+  *out_err = e;
+}
+
+/* seeds (P, S, A), target_T (P, n_ee, 7); out (P, S, A), out_err (P, S).
+ * iopt = {max_iter, adaptive}; dopt = {pos_weight, ori_weight, lambda_init,
+ * prior_weight, kick_scale}; seed (p, s) draws kicks with id gid0 + p S + s. */
+void or_jlm(const double* seeds, int64_t P, int S, int A, const float* twists, const float* parent_tf,
+            const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul, const float* mimic_off,
+            const int32_t* mimic_act_idx, const int32_t* topo_inv, int J, const int32_t* target_jnts, int n_ee,
+            const double* target_T, const double* lower, const double* upper, const int32_t* fixed_mask,
+            const int32_t* iopt, const double* dopt, uint64_t kick_key, double* out, double* out_err,
+            int threads) {
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  const jlm_opts_t o = {iopt[0], dopt[0], dopt[1], dopt[2], iopt[1], dopt[3], dopt[4], kick_key};
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t i = 0; i < P * S; ++i) {
+    const int64_t pb = i / S;
+    const jprob_t pr = {&t, n_ee, target_jnts, target_T + pb * n_ee * 7};
+    jlm_seed(&pr, lower, upper, fixed_mask, &o, seeds + i * A, (uint64_t)i, out + i * A, out_err + i);
+  }
+}
+
+/* _coarse_search_single (_hjcd_ik.py:55-137) for every seed: seed (p, s) runs
+ * k_max Hamiltonian coordinate-descent steps on EE ee_of_seed[s] alone.
+ * out_err (P, S) = sum over all EEs of |f|^2 at the result (:591-597). */
+void or_hjcd_coarse(const double* seeds, int64_t P, int S, int A, const float* twists, const float* parent_tf,
+                    const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul,
+                    const float* mimic_off, const int32_t* mimic_act_idx, const int32_t* topo_inv, int J,
+                    const int32_t* target_jnts, int n_ee, const double* target_T, const int32_t* ee_of_seed,
+                    const double* lower, const double* upper, const int32_t* fixed_mask, int k_max,
+                    double epsilon, double nu, double* out, double* out_err, int threads) {
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t i = 0; i < P * S; ++i) {
+    const int64_t pb = i / S;
+    const int s = (int)(i % S);
+    const jprob_t pr = {&t, n_ee, target_jnts, target_T + pb * n_ee * 7};
+    const int ee = ee_of_seed[s];
+    double c[OR_MAXA], mom[OR_MAXA] = {0}, f[6 * OR_MAXEE], Jm[6 * OR_MAXA];
+    for (int a = 0; a < A; ++a) c[a] = seeds[i * A + a];
+    for (int k = 0; k < k_max; ++k) {
+      jres_jac(&pr, c, ee, ee + 1, f, Jm);
+      const double pe = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+      const double oe = sqrt(f[3] * f[3] + f[4] * f[4] + f[5] * f[5]);
+      if (pe < epsilon && oe < nu) continue;
+      const double gate = pe < 1e-3 ? 1.0 : 0.0;
+      const double so = adaptive_scale(f) * gate;
+      double w[6] = {1, 1, 1, so, so, so};
+      int bj = 0;
+      double bs = -INFINITY, gb = 0, nb = 1;
+      for (int a = 0; a < A; ++a) {
+        double g = 0.0, nsq = 1e-8;
+        for (int r = 0; r < 6; ++r) {
+          const double jw = w[r] * Jm[r * A + a];
+          g += jw * (w[r] * f[r]);
+          nsq += jw * jw;
+        }
+        const double score = fixed_mask[a] ? -1.0 : fabs(g) / sqrt(nsq);
+        if (score > bs) { bs = score; bj = a; gb = g; nb = nsq; }
+      }
+      const double pb2 = 0.9 * mom[bj] - 0.35 * gb / nb;
+      mom[bj] = pb2;
+      if (!fixed_mask[bj]) c[bj] = clampd(c[bj] + pb2, lower[bj], upper[bj]);
+    }
+    jres_jac(&pr, c, 0, n_ee, f, NULL);
+    double e = 0.0;
+    for (int r = 0; r < 6 * n_ee; ++r) e += f[r] * f[r];
+    for (int a = 0; a < A; ++a) out[i * A + a] = c[a];
+    out_err[i] = e;
+  }
+}
+
+/* Residuals (B, n_ee, 6) of configurations cfg (B, A) against one target set
+ * per configuration (B, n_ee, 7), and optionally their Jacobians (B, 6 n_ee, A). */
+void or_jres(const double* cfg, int64_t B, int A, const float* twists, const float* parent_tf,
+             const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul, const float* mimic_off,
+             const int32_t* mimic_act_idx, const int32_t* topo_inv, int J, const int32_t* target_jnts, int n_ee,
+             const double* target_T, double* out_f, double* out_J) {
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  for (int64_t b = 0; b < B; ++b) {
+    const jprob_t pr = {&t, n_ee, target_jnts, target_T + b * n_ee * 7};
+    jres_jac(&pr, cfg + b * A, 0, n_ee, out_f + b * 6 * n_ee, out_J ? out_J + b * 6 * n_ee * A : NULL);
   }
 }

@@ -2,11 +2,13 @@
 
 Public API (same names and arguments as pyroffi):
 
-* ``Robot.from_urdf`` / ``Robot.forward_kinematics``
+* ``Robot.from_urdf`` / ``Robot.forward_kinematics`` (differentiable: its
+  backward is the FK adjoint kernel) / ``Robot.inverse_kinematics`` (HJCD-IK)
 * ``collision.RobotCollisionSpherized``, ``collision.CUDARobotCollisionChecker``,
   ``collision.colldist_from_sdf``
-* ``ls_ik_solve_cuda`` / ``ls_ik_solve_cuda_batch``; ``UnevenLsIkSolver`` for
-  uneven multi-robot batches in one graph
+* ``ls_ik_solve`` (the JAX-convention LM), ``hjcd_solve`` /
+  ``hjcd_solve_batch``, ``ls_ik_solve_cuda`` / ``ls_ik_solve_cuda_batch``;
+  ``UnevenLsIkSolver`` for uneven multi-robot batches in one graph
 * ``costs.pose_hinge_cost`` (differentiable, hand-written adjoint)
 * ``lbfgs_trajopt`` / ``LbfgsTrajOptConfig`` (direct L-BFGS trajectory
   optimisation) and ``lbfgs_trajopt_uneven``
@@ -23,8 +25,11 @@ from ._urdf import URDF, bundled_robot_names, bundled_srdf_path, bundled_urdf_pa
 from .optimization_engines import (
     LbfgsTrajOptConfig,
     UnevenLsIkSolver,
+    hjcd_solve,
+    hjcd_solve_batch,
     lbfgs_trajopt,
     lbfgs_trajopt_uneven,
+    ls_ik_solve,
     ls_ik_solve_cuda,
     ls_ik_solve_cuda_batch,
 )
@@ -47,7 +52,10 @@ __all__ = [
     "collision",
     "costs",
     "kernels",
+    "hjcd_solve",
+    "hjcd_solve_batch",
     "load_bundled_urdf",
+    "ls_ik_solve",
     "ls_ik_solve_cuda",
     "ls_ik_solve_cuda_batch",
 ]

@@ -15,7 +15,8 @@ import sys
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libpyroffi_amd.so")
-SOURCES = ("pk_common.cpp", "pk_fk.hip", "pk_collision.hip", "pk_ik.hip", "pk_grad.hip", "pk_grid.hip", "pk_traj.hip")
+SOURCES = ("pk_common.cpp", "pk_fk.hip", "pk_collision.hip", "pk_distances.hip", "pk_ik.hip", "pk_ik_jax.hip",
+           "pk_grad.hip", "pk_grid.hip", "pk_traj.hip")
 # pk_ik_inst.hip is compiled once per (padded DOF, padded EE count) pair.
 IK_INSTANCES = tuple((m, e) for m in (8, 16) for e in (1, 2, 4))
 
@@ -23,10 +24,11 @@ IK_INSTANCES = tuple((m, e) for m in (8, 16) for e in (1, 2, 4))
 def _units():
     """(source, extra flags, object name) of every translation unit."""
     units = [(s, (), os.path.splitext(s)[0] + ".o") for s in SOURCES]
-    units += [("pk_ik_inst.hip", (f"-DPK_IK_MAXA={m}", f"-DPK_IK_NEE={e}"), f"pk_ik_{m}_{e}.o")
+    units += [(src, (f"-DPK_IK_MAXA={m}", f"-DPK_IK_NEE={e}"), f"{stem}_{m}_{e}.o")
+              for src, stem in (("pk_ik_inst.hip", "pk_ik"), ("pk_ik_jax_inst.hip", "pk_ik_jax"))
               for m, e in IK_INSTANCES]
     return units
-HEADERS = ("pk_common.h", "pk_se3.h", "pk_model.h", "pk_grid.h", "pk_ik.h")
+HEADERS = ("pk_common.h", "pk_se3.h", "pk_model.h", "pk_geom.h", "pk_grid.h", "pk_ik.h", "pk_ik_jax.h")
 
 # Packed FP32 instructions are disabled for all device code; hipcc's host pass
 # warns that it does not know the feature, which is harmless.

@@ -83,6 +83,28 @@ def _to_cuda(x, dtype=torch.float32):
     return torch.as_tensor(arr, dtype=dtype, device="cuda").contiguous(), lambda y: y.cpu().numpy()
 
 
+class _LinkPosesFn(torch.autograd.Function):
+    """Link poses with the hand-written FK adjoint as backward (the torch
+    counterpart of a ``jax.custom_vjp`` over the FK FFI call).  Only the
+    configuration is kept for the backward, which recomputes FK."""
+
+    @staticmethod
+    def forward(ctx, x, tables):
+        from .kernels.ops import fk_links
+
+        ctx.tables = tables
+        ctx.save_for_backward(x)
+        return fk_links(x.detach(), *tables.robot_buffers(), tables.link_parent_joint)
+
+    @staticmethod
+    def backward(ctx, g):
+        from .kernels.ops import fk_vjp
+
+        (x,) = ctx.saved_tensors
+        t = ctx.tables
+        return fk_vjp(x.detach(), g.contiguous(), *t.robot_buffers(), t.link_parent_joint), None
+
+
 class Robot:
     """A robot kinematic tree (joint/link tables + GPU forward kinematics)."""
 
@@ -117,7 +139,9 @@ class Robot:
         """World pose of each link, ``(*batch, n_links, 7)`` wxyz_xyz.
 
         ``unroll_fk`` and ``use_cuda`` are accepted for signature compatibility
-        with the reference; both paths run the HIP kernel.
+        with the reference; both paths run the HIP kernel.  A GPU tensor that
+        requires grad gets a differentiable result: its backward is the FK
+        adjoint kernel (``pk_fk_vjp_f32``).
         """
         del unroll_fk, use_cuda
         from .kernels.ops import fk_links
@@ -127,6 +151,8 @@ class Robot:
             f"cfg has {x.shape[-1]} values, robot has {self.joints.num_actuated_joints} actuated joints"
         )
         t = self.device_tables(x.device)
+        if isinstance(cfg, torch.Tensor) and cfg.requires_grad and torch.is_grad_enabled():
+            return back(_LinkPosesFn.apply(x, t))
         out = fk_links(x, *t.robot_buffers(), t.link_parent_joint)
         return back(out)
 
@@ -139,8 +165,23 @@ class Robot:
         out = fk_hip(x, *t.robot_buffers(), t.fk_level_starts, t.fk_level_joints)
         return back(out)
 
-    def inverse_kinematics(self, *args, **kwargs):
-        raise NotImplementedError(
-            "Robot.inverse_kinematics (HJCD-IK) is not part of this package yet; "
-            "use pyroffi_amd.ls_ik_solve_cuda"
-        )
+    def inverse_kinematics(self, target_link_name: str, target_pose, rng_key=None, previous_cfg=None,
+                           num_seeds: int = 32, coarse_max_iter: int = 20, lm_max_iter: int = 40,
+                           epsilon: float = 0.02, nu: float = float(np.pi / 2), lambda_init: float = 5e-3,
+                           continuity_weight: float = 1e-3, fixed_joint_mask=None):
+        """HJCD-IK for one link pose (``src/pyroffi/_robot.py:111-178``); returns ``(n_act,)``.
+
+        ``previous_cfg`` defaults to the joint-range midpoint and ``rng_key`` to
+        seed 0, as in the reference.
+        """
+        from .optimization_engines._hjcd_ik import hjcd_solve
+
+        if rng_key is None:
+            rng_key = 0
+        if previous_cfg is None:
+            previous_cfg = (self.joints.lower_limits + self.joints.upper_limits) / 2
+        link = self.links.names.index(target_link_name)
+        return hjcd_solve(self, (link,), (target_pose,), rng_key, previous_cfg, num_seeds=num_seeds,
+                          coarse_max_iter=coarse_max_iter, lm_max_iter=lm_max_iter, epsilon=epsilon, nu=nu,
+                          lambda_init=lambda_init, continuity_weight=continuity_weight,
+                          fixed_joint_mask=fixed_joint_mask)

@@ -108,7 +108,7 @@ class CUDARobotCollisionChecker:
             batch, back, _, _, osf = self._coarse_inner._sweep(robot, cfg, None, self_pairs=True)
             if bool((osf > 0).all()):
                 return back(torch.ones((*batch, self._inner.num_pairs), device=osf.device))
-        return self._inner.compute_self_collision_distance(robot, cfg)
+        return self._inner.compute_self_collision_distance(robot, cfg, semantics="cuda")
 
     def compute_min_world_distance(self, robot, cfg, world_geom):
         """``(*batch,)``: min over links and obstacles (the sweep's wave min-reduction)."""

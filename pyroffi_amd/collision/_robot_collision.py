@@ -15,8 +15,16 @@ Restates ``src/pyroffi/collision/_robot_collision.py:429-549`` (model build) and
   ``ignore_immediate_adjacents``) and ``user_ignore_pairs`` -- or, when an SRDF
   is given, minus the SRDF's ``<disable_collisions>`` pairs only.
 
-Distances are computed by the HIP sweep kernel with the reference CUDA
-kernels' semantics (see ``_hip_collision.py``).
+Distances follow the semantics of the reference class of the same name by
+default (``semantics="jax"``): the JAX path's ``+1e-6`` inside the norm
+(``src/pyroffi/collision/_utils.py:45``) and, for self distances, only the
+same-slot ("diagonal") sphere pairs of the two links (``:796-828``).
+``semantics="cuda"`` selects the reference CUDA kernels' semantics instead
+(no bias, padding skipped, the minimum over all S_i x S_j sphere pairs,
+``_collision_cuda_kernel.cu:169-201, 425-459``), which is also what
+``CUDARobotCollisionChecker`` returns.  Both run on HIP kernels: the JAX
+conventions on ``pk_sphere_distances_jax_f32``, the CUDA ones on the fused
+sweep.
 """
 
 from __future__ import annotations
@@ -40,8 +48,13 @@ PAD_RADIUS = -1e9
 class RobotCollisionSpherized:
     """Per-link sphere sets of a robot plus its active self-collision pairs."""
 
+    SEMANTICS = ("jax", "cuda")
+
     def __init__(self, num_links, link_names, coll: Sphere, active_idx_i, active_idx_j,
-                 link_spheres: list[np.ndarray]):
+                 link_spheres: list[np.ndarray], semantics: str = "jax"):
+        if semantics not in self.SEMANTICS:
+            raise ValueError(f"semantics must be one of {self.SEMANTICS}, got {semantics!r}")
+        self.semantics = semantics
         self.num_links = num_links
         self.link_names = tuple(link_names)
         self.coll = coll  # batch axes (N, S)
@@ -80,7 +93,8 @@ class RobotCollisionSpherized:
         return np.concatenate(ents).astype(np.uint32)
 
     @staticmethod
-    def from_urdf(urdf, user_ignore_pairs=(), srdf_path=None, ignore_immediate_adjacents=True):
+    def from_urdf(urdf, user_ignore_pairs=(), srdf_path=None, ignore_immediate_adjacents=True,
+                  semantics: str = "jax"):
         if not isinstance(urdf, URDF):
             urdf = URDF.load(urdf)
         _, link_info = RobotURDFParser.parse(urdf)
@@ -117,7 +131,7 @@ class RobotCollisionSpherized:
         for c, r, s in zip(centers, radii, link_spheres):
             rows = np.concatenate([c[: len(s)], r[: len(s), None]], axis=1)
             compact.append(rows[np.argsort(-rows[:, 3], kind="stable")])
-        return RobotCollisionSpherized(n, names, coll, ai, aj, compact)
+        return RobotCollisionSpherized(n, names, coll, ai, aj, compact, semantics=semantics)
 
     @staticmethod
     def _pairs_from_ignore(ignore: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
@@ -163,6 +177,7 @@ class RobotCollisionSpherized:
                 "self_entries": torch.as_tensor(self.self_entries.view(np.int32), device=device).contiguous(),
                 "pair_i": torch.as_tensor(self.active_idx_i, device=device).contiguous(),
                 "pair_j": torch.as_tensor(self.active_idx_j, device=device).contiguous(),
+                "spheres_padded": torch.as_tensor(self.coll.as_array(), device=device).contiguous(),
             }
         return self._dev_cache[key]
 
@@ -202,15 +217,42 @@ class RobotCollisionSpherized:
         )
         return batch, back, ow, om, osf
 
-    def compute_world_collision_distance(self, robot, cfg, world_geom):
+    def _semantics(self, semantics):
+        sem = self.semantics if semantics is None else semantics
+        if sem not in self.SEMANTICS:
+            raise ValueError(f"semantics must be one of {self.SEMANTICS}, got {sem!r}")
+        return sem
+
+    def _jax_distances(self, robot, cfg, world_arr, *, world: bool, self_pairs: bool):
+        from .._robot import _to_cuda
+        from ..kernels.ops import sphere_distances_jax
+
+        self._check_robot(robot)
+        x, back = _to_cuda(cfg)
+        batch = tuple(x.shape[:-1])
+        poses = robot.forward_kinematics(x.reshape(-1, x.shape[-1]))
+        t = self.device_tables(poses.device)
+        ws = world_arr if world_arr is not None else torch.zeros((0, 4), device=poses.device)
+        ow, osf = sphere_distances_jax(poses, t["spheres_padded"], ws, t["pair_i"], t["pair_j"], world=world,
+                                       self_pairs=self_pairs)
+        return batch, back, ow, osf
+
+    def compute_world_collision_distance(self, robot, cfg, world_geom, semantics: str | None = None):
         """``(*batch, N, M)``: per link, min over its spheres of the sphere-sphere distance."""
         from ._hip_collision import world_sphere_array
 
         ws = torch.as_tensor(world_sphere_array(world_geom), device="cuda")
-        batch, back, ow, _, _ = self._sweep(robot, cfg, ws, world=True)
+        if self._semantics(semantics) == "jax":
+            batch, back, ow, _ = self._jax_distances(robot, cfg, ws, world=True, self_pairs=False)
+        else:
+            batch, back, ow, _, _ = self._sweep(robot, cfg, ws, world=True)
         return back(ow.reshape(*batch, self.num_links, ws.shape[0]))
 
-    def compute_self_collision_distance(self, robot, cfg):
-        """``(*batch, P)``: per active pair, min over all sphere pairs of the two links."""
-        batch, back, _, _, osf = self._sweep(robot, cfg, None, self_pairs=True)
+    def compute_self_collision_distance(self, robot, cfg, semantics: str | None = None):
+        """``(*batch, P)``: per active pair, the minimum over the two links' same-slot
+        sphere pairs (``"jax"``) or over all their sphere pairs (``"cuda"``)."""
+        if self._semantics(semantics) == "jax":
+            batch, back, _, osf = self._jax_distances(robot, cfg, None, world=False, self_pairs=True)
+        else:
+            batch, back, _, _, osf = self._sweep(robot, cfg, None, self_pairs=True)
         return back(osf.reshape(*batch, self.num_pairs))

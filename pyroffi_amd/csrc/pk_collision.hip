@@ -543,6 +543,10 @@ static int launch_sweep(void* stream, const SweepArgs& a) {
   return check_launch("sweep_kernel");
 }
 
+int launch_world_reduced_mixed(void* stream, const float* sc, const float* sr, long long batch, int k,
+                               int n_links, const float* ws, int ms, const float* wc, int mc, const float* wb,
+                               int mb, const float* wh, int mh, float* out);
+
 }  // namespace pk
 
 extern "C" int pk_collision_world_sphere_reduced_f32(
@@ -551,14 +555,20 @@ extern "C" int pk_collision_world_sphere_reduced_f32(
     const float* world_capsules, int32_t n_world_capsules, const float* world_boxes,
     int32_t n_world_boxes, const float* world_halfspaces, int32_t n_world_halfspaces,
     int32_t n_links, float* out) {
-  (void)world_capsules; (void)world_boxes; (void)world_halfspaces;
-  PK_REQUIRE(n_world_capsules == 0 && n_world_boxes == 0 && n_world_halfspaces == 0,
-             "only sphere worlds are supported (got %d capsules, %d boxes, %d halfspaces)",
-             n_world_capsules, n_world_boxes, n_world_halfspaces);
-  PK_REQUIRE(batch >= 0 && k >= 0 && n_world_spheres >= 0 && n_links >= 0, "negative size");
-  if (batch == 0 || n_links == 0 || n_world_spheres == 0) return PK_OK;
+  PK_REQUIRE(batch >= 0 && k >= 0 && n_world_spheres >= 0 && n_world_capsules >= 0 &&
+                 n_world_boxes >= 0 && n_world_halfspaces >= 0 && n_links >= 0,
+             "negative size");
+  const long long n_world =
+      static_cast<long long>(n_world_spheres) + n_world_capsules + n_world_boxes + n_world_halfspaces;
+  if (batch == 0 || n_links == 0 || n_world == 0) return PK_OK;
   PK_REQUIRE(k % n_links == 0, "k=%d is not a multiple of n_links=%d", k, n_links);
-  PK_REQUIRE(sphere_centers && sphere_radii && world_spheres && out, "null pointer");
+  PK_REQUIRE(sphere_centers && sphere_radii && out, "null pointer");
+  if (n_world != n_world_spheres)  // capsules, boxes or half-spaces: the mixed-kind kernel
+    return pk::launch_world_reduced_mixed(stream, sphere_centers, sphere_radii, batch, k, n_links,
+                                          world_spheres, n_world_spheres, world_capsules,
+                                          n_world_capsules, world_boxes, n_world_boxes,
+                                          world_halfspaces, n_world_halfspaces, out);
+  PK_REQUIRE(world_spheres != nullptr, "null pointer");
   pk::SweepArgs a{};
   a.centers_soa = sphere_centers;
   a.radii = sphere_radii;
@@ -607,6 +617,8 @@ extern "C" int pk_sphere_sweep_f32(void* stream, const float* link_poses, int64_
              "null pointer");
   if (out_world || out_world_min) PK_REQUIRE(n_world_spheres == 0 || world_spheres, "null world");
   if (out_self) PK_REQUIRE(n_entries == 0 || self_entries, "null self_entries");
+  // Every entry names a pair slot of out_self, so entries need at least one pair.
+  if (out_self) PK_REQUIRE(n_entries == 0 || n_pairs >= 1, "n_entries=%d with n_pairs=0", n_entries);
   pk::SweepArgs a{};
   a.link_poses = link_poses;
   a.spheres_local = spheres_local;

@@ -90,11 +90,17 @@ _SIGNATURES = {
     "pk_abi_version": (_i32, []),
     "pk_fk_f32": (_i32, [_p, _p, _i64, _i32] + [_p] * 10 + [_i32, _i32, _p]),
     "pk_fk_links_f32": (_i32, [_p, _p, _i64, _i32] + [_p] * 8 + [_i32, _p, _i32, _p]),
+    "pk_fk_vjp_f32": (_i32, [_p, _p, _i64, _i32] + [_p] * 8 + [_i32, _p, _i32, _p, _p]),
     "pk_collision_world_sphere_reduced_f32": (
         _i32,
         [_p, _p, _p, _i64, _i32, _p, _i32, _p, _i32, _p, _i32, _p, _i32, _i32, _p],
     ),
+    "pk_collision_world_sphere_f32": (
+        _i32,
+        [_p, _p, _p, _i64, _i32, _p, _i32, _p, _i32, _p, _i32, _p, _i32, _p],
+    ),
     "pk_collision_self_sphere_f32": (_i32, [_p, _p, _p, _i64, _i32, _i32, _p, _p, _i32, _p]),
+    "pk_sphere_distances_jax_f32": (_i32, [_p, _p, _i64, _i32, _p, _i32, _p, _i32, _p, _p, _i32, _p, _p]),
     "pk_sphere_sweep_f32": (
         _i32,
         [_p, _p, _i64, _i32, _p, _p, _i32, _p, _i32, _p, _i32, _i32, _p, _p, _p],
@@ -111,6 +117,21 @@ _SIGNATURES = {
         + [_p, _p],  # outputs
     ),
     "pk_ls_ik_select_f32": (_i32, [_p, _p, _p, _p, _i32, _i32, _i32, _f32, _p, _p]),
+    "pk_ik_jax_lm_f32": (
+        _i32,
+        [_p, _p, _i32, _i32, _i32]  # stream, seeds, P, S, A
+        + [_p] * 8 + [_i32]  # robot tables, n_joints
+        + [_p, _p, _i32, _p]  # target_jnts, ancestor_masks, n_ee, target_T
+        + [_p, _p, _p]  # lower, upper, fixed_mask
+        + [_i32, _f32, _f32, _f32, _i32, _f32, _f32, ctypes.c_uint64]  # max_iter .. kick_key
+        + [_p, _p],
+    ),
+    "pk_hjcd_coarse_f32": (
+        _i32,
+        [_p, _p, _i32, _i32, _i32] + [_p] * 8 + [_i32] + [_p, _p, _i32, _p, _p] + [_p, _p, _p]
+        + [_i32, _f32, _f32, _p, _p],
+    ),
+    "pk_hjcd_refine_seeds_f32": (_i32, [_p, _p, _p, _i32, _i32, _i32, _i32, _i32, _p, _p, _p, _p, _p]),
     "pk_world_grid_count_f32": (_i32, [_p, _p, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _p]),
     "pk_world_grid_fill_f32": (_i32, [_p, _p, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _i32, _p, _p, _i64]),
     "pk_ls_ik_batched_f32": (

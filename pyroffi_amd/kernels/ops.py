@@ -101,6 +101,29 @@ def fk_links(cfg, twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, 
     return out.reshape(*batch_axes, n_links, 7)
 
 
+def fk_vjp(cfg, cot_links, twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, mimic_act_idx,
+           topo_inv, link_parent_joint) -> torch.Tensor:
+    """``(*batch, n_act)``: the FK adjoint, J^T of the link poses applied to ``cot_links``."""
+    cfg = _dev(cfg, torch.float32)
+    dev = cfg.device
+    batch_axes = tuple(cfg.shape[:-1])
+    n_act = cfg.shape[-1]
+    batch = math.prod(batch_axes)
+    tb = _tables(twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_off, mimic_act_idx, topo_inv, dev)
+    lpj = _dev(link_parent_joint, torch.int32, dev)
+    n_links = lpj.shape[0]
+    cot = _dev(cot_links, torch.float32, dev)
+    if tuple(cot.shape) != batch_axes + (n_links, 7):
+        raise ValueError(f"cotangent shape {tuple(cot.shape)} != {batch_axes + (n_links, 7)}")
+    out = torch.empty((batch, n_act), dtype=torch.float32, device=dev)
+    st = lib().pk_fk_vjp_f32(
+        _stream(), _ptr(cfg), batch, n_act, *[_ptr(t) for t in tb], tb[0].shape[0], _ptr(lpj),
+        n_links, _ptr(cot), _ptr(out),
+    )
+    check(st, "pk_fk_vjp_f32")
+    return out.reshape(*batch_axes, n_act)
+
+
 def collision_world_sphere_reduced(sphere_centers, sphere_radii, world_spheres, world_capsules,
                                    world_boxes, world_halfspaces, n: int) -> torch.Tensor:
     """``[B, N, M]`` per-link world distances from SoA centres ``[3, B, K]``."""
@@ -120,6 +143,49 @@ def collision_world_sphere_reduced(sphere_centers, sphere_radii, world_spheres, 
     )
     check(st, "pk_collision_world_sphere_reduced_f32")
     return out
+
+
+def collision_world_sphere(sphere_centers, sphere_radii, world_spheres, world_capsules, world_boxes,
+                           world_halfspaces) -> torch.Tensor:
+    """``[B, K, M]`` per-sphere world distances from SoA centres ``[3, B, K]``
+    (``M`` = spheres, capsules, boxes, half-spaces in that order)."""
+    sc = _dev(sphere_centers, torch.float32)
+    dev = sc.device
+    sr = _dev(sphere_radii, torch.float32, dev)
+    ws = _dev(world_spheres, torch.float32, dev).reshape(-1, 4)
+    wc = _dev(world_capsules, torch.float32, dev).reshape(-1, 7)
+    wb = _dev(world_boxes, torch.float32, dev).reshape(-1, 15)
+    wh = _dev(world_halfspaces, torch.float32, dev).reshape(-1, 6)
+    _, B, K = sc.shape
+    M = ws.shape[0] + wc.shape[0] + wb.shape[0] + wh.shape[0]
+    out = torch.empty((B, K, M), dtype=torch.float32, device=dev)
+    st = lib().pk_collision_world_sphere_f32(
+        _stream(), _ptr(sc), _ptr(sr), B, K, _ptr(ws), ws.shape[0], _ptr(wc), wc.shape[0],
+        _ptr(wb), wb.shape[0], _ptr(wh), wh.shape[0], _ptr(out),
+    )
+    check(st, "pk_collision_world_sphere_f32")
+    return out
+
+
+def sphere_distances_jax(link_poses, spheres_padded, world_spheres, pair_i, pair_j, *, world: bool = True,
+                         self_pairs: bool = True):
+    """The JAX path's conventions: ``(out_world (B, L, M) | None, out_self (B, P) | None)``
+    from link poses ``(B, L, 7)`` and the padded ``(L, S, 4)`` link-frame sphere table."""
+    lp = _dev(link_poses, torch.float32)
+    dev = lp.device
+    B, L, _ = lp.shape
+    sp = _dev(spheres_padded, torch.float32, dev).reshape(L, -1, 4)
+    ws = _dev(world_spheres, torch.float32, dev).reshape(-1, 4)
+    pi = _dev(pair_i, torch.int32, dev)
+    pj = _dev(pair_j, torch.int32, dev)
+    ow = torch.empty((B, L, ws.shape[0]), dtype=torch.float32, device=dev) if world else None
+    osf = torch.empty((B, pi.shape[0]), dtype=torch.float32, device=dev) if self_pairs else None
+    st = lib().pk_sphere_distances_jax_f32(
+        _stream(), _ptr(lp), B, L, _ptr(sp), sp.shape[1], _ptr(ws), ws.shape[0], _ptr(pi), _ptr(pj),
+        pi.shape[0], _ptr(ow), _ptr(osf),
+    )
+    check(st, "pk_sphere_distances_jax_f32")
+    return ow, osf
 
 
 def collision_self_sphere(centers, radii, pair_i, pair_j) -> torch.Tensor:
@@ -206,6 +272,76 @@ def ls_ik_hip(seeds, twists, parent_tf, parent_idx, act_idx, mimic_mul, mimic_of
 
 
 ls_ik_cuda = ls_ik_hip
+
+
+def jax_ik_lm(seeds, tables, target_jnts, ancestor_masks, target_T, lower, upper, fixed_mask, *, max_iter: int,
+              pos_weight: float = 50.0, ori_weight: float = 10.0, lambda_init: float = 5e-3, adaptive: bool = False,
+              prior_weight: float = 0.0, kick_scale: float = 0.0, kick_key: int = 0):
+    """JAX-convention multi-seed LM (``pk_ik_jax_lm_f32``): ``(best cfgs (P, S, A), winner errors (P, S))``.
+
+    ``tables`` is the robot's 8 FFI buffers (``_DeviceTables.robot_buffers()``).
+    """
+    sd = _dev(seeds, torch.float32)
+    dev = sd.device
+    P, S, A = sd.shape
+    tb = _tables(*tables, dev)
+    tj = _dev(target_jnts, torch.int32, dev).reshape(-1)
+    n_ee = tj.shape[0]
+    am = _dev(ancestor_masks, torch.int32, dev).reshape(n_ee, -1)
+    tT = _dev(target_T, torch.float32, dev).reshape(P, n_ee, 7)
+    lo, hi = _dev(lower, torch.float32, dev), _dev(upper, torch.float32, dev)
+    fm = _dev(fixed_mask, torch.int32, dev)
+    out = torch.empty((P, S, A), dtype=torch.float32, device=dev)
+    err = torch.empty((P, S), dtype=torch.float32, device=dev)
+    st = lib().pk_ik_jax_lm_f32(
+        _stream(), _ptr(sd), P, S, A, *[_ptr(t) for t in tb], tb[0].shape[0], _ptr(tj), _ptr(am), n_ee, _ptr(tT),
+        _ptr(lo), _ptr(hi), _ptr(fm), int(max_iter), float(pos_weight), float(ori_weight), float(lambda_init),
+        int(bool(adaptive)), float(prior_weight), float(kick_scale), ctypes.c_uint64(int(kick_key) & (2 ** 64 - 1)),
+        _ptr(out), _ptr(err),
+    )
+    check(st, "pk_ik_jax_lm_f32")
+    return out, err
+
+
+def hjcd_coarse(seeds, tables, target_jnts, ancestor_masks, target_T, ee_of_seed, lower, upper, fixed_mask, *,
+                k_max: int = 20, epsilon: float = 0.02, nu: float = math.pi / 2):
+    """HJCD coarse coordinate descent (``pk_hjcd_coarse_f32``): ``(cfgs (P, S, A), sum |f|^2 (P, S))``."""
+    sd = _dev(seeds, torch.float32)
+    dev = sd.device
+    P, S, A = sd.shape
+    tb = _tables(*tables, dev)
+    tj = _dev(target_jnts, torch.int32, dev).reshape(-1)
+    n_ee = tj.shape[0]
+    am = _dev(ancestor_masks, torch.int32, dev).reshape(n_ee, -1)
+    tT = _dev(target_T, torch.float32, dev).reshape(P, n_ee, 7)
+    es = _dev(ee_of_seed, torch.int32, dev).reshape(S)
+    lo, hi = _dev(lower, torch.float32, dev), _dev(upper, torch.float32, dev)
+    fm = _dev(fixed_mask, torch.int32, dev)
+    out = torch.empty((P, S, A), dtype=torch.float32, device=dev)
+    err = torch.empty((P, S), dtype=torch.float32, device=dev)
+    st = lib().pk_hjcd_coarse_f32(
+        _stream(), _ptr(sd), P, S, A, *[_ptr(t) for t in tb], tb[0].shape[0], _ptr(tj), _ptr(am), n_ee, _ptr(tT),
+        _ptr(es), _ptr(lo), _ptr(hi), _ptr(fm), int(k_max), float(epsilon), float(nu), _ptr(out), _ptr(err),
+    )
+    check(st, "pk_hjcd_coarse_f32")
+    return out, err
+
+
+def hjcd_refine_seeds(coarse_cfgs, coarse_err, top_k: int, repeats: int, noise, lower, upper, fixed_mask):
+    """Top-k coarse results tiled ``repeats`` times with perturbations: ``(P, top_k * repeats, A)``."""
+    c = _dev(coarse_cfgs, torch.float32)
+    dev = c.device
+    P, S, A = c.shape
+    e = _dev(coarse_err, torch.float32, dev)
+    R = top_k * repeats
+    nz = _dev(noise, torch.float32, dev).reshape(P, R, A)
+    lo, hi = _dev(lower, torch.float32, dev), _dev(upper, torch.float32, dev)
+    fm = _dev(fixed_mask, torch.int32, dev)
+    out = torch.empty((P, R, A), dtype=torch.float32, device=dev)
+    st = lib().pk_hjcd_refine_seeds_f32(_stream(), _ptr(c), _ptr(e), P, S, A, int(top_k), int(repeats), _ptr(nz),
+                                        _ptr(lo), _ptr(hi), _ptr(fm), _ptr(out))
+    check(st, "pk_hjcd_refine_seeds_f32")
+    return out
 
 
 def ls_ik_select(cfgs, errs, previous, continuity_weight: float = 0.0):

@@ -328,29 +328,19 @@ class BatchedLbfgsTrajOpt:
             self.classes.append((None, self.B, self.max_len, 0))
         self._streams = None
 
-    def _launches(self):
+    def _launch_iters(self, subset, count, max_len, threads, b0, b1):
         from ..kernels._lib import check, lib
         from ..kernels.ops import _ptr, _stream
 
-        n = self.cfg.n_iters
-        step = n if self.chunk <= 0 else self.chunk
-        W = self.x0.shape[0]
-        cfg = self.cfg
-        rank_w = (cfg.w_smooth, cfg.w_vel, cfg.w_acc, cfg.w_jerk, cfg.w_limits, cfg.w_collision_max)
+        st = lib().pk_traj_lbfgs_f32(_stream(), self.model.struct_ref(), _ptr(self.x0), _ptr(self.start), self.B,
+                                     self.x0.shape[0], max_len, threads, _ptr(subset), 0 if subset is None else count,
+                                     ctypes.byref(self._cfg), b0, b1, _ptr(self.state), _ptr(self.best),
+                                     _ptr(self.best_cost))
+        check(st, "pk_traj_lbfgs_f32")
 
-        def run_class(subset, count, max_len, threads):
-            for b0 in range(0, max(n, 1), max(step, 1)):
-                b1 = min(b0 + step, n)
-                st = lib().pk_traj_lbfgs_f32(_stream(), self.model.struct_ref(), _ptr(self.x0), _ptr(self.start),
-                                             self.B, W, max_len, threads, _ptr(subset),
-                                             0 if subset is None else count, ctypes.byref(self._cfg), b0, b1,
-                                             _ptr(self.state), _ptr(self.best), _ptr(self.best_cost))
-                check(st, "pk_traj_lbfgs_f32")
-            traj_cost(self.model, self.best, self.start, max_len, rank_w, grad=False, out_comps=self.rank_comps,
-                      block_threads=threads, subset=subset)
-
+    def _per_class(self, fn):
         if len(self.classes) == 1:
-            run_class(*self.classes[0])
+            fn(*self.classes[0])
             return
         main = torch.cuda.current_stream()
         if self._streams is None:
@@ -358,9 +348,31 @@ class BatchedLbfgsTrajOpt:
         for c, s in zip(self.classes, self._streams):
             s.wait_stream(main)
             with torch.cuda.stream(s):
-                run_class(*c)
+                fn(*c)
         for s in self._streams:
             main.wait_stream(s)
+
+    def _launches(self):
+        n = self.cfg.n_iters
+        step = n if self.chunk <= 0 else self.chunk
+        cfg = self.cfg
+        rank_w = (cfg.w_smooth, cfg.w_vel, cfg.w_acc, cfg.w_jerk, cfg.w_limits, cfg.w_collision_max)
+
+        def run_class(subset, count, max_len, threads):
+            for b0 in range(0, max(n, 1), max(step, 1)):
+                self._launch_iters(subset, count, max_len, threads, b0, min(b0 + step, n))
+            traj_cost(self.model, self.best, self.start, max_len, rank_w, grad=False, out_comps=self.rank_comps,
+                      block_threads=threads, subset=subset)
+
+        self._per_class(run_class)
+
+    def run_iterations(self, it0: int, it1: int) -> None:
+        """Run iterations ``[it0, it1)`` of the solve (``it0 == 0`` starts from
+        the initial paths, later ranges resume from the state buffer), without the
+        final ranking; ``final_iterate()`` then holds iterate ``it1``."""
+        if not 0 <= it0 <= it1 <= self.cfg.n_iters:
+            raise ValueError(f"iteration range [{it0}, {it1}) outside [0, {self.cfg.n_iters}]")
+        self._per_class(lambda *c: self._launch_iters(*c, it0, it1))
 
     def final_iterate(self) -> torch.Tensor:
         """The last L-BFGS iterate ``(W, A)`` (a view of the state buffer)."""

@@ -179,6 +179,41 @@ def _unsupported_constraints(constraints) -> None:
         )
 
 
+def ls_ik_solve(robot, target_link_indices, target_poses, rng_key, previous_cfg, num_seeds: int = 32,
+                max_iter: int = 60, pos_weight: float = 50.0, ori_weight: float = 10.0, lambda_init: float = 5e-3,
+                continuity_weight: float = 0.0, fixed_joint_mask=None, constraint_fns=(), constraint_args=(),
+                constraint_weights=None):
+    """The reference's pure-JAX multi-seed LM (``_ls_ik.py:361-518``); returns ``(n_act,)``.
+
+    Residual ``log(T_ee^-1 T_tgt)`` (jaxlie ``SE3.log``) with its exact
+    Jacobian, weights W = (pw, pw, pw, ow, ow, ow), every seed refined by
+    ``_ls_ik_single`` (``:192-350``) in ``pk_ik_jax_lm_f32``; the winner has the
+    lowest sum over EEs of |W f|^2 plus ``continuity_weight * |q - prev|^2``.
+    Seeds: half near ``previous_cfg`` (sigma 0.05; tight/loose halves for
+    several EEs), half uniform, from numpy's PCG64.
+    """
+    _unsupported_constraints(constraint_fns)
+    if isinstance(target_link_indices, int):
+        target_link_indices = (target_link_indices,)
+    links = tuple(int(i) for i in target_link_indices)
+    tT = _target_array(target_poses).reshape(1, len(links), 7)
+    prev = previous_cfg.detach().cpu().numpy() if isinstance(previous_cfg, torch.Tensor) else np.asarray(previous_cfg)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    seeds = make_seeds(rng_from_key(rng_key), prev[None], lo, hi, num_seeds, fixed_joint_mask,
+                       multi_ee=len(links) > 1)
+    n_act = robot.joints.num_actuated_joints
+    fm = np.zeros(n_act, np.int32) if fixed_joint_mask is None else np.asarray(fixed_joint_mask).astype(np.int32)
+    from ..kernels.ops import jax_ik_lm, ls_ik_select
+
+    t = robot.device_tables()
+    tj, masks = ee_tables(robot, links)
+    cfgs, errs = jax_ik_lm(seeds, t.robot_buffers(), tj, masks, tT.astype(np.float32), t.lower, t.upper, fm,
+                           max_iter=max_iter, pos_weight=pos_weight, ori_weight=ori_weight, lambda_init=lambda_init)
+    best, _ = ls_ik_select(cfgs, errs, torch.as_tensor(prev[None], dtype=torch.float32, device=cfgs.device),
+                           continuity_weight)
+    return _like(previous_cfg, best[0])
+
+
 def ls_ik_solve_cuda(robot, target_link_indices, target_poses, rng_key, previous_cfg, num_seeds: int = 32,
                      max_iter: int = 60, pos_weight: float = 50.0, ori_weight: float = 10.0,
                      lambda_init: float = 5e-3, eps_pos: float = 1e-8, eps_ori: float = 1e-8,

@@ -103,3 +103,46 @@ def test_collision_only_gradient_is_zero_when_clear(oracle_mod):
     cost = pose_hinge_cost(robot, coll, qt, tgt.astype(np.float32), far, ee_link=-1)
     (g,) = torch.autograd.grad(cost.sum(), qt)
     assert float(cost.abs().max()) == 0.0 and float(g.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("name,B", [("panda", 200), ("mimic_arm", 65), ("fetch", 64), ("baxter", 33),
+                                    ("ur5", 1)])
+def test_fk_adjoint_vs_fd(oracle_mod, name, B):
+    """The FK adjoint (J^T of the link poses on a random cotangent, through
+    Robot.forward_kinematics' autograd) against FP64 central differences of the
+    oracle FK (h = 1e-4): norm-wise relative error <= 1e-4 per configuration.
+    The mimic arm covers mimic multipliers/offsets, a prismatic and a
+    continuous joint; partial last wave at B = 65 and 33."""
+    robot, _ = load_robot(name)
+    t = oracle_mod.Tables.of(robot)
+    q = random_cfgs(robot, B, seed=B + 7)
+    rng = np.random.default_rng(B)
+    L, A = robot.links.num_links, robot.joints.num_actuated_joints
+    G = rng.standard_normal((B, L, 7))
+    qt = torch.as_tensor(q.astype(np.float32), device="cuda").requires_grad_(True)
+    poses = robot.forward_kinematics(qt)
+    (poses * torch.as_tensor(G, dtype=torch.float32, device="cuda")).sum().backward()
+    got = qt.grad.cpu().numpy().astype(np.float64)
+    h = 1e-4
+    fd = np.empty((B, A))
+    q32 = q.astype(np.float32).astype(np.float64)  # differentiate at the point the GPU saw
+    for a in range(A):
+        e = np.zeros(A)
+        e[a] = h
+        fd[:, a] = np.sum(G * (oracle_mod.fk_links(t, q32 + e) - oracle_mod.fk_links(t, q32 - e)), axis=(1, 2)) / (2 * h)
+    err = np.linalg.norm(got - fd, axis=1) / np.maximum(np.linalg.norm(fd, axis=1), 1e-6)
+    print(f"{name}: max relative error {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-4, err.max()
+    # the forward is the plain FK
+    lp = oracle_mod.fk_links(t, q32)
+    assert np.abs(poses.detach().cpu().numpy() - lp).max() < 1e-5
+
+
+def test_fk_adjoint_root_link_and_zero_cotangent():
+    robot, _ = load_robot("panda")
+    q = torch.as_tensor(random_cfgs(robot, 70, seed=3).astype(np.float32), device="cuda").requires_grad_(True)
+    poses = robot.forward_kinematics(q)
+    g = torch.zeros_like(poses)
+    g[:, 0] = 1.0  # the root link is the constant identity: no gradient
+    (dq,) = torch.autograd.grad(poses, q, g)
+    assert float(dq.abs().max()) == 0.0

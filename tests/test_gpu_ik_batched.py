@@ -10,7 +10,9 @@ chunked iterations and the captured graph.
 * Splitting the iterations into chunks that resume from the state buffer gives
   bit-identical results to one launch.
 * Full uneven solves over four robots (Baxter with two end effectors) match the
-  FP64 oracle on per-problem success within one problem per robot.
+  FP64 oracle on per-problem success within one problem per robot, and on the
+  uneven workload's world and seed counts (64 problems per robot) on per-seed
+  success within 0.5 pp.
 * A captured graph replays to the eager result, replay after replay.
 """
 
@@ -193,6 +195,39 @@ def test_uneven_four_robot_solve_vs_oracle(oracle_mod):
             best = oc[0, int(np.argmin(oe[0]))]
             ok_or[p] = _success(oracle_mod, t, best[None], g.targets[p:p + 1], links)[0]
         assert abs(int(ok_gpu.sum()) - int(ok_or.sum())) <= 1, (g.robot.joints.num_actuated_joints, ok_gpu, ok_or)
+
+
+def test_config4_world_seed_success_vs_oracle(oracle_mod):
+    """The uneven workload's world (256 spheres, radii U[0.02, 0.15]) and seed
+    counts (log-uniform in [16, 256]), 50 iterations, 64 problems per robot
+    (Baxter with both end effectors), grid-culled penalty: per-seed success of
+    the fused solver within 0.5 pp of the FP64 oracle (brute-force penalty) on
+    the same seeds, per robot."""
+    world = _world(256, 61)
+    built = [_group(oracle_mod, n, 64, s, world, lo=16, hi=256) for n, s in (("panda", 62), ("ur5", 63),
+                                                                             ("fetch", 64), ("baxter", 65))]
+    solver = UnevenLsIkSolver([b[0] for b in built], max_iter=50)
+    for g, t, links in built:
+        assert "grid" in g.device
+        cfgs, _ = _run(solver, g)
+        cfgs = cfgs.cpu().numpy()
+        rs, rj, ws = g.collision
+        tj, masks = ee_tables(g.robot, links)
+        ocfgs = np.empty_like(cfgs, dtype=np.float64)
+        for p in range(len(g.seed_start) - 1):
+            s0, s1 = g.seed_start[p], g.seed_start[p + 1]
+            oc, _ = oracle_mod.ls_ik(t, g.seeds[None, s0:s1], tj, masks, g.targets[p:p + 1],
+                                     g.robot.joints.lower_limits, g.robot.joints.upper_limits, max_iter=50,
+                                     robot_spheres=rs, sphere_joint=rj, world=ws, enable_collision=True,
+                                     collision_weight=1e4, collision_margin=0.02)
+            ocfgs[s0:s1] = oc[0]
+        tg = np.repeat(g.targets, np.diff(g.seed_start), axis=0)
+        ok = _success(oracle_mod, t, cfgs, tg, links)
+        ook = _success(oracle_mod, t, ocfgs, tg, links)
+        print(f"{g.robot.joints.num_actuated_joints} DOF, {len(ok)} seeds: seed success {ok.mean():.4f} "
+              f"vs oracle {ook.mean():.4f}", flush=True)
+        assert len(ok) >= 64 * 16
+        assert abs(ok.mean() - ook.mean()) <= 0.005, (ok.mean(), ook.mean())
 
 
 def test_graph_replay_matches_eager_and_per_call(oracle_mod):

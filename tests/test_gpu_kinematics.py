@@ -145,9 +145,11 @@ def test_srdf_pairs_and_empty_world(oracle_mod):
     coll = RobotCollisionSpherized.from_urdf(path, srdf_path=pk.bundled_srdf_path("panda"))
     q = random_cfgs(robot, 33, seed=5).astype(np.float32)
     _, _, selfd = _oracle_distances(oracle_mod, robot, coll, q, _world(1, 0))
-    np.testing.assert_allclose(coll.compute_self_collision_distance(robot, q), selfd, atol=TOL, rtol=0)
-    empty = coll.compute_world_collision_distance(robot, q, np.zeros((0, 4), np.float32))
-    assert empty.shape == (33, 13, 0)
+    np.testing.assert_allclose(coll.compute_self_collision_distance(robot, q, semantics="cuda"), selfd, atol=TOL,
+                               rtol=0)
+    for sem in ("cuda", "jax"):
+        empty = coll.compute_world_collision_distance(robot, q, np.zeros((0, 4), np.float32), semantics=sem)
+        assert empty.shape == (33, 13, 0)
 
 
 def test_reference_layout_kernels(oracle_mod):
@@ -178,7 +180,7 @@ def test_reference_layout_kernels(oracle_mod):
 def test_links_without_spheres_report_sentinel():
     robot, path = load_robot("panda")
     coll = RobotCollisionSpherized.from_urdf(path)
-    got = coll.compute_world_collision_distance(robot, np.zeros((2, 7), np.float32), _world(5, 3))
+    got = coll.compute_world_collision_distance(robot, np.zeros((2, 7), np.float32), _world(5, 3), semantics="cuda")
     for name in ("panda_link8", "panda_grasptarget"):
         assert np.all(got[:, robot.links.names.index(name)] == 1e9)
 
@@ -265,3 +267,68 @@ def test_self_entry_table_runs(oracle_mod, n_entries, n_pairs):
     _, _, got = ops.sphere_sweep(poses, coll.spheres_local, coll.link_sphere_start, np.zeros((0, 4), np.float32),
                                  ents, n_pairs, self_pairs=True)
     np.testing.assert_allclose(got.cpu().numpy(), want, atol=TOL, rtol=0)
+
+
+def _soa(coll, robot, q):
+    """Reference layout of the sphere model at q: SoA centres (3, B, K), radii (B, K), k = s * N + n."""
+    at = coll.at_config(robot, q)  # batch (B, S, N)
+    B, S, N = at.get_batch_axes()
+    c = at.pose.translation().astype(np.float32).reshape(B, S * N, 3)
+    return np.ascontiguousarray(np.transpose(c, (2, 0, 1))), at.radius.astype(np.float32).reshape(B, S * N), S, N
+
+
+@pytest.mark.parametrize("kinds", [(20, 0, 0, 0), (20, 7, 9, 4), (0, 5, 0, 0), (0, 0, 6, 0), (0, 0, 0, 3)])
+def test_world_per_sphere_and_mixed_obstacles(oracle_mod, kinds):
+    """pk_collision_world_sphere_f32 ([B, K, M] per sphere, padding 1e9) and the
+    reduced [B, N, M] entry point over capsules, boxes and half-spaces, against
+    the FP64 oracle's obstacle distances on the same sphere centres."""
+    import worlds
+
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    q = random_cfgs(robot, 37, seed=sum(kinds)).astype(np.float32)
+    sc, sr, S, N = _soa(coll, robot, q)
+    B, K = sr.shape
+    ws, wc, wb, wh = worlds.random_world(np.random.default_rng(len(kinds) + sum(kinds)), *kinds)
+    M = sum(kinds)
+    got = ops.collision_world_sphere(torch.as_tensor(sc, device="cuda"), torch.as_tensor(sr, device="cuda"),
+                                     ws, wc, wb, wh).cpu().numpy()
+    assert got.shape == (B, K, M)
+    ref = oracle_mod.obstacle_dist(np.transpose(sc, (1, 2, 0)).reshape(-1, 3), sr.reshape(-1), ws, wc, wb,
+                                   wh).reshape(B, K, M)
+    pad = sr < 0
+    assert pad.any() and (~pad).any()
+    np.testing.assert_array_equal(got[pad], 1e9)
+    np.testing.assert_allclose(got[~pad], ref[~pad], atol=TOL, rtol=0)
+    red = ops.collision_world_sphere_reduced(torch.as_tensor(sc, device="cuda"), torch.as_tensor(sr, device="cuda"),
+                                             ws, wc, wb, wh, N).cpu().numpy()
+    ref_red = np.where(pad[..., None], np.inf, ref).reshape(B, S, N, M).min(axis=1)
+    ref_red = np.minimum(ref_red, 1e9)
+    np.testing.assert_allclose(red, ref_red, atol=TOL, rtol=0)
+
+
+@pytest.mark.parametrize("name", ("panda", "fetch", "mimic_arm"))
+def test_jax_semantics_vs_oracle(oracle_mod, name):
+    """RobotCollisionSpherized's default (the reference class's JAX semantics):
+    +1e-6 inside the norm, same-slot self pairs, padding evaluated at r = -1e9
+    (values about 1e9 compared as 'at least 1e8')."""
+    robot, path = load_robot(name)
+    coll = RobotCollisionSpherized.from_urdf(path)
+    assert coll.semantics == "jax"
+    q = random_cfgs(robot, 45, seed=4).astype(np.float32)
+    world = _world(50, seed=5)
+    lp = oracle_mod.fk_links(oracle_mod.Tables.of(robot), q.astype(np.float64))
+    sp = oracle_mod.padded_spheres(coll)
+    ref_w, _ = oracle_mod.world_dist(lp, sp, world.astype(np.float64), bias=True)
+    ref_s = oracle_mod.self_dist(lp, sp, coll.active_idx_i, coll.active_idx_j, mode=1)
+    got_w = coll.compute_world_collision_distance(robot, q, world)
+    got_s = coll.compute_self_collision_distance(robot, q)
+    for got, ref in ((got_w, ref_w), (got_s, ref_s)):
+        big = ref >= 1e8
+        np.testing.assert_array_equal(got >= 1e8, big)
+        np.testing.assert_allclose(got[~big], ref[~big], atol=TOL, rtol=0)
+    # the two semantics differ where they should: the full S x S minimum is never larger
+    cuda_s = coll.compute_self_collision_distance(robot, q, semantics="cuda")
+    assert np.all(cuda_s <= np.minimum(got_s, 1e9) + 1e-5)
+    if name == "panda":
+        assert np.mean(cuda_s < got_s - 1e-3) > 0.5

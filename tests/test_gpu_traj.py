@@ -5,9 +5,11 @@
   grids (grid and brute force agree to 1e-6);
 * gradient vs the oracle's analytic gradient (itself pinned by central
   differences in test_oracle_traj.py), norm-wise relative 1e-3 per trajectory;
-* L-BFGS: the first iterations follow the FP64 oracle step for step (1e-4),
-  and after 100 iterations the tracked best costs agree statistically
-  (FP32 and FP64 line searches may pick different steps later on);
+* L-BFGS: the current iterate after 1, 3 and 10 iterations equals the FP64
+  oracle's (1e-4) on the trajectory workload's shapes; after 100 iterations
+  the last iterates' costs agree as a batch (FP32 and FP64 line searches may
+  pick different steps near a tie); in free space the tracked best moves and
+  matches the oracle's;
 * chunked launches = one launch, bit for bit; uneven lengths in one launch =
   each trajectory alone; a captured graph replays to the eager result.
 """
@@ -99,34 +101,123 @@ def _lbfgs_both(oracle_mod, robot, model, om, trajs, cfg):
     return best.cpu().numpy(), best_cost.cpu().numpy(), obest, ocost
 
 
-def test_lbfgs_first_iterations_follow_oracle(oracle_mod):
-    robot, coll, world, model, om = _setup(oracle_mod, M=64, margin=0.05, mode=1, seed=5)
-    trajs = _trajs(robot, [16, 24, 9], seed=6)
-    for n_iters in (1, 3):
-        cfg = LbfgsTrajOptConfig(n_iters=n_iters, collision_margin=0.05)
-        best, cost, obest, ocost = _lbfgs_both(oracle_mod, robot, model, om, trajs, cfg)
-        np.testing.assert_allclose(cost, ocost, rtol=1e-4, atol=1e-6)
-        np.testing.assert_allclose(best, obest, rtol=0, atol=1e-4)
+def _config5_setup(oracle_mod, seed, margin=0.01, far=False):
+    """The trajectory workload's world: 1024 spheres, centres in the 2 m cube, radii U[0.02, 0.1]
+    (``far``: the same spheres moved 100 m away, so no waypoint collides)."""
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    rng = np.random.default_rng(seed)
+    world = np.concatenate([rng.uniform(-1, 1, (1024, 3)), rng.uniform(0.02, 0.1, (1024, 1))], 1)
+    if far:
+        world[:, :3] += 100.0
+    world = world.astype(np.float32)
+    model = TrajModel(robot, coll, world, margin, 1)
+    t = oracle_mod.Tables.of(robot)
+    om = oracle_mod.TrajModel(t, oracle_mod.padded_spheres(coll), coll.active_idx_i, coll.active_idx_j,
+                              world.astype(np.float64), margin, 1)
+    return robot, model, om
 
 
-def test_lbfgs_100_iterations_statistics(oracle_mod):
-    robot, coll, world, model, om = _setup(oracle_mod, M=64, margin=0.01, mode=1, seed=7)
-    trajs = _trajs(robot, [16] * 12 + [32] * 4, seed=8, noise=0.0)
+def _oracle_lbfgs(oracle_mod, robot, om, trajs, cfg):
+    return oracle_mod.traj_lbfgs(
+        om, [t.astype(np.float64) for t in trajs], robot.joints.lower_limits, robot.joints.upper_limits,
+        n_iters=cfg.n_iters, m_lbfgs=cfg.m_lbfgs, escalation_interval=cfg.escalation_interval, w_smooth=cfg.w_smooth,
+        w_vel=cfg.w_vel, w_acc=cfg.w_acc, w_jerk=cfg.w_jerk, w_limits=cfg.w_limits, w_collision=cfg.w_collision,
+        w_collision_max=cfg.w_collision_max, penalty_scale=cfg.penalty_scale, iterates=True)
+
+
+def _max_weight_costs(oracle_mod, robot, om, x_flat, start, cfg):
+    """FP64 cost at the maximum collision weight of each trajectory of a flat (W, A) array."""
+    trajs = [np.asarray(x_flat[start[b]:start[b + 1]], np.float64) for b in range(len(start) - 1)]
+    c, _ = oracle_mod.traj_cost(om, trajs, robot.joints.lower_limits, robot.joints.upper_limits,
+                                (cfg.w_smooth, cfg.w_vel, cfg.w_acc, cfg.w_jerk, cfg.w_limits, cfg.w_collision_max),
+                                grad=False)
+    return c @ [cfg.w_smooth, cfg.w_limits, cfg.w_collision_max]
+
+
+def test_lbfgs_iterates_follow_oracle(oracle_mod):
+    """The current iterate (not the tracked best) after 1, 3 and 10 iterations
+    equals the FP64 oracle's to 1e-4, on the trajectory workload's shapes:
+    straight-line paths of 32, 64 and 128 waypoints through 1024 spheres, the
+    default configuration (colldist, margin 0.01).  Every path is in collision
+    at the start and every iterate moves, so the step list, curvature test,
+    two-loop recursion and direction all act on the compared values."""
+    robot, model, om = _config5_setup(oracle_mod, seed=21)
+    trajs = _trajs(robot, [32, 64, 128, 32, 64, 128], seed=22, noise=0.0)
+    x0 = np.concatenate(trajs)
+    cfg10 = LbfgsTrajOptConfig(n_iters=10)
+    opt = BatchedLbfgsTrajOpt(model, trajs, cfg10)
+    s = opt.start.cpu().numpy()
+    init = oracle_mod.traj_cost(om, [t.astype(np.float64) for t in trajs], robot.joints.lower_limits,
+                                robot.joints.upper_limits, (1, 1, 0.5, 0.1, 1, 1), grad=False)[0]
+    assert np.all(init[:, 2] > 0), "every path starts in collision"
+    done = 0
+    for k in (1, 3, 10):
+        opt.run_iterations(done, k)
+        done = k
+        x_gpu = opt.final_iterate().cpu().numpy()
+        _, _, _, x_or, trace = _oracle_lbfgs(oracle_mod, robot, om, trajs, LbfgsTrajOptConfig(n_iters=k))
+        for b in range(len(trajs)):
+            sl = slice(s[b], s[b + 1])
+            err = np.abs(x_gpu[sl] - x_or[sl]).max()
+            moved = np.abs(x_or[sl] - x0[sl]).max()
+            print(f"iter {k} traj {b} T={s[b + 1] - s[b]}: max|gpu - fp64| {err:.2e}, moved {moved:.3f}, "
+                  f"steps {trace[b, :, 1].astype(int).tolist()}", flush=True)
+            assert moved > 1e-3
+            assert err <= 1e-4, (k, b, err)
+
+
+def test_lbfgs_100_iterations_last_iterate(oracle_mod):
+    """After the full 100 iterations (escalations at 20, 40, 60, 80) the last
+    iterates of FP32 and FP64 may part where a line search is near a tie, so
+    their costs at the maximum collision weight are compared as a batch: the
+    median ratio within 10%, most trajectories within 25%, and both well below
+    the initial cost.  The tracked best never loses to the start, endpoints
+    stay pinned."""
+    robot, model, om = _config5_setup(oracle_mod, seed=23)
+    trajs = _trajs(robot, [32] * 8, seed=24, noise=0.0)
     cfg = LbfgsTrajOptConfig(n_iters=100)
-    best, cost, obest, ocost = _lbfgs_both(oracle_mod, robot, model, om, trajs, cfg)
-    init_cost = np.array([oracle_mod.traj_cost(om, [t.astype(np.float64)], robot.joints.lower_limits,
-                                               robot.joints.upper_limits,
-                                               (1, 1, 0.5, 0.1, 1, 100.0), grad=False)[0][0] @ [1, 1, 100]
-                          for t in trajs])
-    assert np.all(cost <= init_cost * (1 + 1e-5) + 1e-6)  # the best iterate never loses to the start
-    ratio = (cost + 1e-3) / (ocost + 1e-3)
-    assert 0.8 <= np.median(ratio) <= 1.25, ratio
-    assert np.mean(np.abs(np.log(ratio)) < 0.5) >= 0.75, ratio
-    # endpoints pinned
-    s = np.concatenate([[0], np.cumsum([len(t) for t in trajs])])
-    for b, t in enumerate(trajs):
-        np.testing.assert_array_equal(best[s[b]], t[0])
-        np.testing.assert_array_equal(best[s[b + 1] - 1], t[-1])
+    opt = BatchedLbfgsTrajOpt(model, trajs, cfg)
+    best, best_cost, _ = opt.solve()
+    best, best_cost = best.cpu().numpy(), best_cost.cpu().numpy()
+    last = opt.final_iterate().cpu().numpy()
+    s = opt.start.cpu().numpy()
+    obest, ocost, _, olast, _ = _oracle_lbfgs(oracle_mod, robot, om, trajs, cfg)
+    x0 = np.concatenate(trajs)
+    c_init = _max_weight_costs(oracle_mod, robot, om, x0, s, cfg)
+    c_gpu = _max_weight_costs(oracle_mod, robot, om, last, s, cfg)
+    c_or = _max_weight_costs(oracle_mod, robot, om, olast, s, cfg)
+    ratio = (c_gpu + 1e-3) / (c_or + 1e-3)
+    print("initial", np.round(c_init, 3), "\ngpu last", np.round(c_gpu, 3), "\nfp64 last", np.round(c_or, 3),
+          flush=True)
+    assert 0.9 <= np.median(ratio) <= 1.1, ratio
+    assert np.mean(np.abs(np.log(ratio)) < np.log(1.25)) >= 0.75, ratio
+    assert np.median(c_gpu / c_init) < 0.5 and np.median(c_or / c_init) < 0.5
+    assert np.all(best_cost <= c_init * (1 + 1e-5) + 1e-6)
+    for b in range(len(trajs)):
+        np.testing.assert_array_equal(best[s[b]], trajs[b][0])
+        np.testing.assert_array_equal(best[s[b + 1] - 1], trajs[b][-1])
+        np.testing.assert_array_equal(last[s[b]], trajs[b][0])
+        np.testing.assert_array_equal(last[s[b + 1] - 1], trajs[b][-1])
+
+
+def test_lbfgs_best_moves_in_free_space(oracle_mod):
+    """With the world out of reach and noisy initial paths the tracked best
+    follows the iterate: GPU and FP64 best paths agree to 1e-4 after 30
+    iterations, and both are far from the start."""
+    robot, model, om = _config5_setup(oracle_mod, seed=25, far=True)
+    trajs = _trajs(robot, [32, 64, 128], seed=26, noise=0.1)
+    cfg = LbfgsTrajOptConfig(n_iters=30)
+    opt = BatchedLbfgsTrajOpt(model, trajs, cfg)
+    best, best_cost, _ = opt.solve()
+    best, best_cost = best.cpu().numpy(), best_cost.cpu().numpy()
+    obest, ocost, _, olast, trace = _oracle_lbfgs(oracle_mod, robot, om, trajs, cfg)
+    x0 = np.concatenate(trajs)
+    print("max|best - x0| per element", np.abs(obest - x0).max(), "max|gpu - fp64|", np.abs(best - obest).max(),
+          flush=True)
+    assert np.abs(obest - x0).max() > 0.05
+    np.testing.assert_allclose(best, obest, rtol=0, atol=1e-4)
+    np.testing.assert_allclose(best_cost, ocost, rtol=1e-4, atol=1e-6)
 
 
 def test_chunked_uneven_and_graph_replay(oracle_mod):

@@ -1,0 +1,87 @@
+"""Pins of the LM collision penalty's inputs and value (CPU).
+
+* sphere set -- ``prepare_ls_collision_buffers`` keeps exactly the spheres of
+  links that have a parent joint (the root link's are dropped, as in the
+  reference's ``_ls_ik.py:108-123``), each in its link's frame with that
+  link's parent joint; the expected set is read from the URDF XML.
+* value      -- the oracle's error at ``max_iter = 0`` with the penalty on,
+  minus the error with it off, equals w * sum over kept spheres and world
+  spheres of max(0, margin - d)^2, with d from world sphere centres placed by
+  ``independent`` (URDF XML + scipy) and scipy ``cdist``.
+"""
+
+import independent
+import numpy as np
+import pytest
+from conftest import ROBOTS, load_robot, random_cfgs
+from scipy.spatial import distance
+
+from pyroffi_amd.collision import RobotCollisionSpherized
+from pyroffi_amd.optimization_engines._ls_ik import ee_tables, prepare_ls_collision_buffers
+
+# Kept spheres per robot: every sphere of the URDF except the root link's.
+KEPT = {"panda": 58}
+
+
+def _world(seed, m=256):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([rng.uniform(-1, 1, (m, 3)), rng.uniform(0.02, 0.15, (m, 1))], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ROBOTS)
+def test_kept_sphere_set(name):
+    robot, path = load_robot(name)
+    coll = RobotCollisionSpherized.from_urdf(path)
+    rs, rj, ws, ok = prepare_ls_collision_buffers(robot, coll, _world(0))
+    assert ok and ws.shape == (256, 4)
+    links, joints, spheres = independent.read(path)
+    parent_joint = {j["child"]: j["name"] for j in joints}
+    expected = sorted((parent_joint[l], *np.round(s, 6)) for l in links if l in parent_joint
+                      for s in spheres[l] if s[3] > 0)
+    got = sorted((robot.joints.names[j], *np.round(r.astype(np.float64), 6)) for r, j in zip(rs, rj))
+    assert len(got) == len(expected)
+    if name in KEPT:
+        assert len(got) == KEPT[name]
+    root = [l for l in links if l not in parent_joint]
+    assert len(root) == 1
+    n_all = sum(len(spheres[l]) for l in links)
+    assert len(got) == n_all - len(spheres[root[0]])
+    for g, e in zip(got, expected):
+        assert g[0] == e[0]
+        np.testing.assert_allclose(g[1:], e[1:], atol=2e-6)
+
+
+@pytest.mark.parametrize("name", ("panda", "fetch", "baxter"))
+def test_penalty_matches_brute_force(oracle_mod, name):
+    robot, path = load_robot(name)
+    coll = RobotCollisionSpherized.from_urdf(path)
+    world = _world(3)
+    rs, rj, ws, _ = prepare_ls_collision_buffers(robot, coll, world)
+    t = oracle_mod.Tables.of(robot)
+    ee = robot.links.names.index({"panda": "panda_hand", "fetch": "gripper_link",
+                                  "baxter": "left_gripper_base"}[name])
+    tj, masks = ee_tables(robot, (ee,))
+    q = random_cfgs(robot, 12, seed=4)
+    tgt = oracle_mod.fk_links(t, random_cfgs(robot, 2, seed=5))[:, ee][:, None]
+    seeds = q.reshape(2, 6, -1)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    w, margin = 1e4, 0.02
+    _, e_off = oracle_mod.ls_ik(t, seeds, tj, masks, tgt, lo, hi, max_iter=0)
+    _, e_on = oracle_mod.ls_ik(t, seeds, tj, masks, tgt, lo, hi, max_iter=0, robot_spheres=rs, sphere_joint=rj,
+                               world=ws, enable_collision=True, collision_weight=w, collision_margin=margin)
+    links, joints, _ = independent.read(path)
+    children = {j["child"] for j in joints}
+    wd = world.astype(np.float64)
+    expected = np.zeros(len(q))
+    for b in range(len(q)):
+        _, per_link = independent.world_spheres(path, q[b])
+        for l in links:
+            s = per_link[l]
+            if l not in children or not len(s):
+                continue
+            d = distance.cdist(s[:, :3], wd[:, :3]) - s[:, 3:4] - wd[None, :, 3]
+            expected[b] += np.sum(np.maximum(0.0, margin - d) ** 2)
+    expected *= w
+    assert np.count_nonzero(expected) >= len(q) // 2  # most configurations touch the world
+    # the oracle takes the float32 tables the kernels take: ~1e-6 relative from table rounding
+    np.testing.assert_allclose((e_on - e_off).ravel(), expected, rtol=1e-5, atol=1e-6)

@@ -319,3 +319,46 @@ def test_pose_hinge_fd_gradient_sanity(oracle_mod):
     for b in range(3):
         r, Jm = oracle_mod.res_jac(t, q[b], tj, masks, tgt[b])
         np.testing.assert_allclose(fd[b], 2 * Jm[:3].T @ r[:3], rtol=1e-6, atol=1e-9)
+
+
+def test_obstacle_distances_independent(oracle_mod):
+    """Sphere: scipy cdist.  Capsule: the minimum over 20001 points sampled
+    along the segment.  Box: the distance to the clamped (closest) point
+    outside, minus the distance to the nearest face inside.  Half-space: the
+    plane offset.  Padding (r < 0) reads 1e9."""
+    import worlds
+    from scipy.spatial import distance
+
+    rng = np.random.default_rng(12)
+    ws, wc, wb, wh = (x.astype(np.float64) for x in worlds.random_world(rng, 7, 6, 9, 4))
+    c = rng.uniform(-1.2, 1.2, (40, 3))
+    r = rng.uniform(0.0, 0.2, 40)
+    r[5] = -1e9
+    got = oracle_mod.obstacle_dist(c, r, ws, wc, wb, wh)
+    assert got.shape == (40, 26)
+    np.testing.assert_array_equal(got[5], 1e9)
+    keep = np.arange(40) != 5
+    c, r, got = c[keep], r[keep], got[keep]
+    np.testing.assert_allclose(got[:, :7], distance.cdist(c, ws[:, :3]) - r[:, None] - ws[None, :, 3], atol=1e-12)
+    t = np.linspace(0, 1, 20001)
+    for m, cap in enumerate(wc):
+        pts = cap[None, :3] + t[:, None] * (cap[3:6] - cap[:3])[None]
+        ref = distance.cdist(c, pts).min(1) - r - cap[6]
+        np.testing.assert_allclose(got[:, 7 + m], ref, atol=1e-8)
+    for m, box in enumerate(wb):
+        R = np.stack([box[3:6], box[6:9], box[9:12]], 1)  # columns = axes
+        h = box[12:15]
+        loc = (c - box[:3]) @ R
+        clamped = np.clip(loc, -h, h)
+        outside = np.linalg.norm(loc - clamped, axis=1)
+        inside = np.all(np.abs(loc) <= h, axis=1)
+        face = np.min(h - np.abs(loc), axis=1)
+        ref = np.where(inside, -face, outside) - r
+        np.testing.assert_allclose(got[:, 13 + m], ref, atol=1e-12)
+    for m, hs in enumerate(wh):
+        ref = (c - hs[3:]) @ hs[:3] - r
+        np.testing.assert_allclose(got[:, 22 + m], ref, atol=1e-12)
+    # a centre inside a box: negative distance equal to minus the nearest face gap
+    box = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1, 0.5, 0.2, 0.3]], np.float64)
+    d = oracle_mod.obstacle_dist(np.array([[0.1, 0.05, -0.2]]), np.array([0.01]), boxes=box)
+    assert d[0, 0] == pytest.approx(-0.1 - 0.01, abs=1e-12)

@@ -12,7 +12,9 @@ recursion and the direct L-BFGS loop.
 * two-loop   -- the dense BFGS inverse-Hessian recursion H+ = (I - rho s y^T)
   H (I - rho y s^T) + rho s s^T from H0 = gamma I.
 * L-BFGS     -- on a collision-free quadratic (smoothness only, inside the
-  limits) it converges to the minimum-energy path from numpy's linear solve.
+  limits) it converges to the minimum-energy path from numpy's linear solve;
+  in a colliding and in an empty world its accepted steps, last iterate and
+  tracked best equal a dense-algebra restatement of the loop step for step.
 """
 
 import independent
@@ -193,3 +195,89 @@ def test_lbfgs_converges_to_minimum_energy_path(oracle_mod):
     np.testing.assert_allclose(best, xs.reshape(T, 7), atol=1e-5)
     comps, _ = oracle_mod.traj_cost(md, [best], lo, hi, w, grad=False)
     assert cost[0] == pytest.approx(comps[0, 0], rel=1e-9)
+
+
+def _lbfgs_dense(oracle_mod, md, x0, lo, hi, n_iters, w_coll0, w_coll_max, scale=2.0, esc=20, m=6):
+    """The direct L-BFGS loop written from the reference's description with dense
+    algebra: the inverse Hessian is rebuilt each iteration from gamma I and the
+    last m accepted curvature pairs (oldest first), and the cost and gradient
+    come from the pinned ``traj_cost``.  Returns (last, best, best cost, picks)."""
+    alphas = (1.0, 0.5, 0.25, 0.1, 0.025)
+    T, A = x0.shape
+    n = T * A
+    mask = np.ones(n)
+    mask[:A] = 0
+    mask[n - A:] = 0
+
+    def total(x, wc, grad):
+        w = (1.0, 1.0, 0.5, 0.1, 1.0, wc)
+        c, g = oracle_mod.traj_cost(md, [x.reshape(T, A)], lo, hi, w, grad=grad)
+        v = c[0, 0] + c[0, 1] + wc * c[0, 2]
+        return (v, g.ravel() * mask) if grad else v
+
+    x = x0.ravel().copy()
+    wc = w_coll0
+    best_cost, _ = total(x, wc, True)
+    best = x.copy()
+    xp, gp = x.copy(), None
+    pairs, picks, fallback = [], [], 0
+    for k in range(n_iters):
+        if k > 0 and k % esc == 0:
+            wc = min(wc * scale, w_coll_max)
+        cost, g = total(x, wc, True)
+        if k > 0:
+            s, y = x - xp, g - gp
+            if s @ y > 1e-10 * (y @ y) + 1e-30:
+                pairs = (pairs + [(s, y)])[-m:]
+        if pairs:
+            s_n, y_n = pairs[-1]
+            H = (s_n @ y_n) / (y_n @ y_n + 1e-18) * np.eye(n)
+            for s, y in pairs:
+                r = 1.0 / (s @ y + 1e-30)
+                V = np.eye(n) - r * np.outer(y, s)
+                H = V.T @ H @ V + r * np.outer(s, s)
+            d = -H @ g
+        else:
+            d = -g / (np.linalg.norm(g) + 1e-18)
+        d *= mask
+        trials = [total(x + a * d, wc, False) for a in alphas]
+        ok = [i for i, v in enumerate(trials) if v < cost * (1 - 1e-4)]
+        pick = ok[0] if ok else int(np.argmin(trials))
+        fallback += not ok
+        picks.append(pick)
+        xp, gp = x.copy(), g
+        x = x + alphas[pick] * d
+        ev = total(x, w_coll_max, False)
+        if ev < best_cost:
+            best_cost, best = ev, x.copy()
+    return x.reshape(T, A), best.reshape(T, A), best_cost, picks, fallback
+
+
+@pytest.mark.parametrize("noise,M,margin,mode,w_coll", [(0.05, 40, 0.05, 0, 30.0), (0.05, 40, 0.1, 1, 30.0),
+                                                        (0.2, 0, 0.02, 1, 1.0)])
+def test_lbfgs_iterates_match_dense_restatement(oracle_mod, noise, M, margin, mode, w_coll):
+    """Last iterate, accepted steps, best iterate and best cost of the oracle's
+    L-BFGS equal a dense-algebra restatement step for step over 45 iterations
+    (escalations at 20 and 40).  A colliding world (the best stays the start,
+    as in the reference) and an empty one (the best moves with the iterate)."""
+    robot, _, _, t, md, world = _model(oracle_mod, M=max(M, 1), margin=margin, mode=mode, seed=4)
+    if M == 0:
+        md.world = np.array([[100.0, 100.0, 100.0, 0.01]])
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    T = 10
+    rng = np.random.default_rng(8)
+    x0 = _line(robot, T, seed=9)
+    x0[1:-1] += rng.normal(0, noise, (T - 2, 7))
+    last, best, bc, picks, fallback = _lbfgs_dense(oracle_mod, md, x0, lo, hi, 45, w_coll, 100.0)
+    obest, ocost, _, olast, trace = oracle_mod.traj_lbfgs(md, [x0], lo, hi, n_iters=45, w_collision=w_coll,
+                                                          iterates=True)
+    print("picks", np.bincount(picks, minlength=5), "fallback", fallback)
+    assert [int(p) for p in trace[0, :, 1]] == picks
+    # dense and two-loop algebra round differently: 1e-8 after 45 iterations
+    np.testing.assert_allclose(olast, last, atol=1e-7, rtol=0)
+    np.testing.assert_allclose(obest, best, atol=1e-7, rtol=0)
+    assert ocost[0] == pytest.approx(bc, rel=1e-9)
+    if M > 0:
+        assert set(picks) == {0, 1, 2, 3, 4}  # every step size of the list is taken
+    else:
+        assert np.abs(best - x0).max() > 1e-2  # the tracked best moved
