@@ -19,6 +19,7 @@ sample of the same workload, and one object per further workload:
             one plain launch per target, at equal success
   trajopt   512 Panda trajectories (T in {32, 64, 128}), 1024 world spheres,
             100 L-BFGS iterations in one captured graph
+  hjcd      HJCD-IK (Robot.inverse_kinematics' solver) on 1024 Panda targets
 """
 
 from __future__ import annotations
@@ -46,6 +47,13 @@ PEAK_HBM_GBS = 8000.0
 # including the min = 12; SE(3) sphere placement ~ 33 per sphere).
 K_SPHERES = 59
 SELF_SPHERE_PAIRS = 1230
+# LM-IK: algorithmic flops per LM iteration per seed for Panda without
+# collision (FK + Jacobian 1.6 k, normal equations 0.7 k, Cholesky 0.3 k, five
+# line-search FK + residual 7 k); with collision, 33 per robot-sphere placement
+# and 13 per executed sphere-pair test (distance, hinge and accumulate).
+IK_FLOP_PER_SEED_ITER = 9600
+IK_FLOP_PLACEMENT = 33
+IK_FLOP_PAIR = 13
 FLOP_SWEEP = K_SPHERES * M_WORLD * 12 + SELF_SPHERE_PAIRS * 12 + K_SPHERES * 33
 BYTES_FK = 7 * 4 + 13 * 7 * 4  # cfg in + link poses out
 BYTES_SWEEP = 13 * 7 * 4 + 4 + 66 * 4  # link poses in + min + self pairs out
@@ -142,11 +150,16 @@ def ik_bench(reps: int):
 
     solve()
     torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
+    e0.record(stream)
     for _ in range(reps):
         best, cfgs, errs = solve()
+    e1.record(stream)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
+    gpu_ms = e0.elapsed_time(e1) / reps
 
     def success(qs):
         lp = robot.forward_kinematics(np.asarray(qs, np.float32).reshape(-1, 7))[:, link].astype(np.float64)
@@ -168,17 +181,30 @@ def ik_bench(reps: int):
     cpu_dt = time.perf_counter() - c0
     owin01, _ = success(ocfgs[np.arange(16), oerrs.argmin(1)])
     oseed01, _ = success(ocfgs)
+    o32 = oracle.single_precision()
+    c0 = time.perf_counter()
+    o32.ls_ik(t_or, seeds.cpu().numpy(), tj, masks, tgt[:, None].astype(np.float32), lo.astype(np.float32),
+              hi.astype(np.float32), threads=threads, **kw)
+    cpu32_dt = time.perf_counter() - c0
+    flop = IK_FLOP_PER_SEED_ITER * 16 * 64 * 30
+    achieved = flop / (gpu_ms * 1e-3) / 1e12
     return {
         "workload": "panda_lm_ik_16targets_64seeds_30iters",
         "solves_per_s": 16 / dt,
         "ms_per_solve_call": dt * 1e3,
+        "gpu_event_ms_per_call": gpu_ms,
+        "roofline": {"bound": "valu", "kernel": "ls_ik_kernel (pk_ls_ik_f32) + select", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "flop_per_seed_iteration": IK_FLOP_PER_SEED_ITER,
+                     "note": "latency-bound: 1024 seeds (128 waves of 8-lane seed groups) on 256 CUs"},
         "success_1mm_0.01rad": float(win01.mean()),
         "success_1mm_0.05rad": float(win05.mean()),
         "seed_success_1mm_0.01rad": float(seed01.mean()),
         "oracle_success_1mm_0.01rad": float(owin01.mean()),
         "oracle_seed_success_1mm_0.01rad": float(oseed01.mean()),
         "cpu_baseline": {"value": 16 / cpu_dt, "unit": "solves/s", "cores": threads, "kind": "port",
-                         "sample": "the same 16 problems x 64 seeds, FP64 oracle"},
+                         "sample": "the same 16 problems x 64 seeds, FP64 oracle",
+                         "fp32_solves_per_s": 16 / cpu32_dt, "cpu_model": cpu_model()},
     }
 
 
@@ -186,7 +212,8 @@ UNEVEN_EE = {"panda": ("panda_hand",), "ur5": ("ee_link",), "fetch": ("gripper_l
              "baxter": ("left_gripper_base", "right_gripper_base")}
 
 
-def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32, cpu_per_robot: int = 2):
+def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32, cpu_per_robot: int = 2,
+                    batch_per_robot: int = 256):
     """Config 4: 4096 targets over Panda/UR5/Fetch/Baxter (two EEs), seeds per target
     log-uniform in [16, 256], 50 LM iterations, collision penalty against 256 world
     spheres.  Fused = one autotuned persistent launch per robot over flat seeds,
@@ -211,8 +238,68 @@ def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32,
     torch.cuda.synchronize()
     fused_dt = (time.perf_counter() - t0) / reps
     winners = [w.cpu().numpy() for w, _ in out]
-    return _uneven_report(groups, samples, winners, fused_dt, n_seeds, report, per_robot, plain_per_robot,
-                          cpu_per_robot)
+    work = solver.count_work()
+    torch.cuda.synchronize()
+    # The reference's batch call shape: one launch per robot, seeds padded to the largest count.
+    solver.solve_batch_padded(n_problems=4)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    padded = [w.cpu().numpy() for w in solver.solve_batch_padded(n_problems=batch_per_robot)]
+    padded_dt = time.perf_counter() - t0
+    res = _uneven_report(groups, samples, winners, fused_dt, n_seeds, report, per_robot, plain_per_robot,
+                         cpu_per_robot)
+    res.update(_uneven_roofline(groups, work, fused_dt, solver.kw["max_iter"]))
+    res.update(_padded_report(groups, winners, padded, padded_dt, batch_per_robot, per_robot,
+                              res["fused_solves_per_s"]))
+    return res
+
+
+def _uneven_roofline(groups, work, fused_dt, max_iter):
+    """Work-based FLOP/s of the fused solve: per seed max_iter LM iterations at the
+    Panda count scaled by DOF and end effectors, 1 + 6 max_iter penalty evaluations
+    of robot-sphere placements, and the sphere-pair tests the kernel counted."""
+    flop = 0.0
+    per_robot = {}
+    for g, w in zip(groups, work):
+        S = g.seeds.shape[0]
+        A = g.robot.joints.num_actuated_joints
+        K = 0 if g.collision is None else len(g.collision[0])
+        base = S * max_iter * IK_FLOP_PER_SEED_ITER * (A / 7.0) * len(g.target_link_indices)
+        place = S * (1 + 6 * max_iter) * K * IK_FLOP_PLACEMENT
+        pairs = w * IK_FLOP_PAIR
+        flop += base + place + pairs
+        per_robot[g.robot.links.names[0]] = {"seeds": S, "pair_tests": w, "pair_tests_per_seed_eval":
+                                             w / max(1, S * (1 + 6 * max_iter) * max(K, 1))}
+    achieved = flop / fused_dt / 1e12
+    return {"roofline": {"bound": "valu", "kernel": "ls_ik_batched_kernel (pk_ls_ik_batched_f32), all robots",
+                         "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / PEAK_FP32_TFLOPS, "flop_per_batch": flop,
+                         "work": per_robot}}
+
+
+def _padded_report(groups, winners, padded, padded_dt, batch_per_robot, per_robot, fused_rate):
+    from pyroffi_amd._se3 import quat_conj, quat_mul, so3_log
+
+    def ok(g, q):
+        lp = g.robot.forward_kinematics(np.ascontiguousarray(q, np.float32)).astype(np.float64)
+        r = np.ones(len(q), bool)
+        for e, l in enumerate(g.target_link_indices):
+            tt = g.targets[:len(q), e]
+            pos = np.linalg.norm(lp[:, l, 4:] - tt[:, 4:], axis=1)
+            rot = np.linalg.norm(so3_log(quat_mul(quat_conj(tt[:, :4]), lp[:, l, :4])), axis=1)
+            r &= (pos <= 1e-3) & (rot <= 0.01)
+        return r
+
+    n = len(groups) * batch_per_robot
+    rate = n / padded_dt
+    return {"reference_batch_shape": {
+        "solves_per_s": rate, "sample_targets": n, "seconds": padded_dt,
+        "speedup_fused_vs_batch_shape": fused_rate / rate,
+        "success_1mm_0.01rad": float(np.concatenate([ok(g, p) for g, p in zip(groups, padded)]).mean()),
+        "fused_success_same_targets": float(np.concatenate([ok(g, w[:batch_per_robot])
+                                                            for g, w in zip(groups, winners)]).mean()),
+        "shape": "one pk_ls_ik_f32 launch per robot over its first targets, seeds padded to the largest "
+                 "count, brute-force collision penalty (ls_ik_solve_cuda_batch's single seed count)"}}
 
 
 def uneven_workload(per_robot: int = 1024, plain_per_robot: int = 32):
@@ -260,11 +347,12 @@ def _uneven_report(groups, samples, winners, fused_dt, n_seeds, report, per_robo
     plain_dt = time.perf_counter() - t0
     n_plain = sum(len(s.seed_start) - 1 for s in samples)
 
-    def success(g, q):
+    def success(g, q, rows=None):
         lp = g.robot.forward_kinematics(np.ascontiguousarray(q, np.float32)).astype(np.float64)
         ok = np.ones(len(q), bool)
+        rows = np.arange(len(q)) if rows is None else np.asarray(rows)
         for e, l in enumerate(g.target_link_indices):
-            tt = g.targets[:len(q), e]
+            tt = g.targets[rows, e]
             pos = np.linalg.norm(lp[:, l, 4:] - tt[:, 4:], axis=1)
             rot = np.linalg.norm(so3_log(quat_mul(quat_conj(tt[:, :4]), lp[:, l, :4])), axis=1)
             ok &= (pos <= 1e-3) & (rot <= 0.01)
@@ -278,16 +366,20 @@ def _uneven_report(groups, samples, winners, fused_dt, n_seeds, report, per_robo
     threads = cpu_threads()
     c0 = time.perf_counter()
     n_cpu = 0
-    for g in groups:
+    oracle_ok, fused_ok_cpu = [], []
+    for g, ok_g in zip(groups, fused_ok):
         t_or = oracle.Tables.of(g.robot)
         tj, masks = ee_tables(g.robot, g.target_link_indices)
         rs, rj, ws = g.collision
         for p in range(cpu_per_robot):
             s0, s1 = g.seed_start[p], g.seed_start[p + 1]
-            oracle.ls_ik(t_or, g.seeds[None, s0:s1], tj, masks, g.targets[p:p + 1], g.robot.joints.lower_limits,
-                         g.robot.joints.upper_limits, max_iter=50, robot_spheres=rs, sphere_joint=rj, world=ws,
-                         enable_collision=True, collision_weight=1e4, collision_margin=0.02, threads=threads)
+            oc, oe = oracle.ls_ik(t_or, g.seeds[None, s0:s1], tj, masks, g.targets[p:p + 1],
+                                  g.robot.joints.lower_limits, g.robot.joints.upper_limits, max_iter=50,
+                                  robot_spheres=rs, sphere_joint=rj, world=ws, enable_collision=True,
+                                  collision_weight=1e4, collision_margin=0.02, threads=threads)
             n_cpu += 1
+            oracle_ok.append(bool(success(g, oc[0, int(np.argmin(oe[0]))][None], rows=[p])[0]))
+            fused_ok_cpu.append(bool(ok_g[p]))
     cpu_dt = time.perf_counter() - c0
     names = list(UNEVEN_EE)
     return {
@@ -302,10 +394,13 @@ def _uneven_report(groups, samples, winners, fused_dt, n_seeds, report, per_robo
         "fused_success_1mm_0.01rad": {n: float(ok.mean()) for n, ok in zip(names, fused_ok)},
         "fused_success_on_plain_sample": float(np.concatenate(fused_on_sample).mean()),
         "plain_success_on_plain_sample": float(np.concatenate(plain_ok).mean()),
+        "oracle_success_on_cpu_sample": float(np.mean(oracle_ok)),
+        "fused_success_on_cpu_sample": float(np.mean(fused_ok_cpu)),
         "autotune": {n: list(g.tune) for n, g in zip(names, groups)},
         "autotune_ms": [v.get("ms") for v in report.values()],
         "cpu_baseline": {"value": n_cpu / cpu_dt, "unit": "solves/s", "cores": threads, "kind": "port",
-                         "sample": f"first {cpu_per_robot} targets of each robot with their seeds, FP64 oracle"},
+                         "sample": f"first {cpu_per_robot} targets of each robot with their seeds, FP64 oracle",
+                         "cpu_model": cpu_model()},
     }
 
 
@@ -418,6 +513,77 @@ def _trajopt_report(robot, coll, world, trajs, lens, cfg, model, opt, best_cost,
     }
 
 
+def hjcd_bench(reps: int, n_problems: int = 1024, cpu_sample: int = 8):
+    """HJCD-IK (Robot.inverse_kinematics' solver) batched: 1024 FK-generated Panda targets,
+    32 coarse seeds each (top 6 x 5 refinement seeds), 20 coarse + 40 LM iterations,
+    continuity weight 1e-3 from the joint-range midpoint.  The FP64 oracle runs the same
+    phases on the first targets with the GPU run's own seeds, noise and kick key."""
+    import torch
+
+    import oracle
+    import pyroffi_amd as pk
+    from pyroffi_amd.optimization_engines._hjcd_ik import hjcd_solve_batch
+    from pyroffi_amd.optimization_engines._ls_ik import ee_tables
+
+    robot = pk.Robot.from_urdf(pk.bundled_urdf_path("panda"))
+    link = robot.links.names.index("panda_hand")
+    rng = np.random.default_rng(7)
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    tq = rng.uniform(lo, hi, (n_problems, 7)).astype(np.float32)
+    tgt = robot.forward_kinematics(tq)[:, link].astype(np.float64)[:, None]
+    prev = np.tile((lo + hi) / 2, (n_problems, 1))
+    hjcd_solve_batch(robot, (link,), tgt, 0, prev, continuity_weight=1e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        best, d = hjcd_solve_batch(robot, (link,), tgt, r, prev, continuity_weight=1e-3, return_details=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    best = best.cpu().numpy()
+    t_or = oracle.Tables.of(robot)
+    tj, _ = ee_tables(robot, (link,))
+
+    def ok(q, rows):
+        f = oracle.jres(t_or, q.astype(np.float64), tj, tgt[rows])
+        rot = oracle.fk_links(t_or, q.astype(np.float64))[:, link]
+        from scipy.spatial.transform import Rotation
+
+        ang = np.linalg.norm((Rotation.from_quat(tgt[rows, 0][:, [1, 2, 3, 0]]).inv() *
+                              Rotation.from_quat(rot[:, [1, 2, 3, 0]])).as_rotvec(), axis=1)
+        return (np.linalg.norm(rot[:, 4:] - tgt[rows, 0, 4:], axis=1) <= 1e-3) & (ang <= 0.01) & np.isfinite(f[:, 0, 0])
+
+    win = ok(best, np.arange(n_problems))
+    # FP64 oracle: the same phases on the first targets, timed on the host cores.
+    threads = cpu_threads()
+    n = cpu_sample
+    c0 = time.perf_counter()
+    oc, oe = oracle.hjcd_coarse(t_or, d["seeds"][:n], tj, tgt[:n], d["ee_of_seed"], lo, hi, k_max=20,
+                                threads=threads)
+    top_k, repeats = d["top_k"], d["repeats"]
+    R = top_k * repeats
+    ref = np.empty((n, R, 7))
+    for p in range(n):
+        top = np.argsort(np.where(np.isnan(oe[p]), np.inf, oe[p]), kind="stable")[:top_k]
+        base = np.tile(oc[p, top], (repeats, 1))
+        ref[p] = np.where((np.arange(R) < top_k)[:, None], base,
+                          np.clip(base + d["noise"][p] * 0.15 * (hi - lo), lo, hi))
+    cfgs, errs = oracle.jlm(t_or, ref, tj, tgt[:n], lo, hi, max_iter=40, adaptive=True, prior_weight=1e-4,
+                            kick_scale=0.05, kick_key=d["kick_key"], threads=threads)
+    score = errs + 1e-3 * np.sum((cfgs - prev[:n, None]) ** 2, axis=-1)
+    obest = cfgs[np.arange(n), np.argmin(score, axis=1)]
+    cpu_dt = time.perf_counter() - c0
+    return {
+        "workload": "panda_hjcd_ik_1024targets_32seeds_20coarse_40lm",
+        "solves_per_s": n_problems / dt,
+        "ms_per_batch": dt * 1e3,
+        "success_1mm_0.01rad": float(win.mean()),
+        "success_on_oracle_sample": float(win[:n].mean()),
+        "oracle_success_on_sample": float(ok(obest, np.arange(n)).mean()),
+        "cpu_baseline": {"value": n / cpu_dt, "unit": "solves/s", "cores": threads, "kind": "port",
+                         "sample": f"first {n} targets, FP64 oracle of the same phases", "cpu_model": cpu_model()},
+    }
+
+
 def grad_workload():
     """Config 3 inputs: Panda, 16384 configs, FK-generated targets, 512 world spheres.
     Returns (robot, coll, link, q_np, world_np, q, tgt, world, step, run): ``step`` is one
@@ -506,31 +672,61 @@ def grad_bench(reps: int, cpu_sample: int):
 
 
 def cpu_threads() -> int:
+    """Every core this process may run on (its affinity set)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    return max(1, n)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_sweep(o, t, sp, coll, q, world, threads):
+    """FK + world min + self pairs with oracle module ``o`` (FP64 or its FP32 build)."""
+    lp = o.fk_links(t, q, threads)
+    _, mins = o.world_dist(lp, sp, world, want_full=False, threads=threads)
+    selfd = o.self_dist(lp, sp, coll.active_idx_i, coll.active_idx_j, threads=threads)
+    return lp, mins, selfd
 
 
 def cpu_baseline(sample: int):
-    """FP64 oracle (C + OpenMP) on `sample` configurations of the same workload."""
+    """The oracle (C + OpenMP restatement) on `sample` configurations of the same workload:
+    FP64 (the reference JAX path's arithmetic) on every core -- the reported value -- and
+    on one core, and the FP32 build on every core and on one core.  One-core legs run on
+    a sixteenth of the sample."""
     import oracle
 
     robot, coll, q, world = make_inputs()
-    q = q[:sample].astype(np.float64)
     t = oracle.Tables.of(robot)
-    sp = oracle.padded_spheres(coll)
     threads = cpu_threads()
-    oracle.fk_links(t, q[:64], threads)  # warm-up (thread pool, page-in)
-    t0 = time.perf_counter()
-    lp = oracle.fk_links(t, q, threads)
-    _, mins = oracle.world_dist(lp, sp, world.astype(np.float64), want_full=False, threads=threads)
-    selfd = oracle.self_dist(lp, sp, coll.active_idx_i, coll.active_idx_j, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": sample / dt, "unit": "configs/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample} of the {B} configurations, FP64 oracle (FK + world min + self pairs)"}, \
-        (lp, mins, selfd)
+    legs = {}
+    out = None
+    for prec, o in (("fp64", oracle), ("fp32", oracle.single_precision())):
+        R = o._R
+        sp = o.padded_spheres(coll)
+        w = world.astype(R)
+        for cores in (threads, 1):
+            n = sample if cores == threads else max(64, sample // 16)
+            qq = q[:n].astype(R)
+            _oracle_sweep(o, t, sp, coll, qq[:64], w, cores)  # warm-up (thread pool, page-in)
+            t0 = time.perf_counter()
+            res = _oracle_sweep(o, t, sp, coll, qq, w, cores)
+            legs[f"{prec}_{cores}_cores"] = n / (time.perf_counter() - t0)
+            if prec == "fp64" and cores == threads:
+                out = res
+    return {"value": legs[f"fp64_{threads}_cores"], "unit": "configs/s", "cores": threads, "kind": "port",
+            "sample": f"first {sample} of the {B} configurations (one-core legs: the first {max(64, sample // 16)}), "
+                      "oracle FK + world min + self pairs; value = FP64 on every core",
+            "legs_configs_per_s": legs, "cpu_model": cpu_model()}, out
 
 
 def load_traffic():
@@ -664,6 +860,7 @@ def main():
         line["ik"] = ik_bench(reps=10)
         line["grad"] = grad_bench(reps=10, cpu_sample=256)
         line["ik_uneven"] = ik_uneven_bench(reps=5)
+        line["hjcd"] = hjcd_bench(reps=3)
         line["trajopt"] = trajopt_bench(reps=5)
     print(json.dumps(line), flush=True)
 

@@ -194,6 +194,72 @@ int pk_collision_self_sphere_f32(void* stream, const float* centers,
                                  float* out);
 
 /*
+ * World distances of a spherized robot from link poses over the four
+ * obstacle kinds, with the reference CUDA kernels' semantics (padding slots
+ * skipped, 1e9 for a link without spheres): out_world[b, l, m] = min over
+ * link l's spheres of the distance to obstacle m, out_world_min[b] (or NULL)
+ * = its minimum over (l, m).  The world path of CUDARobotCollisionChecker for
+ * worlds with capsules, boxes or half-spaces (the fused sweep serves sphere
+ * worlds); the reference reaches it through CollisionWorldSphereReducedFfi
+ * after placing the spheres (src/pyroffi/collision/_cuda_collision.py:500-540).
+ *   link_poses (batch, n_links, 7), spheres_padded (n_links, s_per_link, 4)
+ *   out_world (batch, n_links, M) required, out_world_min (batch) or NULL
+ */
+int pk_sphere_world_links_f32(void* stream, const float* link_poses, int64_t batch, int32_t n_links,
+                              const float* spheres_padded, int32_t s_per_link,
+                              const float* world_spheres, int32_t n_world_spheres,
+                              const float* world_capsules, int32_t n_world_capsules,
+                              const float* world_boxes, int32_t n_world_boxes,
+                              const float* world_halfspaces, int32_t n_world_halfspaces,
+                              float* out_world, float* out_world_min);
+
+/*
+ * Capsule robot at a configuration: link-frame capsules capsules_local
+ * (n_links, 7) = [a, b, r] moved by link_poses (batch, n_links, 7), written as
+ * out_soa [7, batch, n_links] and/or out_aos [batch, n_links, 7] (either may be
+ * NULL, not both).  Replaces RobotCollision.at_config followed by the
+ * checker's packing (src/pyroffi/collision/_robot_collision.py:323-345,
+ * src/pyroffi/collision/_cuda_collision.py:443-474).
+ */
+int pk_capsules_place_f32(void* stream, const float* link_poses, int64_t batch, int32_t n_links,
+                          const float* capsules_local, float* out_soa, float* out_aos);
+
+/*
+ * out[b] = min over the row_len floats of row b of dist (batch, row_len), 1e9
+ * for an empty row: the per-configuration minimum the reference's coarse
+ * guard takes with jnp.all(coarse_out > 0) over a capsule model's distances
+ * (src/pyroffi/collision/_cuda_collision.py:560-577).
+ */
+int pk_row_min_f32(void* stream, const float* dist, int64_t batch, int64_t row_len, float* out);
+
+/*
+ * World collision of a capsule robot: out[b, n, m] = distance of link
+ * capsule n (segment caps[0..2], caps[3..5], radius caps[6], component-major
+ * (7, batch, n_links)) to obstacle m of the packed world (kinds and order as
+ * for pk_collision_world_sphere_f32).  Replaces CollisionWorldCapsuleFfi
+ * (src/pyroffi/cuda_kernels/_collision_cuda_kernel.cu:993-1002, kernels
+ * :203-407): the capsule-box distance is the box SDF at the segment point
+ * nearest the box centre, as there.
+ *   out (batch, n_links, M) float32
+ */
+int pk_collision_world_capsule_f32(void* stream, const float* caps, int64_t batch, int32_t n_links,
+                                   const float* world_spheres, int32_t n_world_spheres,
+                                   const float* world_capsules, int32_t n_world_capsules,
+                                   const float* world_boxes, int32_t n_world_boxes,
+                                   const float* world_halfspaces, int32_t n_world_halfspaces,
+                                   float* out);
+
+/*
+ * Self collision of a capsule robot: out[b, p] = capsule-capsule distance of
+ * links pair_i[p] and pair_j[p] from AoS capsules (batch, n_links, 7), 1e9 for
+ * an out-of-range link index.  Replaces CollisionSelfCapsuleFfi
+ * (src/pyroffi/cuda_kernels/_collision_cuda_kernel.cu:1114-1214, kernel :474-495).
+ */
+int pk_collision_self_capsule_f32(void* stream, const float* caps, int64_t batch, int32_t n_links,
+                                  const int32_t* pair_i, const int32_t* pair_j, int32_t n_pairs,
+                                  float* out);
+
+/*
  * The JAX path's distance conventions for a spherized robot, from link poses:
  *   out_world[b, l, m] = min over slots s of |(o_m - c_ls) + 1e-6| - (r_ls + R_m)
  *     (RobotCollisionSpherized.compute_world_collision_distance,
@@ -294,8 +360,10 @@ int pk_pose_hinge_cost_f32(void* stream, const float* link_poses, int64_t batch,
  * attributes and outputs, same algorithm (weighted geometric-Jacobian LM with
  * Jacobi column scaling, norm-clipped trust region, 5-alpha line search,
  * all-time-best tracking, merit-only collision penalty).  The damped normal
- * equations are solved in FP32 (the reference uses FP64).  Only sphere worlds
- * are supported (other world counts must be 0).
+ * equations are solved in FP32 (the reference uses FP64).  The collision
+ * penalty takes the four obstacle kinds of the reference kernel (spheres,
+ * capsules, boxes, half-spaces; the non-sphere ones at most 8192 floats in
+ * all).
  *
  *   seeds        (n_problems, n_seeds, n_act)
  *   target_jnts  (n_ee) int32, ancestor_masks (n_ee, n_joints) int32
@@ -368,6 +436,10 @@ int pk_world_grid_fill_f32(void* stream, const float* world_spheres, int32_t n_w
  *                    by the launch that reaches max_iter
  *   grid_waves       number of persistent workgroups (0: one per 64-seed tile)
  *   out_cfgs (total_seeds, n_act), out_err (total_seeds)
+ *   work_counter     NULL, or a device uint64 the launch adds the number of
+ *                    sphere-pair tests of its collision penalty to (the
+ *                    work measure of the solver's roofline; the caller
+ *                    clears it)
  * Replaces one ls_ik_solve_cuda_batch call per robot
  * (src/pyroffi/optimization_engines/_ls_ik.py:1002-1172), which needs the same
  * seed count for every problem, and the per-call path for uneven ones.
@@ -384,7 +456,8 @@ int pk_ls_ik_batched_f32(
     const float* upper, const int32_t* fixed_mask, int32_t max_iter, int32_t iter_begin,
     int32_t iter_end, float* state, float pos_weight, float ori_weight, float lambda_init,
     float eps_pos, float eps_ori, int32_t enable_collision, float collision_weight,
-    float collision_margin, int32_t grid_waves, float* out_cfgs, float* out_err);
+    float collision_margin, int32_t grid_waves, float* out_cfgs, float* out_err,
+    uint64_t* work_counter);
 
 /*
  * Winner selection after pk_ls_ik_batched_f32: per problem, over its CSR seed

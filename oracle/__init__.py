@@ -20,13 +20,19 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
+LIB_F32 = os.path.join(HERE, "_build", "liboracle_f32.so")
 
+_R = np.float64  # the arithmetic type of the loaded build (see single_precision)
 _d = ctypes.POINTER(ctypes.c_double)
 _f = ctypes.POINTER(ctypes.c_float)
 _i = ctypes.POINTER(ctypes.c_int32)
 _I = ctypes.c_int
 _L = ctypes.c_int64
 _D = ctypes.c_double
+
+
+def _empty(shape):
+    return np.empty(shape, _R)
 
 
 def build() -> str:
@@ -49,7 +55,7 @@ def lib() -> ctypes.CDLL:
         h.or_self_dist.argtypes = [_d, _L, _I, _d, _I, _i, _i, _I, _I, _d, _I]
         h.or_ls_ik.argtypes = (
             [_d, _I, _I, _I, _f, _f, _i, _i, _f, _f, _i, _i, _I, _i, _i, _I, _d, _f, _i, _I, _f, _I,
-             _d, _d, _i, _I, _D, _D, _D, _D, _D, _I, _D, _D, _d, _d, _I]
+             _d, _d, _i, _I, _D, _D, _D, _D, _D, _I, _D, _D, _d, _d, _I, _f, _f, _f, _i]
         )
         h.or_res_jac.argtypes = [_d, _I, _f, _f, _i, _i, _f, _f, _i, _i, _I, _i, _i, _I, _d, _d, _d]
         h.or_lm_direction.argtypes = [_d, _d, _I, _I, _d, _D, _i, _d]
@@ -64,6 +70,8 @@ def lib() -> ctypes.CDLL:
         h.or_two_loop.argtypes = [_d, _d, _d, _d, _I, _I, _I, _I, _d]
         h.or_obstacle_dist.argtypes = [_d, _d, _L, _d, _I, _d, _I, _d, _I, _d, _I, _d, _I]
         h.or_obstacle_dist.restype = None
+        h.or_capsule_dist.argtypes = [_d, _L, _d, _I, _d, _I, _d, _I, _d, _I, _d, _I]
+        h.or_capsule_dist.restype = None
         robot8 = [_f, _f, _i, _i, _f, _f, _i, _i, _I]
         h.or_jres.argtypes = [_d, _L, _I] + robot8 + [_i, _I, _d, _d, _d]
         h.or_jres.restype = None
@@ -117,10 +125,10 @@ class Tables:
 
 def fk_joints(tables: Tables, cfg, threads: int = 0) -> np.ndarray:
     """World pose of every joint, FP64, ``(*batch, J, 7)``."""
-    cfg = _c(cfg, np.float64)
+    cfg = _c(cfg, _R)
     batch = cfg.shape[:-1]
     flat = cfg.reshape(-1, tables.A)
-    out = np.empty((flat.shape[0], tables.J, 7))
+    out = _empty((flat.shape[0], tables.J, 7))
     t = tables.ptrs()
     lib().or_fk(_p(flat, _d), flat.shape[0], tables.A, t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7],
                 tables.J, _p(out, _d), threads)
@@ -131,8 +139,8 @@ def fk_links(tables: Tables, cfg, threads: int = 0) -> np.ndarray:
     """World pose of every link, FP64, ``(*batch, L, 7)``."""
     jp = fk_joints(tables, cfg, threads)
     batch = jp.shape[:-2]
-    flat = _c(jp.reshape(-1, tables.J, 7), np.float64)
-    out = np.empty((flat.shape[0], tables.L, 7))
+    flat = _c(jp.reshape(-1, tables.J, 7), _R)
+    out = _empty((flat.shape[0], tables.L, 7))
     lib().or_link_poses(_p(flat, _d), flat.shape[0], tables.J, _p(tables.link_parent_joint, _i), tables.L,
                         _p(out, _d))
     return out.reshape(*batch, tables.L, 7)
@@ -140,20 +148,20 @@ def fk_links(tables: Tables, cfg, threads: int = 0) -> np.ndarray:
 
 def padded_spheres(coll) -> np.ndarray:
     """``(L, S, 4)`` link-frame sphere table with the model's -1e9 padding."""
-    c = np.asarray(coll.coll.pose.translation(), dtype=np.float64)
-    r = np.asarray(coll.coll.radius, dtype=np.float64)
-    return _c(np.concatenate([c, r[..., None]], axis=-1), np.float64)
+    c = np.asarray(coll.coll.pose.translation(), dtype=_R)
+    r = np.asarray(coll.coll.radius, dtype=_R)
+    return _c(np.concatenate([c, r[..., None]], axis=-1), _R)
 
 
 def world_dist(link_poses, spheres, world, bias: bool = False, want_full: bool = True, threads: int = 0):
     """``(out (B, L, M) | None, per-config min (B,))`` from FP64 link poses."""
-    lp = _c(link_poses, np.float64)
+    lp = _c(link_poses, _R)
     B, L, _ = lp.shape
-    sp = _c(spheres, np.float64)
+    sp = _c(spheres, _R)
     S = sp.shape[1]
-    w = _c(world, np.float64).reshape(-1, 4)
-    out = np.empty((B, L, w.shape[0])) if want_full else None
-    mins = np.empty(B)
+    w = _c(world, _R).reshape(-1, 4)
+    out = _empty((B, L, w.shape[0])) if want_full else None
+    mins = _empty(B)
     lib().or_world_dist(_p(lp, _d), B, L, _p(sp, _d), S, _p(w, _d), w.shape[0], int(bias),
                         None if out is None else _p(out, _d), _p(mins, _d), threads)
     return out, mins
@@ -161,12 +169,12 @@ def world_dist(link_poses, spheres, world, bias: bool = False, want_full: bool =
 
 def self_dist(link_poses, spheres, pair_i, pair_j, mode: int = 0, threads: int = 0) -> np.ndarray:
     """``(B, P)``; mode 0 = CUDA semantics (all sphere pairs), 1 = JAX diagonal pairs."""
-    lp = _c(link_poses, np.float64)
+    lp = _c(link_poses, _R)
     B, L, _ = lp.shape
-    sp = _c(spheres, np.float64)
+    sp = _c(spheres, _R)
     pi = _c(pair_i, np.int32)
     pj = _c(pair_j, np.int32)
-    out = np.empty((B, pi.shape[0]))
+    out = _empty((B, pi.shape[0]))
     lib().or_self_dist(_p(lp, _d), B, L, _p(sp, _d), sp.shape[1], _p(pi, _i), _p(pj, _i), pi.shape[0], mode,
                        _p(out, _d), threads)
     return out
@@ -175,39 +183,43 @@ def self_dist(link_poses, spheres, pair_i, pair_j, mode: int = 0, threads: int =
 def ls_ik(tables: Tables, seeds, target_jnts, ancestor_masks, target_T, lower, upper, fixed_mask=None, *,
           max_iter=60, pos_weight=50.0, ori_weight=10.0, lambda_init=5e-3, eps_pos=1e-8, eps_ori=1e-8,
           robot_spheres=None, sphere_joint=None, world=None, enable_collision=False, collision_weight=1e4,
-          collision_margin=0.02, threads: int = 0):
+          collision_margin=0.02, threads: int = 0, world_capsules=None, world_boxes=None, world_halfspaces=None):
     """FP64 restatement of the LM kernel; returns ``(cfgs (P, S, A), errs (P, S))``."""
-    sd = _c(seeds, np.float64)
+    sd = _c(seeds, _R)
     P, S, A = sd.shape
     tj = _c(target_jnts, np.int32).reshape(-1)
     n_ee = tj.shape[0]
     am = _c(ancestor_masks, np.int32).reshape(n_ee, tables.J)
-    tT = _c(target_T, np.float64).reshape(P, n_ee, 7)
+    tT = _c(target_T, _R).reshape(P, n_ee, 7)
     rs = _c(np.zeros((0, 4)) if robot_spheres is None else robot_spheres, np.float32).reshape(-1, 4)
     rj = _c(np.zeros(0) if sphere_joint is None else sphere_joint, np.int32).reshape(-1)
     ws = _c(np.zeros((0, 4)) if world is None else world, np.float32).reshape(-1, 4)
-    lo = _c(lower, np.float64)
-    hi = _c(upper, np.float64)
+    lo = _c(lower, _R)
+    hi = _c(upper, _R)
     fm = _c(np.zeros(A) if fixed_mask is None else fixed_mask, np.int32)
-    out = np.empty((P, S, A))
-    err = np.empty((P, S))
+    out = _empty((P, S, A))
+    err = _empty((P, S))
     t = tables.ptrs()
+    others = [_c(np.zeros((0, w)) if x is None else x, np.float32).reshape(-1, w)
+              for x, w in ((world_capsules, 7), (world_boxes, 15), (world_halfspaces, 6))]
+    counts = _c([o.shape[0] for o in others], np.int32)
+    oth = [_p(o, _f) for o in others]
     lib().or_ls_ik(_p(sd, _d), P, S, A, *t, tables.J, _p(tj, _i), _p(am, _i), n_ee, _p(tT, _d), _p(rs, _f),
                    _p(rj, _i), rs.shape[0], _p(ws, _f), ws.shape[0], _p(lo, _d), _p(hi, _d), _p(fm, _i),
                    int(max_iter), pos_weight, ori_weight, lambda_init, eps_pos, eps_ori, int(bool(enable_collision)),
-                   collision_weight, collision_margin, _p(out, _d), _p(err, _d), threads)
+                   collision_weight, collision_margin, _p(out, _d), _p(err, _d), threads, *oth, _p(counts, _i))
     return out, err
 
 
 def res_jac(tables: Tables, q, target_jnts, ancestor_masks, target_T):
     """Residual ``(6 n_ee,)`` and geometric Jacobian ``(6 n_ee, A)`` at one configuration."""
-    qq = _c(q, np.float64)
+    qq = _c(q, _R)
     tj = _c(target_jnts, np.int32).reshape(-1)
     n_ee = tj.shape[0]
     am = _c(ancestor_masks, np.int32).reshape(n_ee, tables.J)
-    tT = _c(target_T, np.float64).reshape(n_ee, 7)
-    r = np.empty(6 * n_ee)
-    jm = np.empty((6 * n_ee, tables.A))
+    tT = _c(target_T, _R).reshape(n_ee, 7)
+    r = _empty(6 * n_ee)
+    jm = _empty((6 * n_ee, tables.A))
     lib().or_res_jac(_p(qq, _d), tables.A, *tables.ptrs(), tables.J, _p(tj, _i), _p(am, _i), n_ee, _p(tT, _d),
                      _p(r, _d), _p(jm, _d))
     return r, jm
@@ -215,12 +227,12 @@ def res_jac(tables: Tables, q, target_jnts, ancestor_masks, target_T):
 
 def lm_direction(r, jac, weights6, lam, fixed_mask=None):
     """One LM direction (before the trust region) and the Cholesky success flag."""
-    rr = _c(r, np.float64)
-    jm = _c(jac, np.float64)
+    rr = _c(r, _R)
+    jm = _c(jac, _R)
     R, A = jm.shape
-    w = _c(weights6, np.float64)
+    w = _c(weights6, _R)
     fm = _c(np.zeros(A) if fixed_mask is None else fixed_mask, np.int32)
-    out = np.empty(A)
+    out = _empty(A)
     ok = lib().or_lm_direction(_p(rr, _d), _p(jm, _d), R, A, _p(w, _d), float(lam), _p(fm, _i), _p(out, _d))
     return out, bool(ok)
 
@@ -228,11 +240,11 @@ def lm_direction(r, jac, weights6, lam, fixed_mask=None):
 def pose_hinge_cost(tables: Tables, cfg, spheres, world, margin, ee_link, targets, pos_weight=1.0, ori_weight=1.0,
                     threads: int = 0) -> np.ndarray:
     """FP64 pose + hinge cost ``(B,)``; ``spheres`` is the padded ``(L, S, 4)`` table."""
-    q = _c(cfg, np.float64).reshape(-1, tables.A)
-    sp = _c(spheres, np.float64)
-    w = _c(world, np.float64).reshape(-1, 4)
-    tg = _c(np.zeros((q.shape[0], 7)) if targets is None else targets, np.float64).reshape(-1, 7)
-    out = np.empty(q.shape[0])
+    q = _c(cfg, _R).reshape(-1, tables.A)
+    sp = _c(spheres, _R)
+    w = _c(world, _R).reshape(-1, 4)
+    tg = _c(np.zeros((q.shape[0], 7)) if targets is None else targets, _R).reshape(-1, 7)
+    out = _empty(q.shape[0])
     lib().or_pose_hinge_cost(_p(q, _d), q.shape[0], tables.A, *tables.ptrs(), tables.J,
                              _p(tables.link_parent_joint, _i), tables.L, _p(sp, _d), sp.shape[1], _p(w, _d),
                              w.shape[0], float(margin), int(ee_link), _p(tg, _d), float(pos_weight),
@@ -242,13 +254,13 @@ def pose_hinge_cost(tables: Tables, cfg, spheres, world, margin, ee_link, target
 
 def pose_hinge_fd_grad(tables: Tables, cfg, spheres, world, margin, ee_link, targets, h=1e-4, **kw) -> np.ndarray:
     """FP64 central finite-difference gradient ``(B, A)`` of ``pose_hinge_cost``."""
-    q = _c(cfg, np.float64).reshape(-1, tables.A)
+    q = _c(cfg, _R).reshape(-1, tables.A)
     B, A = q.shape
     pert = np.repeat(q[:, None, None, :], 2, axis=1).repeat(A, axis=2)  # (B, 2, A, A)
     eye = np.eye(A) * h
     pert[:, 0] += eye
     pert[:, 1] -= eye
-    tg = np.repeat(np.asarray(targets, np.float64).reshape(B, 1, 7), 2 * A, axis=1).reshape(-1, 7)
+    tg = np.repeat(np.asarray(targets, _R).reshape(B, 1, 7), 2 * A, axis=1).reshape(-1, 7)
     c = pose_hinge_cost(tables, pert.reshape(-1, A), spheres, world, margin, ee_link, tg, **kw).reshape(B, 2, A)
     return (c[:, 0] - c[:, 1]) / (2 * h)
 
@@ -258,10 +270,10 @@ class TrajModel:
 
     def __init__(self, tables: Tables, spheres, pair_i, pair_j, world, margin: float, mode: int):
         self.t = tables
-        self.sph = _c(spheres, np.float64)
+        self.sph = _c(spheres, _R)
         self.pi = _c(pair_i, np.int32)
         self.pj = _c(pair_j, np.int32)
-        self.world = _c(world, np.float64).reshape(-1, 4)
+        self.world = _c(world, _R).reshape(-1, 4)
         self.margin = float(margin)
         self.mode = int(mode)
 
@@ -276,7 +288,7 @@ def _traj_flat(trajs):
     """List of (T_b, A) arrays or a (B, T, A) array -> flat (W, A) + int64 CSR starts (B + 1)."""
     if isinstance(trajs, np.ndarray) and trajs.ndim == 3:
         trajs = list(trajs)
-    x = _c(np.concatenate([np.asarray(t, np.float64) for t in trajs], 0), np.float64)
+    x = _c(np.concatenate([np.asarray(t, _R) for t in trajs], 0), _R)
     start = np.zeros(len(trajs) + 1, np.int64)
     start[1:] = np.cumsum([len(t) for t in trajs])
     return x, start
@@ -289,9 +301,9 @@ def traj_cost(model: TrajModel, trajs, lower, upper, weights, grad: bool = True,
     """
     x, start = _traj_flat(trajs)
     B = len(start) - 1
-    comps = np.empty((B, 3))
+    comps = _empty((B, 3))
     g = np.empty_like(x) if grad else None
-    lo, hi, w = _c(lower, np.float64), _c(upper, np.float64), _c(weights, np.float64)
+    lo, hi, w = _c(lower, _R), _c(upper, _R), _c(weights, _R)
     lib().or_traj_cost(*model.args(), _p(x, _d), start.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), B,
                        _p(lo, _d), _p(hi, _d), _p(w, _d), _p(comps, _d), None if g is None else _p(g, _d), threads)
     return comps, g
@@ -310,12 +322,12 @@ def traj_lbfgs(model: TrajModel, trajs, lower, upper, *, n_iters=100, m_lbfgs=6,
     x, start = _traj_flat(trajs)
     B = len(start) - 1
     best = np.empty_like(x)
-    cost = np.empty(B)
+    cost = _empty(B)
     last = np.empty_like(x) if iterates else None
-    trace = np.empty((B, max(n_iters, 0), 3)) if iterates else None
+    trace = _empty((B, max(n_iters, 0), 3)) if iterates else None
     cfg = _c([n_iters, m_lbfgs, escalation_interval], np.int32)
-    fcfg = _c([w_smooth, w_vel, w_acc, w_jerk, w_limits, w_collision, w_collision_max, penalty_scale], np.float64)
-    lo, hi = _c(lower, np.float64), _c(upper, np.float64)
+    fcfg = _c([w_smooth, w_vel, w_acc, w_jerk, w_limits, w_collision, w_collision_max, penalty_scale], _R)
+    lo, hi = _c(lower, _R), _c(upper, _R)
     lib().or_traj_lbfgs(*model.args(), _p(x, _d), start.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), B,
                         _p(lo, _d), _p(hi, _d), _p(cfg, _i), _p(fcfg, _d), _p(best, _d), _p(cost, _d),
                         None if last is None else _p(last, _d), None if trace is None else _p(trace, _d), threads)
@@ -326,10 +338,10 @@ def traj_lbfgs(model: TrajModel, trajs, lower, upper, *, n_iters=100, m_lbfgs=6,
 
 def two_loop(g, s_buf, y_buf, rho, m_used: int, newest: int):
     """L-BFGS direction -H g from a masked ring buffer (``(m, n)`` s and y)."""
-    gg = _c(g, np.float64)
-    sb, yb, r = _c(s_buf, np.float64), _c(y_buf, np.float64), _c(rho, np.float64)
+    gg = _c(g, _R)
+    sb, yb, r = _c(s_buf, _R), _c(y_buf, _R), _c(rho, _R)
     m, n = sb.shape
-    out = np.empty(n)
+    out = _empty(n)
     lib().or_two_loop(_p(gg, _d), _p(sb, _d), _p(yb, _d), _p(r, _d), int(m_used), int(newest), m, n, _p(out, _d))
     return out
 
@@ -337,12 +349,12 @@ def two_loop(g, s_buf, y_buf, rho, m_used: int, newest: int):
 def obstacle_dist(centres, radii, spheres=None, capsules=None, boxes=None, halfspaces=None, threads: int = 0):
     """``(N, M)`` FP64 signed distances of spheres ``(N, 3)`` / ``(N,)`` to the packed world
     (spheres ``(Ms, 4)``, capsules ``(Mc, 7)``, boxes ``(Mb, 15)``, half-spaces ``(Mh, 6)``)."""
-    c = _c(centres, np.float64).reshape(-1, 3)
-    r = _c(radii, np.float64).reshape(-1)
-    parts = [_c(np.zeros((0, w)) if a is None else a, np.float64).reshape(-1, w)
+    c = _c(centres, _R).reshape(-1, 3)
+    r = _c(radii, _R).reshape(-1)
+    parts = [_c(np.zeros((0, w)) if a is None else a, _R).reshape(-1, w)
              for a, w in ((spheres, 4), (capsules, 7), (boxes, 15), (halfspaces, 6))]
     M = sum(p.shape[0] for p in parts)
-    out = np.empty((c.shape[0], M))
+    out = _empty((c.shape[0], M))
     args = []
     for p in parts:
         args += [_p(p, _d), p.shape[0]]
@@ -353,13 +365,13 @@ def obstacle_dist(centres, radii, spheres=None, capsules=None, boxes=None, halfs
 def jres(tables: Tables, cfg, target_jnts, target_T, jac: bool = False):
     """JAX residual ``log(T_ee^-1 T_tgt)`` per configuration: ``f (B, n_ee, 6)`` and,
     with ``jac``, the exact Jacobian ``(B, 6 n_ee, A)``.  ``target_T`` is ``(B, n_ee, 7)``."""
-    q = _c(cfg, np.float64).reshape(-1, tables.A)
+    q = _c(cfg, _R).reshape(-1, tables.A)
     B = q.shape[0]
     tj = _c(target_jnts, np.int32).reshape(-1)
     n_ee = tj.shape[0]
-    tT = _c(np.broadcast_to(np.asarray(target_T, np.float64).reshape(-1, n_ee, 7), (B, n_ee, 7)), np.float64)
-    f = np.empty((B, n_ee, 6))
-    J = np.empty((B, 6 * n_ee, tables.A)) if jac else None
+    tT = _c(np.broadcast_to(np.asarray(target_T, _R).reshape(-1, n_ee, 7), (B, n_ee, 7)), _R)
+    f = _empty((B, n_ee, 6))
+    J = _empty((B, 6 * n_ee, tables.A)) if jac else None
     t = tables.ptrs()
     lib().or_jres(_p(q, _d), B, tables.A, *t, tables.J, _p(tj, _i), n_ee, _p(tT, _d), _p(f, _d),
                   None if J is None else _p(J, _d))
@@ -370,17 +382,17 @@ def jlm(tables: Tables, seeds, target_jnts, target_T, lower, upper, fixed_mask=N
         pos_weight=50.0, ori_weight=10.0, lambda_init=5e-3, adaptive=False, prior_weight=0.0, kick_scale=0.0,
         kick_key=0, threads: int = 0):
     """FP64 JAX-convention LM per seed: ``(cfgs (P, S, A), winner errors (P, S))``."""
-    sd = _c(seeds, np.float64)
+    sd = _c(seeds, _R)
     P, S, A = sd.shape
     tj = _c(target_jnts, np.int32).reshape(-1)
     n_ee = tj.shape[0]
-    tT = _c(target_T, np.float64).reshape(P, n_ee, 7)
+    tT = _c(target_T, _R).reshape(P, n_ee, 7)
     fm = _c(np.zeros(A) if fixed_mask is None else fixed_mask, np.int32)
     iopt = _c([max_iter, int(bool(adaptive))], np.int32)
-    dopt = _c([pos_weight, ori_weight, lambda_init, prior_weight, kick_scale], np.float64)
-    out = np.empty((P, S, A))
-    err = np.empty((P, S))
-    lo, hi = _c(lower, np.float64), _c(upper, np.float64)
+    dopt = _c([pos_weight, ori_weight, lambda_init, prior_weight, kick_scale], _R)
+    out = _empty((P, S, A))
+    err = _empty((P, S))
+    lo, hi = _c(lower, _R), _c(upper, _R)
     lib().or_jlm(_p(sd, _d), P, S, A, *tables.ptrs(), tables.J, _p(tj, _i), n_ee, _p(tT, _d), _p(lo, _d),
                  _p(hi, _d), _p(fm, _i), _p(iopt, _i), _p(dopt, _d), ctypes.c_uint64(int(kick_key)), _p(out, _d),
                  _p(err, _d), threads)
@@ -390,17 +402,55 @@ def jlm(tables: Tables, seeds, target_jnts, target_T, lower, upper, fixed_mask=N
 def hjcd_coarse(tables: Tables, seeds, target_jnts, target_T, ee_of_seed, lower, upper, fixed_mask=None, *,
                 k_max=20, epsilon=0.02, nu=np.pi / 2, threads: int = 0):
     """FP64 HJCD coarse phase per seed: ``(cfgs (P, S, A), sum over EEs of |f|^2 (P, S))``."""
-    sd = _c(seeds, np.float64)
+    sd = _c(seeds, _R)
     P, S, A = sd.shape
     tj = _c(target_jnts, np.int32).reshape(-1)
     n_ee = tj.shape[0]
-    tT = _c(target_T, np.float64).reshape(P, n_ee, 7)
+    tT = _c(target_T, _R).reshape(P, n_ee, 7)
     es = _c(ee_of_seed, np.int32).reshape(S)
     fm = _c(np.zeros(A) if fixed_mask is None else fixed_mask, np.int32)
-    out = np.empty((P, S, A))
-    err = np.empty((P, S))
-    lo, hi = _c(lower, np.float64), _c(upper, np.float64)
+    out = _empty((P, S, A))
+    err = _empty((P, S))
+    lo, hi = _c(lower, _R), _c(upper, _R)
     lib().or_hjcd_coarse(_p(sd, _d), P, S, A, *tables.ptrs(), tables.J, _p(tj, _i), n_ee, _p(tT, _d), _p(es, _i),
                          _p(lo, _d), _p(hi, _d), _p(fm, _i), int(k_max), float(epsilon), float(nu), _p(out, _d),
                          _p(err, _d), threads)
     return out, err
+
+
+def capsule_dist(caps, spheres=None, capsules=None, boxes=None, halfspaces=None, threads: int = 0):
+    """``(N, M)`` signed distances of capsules ``(N, 7)`` ``[a, b, r]`` to the packed world."""
+    c = _c(caps, _R).reshape(-1, 7)
+    parts = [_c(np.zeros((0, w)) if a is None else a, _R).reshape(-1, w)
+             for a, w in ((spheres, 4), (capsules, 7), (boxes, 15), (halfspaces, 6))]
+    out = _empty((c.shape[0], sum(p.shape[0] for p in parts)))
+    args = []
+    for p in parts:
+        args += [_p(p, _d), p.shape[0]]
+    lib().or_capsule_dist(_p(c, _d), c.shape[0], *args, _p(out, _d), threads)
+    return out
+
+
+_F32 = None
+
+
+def single_precision():
+    """This module's functions bound to ``_build/liboracle_f32.so``: the same
+    restatement compiled with every ``double`` replaced by ``float`` (and the
+    type-generic math functions), for the CPU baseline's single-precision leg.
+    Arrays in and out are float32."""
+    global _F32
+    if _F32 is None:
+        import importlib.util
+
+        spec = importlib.util.spec_from_file_location("oracle_f32", __file__)
+        m = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(m)
+        m._R = np.float32
+        m._d = ctypes.POINTER(ctypes.c_float)
+        m._D = ctypes.c_float
+        m.LIB = m.LIB_F32
+        m._lib = None
+        m._F32 = m
+        _F32 = m
+    return _F32

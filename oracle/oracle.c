@@ -36,6 +36,7 @@
  *   or_obstacle_dist sphere vs sphere / capsule / box / half-space
  *                    (_collision_cuda_helpers.cuh:14-142, packed rows of
  *                    collision/_cuda_collision.py:98-168)
+ *   or_capsule_dist  capsule vs the same four kinds (_collision_cuda_helpers.cuh:21-142)
  *
  * Every function here is pinned by tests/test_oracle.py against checks that
  * share no code with it (known answers, scipy, finite differences).
@@ -297,6 +298,90 @@ static double obstacle_dist(const double* p, double r, int k, const double* o) {
   return (p[0] - o[3]) * o[0] + (p[1] - o[4]) * o[1] + (p[2] - o[5]) * o[2] - r;
 }
 
+/* Closest points of segments [a0, a1] and [b0, b1] (clamped parameters):
+ * returns the distance of the points. */
+static double seg_seg(const double* a0, const double* a1, const double* b0, const double* b1) {
+  const double d1[3] = {a1[0] - a0[0], a1[1] - a0[1], a1[2] - a0[2]};
+  const double d2[3] = {b1[0] - b0[0], b1[1] - b0[1], b1[2] - b0[2]};
+  const double r[3] = {a0[0] - b0[0], a0[1] - b0[1], a0[2] - b0[2]};
+  const double a = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+  const double e = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+  const double f = d2[0] * r[0] + d2[1] * r[1] + d2[2] * r[2];
+  const double eps = 1e-10;
+  double s = 0.0, t = 0.0;
+  if (a > eps || e > eps) {
+    if (a <= eps) {
+      t = f / e; t = t < 0 ? 0 : (t > 1 ? 1 : t);
+    } else {
+      const double c = d1[0] * r[0] + d1[1] * r[1] + d1[2] * r[2];
+      if (e <= eps) {
+        s = -c / a; s = s < 0 ? 0 : (s > 1 ? 1 : s);
+      } else {
+        const double b = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+        const double den = a * e - b * b;
+        if (fabs(den) > eps) { s = (b * f - c * e) / den; s = s < 0 ? 0 : (s > 1 ? 1 : s); }
+        t = (b * s + f) / e;
+        if (t < 0) { t = 0; s = -c / a; s = s < 0 ? 0 : (s > 1 ? 1 : s); }
+        else if (t > 1) { t = 1; s = (b - c) / a; s = s < 0 ? 0 : (s > 1 ? 1 : s); }
+      }
+    }
+  }
+  double d2s = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double v = a0[k] + s * d1[k] - (b0[k] + t * d2[k]);
+    d2s += v * v;
+  }
+  return sqrt(d2s);
+}
+
+/* Signed distance of capsule c [a, b, r] to one packed obstacle of kind k
+ * (_collision_cuda_helpers.cuh:21-142: the capsule-box distance takes the box
+ * SDF at the segment point closest to the box centre). */
+static double capsule_obstacle_dist(const double* c, int k, const double* o) {
+  if (k == 0) {
+    const double cap[7] = {c[0], c[1], c[2], c[3], c[4], c[5], o[3]};
+    return obstacle_dist(o, c[6], 1, cap);
+  }
+  if (k == 1) return seg_seg(c, c + 3, o, o + 3) - (c[6] + o[6]);
+  if (k == 2) {
+    double al[3], u[3];
+    for (int a = 0; a < 3; ++a) {
+      const double* ax = o + 3 + 3 * a;
+      al[a] = (c[0] - o[0]) * ax[0] + (c[1] - o[1]) * ax[1] + (c[2] - o[2]) * ax[2];
+      u[a] = (c[3] - o[0]) * ax[0] + (c[4] - o[1]) * ax[1] + (c[5] - o[2]) * ax[2] - al[a];
+    }
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double t = 0.0;
+    if (uu > 1e-12) { t = -(al[0] * u[0] + al[1] * u[1] + al[2] * u[2]) / uu; t = t < 0 ? 0 : (t > 1 ? 1 : t); }
+    const double p[3] = {al[0] + t * u[0], al[1] + t * u[1], al[2] + t * u[2]};
+    /* box SDF of the box-frame point p: an axis-aligned box at the origin */
+    const double box[15] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1, o[12], o[13], o[14]};
+    return obstacle_dist(p, c[6], 2, box);
+  }
+  const double d1 = (c[0] - o[3]) * o[0] + (c[1] - o[4]) * o[1] + (c[2] - o[5]) * o[2];
+  const double d2 = (c[3] - o[3]) * o[0] + (c[4] - o[4]) * o[1] + (c[5] - o[5]) * o[2];
+  return (d1 < d2 ? d1 : d2) - c[6];
+}
+
+/* out (N, M): capsules (N, 7) against the packed world (order as below). */
+void or_capsule_dist(const double* caps, int64_t N, const double* ws, int ms, const double* wc, int mc,
+                     const double* wb, int mb, const double* wh, int mh, double* out, int threads) {
+  const int M = ms + mc + mb + mh;
+  set_threads(threads);
+#pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < N; ++n) {
+    for (int m = 0; m < M; ++m) {
+      int k = 0, i = m;
+      const double* o;
+      if (i < ms) { o = ws + 4 * i; }
+      else if ((i -= ms) < mc) { k = 1; o = wc + 7 * i; }
+      else if ((i -= mc) < mb) { k = 2; o = wb + 15 * i; }
+      else { i -= mb; k = 3; o = wh + 6 * i; }
+      out[n * M + m] = capsule_obstacle_dist(caps + 7 * n, k, o);
+    }
+  }
+}
+
 /* out (N, M) with M = Ms + Mc + Mb + Mh obstacles in that order; sphere n is
  * centres[3n..3n+2], radius radii[n]; r < 0 is padding and reads 1e9. */
 void or_obstacle_dist(const double* centres, const double* radii, int64_t N, const double* ws, int ms,
@@ -337,6 +422,9 @@ typedef struct {
   int M;
   int coll;
   double wc, margin;
+  /* capsules (mc, 7), boxes (mb, 15), half-spaces (mh, 6); may be NULL/0 */
+  const float *wcap, *wbox, *whs;
+  int mc, mb, mh;
 } ikp_t;
 
 static void residual_ee(const se3* T, const double* tg, double* r) {
@@ -368,6 +456,16 @@ static double penalty(const ikp_t* p, const se3* Tw) {
       const double d = sdist(w, p->rs[4 * i + 3], o, p->ws[4 * m + 3], 0);
       if (d < p->margin) pen += (d - p->margin) * (d - p->margin);
     }
+    /* the other obstacle kinds (_ls_ik_cuda_kernel.cu:195-224) */
+    const int counts[3] = {p->mc, p->mb, p->mh}, width[3] = {7, 15, 6};
+    const float* rows[3] = {p->wcap, p->wbox, p->whs};
+    for (int k = 0; k < 3; ++k)
+      for (int m = 0; m < counts[k]; ++m) {
+        double o[15];
+        for (int c = 0; c < width[k]; ++c) o[c] = rows[k][width[k] * m + c];
+        const double d = obstacle_dist(w, p->rs[4 * i + 3], k + 1, o);
+        if (d < p->margin) pen += (d - p->margin) * (d - p->margin);
+      }
   }
   return p->wc * pen;
 }
@@ -559,6 +657,8 @@ static void solve_seed(const ikp_t* p, const double* seed, const double* lo, con
   *out_err = best_err;
 }
 
+/* world_other = {capsules (mc, 7), boxes (mb, 15), half-spaces (mh, 6)} as
+ * float arrays (NULL when the count is 0), counts = {mc, mb, mh}. */
 void or_ls_ik(const double* seeds, int P, int S, int A, const float* twists, const float* parent_tf,
               const int32_t* parent_idx, const int32_t* act_idx, const float* mimic_mul,
               const float* mimic_off, const int32_t* mimic_act_idx, const int32_t* topo_inv, int J,
@@ -567,7 +667,8 @@ void or_ls_ik(const double* seeds, int P, int S, int A, const float* twists, con
               const double* lower, const double* upper, const int32_t* fixed_mask, int max_iter,
               double pos_weight, double ori_weight, double lambda_init, double eps_pos, double eps_ori,
               int enable_collision, double collision_weight, double collision_margin, double* out_cfgs,
-              double* out_err, int threads) {
+              double* out_err, int threads, const float* w_caps, const float* w_boxes, const float* w_hs,
+              const int32_t* other_counts) {
   tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
   const double W[6] = {pos_weight, pos_weight, pos_weight, ori_weight, ori_weight, ori_weight};
   set_threads(threads);
@@ -575,7 +676,9 @@ void or_ls_ik(const double* seeds, int P, int S, int A, const float* twists, con
   for (int pi = 0; pi < P; ++pi)
     for (int si = 0; si < S; ++si) {
       ikp_t p = {&t, n_ee, target_jnts, ancestor_masks, target_T + (int64_t)pi * n_ee * 7, robot_spheres,
-                 sphere_joint, K, world, M, enable_collision, collision_weight, collision_margin};
+                 sphere_joint, K, world, M, enable_collision, collision_weight, collision_margin,
+                 w_caps, w_boxes, w_hs, other_counts ? other_counts[0] : 0, other_counts ? other_counts[1] : 0,
+                 other_counts ? other_counts[2] : 0};
       const int64_t g = (int64_t)pi * S + si;
       solve_seed(&p, seeds + g * A, lower, upper, fixed_mask, max_iter, W, lambda_init, eps_pos, eps_ori,
                  out_cfgs + g * A, out_err + g);
@@ -588,7 +691,8 @@ void or_res_jac(const double* q, int A, const float* twists, const float* parent
                 const int32_t* mimic_act_idx, const int32_t* topo_inv, int J, const int32_t* target_jnts,
                 const int32_t* ancestor_masks, int n_ee, const double* target_T, double* out_r, double* out_J) {
   tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
-  ikp_t p = {&t, n_ee, target_jnts, ancestor_masks, target_T, NULL, NULL, 0, NULL, 0, 0, 0.0, 0.0};
+  ikp_t p = {&t, n_ee, target_jnts, ancestor_masks, target_T, NULL, NULL, 0, NULL, 0, 0, 0.0, 0.0,
+             NULL, NULL, NULL, 0, 0, 0};
   se3 Tw[OR_MAXJ];
   fk_one(&t, q, Tw);
   for (int e = 0; e < n_ee; ++e) residual_ee(&Tw[target_jnts[e]], target_T + 7 * e, out_r + 6 * e);

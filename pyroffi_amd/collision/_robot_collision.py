@@ -239,10 +239,24 @@ class RobotCollisionSpherized:
 
     def compute_world_collision_distance(self, robot, cfg, world_geom, semantics: str | None = None):
         """``(*batch, N, M)``: per link, min over its spheres of the sphere-sphere distance."""
-        from ._hip_collision import world_sphere_array
+        from ._hip_collision import world_arrays, world_sphere_array
 
+        sem = self._semantics(semantics)
+        arrs = world_arrays(world_geom)
+        if sum(v.shape[0] for k, v in arrs.items() if k != "spheres"):
+            if sem == "jax":
+                raise NotImplementedError("JAX-path distances cover sphere worlds; pass semantics='cuda' for "
+                                          "capsule, box or half-space obstacles")
+            from .._robot import _to_cuda
+            from ..kernels.ops import sphere_world_links
+
+            self._check_robot(robot)
+            x, back = _to_cuda(cfg)
+            poses = robot.forward_kinematics(x.reshape(-1, x.shape[-1]))
+            ow, _ = sphere_world_links(poses, self.device_tables(poses.device)["spheres_padded"], arrs)
+            return back(ow.reshape(*x.shape[:-1], self.num_links, ow.shape[-1]))
         ws = torch.as_tensor(world_sphere_array(world_geom), device="cuda")
-        if self._semantics(semantics) == "jax":
+        if sem == "jax":
             batch, back, ow, _ = self._jax_distances(robot, cfg, ws, world=True, self_pairs=False)
         else:
             batch, back, ow, _, _ = self._sweep(robot, cfg, ws, world=True)
@@ -256,3 +270,125 @@ class RobotCollisionSpherized:
         else:
             batch, back, _, _, osf = self._sweep(robot, cfg, None, self_pairs=True)
         return back(osf.reshape(*batch, self.num_pairs))
+
+
+class RobotCollision:
+    """Capsule robot model: one capsule per link plus the active self-collision pairs
+    (``src/pyroffi/collision/_robot_collision.py:25-427``).
+
+    Distances are those of the reference CUDA capsule kernels
+    (``_collision_cuda_kernel.cu:63-161, 203-407``), which the reference checker
+    returns for this model: per link and obstacle, the capsule-sphere /
+    capsule-capsule / capsule-box / capsule-half-space distance; per active pair,
+    the capsule-capsule distance.  The capsules are placed by
+    ``pk_capsules_place_f32`` and measured by ``pk_collision_world_capsule_f32`` /
+    ``pk_collision_self_capsule_f32``.
+    """
+
+    def __init__(self, num_links, link_names, coll, active_idx_i, active_idx_j):
+        from ._geometry import Capsule
+
+        if not isinstance(coll, Capsule) or coll.get_batch_axes() != (num_links,):
+            raise ValueError(f"coll must be a Capsule with batch axes ({num_links},)")
+        self.num_links = num_links
+        self.link_names = tuple(link_names)
+        self.coll = coll
+        self.active_idx_i = np.asarray(active_idx_i, dtype=np.int32)
+        self.active_idx_j = np.asarray(active_idx_j, dtype=np.int32)
+        self._dev_cache: dict[str, dict[str, torch.Tensor]] = {}
+
+    @property
+    def num_pairs(self) -> int:
+        return int(self.active_idx_i.shape[0])
+
+    @staticmethod
+    def from_urdf(urdf, user_ignore_pairs=(), ignore_immediate_adjacents=True):
+        """One capsule per link bounding its URDF sphere collisions
+        (``Capsule.from_spheres``); the pairs as ``RobotCollision._compute_active_pair_indices``
+        (``:100-156``: lower triangle minus diagonal, adjacent and user pairs)."""
+        from ._geometry import Capsule
+
+        if not isinstance(urdf, URDF):
+            urdf = URDF.load(urdf)
+        _, link_info = RobotURDFParser.parse(urdf)
+        names = link_info.names
+        caps = []
+        for name in names:
+            link = urdf.link_map[name]
+            if link.n_non_sphere_collisions:
+                logger.warning("Unsupported collision geometry type for link %r.", name)
+            sph = np.array([[*s.origin[:3, 3], s.radius] for s in link.spheres], dtype=np.float64).reshape(-1, 4)
+            caps.append(Capsule.from_spheres(sph[:, :3], sph[:, 3]))
+        wxyz_xyz = np.stack([c.pose.wxyz_xyz for c in caps])
+        coll = Capsule(SE3(wxyz_xyz), np.array([float(c.radius) for c in caps]),
+                       np.array([float(c.height) for c in caps]))
+        ai, aj = RobotCollisionSpherized._compute_active_pair_indices(
+            names, urdf, user_ignore_pairs, ignore_immediate_adjacents
+        )
+        return RobotCollision(len(names), names, coll, ai, aj)
+
+    def device_tables(self, device) -> dict[str, torch.Tensor]:
+        key = str(device)
+        if key not in self._dev_cache:
+            self._dev_cache[key] = {
+                "capsules_local": torch.as_tensor(self.coll.as_array(), device=device).contiguous(),
+                "pair_i": torch.as_tensor(self.active_idx_i, device=device).contiguous(),
+                "pair_j": torch.as_tensor(self.active_idx_j, device=device).contiguous(),
+            }
+        return self._dev_cache[key]
+
+    def _check_robot(self, robot) -> None:
+        assert self.link_names == robot.links.names, "Link name mismatch between RobotCollision and Robot kinematics."
+
+    def _placed(self, robot, cfg, *, soa: bool, aos: bool):
+        from .._robot import _to_cuda
+        from ..kernels.ops import capsules_place
+
+        self._check_robot(robot)
+        x, back = _to_cuda(cfg)
+        batch = tuple(x.shape[:-1])
+        poses = robot.forward_kinematics(x.reshape(-1, x.shape[-1]))
+        t = self.device_tables(poses.device)
+        c_soa, c_aos = capsules_place(poses, t["capsules_local"], soa=soa, aos=aos)
+        return batch, back, c_soa, c_aos
+
+    def placed_capsules(self, robot, cfg) -> torch.Tensor:
+        """``(*batch, N, 7)`` world capsules ``[a, b, r]`` on the GPU."""
+        batch, _, _, c = self._placed(robot, cfg, soa=False, aos=True)
+        return c.reshape(*batch, self.num_links, 7)
+
+    def at_config(self, robot, cfg):
+        """World-frame capsules, batch axes ``(*batch, N)`` (``:323-345``)."""
+        from ._geometry import Capsule
+
+        c = self.placed_capsules(robot, cfg).cpu().numpy().astype(np.float64)
+        a, b, r = c[..., :3], c[..., 3:6], c[..., 6]
+        d = b - a
+        h = np.linalg.norm(d, axis=-1)
+        z = np.where(h[..., None] > 0, d / np.maximum(h, 1e-300)[..., None], np.array([0.0, 0.0, 1.0]))
+        from ._geometry import _frame_with_z
+
+        m = np.zeros(c.shape[:-1] + (4, 4))
+        m[..., :3, :3] = _frame_with_z(z)
+        m[..., :3, 3] = 0.5 * (a + b)
+        m[..., 3, 3] = 1.0
+        return Capsule(SE3.from_matrix(m), r, h)
+
+    def compute_self_collision_distance(self, robot, cfg):
+        """``(*batch, P)``: capsule-capsule distance of every active pair (``:347-386``)."""
+        from ..kernels.ops import collision_self_capsule
+
+        batch, back, _, c = self._placed(robot, cfg, soa=False, aos=True)
+        t = self.device_tables(c.device)
+        out = collision_self_capsule(c, t["pair_i"], t["pair_j"])
+        return back(out.reshape(*batch, self.num_pairs))
+
+    def compute_world_collision_distance(self, robot, cfg, world_geom):
+        """``(*batch, N, M)``: every link capsule against every obstacle (``:388-427``)."""
+        from ..kernels.ops import collision_world_capsule
+        from ._hip_collision import world_arrays
+
+        w = world_arrays(world_geom)
+        batch, back, c, _ = self._placed(robot, cfg, soa=True, aos=False)
+        out = collision_world_capsule(c, w["spheres"], w["capsules"], w["boxes"], w["halfspaces"])
+        return back(out.reshape(*batch, self.num_links, out.shape[-1]))

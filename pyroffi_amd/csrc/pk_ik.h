@@ -29,6 +29,7 @@
 // with an FP32 Cholesky in registers.
 #pragma once
 
+#include "pk_geom.h"
 #include "pk_grid.h"
 #include "pk_model.h"
 
@@ -64,6 +65,14 @@ struct IkArgs {
   int n_robot_spheres;
   const float* world;
   int n_world;
+  // Non-sphere obstacles (packed rows of pk_geom.h), tested against every
+  // robot sphere (no grid).
+  const float* world_caps;
+  int n_caps;
+  const float* world_boxes;
+  int n_boxes;
+  const float* world_hs;
+  int n_hs;
   const float* lower;
   const float* upper;
   const int* fixed_mask;
@@ -79,6 +88,9 @@ struct IkArgs {
   float* state;                   // (total_seeds, 2 * n_act + 3) or null
   int iter_begin, iter_end;
   GridArgs grid;
+  // When non-null, the batched kernel adds the number of sphere-pair tests
+  // its collision penalty executed (per seed, summed over its iterations).
+  unsigned long long* work;
 };
 
 // Shared-memory image of one problem, built by the prologue.
@@ -303,12 +315,17 @@ __device__ __forceinline__ float world_penalty_all(float cx, float cy, float cz,
 // (M >= 1 here) and add nothing.
 template <int U>
 __device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz, float r,
-                                                    const float4* ws, int M, const GridArgs& g) {
+                                                    const float4* ws, int M, const GridArgs& g,
+                                                    unsigned& work) {
   const int level = grid_level(g, r);
-  if (level < 0) return world_penalty_all(cx, cy, cz, r, ws, M);
+  if (level < 0) {
+    work += M;
+    return world_penalty_all(cx, cy, cz, r, ws, M);
+  }
   const int* items;
   int n;
   grid_cell_list(g, level, cx, cy, cz, items, n);
+  work += n;
   float pen = 0.f;
   for (int i = 0; i < n; i += U) {
     int m[U];
@@ -329,11 +346,41 @@ __device__ __forceinline__ float world_penalty_grid(float cx, float cy, float cz
   return pen;
 }
 
-// Collision merit penalty (unweighted) over every robot sphere.
+// Capsules, boxes and half-spaces of the world, staged in LDS: every robot
+// sphere tests all of them, sum of min(d - margin, 0)^2.
+struct OtherWorld {
+  const float* caps;   // (nc, 7)
+  const float* boxes;  // (nb, 15)
+  const float* hs;     // (nh, 6)
+  int nc, nb, nh;
+  float margin;
+};
+
+__device__ __forceinline__ float other_penalty(float cx, float cy, float cz, float r, const OtherWorld& ow) {
+  float pen = 0.f;
+  for (int m = 0; m < ow.nc; ++m) {
+    const float e = fminf(dist_sphere_capsule(cx, cy, cz, r, ow.caps + 7 * m) - ow.margin, 0.f);
+    pen = fmaf(e, e, pen);
+  }
+  for (int m = 0; m < ow.nb; ++m) {
+    const float e = fminf(dist_sphere_box(cx, cy, cz, r, ow.boxes + 15 * m) - ow.margin, 0.f);
+    pen = fmaf(e, e, pen);
+  }
+  for (int m = 0; m < ow.nh; ++m) {
+    const float e = fminf(dist_sphere_halfspace(cx, cy, cz, r, ow.hs + 6 * m) - ow.margin, 0.f);
+    pen = fmaf(e, e, pen);
+  }
+  return pen;
+}
+
+// Collision merit penalty (unweighted) over every robot sphere; `work`
+// counts the sphere-pair tests it executes (grid candidates, or the whole
+// world for a sphere no grid level serves).
 template <int MAXA, int NEE>
 __device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, const Pose* P,
                                                 const float4* sph, const float4* ws, int M,
-                                                const GridArgs& grid) {
+                                                const GridArgs& grid, const OtherWorld& ow, unsigned& work) {
+  const int n_other = ow.nc + ow.nb + ow.nh;
   float pen = 0.f;
 #pragma unroll
   for (int g = 0; g <= MAXA; ++g) {
@@ -351,8 +398,13 @@ __device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, c
           // wider instantiations, whose registers are already at the limit
           // (measured on the 14-DOF robot: two are 18% faster than one, four
           // 1% slower than two).
-          pen += (grid.n > 0) ? world_penalty_grid<(MAXA <= 8 ? 4 : 2)>(cx, cy, cz, s.w, ws, M, grid)
+          pen += (grid.n > 0) ? world_penalty_grid<(MAXA <= 8 ? 4 : 2)>(cx, cy, cz, s.w, ws, M, grid, work)
                               : world_penalty_all(cx, cy, cz, s.w, ws, M);
+          if (grid.n == 0) work += M;
+          if (n_other > 0) {
+            pen += other_penalty(cx, cy, cz, s.w, ow);
+            work += n_other;
+          }
         }
       }
     }
@@ -361,13 +413,14 @@ __device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, c
 }
 
 struct IkLds {
-  size_t recs, slot_of, anchor, rel, sph_tmp, sph, ws, total;
+  size_t recs, slot_of, anchor, rel, sph_tmp, sph, ws, other, total;
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~static_cast<size_t>(15); }
 
 // Byte offsets of the IK kernels' LDS regions (float4 regions 16-byte aligned).
-__host__ __device__ inline IkLds ik_lds_layout(size_t shared_bytes, int J, int K, int M) {
+// n_other: floats of the staged capsules, boxes and half-spaces (7, 15, 6 each).
+__host__ __device__ inline IkLds ik_lds_layout(size_t shared_bytes, int J, int K, int M, int n_other = 0) {
   IkLds o;
   o.recs = align16(shared_bytes);
   o.slot_of = align16(o.recs + sizeof(JointRec) * J);
@@ -376,8 +429,13 @@ __host__ __device__ inline IkLds ik_lds_layout(size_t shared_bytes, int J, int K
   o.sph_tmp = o.rel + sizeof(float) * 7 * J;
   o.sph = align16(o.sph_tmp + sizeof(int) * K);
   o.ws = o.sph + sizeof(float4) * K;
-  o.total = o.ws + sizeof(float4) * M;
+  o.other = o.ws + sizeof(float4) * M;
+  o.total = o.other + sizeof(float) * n_other;
   return o;
+}
+
+__host__ __device__ inline int ik_other_floats(const IkArgs& a) {
+  return a.enable_collision ? 7 * a.n_caps + 15 * a.n_boxes + 6 * a.n_hs : 0;
 }
 
 constexpr float kDampFloor = 1e-6f;
@@ -406,6 +464,7 @@ struct LmState {
   float q[MAXA], best_q[MAXA];
   float best_err, lam;
   int done;
+  unsigned work;  // sphere-pair tests of the collision penalty (see IkArgs::work)
 };
 
 // Problem tables staged in LDS by the workgroup prologue.
@@ -417,18 +476,19 @@ struct IkView {
   int M;
   bool coll;
   float wc;
+  OtherWorld ow;
 };
 
 template <int MAXA, int NEE>
 __device__ __forceinline__ float weighted_error(const IkView<MAXA, NEE>& v, const float (&tg)[NEE][7],
                                                 const float* W, const float* q, Pose* P, float* r,
-                                                Pose* Tee, const GridArgs& grid) {
+                                                Pose* Tee, const GridArgs& grid, unsigned& work) {
   node_fk(*v.sh, q, P);
   residuals(*v.sh, P, tg, r, Tee);
   float e = 0.f;
 #pragma unroll
   for (int i = 0; i < 6 * NEE; ++i) { const float x = r[i] * W[i % 6]; e += x * x; }
-  if (v.coll) e += v.wc * sphere_penalty(*v.sh, P, v.sph, v.ws, v.M, grid);
+  if (v.coll) e += v.wc * sphere_penalty(*v.sh, P, v.sph, v.ws, v.M, grid, v.ow, work);
   return e;
 }
 
@@ -463,7 +523,13 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
     float fw[6 * NEE];
 #pragma unroll
     for (int i = 0; i < 6 * NEE; ++i) { fw[i] = r[i] * W[i % 6]; cur += fw[i] * fw[i]; }
-    if (v.coll) cur += v.wc * sphere_penalty(sh, P, v.sph, v.ws, v.M, grid);
+    if (v.coll) {
+      // With G lanes per seed every lane computes the same value; the work
+      // count takes one lane's.
+      unsigned wk = 0;
+      cur += v.wc * sphere_penalty(sh, P, v.sph, v.ws, v.M, grid, v.ow, wk);
+      if (G == 1 || threadIdx.x % G == 0) st.work += wk;
+    }
 
     // Weighted geometric Jacobian, column a from the node(s) driven by a.
     float Jm[6 * NEE][MAXA];
@@ -621,7 +687,7 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
         float qt[MAXA];
 #pragma unroll
         for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + ls_alpha(ai) * delta[i]));
-        const float et = weighted_error(v, tg, W, qt, P, r, Tee, grid);
+        const float et = weighted_error(v, tg, W, qt, P, r, Tee, grid, st.work);
         if (et < best_a_err) { best_a_err = et; best_a = ai; }
       }
     } else {
@@ -630,7 +696,9 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
       float qt[MAXA];
 #pragma unroll
       for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + ls_alpha(ai) * delta[i]));
-      const float et = weighted_error(v, tg, W, qt, P, r, Tee, grid);
+      unsigned wk = 0;
+      const float et = weighted_error(v, tg, W, qt, P, r, Tee, grid, wk);
+      if (threadIdx.x % G < 5) st.work += wk;  // lanes 5..7 repeat step 4
       // Sequential semantics: only errors below 1e30 replace the start value
       // (NaN never does); the first index wins ties.
       float e = (et < 1e30f) ? et : INFINITY;
@@ -665,7 +733,7 @@ __device__ __forceinline__ IkView<MAXA, NEE> ik_prologue(const IkArgs& a, char* 
   using Sh = IkShared<MAXA, NEE>;
   const int J = a.tables.n_joints;
   const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
-  const IkLds off = ik_lds_layout(sizeof(Sh), J, Kin, a.enable_collision ? a.n_world : 0);
+  const IkLds off = ik_lds_layout(sizeof(Sh), J, Kin, a.enable_collision ? a.n_world : 0, ik_other_floats(a));
   Sh& sh = *reinterpret_cast<Sh*>(base);
   JointRec* recs = reinterpret_cast<JointRec*>(base + off.recs);
   int* slot_of = reinterpret_cast<int*>(base + off.slot_of);
@@ -688,13 +756,25 @@ __device__ __forceinline__ IkView<MAXA, NEE> ik_prologue(const IkArgs& a, char* 
     const float* o = a.world + 4 * m;
     ws[m] = make_float4(o[0], o[1], o[2], o[3] + a.collision_margin);
   }
+  float* oth = reinterpret_cast<float*>(base + off.other);
+  const int n_oth = (Kin > 0) ? ik_other_floats(a) : 0;
+  const int nc7 = 7 * a.n_caps, nb15 = 15 * a.n_boxes;
+  for (int i = lane; i < n_oth; i += kWave)
+    oth[i] = (i < nc7) ? a.world_caps[i] : (i < nc7 + nb15 ? a.world_boxes[i - nc7] : a.world_hs[i - nc7 - nb15]);
   __syncthreads();
   IkView<MAXA, NEE> v;
   v.sh = &sh;
   v.sph = sph;
   v.ws = ws;
   v.M = M;
-  v.coll = M > 0 && sh.n_sph > 0;
+  v.ow.caps = oth;
+  v.ow.boxes = oth + nc7;
+  v.ow.hs = oth + nc7 + nb15;
+  v.ow.nc = n_oth > 0 ? a.n_caps : 0;
+  v.ow.nb = n_oth > 0 ? a.n_boxes : 0;
+  v.ow.nh = n_oth > 0 ? a.n_hs : 0;
+  v.ow.margin = a.collision_margin;
+  v.coll = (M > 0 || n_oth > 0) && sh.n_sph > 0;
   v.wc = a.collision_weight;
   return v;
 }
@@ -738,7 +818,8 @@ __global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
   Pose P[MAXA];
   Pose Tee[NEE];
   float r[6 * NEE];
-  st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid);
+  st.work = 0;
+  st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid, st.work);
   st.lam = a.lambda_init;
   st.done = 0;
   lm_iterate<MAXA, NEE, G>(v, tg, W, a, st, 0, a.max_iter, a.grid);
@@ -792,7 +873,8 @@ __global__ __launch_bounds__(64) void ls_ik_batched_kernel(IkArgs a) {
       Pose P[MAXA];
       Pose Tee[NEE];
       float r[6 * NEE];
-      st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid);
+      st.work = 0;
+      st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid, st.work);
       st.lam = a.lambda_init;
       st.done = 0;
     } else {
@@ -805,8 +887,10 @@ __global__ __launch_bounds__(64) void ls_ik_batched_kernel(IkArgs a) {
       st.best_err = sp[2 * A];
       st.lam = sp[2 * A + 1];
       st.done = sp[2 * A + 2] != 0.f;
+      st.work = 0;
     }
     lm_iterate(v, tg, W, a, st, a.iter_begin, a.iter_end, a.grid);
+    if (a.work != nullptr) atomicAdd(a.work, static_cast<unsigned long long>(st.work));
     if (a.state != nullptr && a.iter_end < a.max_iter) {
       float* sp = a.state + s * stride;
 #pragma unroll
@@ -830,7 +914,7 @@ inline int launch_ik(void* stream, const IkArgs& a, bool batched, int grid_waves
   const int J = a.tables.n_joints;
   const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
   const int M = a.enable_collision ? a.n_world : 0;
-  const size_t lds = ik_lds_layout(sizeof(Sh), J, Kin, M).total;
+  const size_t lds = ik_lds_layout(sizeof(Sh), J, Kin, M, ik_other_floats(a)).total;
   PK_REQUIRE(lds <= 150 * 1024, "IK tables need %zu bytes of LDS (limit 150 KiB)", lds);
   if (batched) {
     const long long n_tiles = (a.total_seeds + kWave - 1) / kWave;

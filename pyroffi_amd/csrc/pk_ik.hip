@@ -73,8 +73,9 @@ int fill_ik_args(pk::IkArgs& a, const float* seeds, int32_t n_problems, int32_t 
                  const int32_t* target_jnts, const int32_t* ancestor_masks, int32_t n_ee,
                  const float* target_T, const float* robot_spheres_local,
                  const int32_t* robot_sphere_joint_idx, int32_t n_robot_spheres,
-                 const float* world_spheres, int32_t n_world_spheres, int32_t n_world_capsules,
-                 int32_t n_world_boxes, int32_t n_world_halfspaces, const float* lower,
+                 const float* world_spheres, int32_t n_world_spheres, const float* world_capsules,
+                 int32_t n_world_capsules, const float* world_boxes, int32_t n_world_boxes,
+                 const float* world_halfspaces, int32_t n_world_halfspaces, const float* lower,
                  const float* upper, const int32_t* fixed_mask, int32_t max_iter, float pos_weight,
                  float ori_weight, float lambda_init, float eps_pos, float eps_ori,
                  int32_t enable_collision, float collision_weight, float collision_margin,
@@ -86,9 +87,14 @@ int fill_ik_args(pk::IkArgs& a, const float* seeds, int32_t n_problems, int32_t 
   PK_REQUIRE(n_ee <= pk::kMaxEE, "n_ee=%d exceeds %d", n_ee, pk::kMaxEE);
   PK_REQUIRE(max_iter >= 0, "max_iter must be >= 0");
   const bool coll = enable_collision != 0 && n_robot_spheres > 0;
+  PK_REQUIRE(n_world_capsules >= 0 && n_world_boxes >= 0 && n_world_halfspaces >= 0 &&
+                 7 * n_world_capsules + 15 * n_world_boxes + 6 * n_world_halfspaces <= 8192,
+             "non-sphere obstacles take more than 8192 floats (%d capsules, %d boxes, %d half-spaces)",
+             n_world_capsules, n_world_boxes, n_world_halfspaces);
   if (coll)
-    PK_REQUIRE(n_world_capsules == 0 && n_world_boxes == 0 && n_world_halfspaces == 0,
-               "only sphere worlds are supported in the LM collision penalty");
+    PK_REQUIRE((n_world_capsules == 0 || world_capsules) && (n_world_boxes == 0 || world_boxes) &&
+                   (n_world_halfspaces == 0 || world_halfspaces),
+               "null world array");
   PK_REQUIRE(n_robot_spheres >= 0 && n_robot_spheres <= 4096, "n_robot_spheres=%d outside [0, 4096]",
              n_robot_spheres);
   PK_REQUIRE(n_world_spheres >= 0 && n_world_spheres <= 4096, "n_world_spheres=%d outside [0, 4096]",
@@ -112,6 +118,12 @@ int fill_ik_args(pk::IkArgs& a, const float* seeds, int32_t n_problems, int32_t 
   a.n_robot_spheres = coll ? n_robot_spheres : 0;
   a.world = world_spheres;
   a.n_world = coll ? n_world_spheres : 0;
+  a.world_caps = world_capsules;
+  a.n_caps = coll ? n_world_capsules : 0;
+  a.world_boxes = world_boxes;
+  a.n_boxes = coll ? n_world_boxes : 0;
+  a.world_hs = world_halfspaces;
+  a.n_hs = coll ? n_world_halfspaces : 0;
   a.lower = lower;
   a.upper = upper;
   a.fixed_mask = fixed_mask;
@@ -148,7 +160,6 @@ extern "C" int pk_ls_ik_f32(
     float ori_weight, float lambda_init, float eps_pos, float eps_ori,
     int32_t enable_collision, float collision_weight, float collision_margin,
     float* out_cfgs, float* out_err) {
-  (void)world_capsules; (void)world_boxes; (void)world_halfspaces;
   PK_REQUIRE(n_seeds >= 0, "negative size");
   PK_REQUIRE(n_problems <= 65535, "n_problems=%d exceeds the 65535 grid limit", n_problems);
   if (n_problems == 0 || n_seeds == 0) return PK_OK;
@@ -157,7 +168,8 @@ extern "C" int pk_ls_ik_f32(
                               mimic_mul, mimic_off, mimic_act_idx, topo_inv, n_joints, target_jnts,
                               ancestor_masks, n_ee, target_T, robot_spheres_local,
                               robot_sphere_joint_idx, n_robot_spheres, world_spheres,
-                              n_world_spheres, n_world_capsules, n_world_boxes, n_world_halfspaces,
+                              n_world_spheres, world_capsules, n_world_capsules, world_boxes,
+                              n_world_boxes, world_halfspaces, n_world_halfspaces,
                               lower, upper, fixed_mask, max_iter, pos_weight, ori_weight,
                               lambda_init, eps_pos, eps_ori, enable_collision, collision_weight,
                               collision_margin, out_cfgs, out_err);
@@ -178,7 +190,8 @@ extern "C" int pk_ls_ik_batched_f32(
     const float* upper, const int32_t* fixed_mask, int32_t max_iter, int32_t iter_begin,
     int32_t iter_end, float* state, float pos_weight, float ori_weight, float lambda_init,
     float eps_pos, float eps_ori, int32_t enable_collision, float collision_weight,
-    float collision_margin, int32_t grid_waves, float* out_cfgs, float* out_err) {
+    float collision_margin, int32_t grid_waves, float* out_cfgs, float* out_err,
+    uint64_t* work_counter) {
   PK_REQUIRE(total_seeds >= 0, "negative size");
   if (n_problems == 0 || total_seeds == 0) return PK_OK;
   PK_REQUIRE(problem_seed_start != nullptr, "null problem_seed_start");
@@ -191,7 +204,8 @@ extern "C" int pk_ls_ik_batched_f32(
                               mimic_mul, mimic_off, mimic_act_idx, topo_inv, n_joints, target_jnts,
                               ancestor_masks, n_ee, target_T, robot_spheres_local,
                               robot_sphere_joint_idx, n_robot_spheres, world_spheres,
-                              n_world_spheres, 0, 0, 0, lower, upper, fixed_mask, max_iter,
+                              n_world_spheres, nullptr, 0, nullptr, 0, nullptr, 0, lower, upper,
+                              fixed_mask, max_iter,
                               pos_weight, ori_weight, lambda_init, eps_pos, eps_ori,
                               enable_collision, collision_weight, collision_margin, out_cfgs,
                               out_err);
@@ -201,6 +215,7 @@ extern "C" int pk_ls_ik_batched_f32(
   a.state = state;
   a.iter_begin = iter_begin;
   a.iter_end = iter_end;
+  a.work = reinterpret_cast<unsigned long long*>(work_counter);
   a.grid.n = 0;
   if (a.enable_collision) {
     const int gs = pk::fill_grid_args(grids, n_grids, collision_margin, a.grid);

@@ -101,6 +101,14 @@ _SIGNATURES = {
     ),
     "pk_collision_self_sphere_f32": (_i32, [_p, _p, _p, _i64, _i32, _i32, _p, _p, _i32, _p]),
     "pk_sphere_distances_jax_f32": (_i32, [_p, _p, _i64, _i32, _p, _i32, _p, _i32, _p, _p, _i32, _p, _p]),
+    "pk_collision_world_capsule_f32": (_i32, [_p, _p, _i64, _i32, _p, _i32, _p, _i32, _p, _i32, _p, _i32, _p]),
+    "pk_collision_self_capsule_f32": (_i32, [_p, _p, _i64, _i32, _p, _p, _i32, _p]),
+    "pk_capsules_place_f32": (_i32, [_p, _p, _i64, _i32, _p, _p, _p]),
+    "pk_row_min_f32": (_i32, [_p, _p, _i64, _i64, _p]),
+    "pk_sphere_world_links_f32": (
+        _i32,
+        [_p, _p, _i64, _i32, _p, _i32, _p, _i32, _p, _i32, _p, _i32, _p, _i32, _p, _p],
+    ),
     "pk_sphere_sweep_f32": (
         _i32,
         [_p, _p, _i64, _i32, _p, _p, _i32, _p, _i32, _p, _i32, _i32, _p, _p, _p],
@@ -145,7 +153,7 @@ _SIGNATURES = {
         + [_p, _p, _p]  # limits
         + [_i32, _i32, _i32, _p]  # max_iter, iter range, state
         + [_f32, _f32, _f32, _f32, _f32, _i32, _f32, _f32, _i32]  # attrs, grid_waves
-        + [_p, _p],
+        + [_p, _p, _p],  # outputs, work counter
     ),
     "pk_traj_cost_grad_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _p, _p, _p]),
     "pk_traj_lbfgs_state_floats": (_i64, [_i64, _i32, _i32, _i32]),

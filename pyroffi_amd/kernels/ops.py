@@ -167,6 +167,83 @@ def collision_world_sphere(sphere_centers, sphere_radii, world_spheres, world_ca
     return out
 
 
+def row_min(dist) -> torch.Tensor:
+    """``(B,)`` minimum over everything after the first axis of ``dist`` (1e9 when empty)."""
+    d = _dev(dist, torch.float32)
+    B = d.shape[0]
+    row = d[0].numel() if B else 0
+    out = torch.empty((B,), dtype=torch.float32, device=d.device)
+    check(lib().pk_row_min_f32(_stream(), _ptr(d), B, row, _ptr(out)), "pk_row_min_f32")
+    return out
+
+
+def capsules_place(link_poses, capsules_local, *, soa: bool = True, aos: bool = True):
+    """Capsule robot at link poses ``(B, N, 7)``: ``(caps [7, B, N] | None, caps [B, N, 7] | None)``
+    from the link-frame ``(N, 7)`` ``[a, b, r]`` table."""
+    lp = _dev(link_poses, torch.float32)
+    dev = lp.device
+    B, N, _ = lp.shape
+    cl = _dev(capsules_local, torch.float32, dev).reshape(N, 7)
+    o_soa = torch.empty((7, B, N), dtype=torch.float32, device=dev) if soa else None
+    o_aos = torch.empty((B, N, 7), dtype=torch.float32, device=dev) if aos else None
+    st = lib().pk_capsules_place_f32(_stream(), _ptr(lp), B, N, _ptr(cl), _ptr(o_soa), _ptr(o_aos))
+    check(st, "pk_capsules_place_f32")
+    return o_soa, o_aos
+
+
+def collision_world_capsule(caps, world_spheres, world_capsules, world_boxes, world_halfspaces) -> torch.Tensor:
+    """``[B, N, M]`` capsule-robot world distances from component-major capsules ``[7, B, N]``."""
+    c = _dev(caps, torch.float32)
+    dev = c.device
+    ws = _dev(world_spheres, torch.float32, dev).reshape(-1, 4)
+    wc = _dev(world_capsules, torch.float32, dev).reshape(-1, 7)
+    wb = _dev(world_boxes, torch.float32, dev).reshape(-1, 15)
+    wh = _dev(world_halfspaces, torch.float32, dev).reshape(-1, 6)
+    _, B, N = c.shape
+    M = ws.shape[0] + wc.shape[0] + wb.shape[0] + wh.shape[0]
+    out = torch.empty((B, N, M), dtype=torch.float32, device=dev)
+    st = lib().pk_collision_world_capsule_f32(_stream(), _ptr(c), B, N, _ptr(ws), ws.shape[0], _ptr(wc), wc.shape[0],
+                                              _ptr(wb), wb.shape[0], _ptr(wh), wh.shape[0], _ptr(out))
+    check(st, "pk_collision_world_capsule_f32")
+    return out
+
+
+def collision_self_capsule(caps, pair_i, pair_j) -> torch.Tensor:
+    """``[B, P]`` capsule-robot self distances from AoS capsules ``[B, N, 7]``."""
+    c = _dev(caps, torch.float32)
+    dev = c.device
+    pi, pj = _dev(pair_i, torch.int32, dev), _dev(pair_j, torch.int32, dev)
+    B, N, _ = c.shape
+    out = torch.empty((B, pi.shape[0]), dtype=torch.float32, device=dev)
+    st = lib().pk_collision_self_capsule_f32(_stream(), _ptr(c), B, N, _ptr(pi), _ptr(pj), pi.shape[0], _ptr(out))
+    check(st, "pk_collision_self_capsule_f32")
+    return out
+
+
+def sphere_world_links(link_poses, spheres_padded, world: dict, *, per_link: bool = True):
+    """World distances of a spherized robot over the four obstacle kinds with the CUDA
+    kernels' semantics: ``(out (B, L, M), out_min (B,))`` from link poses ``(B, L, 7)``,
+    the padded ``(L, S, 4)`` link-frame sphere table and ``world`` = packed arrays by kind
+    (``spheres (Ms,4)``, ``capsules (Mc,7)``, ``boxes (Mb,15)``, ``halfspaces (Mh,6)``)."""
+    lp = _dev(link_poses, torch.float32)
+    dev = lp.device
+    B, L, _ = lp.shape
+    sp = _dev(spheres_padded, torch.float32, dev).reshape(L, -1, 4)
+    w = {k: _dev(world[k], torch.float32, dev).reshape(-1, n)
+         for k, n in (("spheres", 4), ("capsules", 7), ("boxes", 15), ("halfspaces", 6))}
+    M = sum(v.shape[0] for v in w.values())
+    out = torch.empty((B, L, M), dtype=torch.float32, device=dev)
+    out_min = torch.empty((B,), dtype=torch.float32, device=dev)
+    st = lib().pk_sphere_world_links_f32(
+        _stream(), _ptr(lp), B, L, _ptr(sp), sp.shape[1],
+        _ptr(w["spheres"]), w["spheres"].shape[0], _ptr(w["capsules"]), w["capsules"].shape[0],
+        _ptr(w["boxes"]), w["boxes"].shape[0], _ptr(w["halfspaces"]), w["halfspaces"].shape[0],
+        _ptr(out), _ptr(out_min),
+    )
+    check(st, "pk_sphere_world_links_f32")
+    return out, out_min
+
+
 def sphere_distances_jax(link_poses, spheres_padded, world_spheres, pair_i, pair_j, *, world: bool = True,
                          self_pairs: bool = True):
     """The JAX path's conventions: ``(out_world (B, L, M) | None, out_self (B, P) | None)``
@@ -449,10 +526,12 @@ def ls_ik_batched(seeds, problem_seed_start, tables, target_jnts, ancestor_masks
                   grid: dict | None = None, pos_weight: float = 50.0, ori_weight: float = 10.0,
                   lambda_init: float = 5e-3, eps_pos: float = 1e-8, eps_ori: float = 1e-8,
                   enable_collision: bool = False, collision_weight: float = 1e4,
-                  collision_margin: float = 0.02, grid_waves: int = 0, out_cfgs=None, out_err=None):
+                  collision_margin: float = 0.02, grid_waves: int = 0, out_cfgs=None, out_err=None, work=None):
     """Uneven batched LM over flat seeds ``(S_total, A)``; ``tables`` = the 8 robot buffers.
 
     Writes into (or allocates) ``out_cfgs (S_total, A)`` and ``out_err (S_total,)``.
+    ``work`` (a one-element int64 device tensor, or None) accumulates the
+    number of sphere-pair tests the collision penalty executed.
     """
     sd = _dev(seeds, torch.float32)
     dev = sd.device
@@ -482,7 +561,7 @@ def ls_ik_batched(seeds, problem_seed_start, tables, target_jnts, ancestor_masks
         grids[0], grids[1], _ptr(lo), _ptr(hi), _ptr(fm), int(max_iter),
         int(iter_begin), int(iter_end), _ptr(state), float(pos_weight), float(ori_weight), float(lambda_init),
         float(eps_pos), float(eps_ori), int(bool(enable_collision)), float(collision_weight),
-        float(collision_margin), int(grid_waves), _ptr(out_cfgs), _ptr(out_err),
+        float(collision_margin), int(grid_waves), _ptr(out_cfgs), _ptr(out_err), _ptr(work),
     )
     check(st, "pk_ls_ik_batched_f32")
     return out_cfgs, out_err

@@ -78,12 +78,15 @@ def ee_tables(robot, target_link_indices) -> tuple[np.ndarray, np.ndarray]:
 
 
 def prepare_ls_collision_buffers(robot, collision_checker: Any | None, world_geom: Any | None):
-    """Robot spheres in joint frames + the sphere world (``_ls_ik.py:77-186``).
+    """Robot spheres in joint frames + the world (``_ls_ik.py:77-186``).
 
     Base-link spheres (no parent joint) and padding are dropped, exactly as in
-    the reference.  Returns ``(spheres (K, 4), joint_idx (K,), world (M, 4), enabled)``.
+    the reference.  Returns ``(spheres (K, 4), joint_idx (K,), world, enabled)``
+    where ``world`` is the ``(M, 4)`` sphere array for a sphere-only world and
+    otherwise the packed arrays by kind (``spheres``, ``capsules``, ``boxes``,
+    ``halfspaces``; ``_cuda_collision.py:98-168``).
     """
-    from ..collision._hip_collision import world_sphere_array
+    from ..collision._hip_collision import world_arrays
 
     empty = (np.zeros((0, 4), np.float32), np.zeros((0,), np.int32), np.zeros((0, 4), np.float32), False)
     if collision_checker is None or world_geom is None:
@@ -108,7 +111,9 @@ def prepare_ls_collision_buffers(robot, collision_checker: Any | None, world_geo
             jidx.append(j)
     if not rows:
         return empty
-    world = world_sphere_array(world_geom)
+    arrs = world_arrays(world_geom)
+    mixed = any(v.shape[0] for k, v in arrs.items() if k != "spheres")
+    world = arrs if mixed else arrs["spheres"]
     return np.asarray(rows, np.float32), np.asarray(jidx, np.int32), world, True
 
 
@@ -131,7 +136,8 @@ def ls_ik_run(robot, seeds, target_T, target_link_indices, *, max_iter=60, pos_w
               collision=None):
     """Run the LM kernel on explicit seeds ``(P, S, A)`` and targets ``(P, n_ee, 7)``.
 
-    ``collision`` is ``(spheres, joint_idx, world, weight, margin)`` or None.
+    ``collision`` is ``(spheres, joint_idx, world, weight, margin)`` or None; ``world``
+    is an ``(M, 4)`` sphere array or packed arrays by kind.
     Returns device tensors ``(cfgs (P, S, A), errs (P, S))``.
     """
     from ..kernels.ops import ls_ik_hip
@@ -153,13 +159,14 @@ def ls_ik_run(robot, seeds, target_T, target_link_indices, *, max_iter=60, pos_w
         enable = False
     rs = np.ascontiguousarray(rs, np.float32)
     rj = np.ascontiguousarray(rj, np.int32)
-    ws = np.ascontiguousarray(ws, np.float32)
-    ckey = ("ls_ik_coll", hashlib.sha1(rs.tobytes() + b"|" + rj.tobytes() + b"|" + ws.tobytes()).digest(),
-            rs.shape, rj.shape, ws.shape)
+    kinds = (("spheres", 4), ("capsules", 7), ("boxes", 15), ("halfspaces", 6))
+    wd = ws if isinstance(ws, dict) else {"spheres": ws}
+    warr = [np.ascontiguousarray(wd.get(k, np.zeros((0, n))), np.float32).reshape(-1, n) for k, n in kinds]
+    ckey = ("ls_ik_coll", hashlib.sha1(b"|".join([rs.tobytes(), rj.tobytes()] + [a.tobytes() for a in warr])).digest(),
+            rs.shape, rj.shape, *(a.shape for a in warr))
     rs_d, rj_d, ws_d, wc_d, wb_d, wh_d = t.derived(
         ckey, lambda: (torch.as_tensor(rs, device=dev), torch.as_tensor(rj, device=dev),
-                       torch.as_tensor(ws, device=dev),
-                       *(torch.zeros((0, n), dtype=torch.float32, device=dev) for n in (7, 15, 6))))
+                       *(torch.as_tensor(a, device=dev) for a in warr)))
     seeds_t = seeds if isinstance(seeds, torch.Tensor) else torch.as_tensor(np.asarray(seeds, np.float32))
     seeds_t = seeds_t.to(dev, torch.float32).contiguous()
     tT = target_T if isinstance(target_T, torch.Tensor) else torch.as_tensor(np.asarray(target_T, np.float32))

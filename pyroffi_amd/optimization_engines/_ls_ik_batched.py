@@ -119,7 +119,7 @@ class UnevenLsIkSolver:
         g.device = d
 
     # ---------------------------------------------------------------- running
-    def _run_group(self, g: RobotGroup, grid_waves: int, chunk: int) -> None:
+    def _run_group(self, g: RobotGroup, grid_waves: int, chunk: int, work=None) -> None:
         d = g.device
         kw = self.kw
         max_iter = kw["max_iter"]
@@ -132,7 +132,7 @@ class UnevenLsIkSolver:
                 pos_weight=kw["pos_weight"], ori_weight=kw["ori_weight"], lambda_init=kw["lambda_init"],
                 eps_pos=kw["eps_pos"], eps_ori=kw["eps_ori"], enable_collision=g.collision is not None,
                 collision_weight=kw["collision_weight"], collision_margin=kw["collision_margin"],
-                grid_waves=grid_waves, out_cfgs=d["out_cfgs"], out_err=d["out_err"],
+                grid_waves=grid_waves, out_cfgs=d["out_cfgs"], out_err=d["out_err"], work=work,
             )
         ls_ik_select_segments(d["out_cfgs"], d["out_err"], d["seed_start"], out_best=d["best"],
                               out_idx=d["best_idx"])
@@ -214,6 +214,16 @@ class UnevenLsIkSolver:
             self.run_eager()
         self.graph = g
 
+    def count_work(self) -> list[int]:
+        """Per group, the sphere-pair tests its collision penalty executes in one
+        solve (one eager run with the kernel's work counter)."""
+        out = []
+        for g in self.groups:
+            w = torch.zeros(1, dtype=torch.int64, device=g.device["seeds"].device)
+            self._run_group(g, *g.tune, work=w)
+            out.append(int(w.item()))
+        return out
+
     def solve(self) -> list[tuple[torch.Tensor, torch.Tensor]]:
         """Run (replay the graph if captured); per group ``(winners (P, A), winner index (P,))``."""
         if self.graph is not None:
@@ -252,6 +262,41 @@ class UnevenLsIkSolver:
         return outs
 
 
+    def solve_batch_padded(self, n_problems: int | None = None) -> list[torch.Tensor]:
+        """The reference's batch call shape: per robot, one ``pk_ls_ik_f32`` launch
+        over its first ``n_problems`` problems (all by default) with every problem's
+        seeds padded to the largest count, brute-force collision penalty, then one
+        selection -- ``ls_ik_solve_cuda_batch`` (``_ls_ik.py:1002-1172``) takes a
+        single seed count for all problems.  Padding rows repeat the problem's
+        first seed, which ties with it and never wins (first index on ties), so
+        the winners are those of the problem's own seeds.  Returns per group the
+        winners ``(n, A)``.
+        """
+        kw = self.kw
+        outs = []
+        for g in self.groups:
+            d = g.device
+            A = g.seeds.shape[1]
+            starts = np.asarray(g.seed_start, np.int64)
+            n = len(starts) - 1 if n_problems is None else min(int(n_problems), len(starts) - 1)
+            counts = np.diff(starts[:n + 1])
+            smax = int(counts.max())
+            rows = np.concatenate([np.r_[np.arange(starts[p], starts[p + 1]), np.full(smax - counts[p], starts[p])]
+                                   for p in range(n)])
+            idx = torch.as_tensor(rows, device=d["seeds"].device)
+            seeds = d["seeds"][idx].reshape(n, smax, A)
+            no = torch.zeros((0, 1), device=d["seeds"].device)
+            cfgs, errs = ls_ik_hip(
+                seeds, *d["tables"], d["tj"], d["masks"], d["targets"][:n], d["rs"], d["rj"], d["ws"], no, no, no,
+                d["lower"], d["upper"], d["fixed"], max_iter=kw["max_iter"], pos_weight=kw["pos_weight"],
+                ori_weight=kw["ori_weight"], lambda_init=kw["lambda_init"], eps_pos=kw["eps_pos"],
+                eps_ori=kw["eps_ori"], enable_collision=g.collision is not None,
+                collision_weight=kw["collision_weight"], collision_margin=kw["collision_margin"])
+            best, _ = ls_ik_select(cfgs, errs, seeds[:, 0])
+            outs.append(best)
+        return outs
+
+
 def log_uniform_seed_counts(rng: np.random.Generator, n: int, lo: int = 16, hi: int = 256) -> np.ndarray:
     """Seeds per problem: floor(exp(U(ln lo, ln hi)))."""
     return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), n))).astype(np.int64)
@@ -284,6 +329,9 @@ def make_group(robot, target_link_indices, targets: np.ndarray, seed_counts: np.
     coll = None
     if collision_checker is not None and world is not None:
         rs, rj, ws, ok = prepare_ls_collision_buffers(robot, collision_checker, world)
+        if ok and isinstance(ws, dict):
+            raise NotImplementedError("the uneven batch solver's penalty grid covers sphere worlds; use "
+                                      "ls_ik_solve_cuda for capsule, box or half-space obstacles")
         if ok:
             coll = (rs, rj, ws)
     return RobotGroup(robot=robot, target_link_indices=tuple(target_link_indices),

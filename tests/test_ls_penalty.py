@@ -85,3 +85,42 @@ def test_penalty_matches_brute_force(oracle_mod, name):
     assert np.count_nonzero(expected) >= len(q) // 2  # most configurations touch the world
     # the oracle takes the float32 tables the kernels take: ~1e-6 relative from table rounding
     np.testing.assert_allclose((e_on - e_off).ravel(), expected, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("kinds", [(0, 40, 0, 0), (0, 0, 40, 0), (0, 0, 0, 2), (10, 10, 10, 1)])
+def test_penalty_other_obstacle_kinds(oracle_mod, kinds):
+    """The penalty over capsules, boxes and half-spaces (``_ls_ik_cuda_kernel.cu:166-229``
+    loops over all four kinds): error with the penalty minus error without equals
+    w * sum of max(0, margin - d)^2 with d from the oracle's obstacle distances on
+    sphere centres placed by ``independent``."""
+    import worlds
+
+    robot, path = load_robot("panda")
+    coll = RobotCollisionSpherized.from_urdf(path)
+    ws, wc, wb, wh = worlds.random_world(np.random.default_rng(sum(kinds)), *kinds, extent=0.6)
+    rs, rj, _, _ = prepare_ls_collision_buffers(robot, coll, _world(0))
+    t = oracle_mod.Tables.of(robot)
+    ee = robot.links.names.index("panda_hand")
+    tj, masks = ee_tables(robot, (ee,))
+    q = random_cfgs(robot, 12, seed=7)
+    tgt = oracle_mod.fk_links(t, random_cfgs(robot, 2, seed=8))[:, ee][:, None]
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    w, margin = 1e4, 0.05
+    _, e_off = oracle_mod.ls_ik(t, q.reshape(2, 6, -1), tj, masks, tgt, lo, hi, max_iter=0)
+    _, e_on = oracle_mod.ls_ik(t, q.reshape(2, 6, -1), tj, masks, tgt, lo, hi, max_iter=0, robot_spheres=rs,
+                               sphere_joint=rj, world=ws, world_capsules=wc, world_boxes=wb, world_halfspaces=wh,
+                               enable_collision=True, collision_weight=w, collision_margin=margin)
+    links, joints, _ = independent.read(path)
+    children = {j["child"] for j in joints}
+    expected = np.zeros(len(q))
+    for b in range(len(q)):
+        _, per_link = independent.world_spheres(path, q[b])
+        for l in links:
+            s = per_link[l]
+            if l not in children or not len(s):
+                continue
+            d = oracle_mod.obstacle_dist(s[:, :3], s[:, 3], ws, wc, wb, wh)
+            expected[b] += np.sum(np.maximum(0.0, margin - d) ** 2)
+    expected *= w
+    assert np.count_nonzero(expected) >= 1
+    np.testing.assert_allclose((e_on - e_off).ravel(), expected, rtol=1e-5, atol=1e-6)

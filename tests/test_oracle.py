@@ -362,3 +362,38 @@ def test_obstacle_distances_independent(oracle_mod):
     box = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1, 0.5, 0.2, 0.3]], np.float64)
     d = oracle_mod.obstacle_dist(np.array([[0.1, 0.05, -0.2]]), np.array([0.01]), boxes=box)
     assert d[0, 0] == pytest.approx(-0.1 - 0.01, abs=1e-12)
+
+
+def test_capsule_distances_independent(oracle_mod):
+    """Capsule robots: sphere and capsule obstacles against the minimum over
+    points sampled along the segments; boxes against the clamped-point
+    distance at the segment point nearest the box centre (found by sampling);
+    half-spaces against the sampled minimum plane offset."""
+    import worlds
+    from scipy.spatial import distance
+
+    rng = np.random.default_rng(14)
+    ws, wc, wb, wh = (x.astype(np.float64) for x in worlds.random_world(rng, 5, 5, 6, 3))
+    a = rng.uniform(-1, 1, (12, 3))
+    caps = np.concatenate([a, a + rng.normal(0, 0.3, (12, 3)), rng.uniform(0.01, 0.1, (12, 1))], 1)
+    caps[3, 3:6] = caps[3, :3]  # a degenerate (point) capsule
+    got = oracle_mod.capsule_dist(caps, ws, wc, wb, wh)
+    t = np.linspace(0, 1, 2001)
+    for n, c in enumerate(caps):
+        pts = c[None, :3] + t[:, None] * (c[3:6] - c[:3])[None]
+        ref_s = distance.cdist(pts, ws[:, :3]).min(0) - c[6] - ws[:, 3]
+        np.testing.assert_allclose(got[n, :5], ref_s, atol=1e-6)
+        for m, o in enumerate(wc):
+            q = o[None, :3] + t[:, None] * (o[3:6] - o[:3])[None]
+            ref = distance.cdist(pts, q).min() - c[6] - o[6]
+            np.testing.assert_allclose(got[n, 5 + m], ref, atol=1e-6)
+        for m, box in enumerate(wb):
+            R = np.stack([box[3:6], box[6:9], box[9:12]], 1)
+            loc = (pts - box[:3]) @ R
+            p = loc[np.argmin(np.linalg.norm(loc, axis=1))]
+            h = box[12:15]
+            inside = np.all(np.abs(p) <= h)
+            ref = (-np.min(h - np.abs(p)) if inside else np.linalg.norm(p - np.clip(p, -h, h))) - c[6]
+            np.testing.assert_allclose(got[n, 10 + m], ref, atol=2e-3)
+        for m, hs in enumerate(wh):
+            np.testing.assert_allclose(got[n, 16 + m], ((pts - hs[3:]) @ hs[:3]).min() - c[6], atol=1e-12)
