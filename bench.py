@@ -248,19 +248,19 @@ def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32,
     padded_dt = time.perf_counter() - t0
     res = _uneven_report(groups, samples, winners, fused_dt, n_seeds, report, per_robot, plain_per_robot,
                          cpu_per_robot)
-    res.update(_uneven_roofline(groups, work, fused_dt, solver.kw["max_iter"]))
+    res.update(_uneven_roofline(groups, work, fused_dt, solver.kw["max_iter"], list(UNEVEN_EE)))
     res.update(_padded_report(groups, winners, padded, padded_dt, batch_per_robot, per_robot,
                               res["fused_solves_per_s"]))
     return res
 
 
-def _uneven_roofline(groups, work, fused_dt, max_iter):
+def _uneven_roofline(groups, work, fused_dt, max_iter, names):
     """Work-based FLOP/s of the fused solve: per seed max_iter LM iterations at the
     Panda count scaled by DOF and end effectors, 1 + 6 max_iter penalty evaluations
     of robot-sphere placements, and the sphere-pair tests the kernel counted."""
     flop = 0.0
     per_robot = {}
-    for g, w in zip(groups, work):
+    for name, g, w in zip(names, groups, work):
         S = g.seeds.shape[0]
         A = g.robot.joints.num_actuated_joints
         K = 0 if g.collision is None else len(g.collision[0])
@@ -268,7 +268,7 @@ def _uneven_roofline(groups, work, fused_dt, max_iter):
         place = S * (1 + 6 * max_iter) * K * IK_FLOP_PLACEMENT
         pairs = w * IK_FLOP_PAIR
         flop += base + place + pairs
-        per_robot[g.robot.links.names[0]] = {"seeds": S, "pair_tests": w, "pair_tests_per_seed_eval":
+        per_robot[name] = {"seeds": S, "pair_tests": w, "pair_tests_per_seed_eval":
                                              w / max(1, S * (1 + 6 * max_iter) * max(K, 1))}
     achieved = flop / fused_dt / 1e12
     return {"roofline": {"bound": "valu", "kernel": "ls_ik_batched_kernel (pk_ls_ik_batched_f32), all robots",
@@ -853,16 +853,23 @@ def main():
             "unit": "GB/s",
         },
     }
+    _progress(f"sweep {g['ms_per_step']:.4f} ms/step")
     if not args.no_cpu_baseline:
         cb, _ = cpu_baseline(args.cpu_sample)
         line["cpu_baseline"] = cb
+        _progress("cpu baseline done")
     if not args.no_ik:
-        line["ik"] = ik_bench(reps=10)
-        line["grad"] = grad_bench(reps=10, cpu_sample=256)
-        line["ik_uneven"] = ik_uneven_bench(reps=5)
-        line["hjcd"] = hjcd_bench(reps=3)
-        line["trajopt"] = trajopt_bench(reps=5)
+        for key, fn in (("ik", lambda: ik_bench(reps=10)), ("grad", lambda: grad_bench(reps=10, cpu_sample=256)),
+                        ("ik_uneven", lambda: ik_uneven_bench(reps=5)), ("hjcd", lambda: hjcd_bench(reps=3)),
+                        ("trajopt", lambda: trajopt_bench(reps=5))):
+            line[key] = fn()
+            _progress(f"{key} done")
     print(json.dumps(line), flush=True)
+
+
+def _progress(msg: str) -> None:
+    """One line on stderr per finished leg (the JSON result stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

@@ -170,7 +170,8 @@ def test_coarse_first_guard_mixed_world(oracle_mod):
     coarse = RobotCollision.from_urdf(path)
     chk = CUDARobotCollisionChecker(fine, coarse_inner=coarse)
     q = random_cfgs(robot, 16, seed=9).astype(np.float32)
-    far = [HalfSpace.from_point_and_normal(np.array([[0.0, 0.0, -3.0]]), np.array([[0.0, 0.0, -1.0]])),
+    # a half-space is solid on the side opposite its normal
+    far = [HalfSpace.from_point_and_normal(np.array([[0.0, 0.0, -3.0]]), np.array([[0.0, 0.0, 1.0]])),
            Box.from_center_and_half_lengths(np.array([[5.0, 5.0, 5.0]]), np.array([[0.1, 0.1, 0.1]]))]
     out = chk.compute_world_collision_distance(robot, q, far)
     np.testing.assert_array_equal(out, np.ones((16, fine.num_links, 2), np.float32))
