@@ -644,6 +644,26 @@ int pk_traj_lbfgs_f32(void* stream, const pk_traj_model* model, const float* x_i
                       int32_t n_subset, const pk_lbfgs_config* cfg, int32_t iter_begin,
                       int32_t iter_end, float* state, float* out_best, float* out_best_cost);
 
+/*
+ * pk_traj_lbfgs_f32 without a workspace, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The solver state
+ * lives in a static device array of 4 Mi floats (16 MiB) that each GPU the
+ * library runs on holds until the process ends: a solve needs
+ * pk_traj_lbfgs_state_floats(n_waypoints, n_act, n_traj, m_hist) <= 4,194,304
+ * floats (about 40,000 waypoints of a 7-joint robot at m_hist = 6), larger
+ * ones are rejected with PK_ERR_INVALID before anything runs.  The call
+ * allocates nothing.  Every call on one GPU shares that GPU's array and leaves
+ * its contents there: a solve split over calls (iter_begin > 0) resumes from
+ * it, so no other solve may run on that GPU in between, and calls that
+ * overlap on one GPU (two streams or host threads) race.
+ */
+int pk_traj_lbfgs_noalloc_f32(void* stream, const pk_traj_model* model, const float* x_init,
+                              const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
+                              int32_t max_len, int32_t block_threads, const int32_t* traj_subset,
+                              int32_t n_subset, const pk_lbfgs_config* cfg, int32_t iter_begin,
+                              int32_t iter_end, float* out_best, float* out_best_cost);
+
 #ifdef __cplusplus
 }
 #endif

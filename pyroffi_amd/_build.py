@@ -16,7 +16,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libpyroffi_amd.so")
 SOURCES = ("pk_common.cpp", "pk_fk.hip", "pk_collision.hip", "pk_distances.hip", "pk_ik.hip", "pk_ik_jax.hip",
-           "pk_grad.hip", "pk_grid.hip", "pk_traj.hip")
+           "pk_grad.hip", "pk_grid.hip", "pk_traj.hip", "pk_traj_noalloc.hip")
 # pk_ik_inst.hip is compiled once per (padded DOF, padded EE count) pair.
 IK_INSTANCES = tuple((m, e) for m in (8, 16) for e in (1, 2, 4))
 

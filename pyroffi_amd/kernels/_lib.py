@@ -158,6 +158,7 @@ _SIGNATURES = {
     "pk_traj_cost_grad_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _p, _p, _p]),
     "pk_traj_lbfgs_state_floats": (_i64, [_i64, _i32, _i32, _i32]),
     "pk_traj_lbfgs_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _p, _i32, _i32, _p, _p, _p]),
+    "pk_traj_lbfgs_noalloc_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _p, _i32, _i32, _p, _p]),
     "pk_ls_ik_select_segments_f32": (_i32, [_p, _p, _p, _p, _p, _i32, _i32, _f32, _p, _p]),
     "pk_pose_hinge_cost_f32": (
         _i32,
