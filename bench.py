@@ -124,14 +124,14 @@ def gpu_bench(steps: int, warmup: int):
     return dict(ms_per_step=ms_per_step, fk_ms=fk_ms, sweep_ms=sweep_ms, results=out)
 
 
-def ik_bench(reps: int):
-    """Config 1 IK: Panda, 16 FK-generated targets x 64 uniform seeds, 30 iterations, lambda 1e-2."""
+def ik_workload():
+    """Config 1 IK: Panda, 16 FK-generated targets x 64 uniform seeds, 30 iterations, lambda 1e-2.
+    Returns ``(robot, link, targets (16, 7), seeds, kw, solve)``."""
     import torch
 
-    import oracle
     import pyroffi_amd as pk
     from pyroffi_amd.kernels.ops import ls_ik_select
-    from pyroffi_amd.optimization_engines._ls_ik import ee_tables, ls_ik_run
+    from pyroffi_amd.optimization_engines._ls_ik import ls_ik_run
 
     robot = pk.Robot.from_urdf(pk.bundled_urdf_path("panda"))
     link = robot.links.names.index("panda_hand")
@@ -148,6 +148,18 @@ def ik_bench(reps: int):
         cfgs, errs = ls_ik_run(robot, seeds, tT, (link,), **kw)
         return ls_ik_select(cfgs, errs, prev, 0.0)[0], cfgs, errs
 
+    return robot, link, tgt, seeds, kw, solve
+
+
+def ik_bench(reps: int):
+    """Config 1 IK (``ik_workload``): GPU time per call, success, FP64/FP32 oracle on the same seeds."""
+    import torch
+
+    import oracle
+    from pyroffi_amd.optimization_engines._ls_ik import ee_tables
+
+    robot, link, tgt, seeds, kw, solve = ik_workload()
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
     solve()
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
@@ -752,6 +764,10 @@ def profile_workload(name: str):
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)  # reads 256 MiB, writes 256 MiB
+    elif name == "ik":
+        *_, solve = ik_workload()
+        for _ in range(20):
+            solve()
     elif name == "ik_uneven":
         from pyroffi_amd.optimization_engines._ls_ik_batched import UnevenLsIkSolver
 
@@ -790,10 +806,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=B)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ik", action="store_true")
-    ap.add_argument("--profile", nargs="?", const="sweep", choices=("sweep", "grad", "ik_uneven", "trajopt"),
+    ap.add_argument("--profile", nargs="?", const="sweep", choices=("sweep", "grad", "ik", "ik_uneven", "trajopt"),
                     help="run for rocprofv3: 'sweep' = the headline workload's 20 + 200 steps and 200 timed "
                          "launches of each kernel, plus a 256 MiB device copy (the byte-count calibration for the HBM counters); 'grad' = 20 launches "
-                         "of the config-3 adjoint kernel plus the same copy; 'ik_uneven' / 'trajopt' = 3 solves "
+                         "of the config-3 adjoint kernel plus the same copy; 'ik' = 20 config-1 solve calls; "
+                         "'ik_uneven' / 'trajopt' = 3 solves "
                          "of the uneven LM-IK / trajectory workload")
     args = ap.parse_args()
     if args.profile == "sweep":

@@ -644,6 +644,67 @@ int pk_traj_lbfgs_f32(void* stream, const pk_traj_model* model, const float* x_i
                       int32_t n_subset, const pk_lbfgs_config* cfg, int32_t iter_begin,
                       int32_t iter_end, float* state, float* out_best, float* out_best_cost);
 
+/* ChompTrajOptConfig (src/pyroffi/optimization_engines/_chomp_optimization.py:31-90). */
+typedef struct pk_chomp_config {
+  int32_t n_iters, use_covariant_update, n_cg_iters, early_stop_patience;
+  float step_size, w_smooth, w_acc, w_jerk, w_collision, w_collision_max, collision_penalty_scale;
+  float w_limits, smoothness_reg, cg_tol, grad_clip_norm, max_delta_per_step, min_cost_improve;
+} pk_chomp_config;
+
+/*
+ * CHOMP trajectory optimisation of many trajectories of different lengths
+ * (x_init / traj_start as for pk_traj_lbfgs_f32; endpoints stay where x_init
+ * has them), one workgroup per trajectory, all iterations in one launch:
+ * _optimize_single_traj of src/pyroffi/optimization_engines/_chomp_optimization.py:232-330
+ * (clipped gradient, covariant direction by truncated CG per joint, six
+ * clamped trials, early stop, collision-weight continuation).  The model's
+ * mode must be 0 (hinge^2) for the reference's cost.  out_traj (W, A) = final
+ * iterates, out_cost (B) = their cost at w_collision_max (_eval_cost,
+ * :333-348); out_trace (B, n_iters, 3) or NULL = per iteration the current
+ * cost, the picked trial (0-5) and whether it was accepted (-1 once stopped).
+ */
+int pk_traj_chomp_f32(void* stream, const pk_traj_model* model, const float* x_init,
+                      const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints, int32_t max_len,
+                      int32_t block_threads, const pk_chomp_config* cfg, float* out_traj,
+                      float* out_cost, float* out_trace);
+
+/* ScoTrajOptConfig (src/pyroffi/optimization_engines/_sco_optimization.py:49-110). */
+typedef struct pk_sco_config {
+  int32_t n_outer_iters, n_inner_iters, m_lbfgs;
+  float w_smooth, w_vel, w_acc, w_jerk, w_collision, w_collision_max, penalty_scale;
+  float collision_margin, w_trust, w_limits, smooth_min_temperature;
+} pk_sco_config;
+
+/*
+ * SCO's linearisation (_collision_dists_reduced + _compute_coll_dists_and_jacs,
+ * src/pyroffi/optimization_engines/_sco_optimization.py:224-288) of every
+ * waypoint of x (layout as pk_traj_lbfgs_f32): per collision group the smooth
+ * minimum -T logsumexp(-d / T) of its JAX distances and its gradient.  Group
+ * 0 = the model's active self pairs, group 1 + g = every (link, world sphere)
+ * pair of world spheres [world_group_start[g], world_group_start[g + 1]) (the
+ * reference's one group per world geometry; at most 7).  An empty group reads
+ * +inf with a zero row.  out_d (W, 1 + n_world_groups), out_J (W, 1 +
+ * n_world_groups, n_act).  The model's margin, mode and grids are not used.
+ */
+int pk_traj_sco_linearize_f32(void* stream, const pk_traj_model* model, const float* x,
+                              const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints, int32_t max_len,
+                              int32_t block_threads, const int32_t* world_group_start, int32_t n_world_groups,
+                              float temperature, float* out_d, float* out_J);
+
+/*
+ * SCO's inner problem (_lbfgs_inner_solve, _sco_optimization.py:291-418) for
+ * every trajectory from its linearisation point x: n_inner_iters L-BFGS steps
+ * on smoothness + limits + w_collision * linearised hinge^2 (d, J from
+ * pk_traj_sco_linearize_f32, n_groups = 1 + n_world_groups) + w_trust |x -
+ * x_lin|^2, endpoints fixed; out_x (W, A) = each trajectory's best inner
+ * iterate.  One workgroup per trajectory with every vector in LDS:
+ * 4 (9 + 2 m) max_len n_act bytes plus the linearisation must fit 150 KiB.
+ */
+int pk_traj_sco_inner_f32(void* stream, const float* x, const int64_t* traj_start, int32_t n_traj,
+                          int64_t n_waypoints, int32_t max_len, int32_t n_act, int32_t n_groups, const float* d,
+                          const float* J, const float* lower, const float* upper, const pk_sco_config* cfg,
+                          float w_collision, float* out_x);
+
 /*
  * pk_traj_lbfgs_f32 without a workspace, for callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops, harnesses that

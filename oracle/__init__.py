@@ -68,6 +68,16 @@ def lib() -> ctypes.CDLL:
         h.or_traj_lbfgs.argtypes = traj_model + [_d, ctypes.POINTER(ctypes.c_int64), _L, _d, _d, _i, _d, _d, _d,
                                                  _d, _d, _I]
         h.or_two_loop.argtypes = [_d, _d, _d, _d, _I, _I, _I, _I, _d]
+        h.or_traj_chomp.argtypes = traj_model + [_d, ctypes.POINTER(ctypes.c_int64), _L, _d, _d, _i, _d, _d, _d,
+                                                 _d, _I]
+        h.or_traj_chomp.restype = None
+        h.or_chomp_cg.argtypes = [_d, _I, _D, _I, _D, _d]
+        h.or_sco_linearize.argtypes = traj_model + [_d, _L, _i, _I, _D, _d, _d, _I]
+        h.or_sco_linearize.restype = None
+        h.or_traj_sco.argtypes = traj_model + [_d, ctypes.POINTER(ctypes.c_int64), _L, _d, _d, _i, _I, _i, _d, _d,
+                                               _d, _I]
+        h.or_traj_sco.restype = None
+        h.or_chomp_cg.restype = None
         h.or_obstacle_dist.argtypes = [_d, _d, _L, _d, _I, _d, _I, _d, _I, _d, _I, _d, _I]
         h.or_obstacle_dist.restype = None
         h.or_capsule_dist.argtypes = [_d, _L, _d, _I, _d, _I, _d, _I, _d, _I, _d, _I]
@@ -334,6 +344,73 @@ def traj_lbfgs(model: TrajModel, trajs, lower, upper, *, n_iters=100, m_lbfgs=6,
     if iterates:
         return best, cost, start, last, trace
     return best, cost, start
+
+
+def traj_chomp(model: TrajModel, trajs, lower, upper, *, n_iters=100, step_size=0.05, w_smooth=1.0, w_acc=0.5,
+               w_jerk=0.1, w_collision=3.0, w_collision_max=50.0, collision_penalty_scale=1.05, w_limits=1.0,
+               use_covariant_update=True, smoothness_reg=1e-3, n_cg_iters=24, cg_tol=1e-4, grad_clip_norm=10.0,
+               max_delta_per_step=0.05, early_stop_patience=15, min_cost_improve=1e-5, threads: int = 0,
+               trace: bool = False):
+    """FP64 CHOMP per trajectory: ``(final (W, A) flat, cost at w_collision_max (B,), starts (B + 1))``,
+    plus the per-iteration trace ``(B, n_iters, 3)`` = (current cost, picked trial, accepted) with ``trace``."""
+    x, start = _traj_flat(trajs)
+    B = len(start) - 1
+    out = np.empty_like(x)
+    cost = _empty(B)
+    tr = _empty((B, max(n_iters, 0), 3)) if trace else None
+    icfg = _c([n_iters, int(bool(use_covariant_update)), n_cg_iters, early_stop_patience], np.int32)
+    fcfg = _c([step_size, w_smooth, w_acc, w_jerk, w_collision, w_collision_max, collision_penalty_scale, w_limits,
+               smoothness_reg, cg_tol, grad_clip_norm, max_delta_per_step, min_cost_improve], _R)
+    lo, hi = _c(lower, _R), _c(upper, _R)
+    lib().or_traj_chomp(*model.args(), _p(x, _d), start.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), B,
+                        _p(lo, _d), _p(hi, _d), _p(icfg, _i), _p(fcfg, _d), _p(out, _d), _p(cost, _d),
+                        None if tr is None else _p(tr, _d), threads)
+    if trace:
+        return out, cost, start, tr
+    return out, cost, start
+
+
+def sco_linearize(model: TrajModel, q, world_group_start, temperature: float = 0.05, threads: int = 0):
+    """SCO's per-waypoint group smooth minima: ``(d (W, G), J (W, G, A))`` for configurations ``q (W, A)``;
+    G = 1 (self pairs) + one per world group (CSR ``world_group_start`` over the model's world spheres)."""
+    qq = _c(q, _R).reshape(-1, model.t.A)
+    gs = _c(world_group_start, np.int32)
+    G = gs.shape[0]  # 1 + n_world_groups = len(CSR)
+    W = qq.shape[0]
+    d = _empty((W, G))
+    J = _empty((W, G, model.t.A))
+    lib().or_sco_linearize(*model.args(), _p(qq, _d), W, _p(gs, _i), G - 1, float(temperature),
+                           _p(d, _d), _p(J, _d), threads)
+    return d, J
+
+
+def traj_sco(model: TrajModel, trajs, lower, upper, world_group_start, *, n_outer_iters=10, n_inner_iters=30,
+             m_lbfgs=6, w_smooth=1.0, w_vel=1.0, w_acc=0.5, w_jerk=0.1, w_collision=1.0, w_collision_max=100.0,
+             penalty_scale=3.0, collision_margin=0.01, w_trust=0.5, w_limits=1.0, smooth_min_temperature=0.05,
+             threads: int = 0):
+    """FP64 SCO per trajectory: ``(final (W, A) flat, final cost at w_collision_max (B,), starts (B + 1))``.
+    The model's margin/mode serve the final cost (mode 1, colldist)."""
+    x, start = _traj_flat(trajs)
+    B = len(start) - 1
+    out = np.empty_like(x)
+    cost = _empty(B)
+    gs = _c(world_group_start, np.int32)
+    icfg = _c([n_outer_iters, n_inner_iters, m_lbfgs], np.int32)
+    fcfg = _c([w_smooth, w_vel, w_acc, w_jerk, w_limits, w_collision, w_collision_max, penalty_scale,
+               collision_margin, w_trust, smooth_min_temperature], _R)
+    lo, hi = _c(lower, _R), _c(upper, _R)
+    lib().or_traj_sco(*model.args(), _p(x, _d), start.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), B,
+                      _p(lo, _d), _p(hi, _d), _p(gs, _i), gs.shape[0] - 1, _p(icfg, _i), _p(fcfg, _d), _p(out, _d),
+                      _p(cost, _d), threads)
+    return out, cost, start
+
+
+def chomp_cg(b, reg: float = 1e-3, n_iters: int = 24, tol: float = 1e-4):
+    """Truncated CG of CHOMP's covariant step: ``x`` with ``M x ~= b`` for one joint's interior timeline."""
+    bb = _c(b, _R).reshape(-1)
+    out = _empty(bb.shape[0])
+    lib().or_chomp_cg(_p(bb, _d), bb.shape[0], float(reg), int(n_iters), float(tol), _p(out, _d))
+    return out
 
 
 def two_loop(g, s_buf, y_buf, rho, m_used: int, newest: int):

@@ -1145,6 +1145,422 @@ void or_traj_lbfgs(const float* twists, const float* parent_tf, const int32_t* p
     free(x); free(xp); free(gp); free(g); free(dir); free(xt); free(best); free(sb); free(yb);
   }
 }
+/* CHOMP's interior smoothness precision operator on one joint's timeline
+ * (_chomp_optimization.py:156-166): y = 6x - 4(x[i-1] + x[i+1]) + x[i-2] + x[i+2]
+ * + reg x, terms outside [0, n) dropped. */
+static void chomp_metric(const double* x, int n, double reg, double* y) {
+  for (int i = 0; i < n; ++i) {
+    double v = (6.0 + reg) * x[i];
+    if (i >= 1) v -= 4.0 * x[i - 1];
+    if (i + 1 < n) v -= 4.0 * x[i + 1];
+    if (i >= 2) v += x[i - 2];
+    if (i + 2 < n) v += x[i + 2];
+    y[i] = v;
+  }
+}
+
+/* Truncated CG for M x = b (_chomp_optimization.py:169-205): zero start,
+ * n_iters steps, a step taken only while the squared residual stays above
+ * (tol |b| + 1e-12)^2. */
+static void chomp_cg(const double* b, int n, double reg, int n_iters, double tol, double* x) {
+  double* r = (double*)malloc(sizeof(double) * n);
+  double* p = (double*)malloc(sizeof(double) * n);
+  double* ap = (double*)malloc(sizeof(double) * n);
+  double rr = 0.0, bb = 0.0;
+  for (int i = 0; i < n; ++i) { x[i] = 0.0; r[i] = b[i]; p[i] = b[i]; rr += b[i] * b[i]; }
+  bb = rr;
+  const double tb = tol * sqrt(bb + 1e-18) + 1e-12;
+  const double tol2 = tb * tb;
+  int active = 1;
+  for (int it = 0; it < n_iters && active; ++it) {
+    chomp_metric(p, n, reg, ap);
+    double pap = 0.0;
+    for (int i = 0; i < n; ++i) pap += p[i] * ap[i];
+    const double alpha = rr / (pap + 1e-9);
+    double rrn = 0.0;
+    for (int i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * ap[i];
+      rrn += r[i] * r[i];
+    }
+    const double beta = rrn / (rr + 1e-9);
+    for (int i = 0; i < n; ++i) p[i] = r[i] + beta * p[i];
+    rr = rrn;
+    active = rrn > tol2;
+  }
+  free(r); free(p); free(ap);
+}
+
+/* The CHOMP trajectory optimiser of _chomp_optimization.py:208-348 for each
+ * trajectory (endpoints stay where x_init has them; the model's mode is the
+ * hinge^2 activation, 0).  icfg = {n_iters, use_covariant_update, n_cg_iters,
+ * early_stop_patience}; fcfg = {step_size, w_smooth, w_acc, w_jerk,
+ * w_collision, w_collision_max, collision_penalty_scale, w_limits,
+ * smoothness_reg, cg_tol, grad_clip_norm, max_delta_per_step,
+ * min_cost_improve}.  Per iteration while active: the gradient of the cost at
+ * the current collision weight, clipped to norm grad_clip_norm, endpoint rows
+ * zeroed; the direction -M^-1 g per joint over the interior (truncated CG) or
+ * -g; the six trials x + clip(step_size alpha d, +-max_delta), alpha in
+ * {1, .5, .25, .1, .05, 0}, the cheapest (first on ties) kept when it beats the
+ * current cost by min_cost_improve; stall count and early stop; collision
+ * weight x penalty_scale up to w_collision_max.  out_traj (W, A) = the final
+ * iterate, out_cost (B) = its cost at w_collision_max; out_trace (B, n_iters,
+ * 3), when non-NULL, = per iteration the current cost, the picked trial and
+ * whether it was accepted (-1 in all three once stopped). */
+void or_traj_chomp(const float* twists, const float* parent_tf, const int32_t* parent_idx, const int32_t* act_idx,
+                   const float* mimic_mul, const float* mimic_off, const int32_t* mimic_act_idx,
+                   const int32_t* topo_inv, int J, int A, const int32_t* link_parent_joint, int L,
+                   const double* spheres, int S, const int32_t* pair_i, const int32_t* pair_j, int P,
+                   const double* world, int M, double margin, int mode, const double* x_init,
+                   const int64_t* start, int64_t B, const double* lower, const double* upper, const int32_t* icfg,
+                   const double* fcfg, double* out_traj, double* out_cost, double* out_trace, int threads) {
+  static const double alphas[6] = {1.0, 0.5, 0.25, 0.1, 0.05, 0.0};
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  traj_model_t md = {&t, link_parent_joint, L, spheres, S, pair_i, pair_j, P, world, M, margin, mode};
+  const int n_iters = icfg[0], covariant = icfg[1], n_cg = icfg[2], patience = icfg[3];
+  const double step = fcfg[0], w_smooth = fcfg[1], w_acc = fcfg[2], w_jerk = fcfg[3];
+  const double w_c0 = fcfg[4], w_cmax = fcfg[5], scale = fcfg[6], w_lim = fcfg[7];
+  const double reg = fcfg[8], cg_tol = fcfg[9], clip = fcfg[10], max_delta = fcfg[11], min_imp = fcfg[12];
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t b = 0; b < B; ++b) {
+    const int T = (int)(start[b + 1] - start[b]);
+    const int n = T * A;
+    double* x = (double*)malloc(sizeof(double) * n);
+    double* g = (double*)malloc(sizeof(double) * n);
+    double* dir = (double*)malloc(sizeof(double) * n);
+    double* xt = (double*)malloc(sizeof(double) * n);
+    double* xb = (double*)malloc(sizeof(double) * n);
+    const int tin = T > 2 ? T - 2 : 0;
+    double* col = (double*)malloc(sizeof(double) * (tin > 0 ? tin : 1));
+    double* sol = (double*)malloc(sizeof(double) * (tin > 0 ? tin : 1));
+    memcpy(x, x_init + start[b] * A, sizeof(double) * n);
+    double w_coll = w_c0;
+    int stall = 0, active = 1;
+    for (int k = 0; k < n_iters; ++k) {
+      double* tr = out_trace ? out_trace + ((size_t)b * n_iters + k) * 3 : NULL;
+      if (!active) {
+        if (tr) tr[0] = tr[1] = tr[2] = -1.0;
+        continue;
+      }
+      double w[6] = {w_smooth, 0.0, w_acc, w_jerk, w_lim, w_coll};
+      double comps[3];
+      traj_cost_one(&md, x, T, lower, upper, w, comps, g);
+      const double cost = w_smooth * comps[0] + w_lim * comps[1] + w_coll * comps[2];
+      double gn = 0.0;
+      for (int i = 0; i < n; ++i) gn += g[i] * g[i];
+      gn = sqrt(gn);
+      const double cs = (clip > 0.0 && gn > clip) ? clip / (gn + 1e-12) : 1.0;
+      for (int i = 0; i < n; ++i) g[i] *= cs;
+      for (int a = 0; a < A; ++a) { g[a] = 0.0; g[n - A + a] = 0.0; }
+      if (covariant && T > 2) {
+        for (int i = 0; i < n; ++i) dir[i] = 0.0;
+        for (int a = 0; a < A; ++a) {
+          for (int i = 0; i < tin; ++i) col[i] = g[(i + 1) * A + a];
+          chomp_cg(col, tin, reg, n_cg, cg_tol, sol);
+          for (int i = 0; i < tin; ++i) dir[(i + 1) * A + a] = -sol[i];
+        }
+      } else {
+        for (int i = 0; i < n; ++i) dir[i] = -g[i];
+      }
+      int pick = -1;
+      double best = INFINITY;
+      for (int j = 0; j < 6; ++j) {
+        const double al = step * alphas[j];
+        for (int i = 0; i < n; ++i) {
+          double d = al * dir[i];
+          d = d < -max_delta ? -max_delta : (d > max_delta ? max_delta : d);
+          xt[i] = x[i] + d;
+        }
+        double tc[3];
+        traj_cost_one(&md, xt, T, lower, upper, w, tc, NULL);
+        const double c = w_smooth * tc[0] + w_lim * tc[1] + w_coll * tc[2];
+        if (c < best) { best = c; pick = j; memcpy(xb, xt, sizeof(double) * n); }
+      }
+      const int improved = best < cost - min_imp;
+      if (improved) memcpy(x, xb, sizeof(double) * n);
+      stall = improved ? 0 : stall + 1;
+      active = stall < patience;
+      w_coll = fmin(w_coll * scale, w_cmax);
+      if (tr) { tr[0] = cost; tr[1] = pick; tr[2] = improved; }
+    }
+    double w[6] = {w_smooth, 0.0, w_acc, w_jerk, w_lim, w_cmax};
+    double comps[3];
+    traj_cost_one(&md, x, T, lower, upper, w, comps, NULL);
+    out_cost[b] = w_smooth * comps[0] + w_lim * comps[1] + w_cmax * comps[2];
+    memcpy(out_traj + start[b] * A, x, sizeof(double) * n);
+    free(x); free(g); free(dir); free(xt); free(xb); free(col); free(sol);
+  }
+}
+
+void or_chomp_cg(const double* b, int n, double reg, int n_iters, double tol, double* x) {
+  chomp_cg(b, n, reg, n_iters, tol, x);
+}
+
+/* ----------------------------------------------------------------- SCO */
+
+/* SCO's linearisation at one waypoint (_sco_optimization.py:224-288): per
+ * collision group the smooth minimum  -tau logsumexp(-d / tau)  of its
+ * distances and its gradient (the softmax-weighted distance gradients; the
+ * gradient of a distance flows to the closest sphere, the first on ties, as
+ * jnp.min's does).  Group 0: every active self pair (same-slot spheres);
+ * group 1 + g: every (link, world sphere) pair of world group g, spheres
+ * [gstart[g], gstart[g + 1]).  All with the JAX +1e-6 bias; an empty group
+ * reads +inf with a zero gradient.  dk (G), Jk (G, A). */
+static void sco_linearize_wp(const traj_model_t* md, const double* q, const int32_t* gstart, int n_wg, double tau,
+                             double* dk, double* Jk) {
+  const int L = md->L, S = md->S, A = md->t->A;
+  const int G = 1 + n_wg;
+  se3 Tw[OR_MAXJ];
+  fk_one(md->t, q, Tw);
+  double* cw = (double*)malloc(sizeof(double) * 3 * L * S);
+  for (int l = 0; l < L; ++l) {
+    const int j = md->lpj[l];
+    se3 T = (j < 0) ? (se3){1, 0, 0, 0, 0, 0, 0} : Tw[j];
+    for (int sl = 0; sl < S; ++sl) {
+      double* o = cw + 3 * (l * S + sl);
+      qrot(&T, md->sph + (l * S + sl) * 4, o);
+      o[0] += T.tx; o[1] += T.ty; o[2] += T.tz;
+    }
+  }
+  const double e = 1e-6;
+  for (int g = 0; g < G; ++g) {
+    const int n_terms = (g == 0) ? md->P : L * (gstart[g] - gstart[g - 1]);
+    double* J = Jk + g * A;
+    for (int a = 0; a < A; ++a) J[a] = 0.0;
+    if (n_terms <= 0) { dk[g] = INFINITY; continue; }
+    double* d = (double*)malloc(sizeof(double) * n_terms);
+    int* arg = (int*)malloc(sizeof(int) * n_terms);
+    double dmin = INFINITY;
+    for (int i = 0; i < n_terms; ++i) {
+      double best = INFINITY;
+      int bs = 0;
+      if (g == 0) {
+        const int li = md->pi[i], lj = md->pj[i];
+        for (int sl = 0; sl < S; ++sl) {
+          const double v = sdist(cw + 3 * (li * S + sl), md->sph[(li * S + sl) * 4 + 3], cw + 3 * (lj * S + sl),
+                                 md->sph[(lj * S + sl) * 4 + 3], 1);
+          if (v < best) { best = v; bs = sl; }
+        }
+      } else {
+        const int l = i / (gstart[g] - gstart[g - 1]);
+        const int m = gstart[g - 1] + i % (gstart[g] - gstart[g - 1]);
+        const double* o = md->world + 4 * m;
+        for (int sl = 0; sl < S; ++sl) {
+          const double v = sdist(cw + 3 * (l * S + sl), md->sph[(l * S + sl) * 4 + 3], o, o[3], 1);
+          if (v < best) { best = v; bs = sl; }
+        }
+      }
+      d[i] = best;
+      arg[i] = bs;
+      if (best < dmin) dmin = best;
+    }
+    double z = 0.0;
+    for (int i = 0; i < n_terms; ++i) z += exp(-(d[i] - dmin) / tau);
+    dk[g] = dmin - tau * log(z);
+    for (int i = 0; i < n_terms; ++i) {
+      const double wgt = exp(-(d[i] - dmin) / tau) / z;
+      if (wgt == 0.0) continue;
+      const int sl = arg[i];
+      if (g == 0) {
+        const int li = md->pi[i], lj = md->pj[i];
+        const double* ci = cw + 3 * (li * S + sl);
+        const double* cj = cw + 3 * (lj * S + sl);
+        const double x[3] = {cj[0] - ci[0] + e, cj[1] - ci[1] + e, cj[2] - ci[2] + e};
+        const double n = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        const double gj[3] = {wgt * x[0] / n, wgt * x[1] / n, wgt * x[2] / n};
+        const double gi[3] = {-gj[0], -gj[1], -gj[2]};
+        pull_back(md, Tw, lj, cj, gj, J);
+        pull_back(md, Tw, li, ci, gi, J);
+      } else {
+        const int l = i / (gstart[g] - gstart[g - 1]);
+        const int m = gstart[g - 1] + i % (gstart[g] - gstart[g - 1]);
+        const double* o = md->world + 4 * m;
+        const double* c = cw + 3 * (l * S + sl);
+        const double x[3] = {o[0] - c[0] + e, o[1] - c[1] + e, o[2] - c[2] + e};
+        const double n = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        const double gc[3] = {-wgt * x[0] / n, -wgt * x[1] / n, -wgt * x[2] / n};
+        pull_back(md, Tw, l, c, gc, J);
+      }
+    }
+    free(d);
+    free(arg);
+  }
+  free(cw);
+}
+
+void or_sco_linearize(const float* twists, const float* parent_tf, const int32_t* parent_idx, const int32_t* act_idx,
+                      const float* mimic_mul, const float* mimic_off, const int32_t* mimic_act_idx,
+                      const int32_t* topo_inv, int J, int A, const int32_t* link_parent_joint, int L,
+                      const double* spheres, int S, const int32_t* pair_i, const int32_t* pair_j, int P,
+                      const double* world, int M, double margin, int mode, const double* q, int64_t W,
+                      const int32_t* gstart, int n_wg, double tau, double* dk, double* Jk, int threads) {
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  traj_model_t md = {&t, link_parent_joint, L, spheres, S, pair_i, pair_j, P, world, M, margin, mode};
+  const int G = 1 + n_wg;
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t w = 0; w < W; ++w) sco_linearize_wp(&md, q + w * A, gstart, n_wg, tau, dk + w * G, Jk + w * G * A);
+}
+
+/* The inner objective of _lbfgs_inner_solve (_sco_optimization.py:291-339) for
+ * one trajectory: smoothness (velocity, acceleration, jerk) + limits + w_coll
+ * sum over waypoints and groups of max(0, margin - d_lin)^2 with d_lin =
+ * d_k + J_k (x_t - q_k,t) + w_trust |x - q_k|^2; grad (T, A) when non-NULL. */
+static double sco_inner_cost(const double* x, const double* qk, const double* dk, const double* Jk, int T, int A,
+                             int G, const double* lower, const double* upper, const double* w, double w_coll,
+                             double margin, double w_trust, double* grad) {
+  tables_t dummy = {0, A, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL};
+  traj_model_t md = {&dummy, NULL, 0, NULL, 0, NULL, NULL, 0, NULL, 0, 0.0, 0};
+  double comps[3];
+  const double ws[6] = {w[0], w[1], w[2], w[3], w[4], 0.0};
+  traj_cost_one(&md, x, T, lower, upper, ws, comps, grad);
+  double c = w[0] * comps[0] + w[4] * comps[1];
+  for (int t = 0; t < T; ++t) {
+    for (int g = 0; g < G; ++g) {
+      const double* Jr = Jk + ((size_t)t * G + g) * A;
+      double dl = dk[t * G + g];
+      for (int a = 0; a < A; ++a) dl += Jr[a] * (x[t * A + a] - qk[t * A + a]);
+      const double h = margin - dl;
+      if (h > 0.0) {
+        c += w_coll * h * h;
+        if (grad)
+          for (int a = 0; a < A; ++a) grad[t * A + a] -= 2.0 * w_coll * h * Jr[a];
+      }
+    }
+    for (int a = 0; a < A; ++a) {
+      const double dq = x[t * A + a] - qk[t * A + a];
+      c += w_trust * dq * dq;
+      if (grad) grad[t * A + a] += 2.0 * w_trust * dq;
+    }
+  }
+  return c;
+}
+
+/* SCO per trajectory (_sco_optimization.py:291-503): n_outer rounds of
+ * linearise at the current path, solve the inner problem with n_inner L-BFGS
+ * steps from it (history m, the five-step line search with sufficient
+ * decrease 1e-4, the best inner iterate returned), collision weight x
+ * penalty_scale up to w_collision_max.  icfg = {n_outer, n_inner, m};
+ * fcfg = {w_smooth, w_vel, w_acc, w_jerk, w_limits, w_collision,
+ * w_collision_max, penalty_scale, margin, w_trust, tau}.  Endpoints stay where
+ * x_init has them.  out_traj (W, A); out_cost (B) = the final nonlinear cost
+ * at w_collision_max with the colldist activation (_eval_cost, :421-447; the
+ * model's mode must be 1). */
+void or_traj_sco(const float* twists, const float* parent_tf, const int32_t* parent_idx, const int32_t* act_idx,
+                 const float* mimic_mul, const float* mimic_off, const int32_t* mimic_act_idx,
+                 const int32_t* topo_inv, int J, int A, const int32_t* link_parent_joint, int L,
+                 const double* spheres, int S, const int32_t* pair_i, const int32_t* pair_j, int P,
+                 const double* world, int M, double margin_model, int mode, const double* x_init,
+                 const int64_t* start, int64_t B, const double* lower, const double* upper, const int32_t* gstart,
+                 int n_wg, const int32_t* icfg, const double* fcfg, double* out_traj, double* out_cost,
+                 int threads) {
+  static const double alphas[5] = {1.0, 0.5, 0.25, 0.1, 0.025};
+  tables_t t = {J, A, twists, parent_tf, mimic_mul, mimic_off, parent_idx, act_idx, mimic_act_idx, topo_inv};
+  traj_model_t md = {&t, link_parent_joint, L, spheres, S, pair_i, pair_j, P, world, M, margin_model, mode};
+  const int n_outer = icfg[0], n_inner = icfg[1], m = icfg[2];
+  const double w[5] = {fcfg[0], fcfg[1], fcfg[2], fcfg[3], fcfg[4]};
+  const double w_c0 = fcfg[5], w_cmax = fcfg[6], scale = fcfg[7], margin = fcfg[8], w_trust = fcfg[9];
+  const double tau = fcfg[10];
+  const int G = 1 + n_wg;
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t b = 0; b < B; ++b) {
+    const int T = (int)(start[b + 1] - start[b]);
+    const int n = T * A;
+    double* x = (double*)malloc(sizeof(double) * n);
+    double* qk = (double*)malloc(sizeof(double) * n);
+    double* dk = (double*)malloc(sizeof(double) * T * G);
+    double* Jk = (double*)malloc(sizeof(double) * T * G * A);
+    double* xi = (double*)malloc(sizeof(double) * n);
+    double* xp = (double*)malloc(sizeof(double) * n);
+    double* g = (double*)malloc(sizeof(double) * n);
+    double* gp = (double*)malloc(sizeof(double) * n);
+    double* dir = (double*)malloc(sizeof(double) * n);
+    double* xt = (double*)malloc(sizeof(double) * n);
+    double* best = (double*)malloc(sizeof(double) * n);
+    double* sb = (double*)calloc((size_t)m * n, sizeof(double));
+    double* yb = (double*)calloc((size_t)m * n, sizeof(double));
+    memcpy(x, x_init + start[b] * A, sizeof(double) * n);
+    double w_coll = w_c0;
+    for (int it = 0; it < n_outer; ++it) {
+      memcpy(qk, x, sizeof(double) * n);
+      for (int tt = 0; tt < T; ++tt) sco_linearize_wp(&md, qk + tt * A, gstart, n_wg, tau, dk + tt * G, Jk + tt * G * A);
+      /* inner L-BFGS from x = q_k */
+      double rho[16] = {0};
+      int m_used = 0, newest = 0;
+      memset(sb, 0, sizeof(double) * m * n);
+      memset(yb, 0, sizeof(double) * m * n);
+      memcpy(xi, qk, sizeof(double) * n);
+      double c0 = sco_inner_cost(xi, qk, dk, Jk, T, A, G, lower, upper, w, w_coll, margin, w_trust, g);
+      for (int a = 0; a < A; ++a) { g[a] = 0; g[n - A + a] = 0; }
+      double best_cost = c0;
+      memcpy(best, xi, sizeof(double) * n);
+      memcpy(xp, xi, sizeof(double) * n);
+      memcpy(gp, g, sizeof(double) * n);
+      for (int k = 0; k < n_inner; ++k) {
+        const double cost = sco_inner_cost(xi, qk, dk, Jk, T, A, G, lower, upper, w, w_coll, margin, w_trust, g);
+        for (int a = 0; a < A; ++a) { g[a] = 0; g[n - A + a] = 0; }
+        double sy = 0, yy = 0;
+        for (int i = 0; i < n; ++i) {
+          const double sv = xi[i] - xp[i], yv = g[i] - gp[i];
+          sy += sv * yv;
+          yy += yv * yv;
+        }
+        if (sy > 1e-10 * yy + 1e-30 && k > 0) {
+          newest = (newest + 1) % m;
+          for (int i = 0; i < n; ++i) { sb[newest * n + i] = xi[i] - xp[i]; yb[newest * n + i] = g[i] - gp[i]; }
+          rho[newest] = 1.0 / (sy + 1e-30);
+          if (m_used < m) ++m_used;
+        }
+        if (m_used > 0) {
+          two_loop(g, sb, yb, rho, m_used, newest, m, n, dir);
+        } else {
+          double gn = 0;
+          for (int i = 0; i < n; ++i) gn += g[i] * g[i];
+          gn = sqrt(gn) + 1e-18;
+          for (int i = 0; i < n; ++i) dir[i] = -g[i] / gn;
+        }
+        for (int a = 0; a < A; ++a) { dir[a] = 0; dir[n - A + a] = 0; }
+        const double thresh = cost * (1.0 - 1e-4);
+        double trial[5];
+        for (int i = 0; i < 5; ++i) {
+          for (int k2 = 0; k2 < n; ++k2) xt[k2] = xi[k2] + alphas[i] * dir[k2];
+          trial[i] = sco_inner_cost(xt, qk, dk, Jk, T, A, G, lower, upper, w, w_coll, margin, w_trust, NULL);
+        }
+        int pick = -1;
+        for (int i = 0; i < 5; ++i)
+          if (trial[i] < thresh) { pick = i; break; }
+        if (pick < 0) {
+          pick = 0;
+          for (int i = 1; i < 5; ++i)
+            if (trial[i] < trial[pick]) pick = i;
+        }
+        memcpy(xp, xi, sizeof(double) * n);
+        memcpy(gp, g, sizeof(double) * n);
+        for (int i = 0; i < n; ++i) xi[i] += alphas[pick] * dir[i];
+        if (trial[pick] < best_cost) {
+          best_cost = trial[pick];
+          memcpy(best, xi, sizeof(double) * n);
+        }
+      }
+      memcpy(x, best, sizeof(double) * n);
+      for (int a = 0; a < A; ++a) {  /* re-pin the endpoints */
+        x[a] = x_init[start[b] * A + a];
+        x[n - A + a] = x_init[start[b] * A + n - A + a];
+      }
+      w_coll = fmin(w_coll * scale, w_cmax);
+    }
+    double comps[3];
+    const double we[6] = {w[0], w[1], w[2], w[3], w[4], w_cmax};
+    traj_cost_one(&md, x, T, lower, upper, we, comps, NULL);
+    out_cost[b] = w[0] * comps[0] + w[4] * comps[1] + w_cmax * comps[2];
+    memcpy(out_traj + start[b] * A, x, sizeof(double) * n);
+    free(x); free(qk); free(dk); free(Jk); free(xi); free(xp); free(g); free(gp); free(dir); free(xt); free(best);
+    free(sb); free(yb);
+  }
+}
+
 /* ------------------------------------------------- JAX-convention LM / HJCD */
 
 /* jaxlie 1.5.0 (a third-party dependency, pinned in pyproject.toml:27, absent

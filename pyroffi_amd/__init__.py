@@ -11,7 +11,10 @@ Public API (same names and arguments as pyroffi):
   ``UnevenLsIkSolver`` for uneven multi-robot batches in one graph
 * ``costs.pose_hinge_cost`` (differentiable, hand-written adjoint)
 * ``lbfgs_trajopt`` / ``LbfgsTrajOptConfig`` (direct L-BFGS trajectory
-  optimisation) and ``lbfgs_trajopt_uneven``
+  optimisation) and ``lbfgs_trajopt_uneven``; ``chomp_trajopt`` /
+  ``ChompTrajOptConfig`` (CHOMP with the covariant update);
+  ``sco_trajopt`` / ``ScoTrajOptConfig`` / ``make_init_trajs`` (sequential
+  convex optimisation)
 
 All computation runs in the HIP library ``libpyroffi_amd.so`` (C ABI in
 ``include/pyroffi_amd.h``); torch supplies device memory and the stream.
@@ -23,8 +26,11 @@ from ._robot_urdf_parser import JointInfo, LinkInfo, RobotURDFParser
 from ._se3 import SE3, SO3
 from ._urdf import URDF, bundled_robot_names, bundled_srdf_path, bundled_urdf_path, load_bundled_urdf
 from .optimization_engines import (
+    ChompTrajOptConfig,
     LbfgsTrajOptConfig,
+    ScoTrajOptConfig,
     UnevenLsIkSolver,
+    chomp_trajopt,
     hjcd_solve,
     hjcd_solve_batch,
     lbfgs_trajopt,
@@ -32,10 +38,17 @@ from .optimization_engines import (
     ls_ik_solve,
     ls_ik_solve_cuda,
     ls_ik_solve_cuda_batch,
+    make_init_trajs,
+    sco_trajopt,
 )
 
 __all__ = [
+    "ChompTrajOptConfig",
     "LbfgsTrajOptConfig",
+    "ScoTrajOptConfig",
+    "chomp_trajopt",
+    "make_init_trajs",
+    "sco_trajopt",
     "UnevenLsIkSolver",
     "lbfgs_trajopt",
     "lbfgs_trajopt_uneven",

@@ -578,13 +578,14 @@ __device__ __forceinline__ void for_each_waypoint(const TrajModelDev& md, const 
   }
 }
 
-// Cost components {smooth, limits, coll} of the trajectory in L.vx (T, A), all
+// Cost components {smooth, limits, coll} of the trajectory X (T, A) in LDS, all
 // threads receiving the totals; with GRAD, L.vg = gradient of
 // w_smooth smooth + w_limits limits + w_coll coll (endpoint rows zeroed when
 // mask_ends).  Leaves the accelerations of x in L.vt; uses L.vd as scratch.
 template <int MAXN, bool GRAD>
-__device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
-                                float w_coll, bool mask_ends, float (&comps)[3]) {
+__device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, const float* X, int T,
+                                                const TrajWeights& w, float w_coll, bool mask_ends,
+                                                float (&comps)[3]) {
   const int A = md.n_act;
   const int n = T * A;
   const int tid = threadIdx.x;
@@ -594,7 +595,7 @@ __device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const Tr
       md, L, T,
       [&](int t, float (&q)[MAXN]) {
 #pragma unroll
-        for (int a = 0; a < MAXN; ++a) q[a] = (a < A) ? L.vx[t * A + a] : 0.f;
+        for (int a = 0; a < MAXN; ++a) q[a] = (a < A) ? X[t * A + a] : 0.f;
       },
       [&](int t, float c, const float (&gq)[MAXN]) {
         if (!leader) return;
@@ -608,18 +609,18 @@ __device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const Tr
   const int na = T - 4;
   for (int i = tid; i < n; i += blockDim.x) {
     const int a = i % A;
-    const float xv = L.vx[i];
+    const float xv = X[i];
     const float v = fmaxf(xv - L.hi[a], 0.f) + fmaxf(L.lo[a] - xv, 0.f);
     lim += v * v;
     if (GRAD)
       L.vg[i] += w.w_limits * 2.f * v * ((xv > L.hi[a] ? 1.f : 0.f) - (L.lo[a] > xv ? 1.f : 0.f));
     if (i + A < n) {
-      const float dv = L.vx[i + A] - xv;
+      const float dv = X[i + A] - xv;
       sm += w.w_vel * dv * dv;
     }
     if (i < na * A) {
-      const float acc = (-xv + 16.f * L.vx[i + A] - 30.f * L.vx[i + 2 * A] + 16.f * L.vx[i + 3 * A] -
-                         L.vx[i + 4 * A]) / 12.f;
+      const float acc = (-xv + 16.f * X[i + A] - 30.f * X[i + 2 * A] + 16.f * X[i + 3 * A] -
+                         X[i + 4 * A]) / 12.f;
       L.vt[i] = acc;
       sm += w.w_acc * acc * acc;
     }
@@ -645,8 +646,8 @@ __device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const Tr
     for (int i = tid; i < n; i += blockDim.x) {
       const int t = i / A;
       float gs = 0.f;
-      if (t >= 1) gs += 2.f * w.w_vel * (L.vx[i] - L.vx[i - A]);
-      if (t + 1 < T) gs -= 2.f * w.w_vel * (L.vx[i + A] - L.vx[i]);
+      if (t >= 1) gs += 2.f * w.w_vel * (X[i] - X[i - A]);
+      if (t + 1 < T) gs -= 2.f * w.w_vel * (X[i + A] - X[i]);
 #pragma unroll
       for (int o = 0; o < 5; ++o) {
         const int tt = t - o;
@@ -662,6 +663,12 @@ __device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const Tr
   comps[0] = v3[0];
   comps[1] = v3[1];
   comps[2] = v3[2];
+}
+
+template <int MAXN, bool GRAD>
+__device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const TrajLds& L, int T, const TrajWeights& w,
+                                                float w_coll, bool mask_ends, float (&comps)[3]) {
+  traj_value_grad<MAXN, GRAD>(md, L, L.vx, T, w, w_coll, mask_ends, comps);
 }
 
 // Line search of _lbfgs_direct_solve on x + alpha_k d, alpha in {1, .5, .25,
@@ -967,6 +974,664 @@ __global__ __launch_bounds__(kMaxTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a
   }
 }
 
+// CHOMP (_chomp_optimization.py:208-330): per iteration the gradient of the
+// cost at the current collision weight, clipped to a global norm, endpoint
+// rows zeroed; the covariant direction -M^-1 g, M the interior second-
+// difference precision operator per joint, by truncated CG (all joints in
+// lockstep, one LDS reduction of A sums per CG half-step); the six trials
+// x + clip(step alpha d, +-max_delta) evaluated in full and the cheapest (first
+// on ties) kept when it beats the current cost by min_cost_improve; stall count,
+// early stop and collision-weight continuation.  The alpha = 0 trial is x itself,
+// whose cost is the current one.  LDS use: vx iterate, vg gradient / CG search
+// vector / trial point, vd CG solution (the direction is its negation), vt CG
+// residual / accelerations.
+struct ChompArgs {
+  TrajModelDev md;
+  const float* x_init;
+  const long long* start;
+  int n_traj, max_len;
+  long long n_waypoints;
+  TrajWeights w;  // w_vel = 0: CHOMP's smoothness has no velocity term
+  float step, w_coll0, w_coll_max, scale, reg, cg_tol, clip, max_delta, min_improve;
+  int n_iters, covariant, n_cg, patience;
+  float* out_traj;   // (W, A)
+  float* out_cost;   // (B)
+  float* out_trace;  // (B, n_iters, 3) or null
+};
+
+__device__ __forceinline__ float chomp_alpha(int k) {
+  return k == 0 ? 1.f : (k == 1 ? 0.5f : (k == 2 ? 0.25f : (k == 3 ? 0.1f : (k == 4 ? 0.05f : 0.f))));
+}
+
+// M p at element i = t * A + a of a (T, A) array whose rows 0 and T - 1 are zero.
+__device__ __forceinline__ float chomp_mp(const float* p, int i, int t, int T, int A, float reg) {
+  float v = (6.f + reg) * p[i] - 4.f * (p[i - A] + p[i + A]);
+  if (t >= 2) v += p[i - 2 * A];
+  if (t + 2 <= T - 1) v += p[i + 2 * A];
+  return v;
+}
+
+template <int MAXN>
+__device__ __forceinline__ void add_at(float (&v)[MAXN], int a, float x) {
+#pragma unroll
+  for (int k = 0; k < MAXN; ++k)
+    if (k == a) v[k] += x;
+}
+
+template <int MAXN>
+__device__ __forceinline__ float get_at(const float (&v)[MAXN], int a) {
+  float r = 0.f;
+#pragma unroll
+  for (int k = 0; k < MAXN; ++k)
+    if (k == a) r = v[k];
+  return r;
+}
+
+template <int MAXN>
+__global__ __launch_bounds__(kMaxTrajThreads) void traj_chomp_kernel(ChompArgs a) {
+  extern __shared__ float4 smem4[];
+  const TrajLds L = stage_traj_model(a.md, a.max_len, reinterpret_cast<char*>(smem4));
+  const int b = blockIdx.x;
+  if (b >= a.n_traj) return;  // uniform per block
+  const long long w0 = a.start[b], w1 = a.start[b + 1];
+  if (w0 < 0 || w1 < w0 || w1 > a.n_waypoints || w1 - w0 > a.max_len) return;  // uniform per block
+  const int T = static_cast<int>(w1 - w0);
+  const int A = a.md.n_act;
+  const int n = T * A;
+  const int tid = threadIdx.x;
+  const long long off = w0 * A;
+  for (int i = tid; i < n; i += blockDim.x) L.vx[i] = a.x_init[off + i];
+  __syncthreads();
+  float w_coll = a.w_coll0;
+  int stall = 0;
+  bool active = true;
+  for (int k = 0; k < a.n_iters; ++k) {
+    float* tr = a.out_trace ? a.out_trace + (static_cast<long long>(b) * a.n_iters + k) * 3 : nullptr;
+    if (!active) {
+      if (tr && tid == 0) tr[0] = tr[1] = tr[2] = -1.f;
+      continue;
+    }
+    float comps[3];
+    traj_value_grad<MAXN, true>(a.md, L, L.vx, T, a.w, w_coll, false, comps);
+    const float cost = a.w.w_smooth * comps[0] + a.w.w_limits * comps[1] + w_coll * comps[2];
+    float g2 = 0.f;
+    for (int i = tid; i < n; i += blockDim.x) g2 += L.vg[i] * L.vg[i];
+    const float gn = sqrtf(block_sum1(g2, L.red));
+    const float cs = (a.clip > 0.f && gn > a.clip) ? a.clip / (gn + 1e-12f) : 1.f;
+    for (int i = tid; i < n; i += blockDim.x) {
+      const int t = i / A;
+      L.vg[i] = (t == 0 || t == T - 1) ? 0.f : L.vg[i] * cs;
+    }
+    __syncthreads();
+    if (a.covariant && T > 2) {
+      // CG on every joint's interior timeline: x = 0 (vd), r = b (vt), p = b (vg).
+      float rr[MAXN], tol2[MAXN];
+      bool act[MAXN];
+#pragma unroll
+      for (int c = 0; c < MAXN; ++c) rr[c] = 0.f;
+      for (int i = tid; i < n; i += blockDim.x) {
+        const float gv = L.vg[i];
+        L.vd[i] = 0.f;
+        L.vt[i] = gv;
+        add_at(rr, i % A, gv * gv);
+      }
+      block_sum<MAXN>(rr, L.red);
+#pragma unroll
+      for (int c = 0; c < MAXN; ++c) {
+        const float tb = a.cg_tol * sqrtf(rr[c] + 1e-18f) + 1e-12f;
+        tol2[c] = tb * tb;
+        act[c] = c < A;
+      }
+      for (int it = 0; it < a.n_cg; ++it) {
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < MAXN; ++c) any = any || act[c];
+        if (!any) break;  // uniform: every thread holds the same flags
+        float pap[MAXN];
+#pragma unroll
+        for (int c = 0; c < MAXN; ++c) pap[c] = 0.f;
+        for (int i = tid; i < n; i += blockDim.x) {
+          const int t = i / A;
+          if (t == 0 || t == T - 1) continue;
+          add_at(pap, i % A, L.vg[i] * chomp_mp(L.vg, i, t, T, A, a.reg));
+        }
+        block_sum<MAXN>(pap, L.red);
+        float al[MAXN];
+#pragma unroll
+        for (int c = 0; c < MAXN; ++c) al[c] = act[c] ? rr[c] / (pap[c] + 1e-9f) : 0.f;
+        float rn[MAXN];
+#pragma unroll
+        for (int c = 0; c < MAXN; ++c) rn[c] = 0.f;
+        for (int i = tid; i < n; i += blockDim.x) {
+          const int t = i / A;
+          if (t == 0 || t == T - 1) continue;
+          const int c = i % A;
+          const float alc = get_at(al, c);
+          const float mp = chomp_mp(L.vg, i, t, T, A, a.reg);
+          L.vd[i] += alc * L.vg[i];
+          const float r = L.vt[i] - alc * mp;
+          L.vt[i] = r;
+          add_at(rn, c, r * r);
+        }
+        block_sum<MAXN>(rn, L.red);  // its barriers also order the p update after every M p read
+        float be[MAXN];
+#pragma unroll
+        for (int c = 0; c < MAXN; ++c) be[c] = rn[c] / (rr[c] + 1e-9f);
+        for (int i = tid; i < n; i += blockDim.x) {
+          const int t = i / A;
+          if (t == 0 || t == T - 1) continue;
+          const int c = i % A;
+          bool ac = false;
+#pragma unroll
+          for (int cc = 0; cc < MAXN; ++cc)
+            if (cc == c) ac = act[cc];
+          if (ac) L.vg[i] = L.vt[i] + get_at(be, c) * L.vg[i];
+        }
+#pragma unroll
+        for (int c = 0; c < MAXN; ++c) {
+          if (act[c]) {
+            rr[c] = rn[c];
+            act[c] = rn[c] > tol2[c];
+          }
+        }
+        __syncthreads();
+      }
+      // The residual and search vectors of an inactive joint froze with it; the
+      // solution is complete.  Direction = -x.
+      for (int i = tid; i < n; i += blockDim.x) L.vd[i] = -L.vd[i];
+    } else {
+      for (int i = tid; i < n; i += blockDim.x) L.vd[i] = -L.vg[i];
+    }
+    __syncthreads();
+    // Trials alpha_k, k = 0..4 (k = 5 is x itself, at the current cost).
+    int pick = 5;
+    float best = cost;
+    for (int kk = 0; kk < 5; ++kk) {
+      const float al = a.step * chomp_alpha(kk);
+      for (int i = tid; i < n; i += blockDim.x)
+        L.vg[i] = L.vx[i] + fminf(fmaxf(al * L.vd[i], -a.max_delta), a.max_delta);
+      __syncthreads();
+      float tc[3];
+      traj_value_grad<MAXN, false>(a.md, L, L.vg, T, a.w, w_coll, false, tc);
+      const float c = a.w.w_smooth * tc[0] + a.w.w_limits * tc[1] + w_coll * tc[2];
+      if (c < best || (kk < pick && c == best)) {
+        best = c;
+        pick = kk;
+      }
+    }
+    const bool improved = best < cost - a.min_improve;
+    if (improved) {
+      const float al = a.step * chomp_alpha(pick);
+      for (int i = tid; i < n; i += blockDim.x)
+        L.vx[i] += fminf(fmaxf(al * L.vd[i], -a.max_delta), a.max_delta);
+    }
+    stall = improved ? 0 : stall + 1;
+    active = stall < a.patience;
+    w_coll = fminf(w_coll * a.scale, a.w_coll_max);
+    if (tr && tid == 0) {
+      tr[0] = cost;
+      tr[1] = static_cast<float>(pick);
+      tr[2] = improved ? 1.f : 0.f;
+    }
+    __syncthreads();
+  }
+  float comps[3];
+  traj_value_grad<MAXN, false>(a.md, L, L.vx, T, a.w, a.w_coll_max, false, comps);
+  for (int i = tid; i < n; i += blockDim.x) a.out_traj[off + i] = L.vx[i];
+  if (tid == 0) a.out_cost[b] = a.w.w_smooth * comps[0] + a.w.w_limits * comps[1] + a.w_coll_max * comps[2];
+}
+
+// ---------------------------------------------------------------- SCO
+// _sco_optimization.py:224-503.  Linearisation: per waypoint and collision
+// group (0 = the active self pairs, 1 + g = every (link, world sphere) pair of
+// world group g) the smooth minimum dmin - tau log sum exp(-(d - dmin) / tau)
+// of the JAX distances (closest same-slot pair / closest sphere of the link,
+// +1e-6 bias) and its gradient: softmax weights times each distance's gradient
+// at its closest sphere(s).  Links without spheres take no part (the
+// reference's padded distances of about 1e9 have weight 0).  Two passes per
+// group over the group's lanes: the minimum, then the weights, accumulated per
+// link as a wrench (F, c x F) and pulled back to the link's ancestor nodes
+// once per link: dq_a += v_a . F + w_a . (tau - p_a x F).
+constexpr int kMaxScoGroups = 8;
+
+__device__ __forceinline__ float group8_min(float v) {
+  v = fminf(v, __shfl_xor(v, 1, kWave));
+  v = fminf(v, __shfl_xor(v, 2, kWave));
+  return fminf(v, __shfl_xor(v, 4, kWave));
+}
+
+template <int MAXN>
+__device__ __forceinline__ void pull_wrench(const WaypointNodes<MAXN, true>& K, int mask, const float (&F)[3],
+                                            const float (&tq)[3], float (&gn)[MAXN]) {
+#pragma unroll
+  for (int a = 0; a < MAXN; ++a) {
+    if (mask & (1 << a)) {
+      const float px = K.P[a].tx, py = K.P[a].ty, pz = K.P[a].tz;
+      const float mx = tq[0] - (py * F[2] - pz * F[1]);
+      const float my = tq[1] - (pz * F[0] - px * F[2]);
+      const float mz = tq[2] - (px * F[1] - py * F[0]);
+      gn[a] += K.v[a][0] * F[0] + K.v[a][1] * F[1] + K.v[a][2] * F[2] + K.w[a][0] * mx + K.w[a][1] * my +
+               K.w[a][2] * mz;
+    }
+  }
+}
+
+// Closest sphere of link spheres [k0, k1) to world sphere o: distance and the
+// (unnormalised) direction o - c + e and its length.
+__device__ __forceinline__ float link_world_dist(const float4* gc, int k0, int k1, float4 o, float& bx, float& by,
+                                                 float& bz, float& bn, float4& bc) {
+  const float e = 1e-6f;
+  float best = INFINITY;
+  for (int s = k0; s < k1; ++s) {
+    const float4 c = gc[s];
+    const float dx = o.x - c.x + e, dy = o.y - c.y + e, dz = o.z - c.z + e;
+    const float n = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float d = n - c.w - o.w;
+    if (d < best) { best = d; bx = dx; by = dy; bz = dz; bn = n; bc = c; }
+  }
+  return best;
+}
+
+// Closest same-slot pair of active pair p: distance, direction cj - ci + e, its length, the centres.
+__device__ __forceinline__ float pair_dist(const TrajLds& L, const float4* gc, int p, float& bx, float& by, float& bz,
+                                           float& bn, float4& bci, float4& bcj) {
+  const float e = 1e-6f;
+  float best = INFINITY;
+  for (int it = L.pair_start[p]; it < L.pair_start[p + 1]; ++it) {
+    const float4 ci = gc[L.pair_items[2 * it]];
+    const float4 cj = gc[L.pair_items[2 * it + 1]];
+    const float dx = cj.x - ci.x + e, dy = cj.y - ci.y + e, dz = cj.z - ci.z + e;
+    const float n = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float d = n - ci.w - cj.w;
+    if (d < best) { best = d; bx = dx; by = dy; bz = dz; bn = n; bci = ci; bcj = cj; }
+  }
+  return best;
+}
+
+template <int MAXN>
+__device__ __forceinline__ void group_smooth_min(const TrajModelDev& md, const TrajLds& L,
+                                                 const WaypointNodes<MAXN, true>& K, const float4* gc,
+                                                 const int* gstart, int gi, float tau, float& dk,
+                                                 float (&gn)[MAXN]) {
+  const int g = threadIdx.x % kGroup;
+#pragma unroll
+  for (int a = 0; a < MAXN; ++a) gn[a] = 0.f;
+  float bx, by, bz, bn;
+  float4 bc, bc2;
+  // Pass 1: the group minimum.
+  float dmin = INFINITY;
+  if (gi == 0) {
+    for (int p = g; p < md.n_pairs; p += kGroup) dmin = fminf(dmin, pair_dist(L, gc, p, bx, by, bz, bn, bc, bc2));
+  } else {
+    const int m0 = gstart[gi - 1], m1 = gstart[gi];
+    for (int l = 0; l < md.n_links; ++l) {
+      const int k0 = L.link_sph[l], k1 = L.link_sph[l + 1];
+      if (k0 >= k1) continue;
+      for (int m = m0 + g; m < m1; m += kGroup)
+        dmin = fminf(dmin, link_world_dist(gc, k0, k1, L.world[m], bx, by, bz, bn, bc));
+    }
+  }
+  dmin = group8_min(dmin);
+  if (!(dmin < INFINITY)) {  // empty group
+    dk = INFINITY;
+    return;
+  }
+  // Pass 2: weights and their gradients.
+  float z = 0.f;
+  if (gi == 0) {
+    for (int p = g; p < md.n_pairs; p += kGroup) {
+      const float d = pair_dist(L, gc, p, bx, by, bz, bn, bc, bc2);
+      const float w = __expf(-(d - dmin) / tau);
+      if (!(w > 0.f)) continue;
+      z += w;
+      const float k = w / bn;
+      const int li = L.pair_links[2 * p], lj = L.pair_links[2 * p + 1];
+      const int ni = L.link_node[li], nj = L.link_node[lj];
+      pull_point<MAXN, true>(K, nj >= 0 ? L.anc[nj] : 0, bc2.x, bc2.y, bc2.z, k * bx, k * by, k * bz, gn);
+      pull_point<MAXN, true>(K, ni >= 0 ? L.anc[ni] : 0, bc.x, bc.y, bc.z, -k * bx, -k * by, -k * bz, gn);
+    }
+  } else {
+    const int m0 = gstart[gi - 1], m1 = gstart[gi];
+    for (int l = 0; l < md.n_links; ++l) {
+      const int k0 = L.link_sph[l], k1 = L.link_sph[l + 1];
+      if (k0 >= k1) continue;
+      float F[3] = {0.f, 0.f, 0.f}, tq[3] = {0.f, 0.f, 0.f};
+      for (int m = m0 + g; m < m1; m += kGroup) {
+        const float d = link_world_dist(gc, k0, k1, L.world[m], bx, by, bz, bn, bc);
+        const float w = __expf(-(d - dmin) / tau);
+        if (!(w > 0.f)) continue;
+        z += w;
+        const float k = -w / bn;  // d(distance)/dc = -(o - c + e) / |o - c + e|
+        const float gx = k * bx, gy = k * by, gz = k * bz;
+        F[0] += gx; F[1] += gy; F[2] += gz;
+        tq[0] += bc.y * gz - bc.z * gy;
+        tq[1] += bc.z * gx - bc.x * gz;
+        tq[2] += bc.x * gy - bc.y * gx;
+      }
+      const int node = L.link_node[l];
+      pull_wrench<MAXN>(K, node >= 0 ? L.anc[node] : 0, F, tq, gn);
+    }
+  }
+  z = group_sum(z);
+  dk = dmin - tau * __logf(z);
+  const float inv = 1.f / z;
+#pragma unroll
+  for (int a = 0; a < MAXN; ++a) gn[a] = group_sum(gn[a]) * inv;
+}
+
+struct ScoLinArgs {
+  TrajModelDev md;
+  const float* x;
+  const long long* start;
+  int n_traj, max_len;
+  long long n_waypoints;
+  const int* gstart;  // (n_wg + 1) CSR of the world groups over the world spheres
+  int n_wg;
+  float tau;
+  float* out_d;  // (W, G)
+  float* out_J;  // (W, G, A)
+};
+
+template <int MAXN>
+__global__ __launch_bounds__(kMaxTrajThreads) void traj_sco_linearize_kernel(ScoLinArgs a) {
+  extern __shared__ float4 smem4[];
+  const TrajLds L = stage_traj_model(a.md, a.max_len, reinterpret_cast<char*>(smem4));
+  const int b = blockIdx.x;
+  if (b >= a.n_traj) return;  // uniform per block
+  const long long w0 = a.start[b], w1 = a.start[b + 1];
+  if (w0 < 0 || w1 < w0 || w1 > a.n_waypoints || w1 - w0 > a.max_len) return;  // uniform per block
+  const int T = static_cast<int>(w1 - w0);
+  const int A = a.md.n_act;
+  const int G = 1 + a.n_wg;
+  const int grp = threadIdx.x / kGroup;
+  const int g = threadIdx.x % kGroup;
+  float4* gc = L.gc + grp * (a.md.n_spheres + a.md.n_links);
+  const int n_groups = blockDim.x / kGroup;
+  const int rounds = (T + n_groups - 1) / n_groups;
+  for (int rd = 0; rd < rounds; ++rd) {
+    const int t = rd * n_groups + grp;
+    const bool active = t < T;
+    WaypointNodes<MAXN, true> K;
+    if (active) {
+      float q[MAXN];
+#pragma unroll
+      for (int c = 0; c < MAXN; ++c) q[c] = (c < A) ? a.x[(w0 + t) * A + c] : 0.f;
+      waypoint_fk<MAXN, true>(a.md, L, q, K);
+      waypoint_centres<MAXN, true>(a.md, L, K, gc);
+    }
+    __syncthreads();
+    if (active) {
+      for (int gi = 0; gi < G; ++gi) {
+        float dk, gn[MAXN], gq[MAXN];
+        group_smooth_min<MAXN>(a.md, L, K, gc, a.gstart, gi, a.tau, dk, gn);
+        if (!(dk < INFINITY)) {
+#pragma unroll
+          for (int c = 0; c < MAXN; ++c) gq[c] = 0.f;
+        } else {
+          node_to_actuated<MAXN>(L, a.md.n_nodes, gn, gq);
+        }
+        const long long row = (w0 + t) * G + gi;
+        if (g == 0) a.out_d[row] = dk;
+#pragma unroll
+        for (int c = 0; c < MAXN; ++c)
+          if (c < A && c % kGroup == g) a.out_J[row * A + c] = gq[c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// The inner problem of _lbfgs_inner_solve (_sco_optimization.py:291-418) for
+// one trajectory per workgroup, entirely in LDS: smoothness (velocity,
+// acceleration, jerk) + limits + w_coll sum max(0, margin - d_lin)^2 with
+// d_lin = d_k + J_k (x_t - q_k,t) + w_trust |x - q_k|^2; n_inner L-BFGS steps
+// from x = q_k (history m, curvature test, two-loop recursion, five-step line
+// search with sufficient decrease 1e-4 evaluated in order), returning the best
+// iterate.
+struct ScoInnerArgs {
+  const float* x;       // (W, A) linearisation point = start
+  const long long* start;
+  int n_traj, max_len, n_act, G, m_hist, n_inner;
+  long long n_waypoints;
+  const float* d;       // (W, G)
+  const float* J;       // (W, G, A)
+  const float* lower;
+  const float* upper;
+  TrajWeights w;
+  float w_coll, margin, w_trust;
+  float* out_x;         // (W, A)
+};
+
+__host__ __device__ inline size_t sco_inner_lds_floats(int max_len, int A, int G, int m) {
+  const size_t n = static_cast<size_t>(max_len) * A;
+  return (9 + 2 * static_cast<size_t>(m)) * n + static_cast<size_t>(max_len) * G * (A + 1) + 2 * A +
+         static_cast<size_t>(kMaxTrajWaves) * kRedSlots;
+}
+
+// Inner cost at y (LDS) and, with grad, its gradient into gout (endpoint rows zeroed).
+__device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const float* y, const float* qk,
+                                                const float* dk, const float* Jk, const float* lo, const float* hi,
+                                                int T, float* acc, float* gout, float* red, bool grad) {
+  const int A = a.n_act, G = a.G;
+  const int n = T * A;
+  const int tid = threadIdx.x;
+  const int na = T - 4;
+  float sm = 0.f, rest = 0.f;
+  for (int i = tid; i < n; i += blockDim.x) {
+    const int c = i % A;
+    const float xv = y[i];
+    const float v = fmaxf(xv - hi[c], 0.f) + fmaxf(lo[c] - xv, 0.f);
+    rest += a.w.w_limits * v * v;
+    const float dq = xv - qk[i];
+    rest += a.w_trust * dq * dq;
+    if (i + A < n) {
+      const float dv = y[i + A] - xv;
+      sm += a.w.w_vel * dv * dv;
+    }
+    if (i < na * A) {
+      const float ac = (-xv + 16.f * y[i + A] - 30.f * y[i + 2 * A] + 16.f * y[i + 3 * A] - y[i + 4 * A]) / 12.f;
+      acc[i] = ac;
+      sm += a.w.w_acc * ac * ac;
+    }
+    if (grad) gout[i] = a.w.w_limits * 2.f * v * ((xv > hi[c] ? 1.f : 0.f) - (lo[c] > xv ? 1.f : 0.f)) +
+                        2.f * a.w_trust * dq;
+  }
+  // Linearised hinge per (waypoint, group).
+  for (int r = tid; r < T * G; r += blockDim.x) {
+    const int t = r / G;
+    const float* Jr = Jk + static_cast<long long>(r) * A;
+    float dl = dk[r];
+    for (int c = 0; c < A; ++c) dl += Jr[c] * (y[t * A + c] - qk[t * A + c]);
+    const float h = a.margin - dl;
+    if (h > 0.f) rest += a.w_coll * h * h;
+  }
+  __syncthreads();
+  for (int i = tid; i < na * A; i += blockDim.x) {
+    const int t = i / A;
+    if (t + 1 < na) {
+      const float jk = acc[i + A] - acc[i];
+      sm += a.w.w_jerk * jk * jk;
+    }
+  }
+  if (grad) {
+    // hinge gradient, one (waypoint, group) row per thread-iteration, added per row element
+    for (int i = tid; i < n; i += blockDim.x) {
+      const int t = i / A, c = i % A;
+      float gh = 0.f;
+      for (int gg = 0; gg < G; ++gg) {
+        const int r = t * G + gg;
+        const float* Jr = Jk + static_cast<long long>(r) * A;
+        float dl = dk[r];
+        for (int cc = 0; cc < A; ++cc) dl += Jr[cc] * (y[t * A + cc] - qk[t * A + cc]);
+        const float h = a.margin - dl;
+        if (h > 0.f) gh -= 2.f * a.w_coll * h * Jr[c];
+      }
+      gout[i] += gh;
+    }
+    __syncthreads();
+    const float c5[5] = {-1.f / 12.f, 16.f / 12.f, -30.f / 12.f, 16.f / 12.f, -1.f / 12.f};
+    for (int i = tid; i < n; i += blockDim.x) {
+      const int t = i / A;
+      float gs = 0.f;
+      if (t >= 1) gs += 2.f * a.w.w_vel * (y[i] - y[i - A]);
+      if (t + 1 < T) gs -= 2.f * a.w.w_vel * (y[i + A] - y[i]);
+#pragma unroll
+      for (int o = 0; o < 5; ++o) {
+        const int tt = t - o;
+        if (tt >= 0 && tt < na) {
+          const int j = i - o * A;
+          float ga = 2.f * a.w.w_acc * acc[j];
+          if (tt + 1 < na) ga -= 2.f * a.w.w_jerk * (acc[j + A] - acc[j]);
+          if (tt >= 1) ga += 2.f * a.w.w_jerk * (acc[j] - acc[j - A]);
+          gs += ga * c5[o];
+        }
+      }
+      float gv = gout[i] + a.w.w_smooth * gs;
+      if (t == 0 || t == T - 1) gv = 0.f;
+      gout[i] = gv;
+    }
+  }
+  float v2[2] = {sm, rest};
+  block_sum<2>(v2, red);
+  return a.w.w_smooth * v2[0] + v2[1];
+}
+
+__global__ __launch_bounds__(256) void traj_sco_inner_kernel(ScoInnerArgs a) {
+  extern __shared__ float4 smem4[];
+  const int b = blockIdx.x;
+  if (b >= a.n_traj) return;  // uniform per block
+  const long long w0 = a.start[b], w1 = a.start[b + 1];
+  if (w0 < 0 || w1 < w0 || w1 > a.n_waypoints || w1 - w0 > a.max_len) return;  // uniform per block
+  const int T = static_cast<int>(w1 - w0);
+  const int A = a.n_act, G = a.G, m = a.m_hist;
+  const int n = T * A;
+  const int cap = a.max_len * A;
+  const int tid = threadIdx.x;
+  float* base = reinterpret_cast<float*>(smem4);
+  float* qk = base;
+  float* x = qk + cap;
+  float* g = x + cap;
+  float* dir = g + cap;
+  float* xp = dir + cap;
+  float* gp = xp + cap;
+  float* best = gp + cap;
+  float* acc = best + cap;
+  float* xt = acc + cap;
+  float* sb = xt + cap;             // (m, cap)
+  float* yb = sb + m * cap;         // (m, cap)
+  float* dk = yb + m * cap;         // (max_len, G)
+  float* Jk = dk + a.max_len * G;   // (max_len, G, A)
+  float* lo = Jk + a.max_len * G * A;
+  float* hi = lo + A;
+  float* red = hi + A;
+  for (int i = tid; i < n; i += blockDim.x) {
+    qk[i] = a.x[w0 * A + i];
+    x[i] = qk[i];
+  }
+  for (int i = tid; i < T * G; i += blockDim.x) dk[i] = a.d[w0 * G + i];
+  for (int i = tid; i < T * G * A; i += blockDim.x) Jk[i] = a.J[w0 * G * A + i];
+  for (int c = tid; c < A; c += blockDim.x) { lo[c] = a.lower[c]; hi[c] = a.upper[c]; }
+  __syncthreads();
+  float rho[kMaxHist];
+#pragma unroll
+  for (int h = 0; h < kMaxHist; ++h) rho[h] = 0.f;
+  int m_used = 0, newest = 0;
+  float best_cost = sco_inner_eval(a, x, qk, dk, Jk, lo, hi, T, acc, g, red, true);
+  for (int i = tid; i < n; i += blockDim.x) { best[i] = x[i]; xp[i] = x[i]; gp[i] = g[i]; }
+  __syncthreads();
+  for (int k = 0; k < a.n_inner; ++k) {
+    const float cost = sco_inner_eval(a, x, qk, dk, Jk, lo, hi, T, acc, g, red, true);
+    float syy[2] = {0.f, 0.f};
+    for (int i = tid; i < n; i += blockDim.x) {
+      const float sv = x[i] - xp[i], yv = g[i] - gp[i];
+      syy[0] += sv * yv;
+      syy[1] += yv * yv;
+    }
+    block_sum<2>(syy, red);
+    if (syy[0] > 1e-10f * syy[1] + 1e-30f && k > 0) {
+      newest = (newest + 1) % m;
+      for (int i = tid; i < n; i += blockDim.x) {
+        sb[newest * cap + i] = x[i] - xp[i];
+        yb[newest * cap + i] = g[i] - gp[i];
+      }
+      const float r = 1.f / (syy[0] + 1e-30f);
+#pragma unroll
+      for (int h = 0; h < kMaxHist; ++h)
+        if (h == newest) rho[h] = r;
+      if (m_used < m) ++m_used;
+    }
+    __syncthreads();
+    if (m_used > 0) {
+      float alph[kMaxHist];
+#pragma unroll
+      for (int h = 0; h < kMaxHist; ++h) alph[h] = 0.f;
+      for (int i = tid; i < n; i += blockDim.x) dir[i] = g[i];
+      for (int j = 0; j < m_used; ++j) {
+        const int idx = (newest - j + m) % m;
+        float sq = 0.f;
+        for (int i = tid; i < n; i += blockDim.x) sq += sb[idx * cap + i] * dir[i];
+        const float al = sel_hist(rho, idx) * block_sum1(sq, red);
+#pragma unroll
+        for (int h = 0; h < kMaxHist; ++h)
+          if (h == idx) alph[h] = al;
+        for (int i = tid; i < n; i += blockDim.x) dir[i] -= al * yb[idx * cap + i];
+      }
+      float g2[2] = {0.f, 0.f};
+      for (int i = tid; i < n; i += blockDim.x) {
+        g2[0] += sb[newest * cap + i] * yb[newest * cap + i];
+        g2[1] += yb[newest * cap + i] * yb[newest * cap + i];
+      }
+      block_sum<2>(g2, red);
+      const float gamma = g2[0] / (g2[1] + 1e-18f);
+      for (int i = tid; i < n; i += blockDim.x) dir[i] *= gamma;
+      for (int st = 0; st < m_used; ++st) {
+        const int idx = (newest - m_used + 1 + st + 2 * m) % m;
+        float yr = 0.f;
+        for (int i = tid; i < n; i += blockDim.x) yr += yb[idx * cap + i] * dir[i];
+        const float beta = sel_hist(rho, idx) * block_sum1(yr, red);
+        const float coef = sel_hist(alph, idx) - beta;
+        for (int i = tid; i < n; i += blockDim.x) dir[i] += sb[idx * cap + i] * coef;
+      }
+      for (int i = tid; i < n; i += blockDim.x) dir[i] = -dir[i];
+    } else {
+      float gg = 0.f;
+      for (int i = tid; i < n; i += blockDim.x) gg += g[i] * g[i];
+      const float inv = 1.f / (sqrtf(block_sum1(gg, red)) + 1e-18f);
+      for (int i = tid; i < n; i += blockDim.x) dir[i] = -g[i] * inv;
+    }
+    for (int i = tid; i < n; i += blockDim.x) {
+      const int t = i / A;
+      if (t == 0 || t == T - 1) dir[i] = 0.f;
+    }
+    __syncthreads();
+    const float thresh = cost * (1.f - 1e-4f);
+    int pick = -1;
+    float pick_cost = INFINITY;
+    int first_best = 0;
+    for (int kk = 0; kk < kNumAlphas; ++kk) {
+      const float al = traj_alpha(kk);
+      for (int i = tid; i < n; i += blockDim.x) xt[i] = x[i] + al * dir[i];
+      __syncthreads();
+      const float c = sco_inner_eval(a, xt, qk, dk, Jk, lo, hi, T, acc, nullptr, red, false);
+      if (kk == 0 || c < pick_cost) { pick_cost = c; first_best = kk; }
+      if (c < thresh) { pick = kk; pick_cost = c; break; }
+    }
+    if (pick < 0) pick = first_best;
+    const float al = traj_alpha(pick);
+    for (int i = tid; i < n; i += blockDim.x) {
+      xp[i] = x[i];
+      gp[i] = g[i];
+      x[i] += al * dir[i];
+    }
+    if (pick_cost < best_cost) {
+      best_cost = pick_cost;
+      for (int i = tid; i < n; i += blockDim.x) best[i] = x[i];
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += blockDim.x) a.out_x[w0 * A + i] = best[i];
+}
+
 }  // namespace pk
 
 namespace {
@@ -1126,4 +1791,134 @@ extern "C" int pk_traj_lbfgs_f32(void* stream, const pk_traj_model* model, const
   else
     hipLaunchKernelGGL(pk::traj_lbfgs_kernel<16>, dim3(blocks), dim3(threads), lds, s, a);
   return pk::check_launch("traj_lbfgs_kernel");
+}
+
+extern "C" int pk_traj_chomp_f32(void* stream, const pk_traj_model* model, const float* x_init,
+                                 const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints, int32_t max_len,
+                                 int32_t block_threads, const pk_chomp_config* cfg, float* out_traj,
+                                 float* out_cost, float* out_trace) {
+  pk::TrajModelDev d{};
+  int maxn = 8;
+  const int st = fill_traj_model(model, d, maxn);
+  if (st != PK_OK) return st;
+  PK_REQUIRE(cfg != nullptr, "null config");
+  PK_REQUIRE(n_traj >= 0 && n_waypoints >= 0 && max_len >= 1, "invalid trajectory sizes");
+  PK_REQUIRE(cfg->n_iters >= 0 && cfg->n_cg_iters >= 0 && cfg->early_stop_patience >= 0,
+             "invalid CHOMP configuration (negative iteration counts)");
+  PK_REQUIRE(cfg->max_delta_per_step >= 0.f, "negative max_delta_per_step");
+  if (n_traj == 0) return PK_OK;
+  PK_REQUIRE(x_init && traj_start && out_traj && out_cost, "null pointer");
+  int threads = 0;
+  const int ts = pick_threads(block_threads, max_len, threads);
+  if (ts != PK_OK) return ts;
+  const size_t lds = traj_lds_bytes(d, max_len, threads);
+  PK_REQUIRE(lds <= 150 * 1024, "trajectory model needs %zu bytes of LDS (limit 150 KiB)", lds);
+  pk::ChompArgs a{};
+  a.md = d;
+  a.x_init = x_init;
+  a.start = reinterpret_cast<const long long*>(traj_start);
+  a.n_traj = n_traj;
+  a.max_len = max_len;
+  a.n_waypoints = n_waypoints;
+  a.w = pk::TrajWeights{cfg->w_smooth, 0.f, cfg->w_acc, cfg->w_jerk, cfg->w_limits};
+  a.step = cfg->step_size;
+  a.w_coll0 = cfg->w_collision;
+  a.w_coll_max = cfg->w_collision_max;
+  a.scale = cfg->collision_penalty_scale;
+  a.reg = cfg->smoothness_reg;
+  a.cg_tol = cfg->cg_tol;
+  a.clip = cfg->grad_clip_norm;
+  a.max_delta = cfg->max_delta_per_step;
+  a.min_improve = cfg->min_cost_improve;
+  a.n_iters = cfg->n_iters;
+  a.covariant = cfg->use_covariant_update;
+  a.n_cg = cfg->n_cg_iters;
+  a.patience = cfg->early_stop_patience;
+  a.out_traj = out_traj;
+  a.out_cost = out_cost;
+  a.out_trace = out_trace;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (maxn == 8)
+    hipLaunchKernelGGL(pk::traj_chomp_kernel<8>, dim3(n_traj), dim3(threads), lds, s, a);
+  else
+    hipLaunchKernelGGL(pk::traj_chomp_kernel<16>, dim3(n_traj), dim3(threads), lds, s, a);
+  return pk::check_launch("traj_chomp_kernel");
+}
+
+extern "C" int pk_traj_sco_linearize_f32(void* stream, const pk_traj_model* model, const float* x,
+                                         const int64_t* traj_start, int32_t n_traj, int64_t n_waypoints,
+                                         int32_t max_len, int32_t block_threads, const int32_t* world_group_start,
+                                         int32_t n_world_groups, float temperature, float* out_d, float* out_J) {
+  pk::TrajModelDev d{};
+  int maxn = 8;
+  const int st = fill_traj_model(model, d, maxn);
+  if (st != PK_OK) return st;
+  PK_REQUIRE(n_traj >= 0 && n_waypoints >= 0 && max_len >= 1, "invalid trajectory sizes");
+  PK_REQUIRE(n_world_groups >= 0 && n_world_groups + 1 <= pk::kMaxScoGroups,
+             "n_world_groups=%d outside [0, %d]", n_world_groups, pk::kMaxScoGroups - 1);
+  PK_REQUIRE(temperature > 0.f, "temperature must be > 0");
+  if (n_traj == 0) return PK_OK;
+  PK_REQUIRE(x && traj_start && out_d && out_J && (n_world_groups == 0 || world_group_start), "null pointer");
+  int threads = 0;
+  const int ts = pick_threads(block_threads, max_len, threads);
+  if (ts != PK_OK) return ts;
+  const size_t lds = traj_lds_bytes(d, max_len, threads);
+  PK_REQUIRE(lds <= 150 * 1024, "trajectory model needs %zu bytes of LDS (limit 150 KiB)", lds);
+  pk::ScoLinArgs a{};
+  a.md = d;
+  a.x = x;
+  a.start = reinterpret_cast<const long long*>(traj_start);
+  a.n_traj = n_traj;
+  a.max_len = max_len;
+  a.n_waypoints = n_waypoints;
+  a.gstart = world_group_start;
+  a.n_wg = n_world_groups;
+  a.tau = temperature;
+  a.out_d = out_d;
+  a.out_J = out_J;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (maxn == 8)
+    hipLaunchKernelGGL(pk::traj_sco_linearize_kernel<8>, dim3(n_traj), dim3(threads), lds, s, a);
+  else
+    hipLaunchKernelGGL(pk::traj_sco_linearize_kernel<16>, dim3(n_traj), dim3(threads), lds, s, a);
+  return pk::check_launch("traj_sco_linearize_kernel");
+}
+
+extern "C" int pk_traj_sco_inner_f32(void* stream, const float* x, const int64_t* traj_start, int32_t n_traj,
+                                     int64_t n_waypoints, int32_t max_len, int32_t n_act, int32_t n_groups,
+                                     const float* d, const float* J, const float* lower, const float* upper,
+                                     const pk_sco_config* cfg, float w_collision, float* out_x) {
+  PK_REQUIRE(cfg != nullptr, "null config");
+  PK_REQUIRE(n_traj >= 0 && n_waypoints >= 0 && max_len >= 1, "invalid trajectory sizes");
+  PK_REQUIRE(n_act >= 1 && n_act <= 16, "n_act=%d outside [1, 16]", n_act);
+  PK_REQUIRE(n_groups >= 1 && n_groups <= pk::kMaxScoGroups, "n_groups=%d outside [1, %d]", n_groups,
+             pk::kMaxScoGroups);
+  PK_REQUIRE(cfg->m_lbfgs >= 1 && cfg->m_lbfgs <= pk::kMaxHist && cfg->n_inner_iters >= 0,
+             "invalid SCO configuration (m_lbfgs in [1, %d])", pk::kMaxHist);
+  if (n_traj == 0) return PK_OK;
+  PK_REQUIRE(x && traj_start && d && J && lower && upper && out_x, "null pointer");
+  const size_t lds = sizeof(float) * pk::sco_inner_lds_floats(max_len, n_act, n_groups, cfg->m_lbfgs);
+  PK_REQUIRE(lds <= 150 * 1024, "SCO inner problem needs %zu bytes of LDS (limit 150 KiB): max_len too long",
+             lds);
+  pk::ScoInnerArgs a{};
+  a.x = x;
+  a.start = reinterpret_cast<const long long*>(traj_start);
+  a.n_traj = n_traj;
+  a.max_len = max_len;
+  a.n_act = n_act;
+  a.G = n_groups;
+  a.m_hist = cfg->m_lbfgs;
+  a.n_inner = cfg->n_inner_iters;
+  a.n_waypoints = n_waypoints;
+  a.d = d;
+  a.J = J;
+  a.lower = lower;
+  a.upper = upper;
+  a.w = pk::TrajWeights{cfg->w_smooth, cfg->w_vel, cfg->w_acc, cfg->w_jerk, cfg->w_limits};
+  a.w_coll = w_collision;
+  a.margin = cfg->collision_margin;
+  a.w_trust = cfg->w_trust;
+  a.out_x = out_x;
+  hipLaunchKernelGGL(pk::traj_sco_inner_kernel, dim3(n_traj), dim3(256), lds, static_cast<hipStream_t>(stream), a);
+  return pk::check_launch("traj_sco_inner_kernel");
 }

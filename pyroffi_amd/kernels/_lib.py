@@ -68,6 +68,26 @@ class TrajModel(ctypes.Structure):
     ]
 
 
+class ChompConfig(ctypes.Structure):
+    """``pk_chomp_config``."""
+
+    _fields_ = [(n, ctypes.c_int32) for n in ("n_iters", "use_covariant_update", "n_cg_iters",
+                                               "early_stop_patience")] + \
+               [(n, ctypes.c_float) for n in ("step_size", "w_smooth", "w_acc", "w_jerk", "w_collision",
+                                               "w_collision_max", "collision_penalty_scale", "w_limits",
+                                               "smoothness_reg", "cg_tol", "grad_clip_norm", "max_delta_per_step",
+                                               "min_cost_improve")]
+
+
+class ScoConfig(ctypes.Structure):
+    """``pk_sco_config``."""
+
+    _fields_ = [(n, ctypes.c_int32) for n in ("n_outer_iters", "n_inner_iters", "m_lbfgs")] + \
+               [(n, ctypes.c_float) for n in ("w_smooth", "w_vel", "w_acc", "w_jerk", "w_collision",
+                                               "w_collision_max", "penalty_scale", "collision_margin", "w_trust",
+                                               "w_limits", "smooth_min_temperature")]
+
+
 class LbfgsConfig(ctypes.Structure):
     """``pk_lbfgs_config``."""
 
@@ -158,6 +178,9 @@ _SIGNATURES = {
     "pk_traj_cost_grad_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _p, _p, _p]),
     "pk_traj_lbfgs_state_floats": (_i64, [_i64, _i32, _i32, _i32]),
     "pk_traj_lbfgs_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _p, _i32, _i32, _p, _p, _p]),
+    "pk_traj_sco_linearize_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _f32, _p, _p]),
+    "pk_traj_sco_inner_f32": (_i32, [_p, _p, _p, _i32, _i64, _i32, _i32, _i32, _p, _p, _p, _p, _p, _f32, _p]),
+    "pk_traj_chomp_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _p, _p, _p]),
     "pk_traj_lbfgs_noalloc_f32": (_i32, [_p, _p, _p, _p, _i32, _i64, _i32, _i32, _p, _i32, _p, _i32, _i32, _p, _p]),
     "pk_ls_ik_select_segments_f32": (_i32, [_p, _p, _p, _p, _p, _i32, _i32, _f32, _p, _p]),
     "pk_pose_hinge_cost_f32": (

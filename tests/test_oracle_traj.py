@@ -15,6 +15,11 @@ recursion and the direct L-BFGS loop.
   limits) it converges to the minimum-energy path from numpy's linear solve;
   in a colliding and in an empty world its accepted steps, last iterate and
   tracked best equal a dense-algebra restatement of the loop step for step.
+* CHOMP      -- the truncated CG solves the dense pentadiagonal precision
+  operator (numpy) when run long enough and stops where its tolerance says;
+  one CHOMP step (clip, endpoint rows, covariant or plain direction, six
+  clamped trials, first-minimum pick, accept rule) equals a dense numpy step
+  on the pinned cost and gradient; early stopping and monotone accepted costs.
 """
 
 import independent
@@ -281,3 +286,173 @@ def test_lbfgs_iterates_match_dense_restatement(oracle_mod, noise, M, margin, mo
         assert set(picks) == {0, 1, 2, 3, 4}  # every step size of the list is taken
     else:
         assert np.abs(best - x0).max() > 1e-2  # the tracked best moved
+
+
+# ----------------------------------------------------------------- CHOMP
+def _chomp_matrix(n, reg):
+    """Dense interior precision operator of _chomp_metric_apply_1d: the pentadiagonal (1, -4, 6, -4, 1) + reg I."""
+    M = (6.0 + reg) * np.eye(n)
+    for k, v in ((1, -4.0), (2, 1.0)):
+        M += v * (np.eye(n, k=k) + np.eye(n, k=-k))
+    return M
+
+
+@pytest.mark.parametrize("n,reg", [(5, 1e-3), (30, 1e-3), (126, 0.5)])
+def test_chomp_cg_solves_the_precision_operator(oracle_mod, n, reg):
+    """Enough CG steps solve M x = b (scaled so the reference's 1e-9 guards are negligible)."""
+    b = np.random.default_rng(n).normal(size=n) * 1e3
+    x = oracle_mod.chomp_cg(b, reg, 4 * n + 50, 1e-15)
+    ref = np.linalg.solve(_chomp_matrix(n, reg), b)
+    np.testing.assert_allclose(x, ref, rtol=0, atol=1e-6 * np.abs(ref).max())
+
+
+def test_chomp_cg_truncation(oracle_mod):
+    """n steps of CG on an n x n system reach the solution, fewer steps do not; a
+    loose tolerance stops early (the stopping test compares |r|^2 with (tol |b|)^2)."""
+    n = 8
+    b = np.random.default_rng(1).normal(size=n) * 1e3
+    M = _chomp_matrix(n, 0.5)
+    ref = np.linalg.solve(M, b)
+    full = oracle_mod.chomp_cg(b, 0.5, n, 1e-15)
+    np.testing.assert_allclose(full, ref, atol=1e-6 * np.abs(ref).max())
+    part = oracle_mod.chomp_cg(b, 0.5, 2, 1e-15)
+    assert np.abs(part - ref).max() > 1e-3 * np.abs(ref).max()
+    loose = oracle_mod.chomp_cg(b, 0.5, 50, 0.5)
+    assert np.linalg.norm(M @ loose - b) <= 0.5 * np.linalg.norm(b) + 1e-9
+    assert np.abs(loose - ref).max() > 1e-6 * np.abs(ref).max()
+
+
+def _chomp_one_step(oracle_mod, md, x0, lo, hi, covariant, step=0.05, clip=10.0, max_delta=0.05, w_coll=3.0):
+    """One CHOMP iteration in dense numpy around the oracle's pinned cost and gradient."""
+    w = (1.0, 0.0, 0.5, 0.1, 1.0, w_coll)
+    comps, g = oracle_mod.traj_cost(md, [x0], lo, hi, w, grad=True)
+    cost = comps[0] @ [1.0, 1.0, w_coll]
+    gn = np.linalg.norm(g)
+    if clip > 0 and gn > clip:
+        g = g * clip / (gn + 1e-12)
+    g[0] = g[-1] = 0.0
+    if covariant:
+        T = len(x0)
+        d = np.zeros_like(g)
+        d[1:-1] = -np.linalg.solve(_chomp_matrix(T - 2, 1e-3), g[1:-1])
+    else:
+        d = -g
+    trials = []
+    for a in (1.0, 0.5, 0.25, 0.1, 0.05, 0.0):
+        xt = x0 + np.clip(step * a * d, -max_delta, max_delta)
+        c, _ = oracle_mod.traj_cost(md, [xt], lo, hi, w, grad=False)
+        trials.append((c[0] @ [1.0, 1.0, w_coll], xt))
+    k = int(np.argmin([c for c, _ in trials]))
+    improved = trials[k][0] < cost - 1e-5
+    return (trials[k][1] if improved else x0), k, improved
+
+
+@pytest.mark.parametrize("covariant", [False, True])
+@pytest.mark.parametrize("noise", [0.0, 0.05])
+def test_chomp_step_matches_dense_restatement(oracle_mod, covariant, noise):
+    """The oracle's first iterate, picked trial and accept decision equal a dense
+    numpy step built on the (separately pinned) cost and gradient, with the
+    covariant direction from numpy's linear solve (CG run to convergence)."""
+    robot, _, _, t, md, _ = _model(oracle_mod, M=60, margin=0.05, mode=0, seed=3)
+    rng = np.random.default_rng(4)
+    x0 = _line(robot, 40, seed=5) + rng.normal(0, noise, (40, 7))
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    want, k, imp = _chomp_one_step(oracle_mod, md, x0, lo, hi, covariant)
+    got, _, _, tr = oracle_mod.traj_chomp(md, [x0], lo, hi, n_iters=1, use_covariant_update=covariant,
+                                          n_cg_iters=400, cg_tol=1e-14, trace=True)
+    assert tr[0, 0, 1] == k and tr[0, 0, 2] == imp
+    moved = np.abs(want - x0).max()
+    assert imp and moved > 1e-3
+    # The reference's CG guards (+1e-9 in both divisions) stall its residual near
+    # |r|^2 ~ 1e-9, so the covariant step matches the exact solve to ~1e-3 of the step.
+    np.testing.assert_allclose(got, want, atol=(2e-3 * moved if covariant else 1e-9), rtol=0)
+
+
+def test_chomp_early_stop_and_weight_schedule(oracle_mod):
+    """A path that cannot improve (straight, in free space) is stopped after
+    `patience` iterations; on a colliding path the accepted costs fall while the
+    collision weight is held (scale 1)."""
+    robot, _, _, t, md, _ = _model(oracle_mod, M=0, pairs=False)
+    md.world = np.array([[100.0, 100.0, 100.0, 0.01]])
+    lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
+    q0 = np.array([0.0, -0.785, 0.0, -2.356, 0.0, 1.571, 0.785])
+    s = np.linspace(0, 1, 30)[:, None]
+    line = (1 - s) * q0 + s * (q0 + 0.2)
+    out, _, _, tr = oracle_mod.traj_chomp(md, [line], lo, hi, n_iters=10, early_stop_patience=3, trace=True)
+    np.testing.assert_array_equal(tr[0, :3, 2], 0)
+    np.testing.assert_array_equal(tr[0, 3:], -1)
+    np.testing.assert_array_equal(out, line)
+    robot, _, _, t, md, _ = _model(oracle_mod, M=80, margin=0.05, mode=0, seed=7)
+    x0 = _line(robot, 32, seed=8)
+    _, _, _, tr = oracle_mod.traj_chomp(md, [x0], lo, hi, n_iters=25, collision_penalty_scale=1.0, trace=True)
+    c = tr[0, :, 0]
+    run = c[tr[0, :, 2] >= 0]
+    assert np.all(np.diff(run) <= 1e-12)
+
+
+# ----------------------------------------------------------------- SCO
+def _sco_model(oracle_mod, M=120, seed=11):
+    robot, path, coll, t, md, world = _model(oracle_mod, M=M, margin=0.01, mode=1, seed=seed)
+    gstart = np.array([0, M // 2, M], np.int32)
+    return robot, coll, t, md, world, gstart
+
+
+def test_sco_linearize_gradient_matches_finite_differences(oracle_mod):
+    robot, _, _, md, _, gs = _sco_model(oracle_mod)
+    q = random_cfgs(robot, 6, seed=12)
+    d, J = oracle_mod.sco_linearize(md, q, gs, 0.05)
+    assert d.shape == (6, 3) and J.shape == (6, 3, 7)
+    h = 1e-6
+    for a in range(7):
+        qp, qm = q.copy(), q.copy()
+        qp[:, a] += h
+        qm[:, a] -= h
+        fd = (oracle_mod.sco_linearize(md, qp, gs, 0.05)[0] - oracle_mod.sco_linearize(md, qm, gs, 0.05)[0]) / (2 * h)
+        np.testing.assert_allclose(J[:, :, a], fd, atol=1e-6 * max(1.0, np.abs(J).max()))
+
+
+def test_sco_smooth_min_bounds_and_limit(oracle_mod):
+    """dmin - T log(n) <= smooth min <= dmin, and it tends to the group minimum of the
+    JAX distances (same-slot self pairs; closest sphere per link and world sphere)."""
+    robot, coll, t, md, world, gs = _sco_model(oracle_mod)
+    q = random_cfgs(robot, 5, seed=13)
+    lp = oracle_mod.fk_links(t, q)
+    sp = oracle_mod.padded_spheres(coll)
+    selfd = oracle_mod.self_dist(lp, sp, coll.active_idx_i, coll.active_idx_j, mode=1)
+    full, _ = oracle_mod.world_dist(lp, sp, world, bias=True)
+    mins = np.stack([selfd.min(1), full[:, :, :60].min((1, 2)), full[:, :, 60:].min((1, 2))], 1)
+    counts = np.array([selfd.shape[1], full.shape[1] * 60, full.shape[1] * 60])
+    for tau in (0.05, 0.01):
+        d, _ = oracle_mod.sco_linearize(md, q, gs, tau)
+        assert np.all(d <= mins + 1e-12) and np.all(d >= mins - tau * np.log(counts) - 1e-12)
+    d, _ = oracle_mod.sco_linearize(md, q, gs, 1e-5)
+    np.testing.assert_allclose(d, mins, atol=1e-5 * np.log(counts).max())
+
+
+def test_sco_inner_without_contact_reaches_the_quadratic_minimum(oracle_mod):
+    """With the linearised hinge inactive (a negative margin) the inner problem is
+    smoothness + trust region, a quadratic with pinned endpoints: enough L-BFGS
+    steps reach numpy's linear solve of its normal equations."""
+    robot, _, _, md, _, gs = _sco_model(oracle_mod)
+    T, A = 16, 7
+    q0, q1 = random_cfgs(robot, 2, seed=14, margin=0.3)
+    s = np.linspace(0, 1, T)[:, None]
+    x0 = (1 - s) * q0 + s * q1 + np.random.default_rng(15).normal(0, 0.05, (T, A))
+    x0[0], x0[-1] = q0, q1
+    w_vel, w_acc, w_jerk, w_trust = 1.0, 0.5, 0.1, 0.5
+    out, _, _ = oracle_mod.traj_sco(md, [x0], robot.joints.lower_limits, robot.joints.upper_limits, gs,
+                                    n_outer_iters=1, n_inner_iters=300, collision_margin=-10.0, w_trust=w_trust,
+                                    w_limits=0.0)
+    # dense quadratic: smooth = x^T H x per joint, H from the stencils
+    D1 = np.diff(np.eye(T), axis=0)
+    D2 = np.zeros((T - 4, T))
+    for i in range(T - 4):
+        D2[i, i:i + 5] = np.array([-1, 16, -30, 16, -1]) / 12.0
+    D3 = np.diff(D2, axis=0)
+    H = w_vel * D1.T @ D1 + w_acc * D2.T @ D2 + w_jerk * D3.T @ D3 + w_trust * np.eye(T)
+    want = x0.copy()
+    inner = slice(1, T - 1)
+    for a in range(A):
+        rhs = w_trust * x0[:, a] - H[:, [0, T - 1]] @ x0[[0, T - 1], a]
+        want[inner, a] = np.linalg.solve(H[inner, inner], rhs[inner])
+    np.testing.assert_allclose(out, want, atol=1e-5)
