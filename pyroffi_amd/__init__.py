@@ -16,11 +16,15 @@ Public API (same names and arguments as pyroffi):
   ``sco_trajopt`` / ``ScoTrajOptConfig`` / ``make_init_trajs`` (sequential
   convex optimisation)
 
+``save_model`` / ``load_model`` store a robot's tables and collision model on
+disk (no URDF needed to load them).
+
 All computation runs in the HIP library ``libpyroffi_amd.so`` (C ABI in
 ``include/pyroffi_amd.h``); torch supplies device memory and the stream.
 """
 
 from . import collision, costs, kernels
+from ._model_file import load_model, save_model
 from ._robot import Robot
 from ._robot_urdf_parser import JointInfo, LinkInfo, RobotURDFParser
 from ._se3 import SE3, SO3
@@ -68,6 +72,8 @@ __all__ = [
     "hjcd_solve",
     "hjcd_solve_batch",
     "load_bundled_urdf",
+    "load_model",
+    "save_model",
     "ls_ik_solve",
     "ls_ik_solve_cuda",
     "ls_ik_solve_cuda_batch",

@@ -1,14 +1,22 @@
 // Forward kinematics for gfx950: one lane = one configuration.
 //
 // A 64-lane workgroup owns 64 consecutive configurations.  It stages the joint
-// records and the 64 x n_act configuration tile in LDS (coalesced loads), every
-// lane walks the joint tree for its configuration with all poses held in an
-// LDS tile, and the finished tile -- 64 x (n_out x 7) floats, contiguous in
-// HBM -- is copied out with 16-byte stores.  The output is 28 B per joint (or
-// link) per configuration; with one wave per SIMD (a 64-lane workgroup holds
-// about 21 KB of LDS for Panda) the three phases -- staging, walk, copy-out --
-// run back to back, so the kernel is latency-bound rather than at the write
-// bandwidth.
+// records and the 64 x n_act configuration tile in LDS (coalesced loads); every
+// lane walks the joint tree for its configuration and keeps each joint's world
+// pose in LDS directly at its place in the lane's output row (a link pose is
+// the pose of its parent joint, and in a URDF every joint has exactly one
+// child link).  The finished tile -- 64 rows of n_out x 7 floats, the exact
+// image of the block's contiguous output -- is copied out with 16-byte loads
+// and non-temporal 16-byte stores, with no per-element index lookups.  Output
+// positions the walk does not write (root links: identity; a second link on
+// the same joint: a copy) are filled after the walk.  The row stride is odd
+// (the row padded by one float when 7 n_out is even), so the per-lane walk
+// accesses are free of bank conflicts.
+//
+// Measured phases for 65,536 Panda configurations (one wave per SIMD, run back
+// to back): staging ~4.9 us (kernel start plus three dependent table loads),
+// walk ~6.3 us, copy-out of 24 MB ~5.7 us with the earlier per-element lookups
+// costing another ~5 us on top of the store floor.
 //
 // Reference behaviour: FkCudaImpl / fk_kernel_warp
 // (src/pyroffi/cuda_kernels/_fk_cuda_kernel.cu:39-137, 236-454) and the link
@@ -19,21 +27,51 @@ namespace pk {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// out_mode 0: joint poses (B, J, 7); 1: link poses (B, L, 7).
+// LDS image of one FK workgroup (float offsets into the dynamic LDS array).
+struct FkLds {
+  int recs, slot_of, pos, fill, lmap, flag, cfg, tile, total;
+};
+
+// n_extra: positions past the output row for joints without a child link.
+__host__ __device__ inline int fk_row_stride(int n_out, int n_extra) { return (7 * (n_out + n_extra)) | 1; }
+
+__host__ __device__ inline FkLds fk_lds_layout(int J, int n_act, int n_out, int n_extra) {
+  FkLds o;
+  o.recs = 0;
+  o.slot_of = o.recs + static_cast<int>(sizeof(JointRec) / 4) * J;
+  o.pos = o.slot_of + J;
+  o.fill = o.pos + J;
+  o.lmap = o.fill + n_out;
+  o.flag = o.lmap + n_out;
+  o.cfg = o.flag + 1;
+  o.tile = ((o.cfg + n_act * kLaneStride) + 3) & ~3;  // 16-byte aligned
+  o.total = o.tile + kWave * fk_row_stride(n_out, n_extra);
+  return o;
+}
+
+// out_mode 0: joint poses (B, J, 7) in joint order; 1: link poses (B, L, 7).
+// In mode 1 the link map must give each joint at most one position of its
+// own: joints without a child link use the n_extra spare positions, and a
+// workgroup that would need more writes NaN rows (a URDF-derived link map
+// never does: every joint has exactly one child link).
 __global__ __launch_bounds__(64) void fk_kernel(
     const float* __restrict__ cfg, long long batch, int n_act, RobotTables tables,
-    const int* __restrict__ link_parent_joint, int n_links, int out_mode,
+    const int* __restrict__ link_parent_joint, int n_links, int out_mode, int n_extra,
     float* __restrict__ out) {
   extern __shared__ float smem[];
   const int J = tables.n_joints;
-  JointRec* recs = reinterpret_cast<JointRec*>(smem);
-  int* slot_of = reinterpret_cast<int*>(recs + J);
   const int n_out = out_mode ? n_links : J;
-  const int row = n_out * 7;
-  int* src_off = slot_of + J;   // per output float of a row: LDS offset of the pose component,
-                                // -1 for an identity 1, -2 for an identity 0
-  float* cfg_tile = reinterpret_cast<float*>(src_off + row);      // [a][lane], stride 65
-  float* poses = cfg_tile + n_act * kLaneStride;                  // [slot*7+c][lane]
+  const int row = 7 * n_out;
+  const int stride = fk_row_stride(n_out, n_extra);
+  const FkLds lay = fk_lds_layout(J, n_act, n_out, n_extra);
+  JointRec* recs = reinterpret_cast<JointRec*>(smem + lay.recs);
+  int* slot_of = reinterpret_cast<int*>(smem + lay.slot_of);
+  int* pos = reinterpret_cast<int*>(smem + lay.pos);    // per slot: float offset of its pose in a row
+  int* fill = reinterpret_cast<int*>(smem + lay.fill);  // per output position: -1 walk, -2 identity, else copy source
+  int* lmap = reinterpret_cast<int*>(smem + lay.lmap);  // per output position: its joint slot, or -1
+  int* flag = reinterpret_cast<int*>(smem + lay.flag);
+  float* cfg_tile = smem + lay.cfg;                     // [a][lane], stride 65
+  float* tile = smem + lay.tile;                        // [lane * stride + offset]
 
   const int lane = threadIdx.x;
   const long long b0 = static_cast<long long>(blockIdx.x) * kWave;
@@ -41,8 +79,6 @@ __global__ __launch_bounds__(64) void fk_kernel(
 
   // The configuration block and the link -> joint map are loaded into
   // registers first, so their latency overlaps the table staging.
-  // Configuration tile: element f = lane + 64 i of the contiguous
-  // (n_valid x n_act) block, at most kMaxAct elements per lane.
   const float* cfg_blk = cfg + b0 * n_act;
   const int n_cfg = n_valid * n_act;
   float cv[kMaxAct];
@@ -51,23 +87,22 @@ __global__ __launch_bounds__(64) void fk_kernel(
     const int f = lane + i * kWave;
     cv[i] = (f < n_cfg) ? cfg_blk[f] : 0.f;
   }
-  constexpr int kRowIters = (7 * (kMaxJoints + 1) + kWave - 1) / kWave;
-  int pjv[kRowIters];
+  constexpr int kOutIters = (kMaxJoints + 1 + kWave - 1) / kWave;
+  int pjv[kOutIters];
 #pragma unroll
-  for (int i = 0; i < kRowIters; ++i) {
-    const int r = lane + i * kWave;
-    pjv[i] = (out_mode && r < row) ? link_parent_joint[r / 7] : -1;
+  for (int i = 0; i < kOutIters; ++i) {
+    const int o = lane + i * kWave;
+    pjv[i] = (o < n_out) ? (out_mode ? link_parent_joint[o] : o) : -1;
   }
-
-  stage_joint_records(tables, recs, slot_of);
+  if (lane == 0) *flag = 0;
+  for (int i = lane; i < J; i += kWave) pos[i] = -1;
+  stage_joint_records(tables, recs, slot_of);  // ends with a barrier
 #pragma unroll
-  for (int i = 0; i < kRowIters; ++i) {
-    const int r = lane + i * kWave;
-    if (r < row) {
-      const int o = r / 7, c = r - 7 * o;
-      const int pj = out_mode ? pjv[i] : o;
-      const int s = (pj >= 0 && pj < J) ? slot_of[pj] : -1;
-      src_off[r] = (s >= 0) ? (s * 7 + c) * kLaneStride : (c == 0 ? -1 : -2);
+  for (int i = 0; i < kOutIters; ++i) {
+    const int o = lane + i * kWave;
+    if (o < n_out) {
+      const int pj = pjv[i];
+      lmap[o] = (pj >= 0 && pj < J) ? slot_of[pj] : -1;
     }
   }
 #pragma unroll
@@ -79,49 +114,108 @@ __global__ __launch_bounds__(64) void fk_kernel(
     }
   }
   __syncthreads();
-
-  if (lane < n_valid) {
-    fk_walk(recs, J, [&](int a) { return cfg_tile[a * kLaneStride + lane]; }, poses, lane);
+  // Positions: the first output position of a slot is where the walk writes
+  // its pose; later positions of the same slot copy it; positions without a
+  // slot read the identity.
+#pragma unroll
+  for (int i = 0; i < kOutIters; ++i) {
+    const int o = lane + i * kWave;
+    if (o < n_out) {
+      const int s = lmap[o];
+      int first = o;
+      for (int o2 = 0; o2 < o; ++o2)
+        if (s >= 0 && lmap[o2] == s) { first = o2; break; }
+      fill[o] = (s < 0) ? -2 : (first == o ? -1 : 7 * first);
+      if (s >= 0 && first == o) pos[s] = 7 * o;
+    }
+  }
+  __syncthreads();
+  // Slots without an output position take the spare positions past the row.
+  for (int i = lane; i < J; i += kWave) {
+    if (pos[i] < 0) {
+      int k = 0;
+      for (int i2 = 0; i2 < i; ++i2) k += (pos[i2] < 0) ? 1 : 0;
+      if (k < n_extra) {
+        pos[i] = 7 * (n_out + k);
+      } else {
+        pos[i] = 0;  // no room: the rows are overwritten with NaN below
+        *flag = 1;
+      }
+    }
   }
   __syncthreads();
 
-  // Copy-out: the block's rows are contiguous, n_valid * n_out * 7 floats,
-  // written as 16-byte non-temporal stores (the block starts 16-byte
-  // aligned: 64 rows of 7 n_out floats).  Lane f writes floats 4f .. 4f + 3;
-  // float rem of configuration l comes from LDS offset src_off[rem] + l.
+  float* trow = tile + lane * stride;
+  if (lane < n_valid) {
+    for (int i = 0; i < J; ++i) {
+      const JointRec& r = recs[i];
+      const float qref = (r.src >= 0) ? cfg_tile[r.src * kLaneStride + lane] : 0.f;
+      const float q = qref * r.mul + r.off;
+      const Pose tpc = joint_local(r, q);
+      const Pose tw = (r.parent_slot < 0) ? tpc : compose(load_pose(trow + pos[r.parent_slot]), tpc);
+      float* d = trow + pos[i];
+      d[0] = tw.w; d[1] = tw.x; d[2] = tw.y; d[3] = tw.z;
+      d[4] = tw.tx; d[5] = tw.ty; d[6] = tw.tz;
+    }
+    const bool bad = *flag != 0;
+    for (int o = 0; o < n_out; ++o) {
+      const int fo = fill[o];
+      if (fo == -1 && !bad) continue;
+      float* d = trow + 7 * o;
+      if (bad) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) d[c] = __builtin_nanf("");
+      } else if (fo == -2) {
+        d[0] = 1.f;
+#pragma unroll
+        for (int c = 1; c < 7; ++c) d[c] = 0.f;
+      } else {
+        float v[7];
+#pragma unroll
+        for (int c = 0; c < 7; ++c) v[c] = trow[fo + c];
+#pragma unroll
+        for (int c = 0; c < 7; ++c) d[c] = v[c];
+      }
+    }
+  }
+  __syncthreads();
+
+  // Copy-out of the block's n_valid * row contiguous floats.
   float* out_blk = out + b0 * row;
   const int total = n_valid * row;
+  if (stride == row) {
+    // The tile is the output image: 16-byte loads and non-temporal stores
+    // (the block starts 16-byte aligned: 64 rows of 7 n_out floats).
+    const int n4 = total / 4;
+    const v4f* t4 = reinterpret_cast<const v4f*>(tile);
+    for (int f4 = lane; f4 < n4; f4 += kWave) __builtin_nontemporal_store(t4[f4], reinterpret_cast<v4f*>(out_blk) + f4);
+    for (int f = 4 * n4 + lane; f < total; f += kWave) out_blk[f] = tile[f];
+    return;
+  }
+  // Padded rows: element f = (configuration l, float r) of the block.
   const int n4 = total / 4;
   int l = (4 * lane) / row;
   int rem = 4 * lane - l * row;
-  int adv_l = (4 * kWave) / row;            // per pass: 256 floats = adv_l rows + adv_r
+  const int adv_l = (4 * kWave) / row;  // per pass: 256 floats = adv_l rows + adv_r
   const int adv_r = 4 * kWave - adv_l * row;
   for (int f4 = lane; f4 < n4; f4 += kWave) {
     float v[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       int r = rem + k, lk = l;
-      if (r >= row) { r -= row; ++lk; }      // row >= 7 > 3: at most one wrap
-      const int so = src_off[r];
-      v[k] = (so >= 0) ? poses[so + lk] : (so == -1 ? 1.f : 0.f);
+      if (r >= row) { r -= row; ++lk; }  // row >= 7 > 3: at most one wrap
+      v[k] = tile[lk * stride + r];
     }
     const v4f w = {v[0], v[1], v[2], v[3]};
-    __builtin_nontemporal_store(w, reinterpret_cast<v4f*>(out_blk) + f4);  // streamed past L2
+    __builtin_nontemporal_store(w, reinterpret_cast<v4f*>(out_blk) + f4);
     rem += adv_r;
     l += adv_l;
     if (rem >= row) { rem -= row; ++l; }
   }
-  // Tail of a partial last block (total not a multiple of 4).
   for (int f = 4 * n4 + lane; f < total; f += kWave) {
     const int lk = f / row, r = f - lk * row;
-    const int so = src_off[r];
-    out_blk[f] = (so >= 0) ? poses[so + lk] : (so == -1 ? 1.f : 0.f);
+    out_blk[f] = tile[lk * stride + r];
   }
-}
-
-static size_t fk_lds_bytes(int J, int n_act, int n_out) {
-  return sizeof(JointRec) * J + sizeof(int) * (J + 7 * n_out) +
-         sizeof(float) * (static_cast<size_t>(n_act) + 7 * J) * kLaneStride;
 }
 
 static int launch_fk(void* stream, const float* cfg, int64_t batch, int32_t n_act,
@@ -143,10 +237,13 @@ static int launch_fk(void* stream, const float* cfg, int64_t batch, int32_t n_ac
   if (out_mode) PK_REQUIRE(link_parent_joint != nullptr, "null link_parent_joint");
   const long long blocks = (batch + kWave - 1) / kWave;
   PK_REQUIRE(blocks <= 0x7fffffffLL, "batch too large");
-  const size_t lds = fk_lds_bytes(t.n_joints, n_act, n_out);
+  // A tree has one more link than joints; spare positions cover joints beyond that.
+  const int n_extra = out_mode ? std::max(0, t.n_joints - (n_links - 1)) : 0;
+  const size_t lds = sizeof(float) * static_cast<size_t>(fk_lds_layout(t.n_joints, n_act, n_out, n_extra).total);
+  PK_REQUIRE(lds <= 160 * 1024, "FK tile needs %zu bytes of LDS", lds);
   hipLaunchKernelGGL(fk_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kWave), lds,
                      static_cast<hipStream_t>(stream), cfg, static_cast<long long>(batch),
-                     n_act, t, link_parent_joint, n_links, out_mode, out);
+                     n_act, t, link_parent_joint, n_links, out_mode, n_extra, out);
   return check_launch("fk_kernel");
 }
 

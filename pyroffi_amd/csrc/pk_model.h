@@ -105,79 +105,6 @@ __device__ inline void stage_joint_records(const RobotTables& t, JointRec* recs,
   __syncthreads();
 }
 
-// The same records from one round of global loads: lane i copies joint i's raw
-// table entries (all indexed by i, so the loads are independent and coalesced)
-// and its topological slot into `tmp` (>= 20 J floats of LDS that the caller
-// overwrites afterwards), then slot i's record is assembled from LDS.
-// stage_joint_records needs three dependent rounds (topo_inv, then the slot
-// inverse, then the tables of joint topo_inv[i]); at a kernel's start every
-// round is a full memory latency.
-__device__ inline void stage_joint_records_1r(const RobotTables& t, JointRec* recs, int* slot_of, float* tmp) {
-  const int J = t.n_joints;
-  for (int i = threadIdx.x; i < J; i += blockDim.x) slot_of[i] = -1;
-  __syncthreads();
-  for (int i = threadIdx.x; i < J; i += blockDim.x) {
-    float v[20];
-#pragma unroll
-    for (int c = 0; c < 7; ++c) v[c] = t.parent_tf[i * 7 + c];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) v[7 + c] = t.twists[i * 6 + c];
-    v[13] = t.mimic_mul[i];
-    v[14] = t.mimic_off[i];
-    v[15] = __int_as_float(t.mimic_act_idx[i]);
-    v[16] = __int_as_float(t.act_idx[i]);
-    v[17] = __int_as_float(t.parent_idx[i]);
-    const int ti = t.topo_inv[i];
-    v[18] = __int_as_float(ti);
-    v[19] = 0.f;
-#pragma unroll
-    for (int c = 0; c < 20; ++c) tmp[i * 20 + c] = v[c];
-    if (ti >= 0 && ti < J) slot_of[ti] = i;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < J; i += blockDim.x) {
-    int j = __float_as_int(tmp[i * 20 + 18]);
-    if (j < 0 || j >= J) j = 0;
-    const float* raw = tmp + j * 20;
-    JointRec r;
-#pragma unroll
-    for (int c = 0; c < 7; ++c) r.ptf[c] = raw[c];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      r.lin[c] = raw[7 + c];
-      r.ang[c] = raw[10 + c];
-    }
-    r.mul = raw[13];
-    r.off = raw[14];
-    const int m = __float_as_int(raw[15]);
-    int src = (m != -1) ? m : __float_as_int(raw[16]);
-    if (src < 0 || src >= t.n_act) src = -1;
-    r.src = src;
-    const int p = __float_as_int(raw[17]);
-    int ps = (p >= 0 && p < J) ? slot_of[p] : -1;
-    if (ps >= i) ps = -1;  // not topologically before us: invalid table
-    r.parent_slot = ps;
-    r.joint = j;
-    const float l2 = r.lin[0] * r.lin[0] + r.lin[1] * r.lin[1] + r.lin[2] * r.lin[2];
-    const float a2 = r.ang[0] * r.ang[0] + r.ang[1] * r.ang[1] + r.ang[2] * r.ang[2];
-    int kind;
-    if (src == -1 && r.off == 0.f) kind = 0;
-    else if (a2 == 0.f) kind = 2;
-    else if (l2 == 0.f) kind = 4;
-    else kind = 3;
-    if (kind == 4) {
-      const float na = sqrtf(a2);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) r.ang[c] /= na;
-      r.lin[0] = na;
-    }
-    r.kind = kind;
-    r.pad = 0;
-    recs[i] = r;
-  }
-  __syncthreads();
-}
-
 __device__ __forceinline__ Pose load_pose(const float* p) {
   Pose o;
   o.w = p[0]; o.x = p[1]; o.y = p[2]; o.z = p[3];
