@@ -458,6 +458,15 @@ __device__ __forceinline__ void ls_argmin_step(float& e, int& i) {
   if (pe < e || (pe == e && pi < i)) { e = pe; i = pi; }
 }
 
+// A pose from lane src of the wave (ds_bpermute per component).
+__device__ __forceinline__ Pose shfl_pose(const Pose& p, int src) {
+  Pose o;
+  o.w = __shfl(p.w, src, kWave); o.x = __shfl(p.x, src, kWave);
+  o.y = __shfl(p.y, src, kWave); o.z = __shfl(p.z, src, kWave);
+  o.tx = __shfl(p.tx, src, kWave); o.ty = __shfl(p.ty, src, kWave); o.tz = __shfl(p.tz, src, kWave);
+  return o;
+}
+
 // Per-seed solver state (registers).
 template <int MAXA>
 struct LmState {
@@ -504,9 +513,19 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
   Pose P[MAXA];
   Pose Tee[NEE];
   float r[6 * NEE];
+  // G = 8 (8-wide instantiations; the 16-wide ones have no registers to
+  // spare): the kinematics and error of the iterate carried into the next
+  // iteration are the accepted line-search point's, computed by one lane of
+  // the group (the same arithmetic on the same q), so they are taken from that
+  // lane instead of being recomputed; a rejected step keeps the current ones.
+  constexpr bool kReuse = (G == 8) && (MAXA == 8);
+  bool have = false;
+  float cur_next = 0.f;
   for (int it = it0; it < it1 && !st.done; ++it) {
-    node_fk(sh, st.q, P);
-    residuals(sh, P, tg, r, Tee);
+    if (!kReuse || !have) {
+      node_fk(sh, st.q, P);
+      residuals(sh, P, tg, r, Tee);
+    }
     bool conv = true;
     float max_p = 0.f, max_o = 0.f;
 #pragma unroll
@@ -523,7 +542,9 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
     float fw[6 * NEE];
 #pragma unroll
     for (int i = 0; i < 6 * NEE; ++i) { fw[i] = r[i] * W[i % 6]; cur += fw[i] * fw[i]; }
-    if (v.coll) {
+    if (kReuse && have) {
+      cur = cur_next;  // the same sum (and penalty) the accepting lane evaluated
+    } else if (v.coll) {
       // With G lanes per seed every lane computes the same value; the work
       // count takes one lane's.
       unsigned wk = 0;
@@ -697,7 +718,10 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
 #pragma unroll
       for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + ls_alpha(ai) * delta[i]));
       unsigned wk = 0;
-      const float et = weighted_error(v, tg, W, qt, P, r, Tee, grid, wk);
+      Pose P2[MAXA], Tee2[NEE];
+      float r2[6 * NEE];
+      const float et = kReuse ? weighted_error(v, tg, W, qt, P2, r2, Tee2, grid, wk)
+                              : weighted_error(v, tg, W, qt, P, r, Tee, grid, wk);
       if (threadIdx.x % G < 5) st.work += wk;  // lanes 5..7 repeat step 4
       // Sequential semantics: only errors below 1e30 replace the start value
       // (NaN never does); the first index wins ties.
@@ -707,6 +731,20 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
       ls_argmin_step<0x4e>(e, idx);   // quad_perm [2,3,0,1]
       ls_argmin_step<0x141>(e, idx);  // row_half_mirror
       if (e != INFINITY) { best_a_err = e; best_a = idx; }
+      if (kReuse && best_a_err < cur * (1.f - 1e-4f)) {
+        // Accepted: the next iterate's poses, EE poses and residuals from lane best_a.
+        const int src = (static_cast<int>(threadIdx.x) & ~(G - 1)) + best_a;
+#pragma unroll
+        for (int n = 0; n < MAXA; ++n) P[n] = shfl_pose(P2[n], src);
+#pragma unroll
+        for (int e2 = 0; e2 < NEE; ++e2) Tee[e2] = shfl_pose(Tee2[e2], src);
+#pragma unroll
+        for (int i = 0; i < 6 * NEE; ++i) r[i] = __shfl(r2[i], src, kWave);
+        cur_next = best_a_err;
+      } else {
+        cur_next = cur;
+      }
+      have = true;
     }
     const float al = ls_alpha(best_a);
     float qt[MAXA];
