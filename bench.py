@@ -447,6 +447,137 @@ def trajopt_bench(reps: int, cpu_sample: int = 16):
                            cpu_sample)
 
 
+def chomp_bench(reps: int = 3, cpu_sample: int = 4):
+    """CHOMP (chomp_trajopt's solver, default ChompTrajOptConfig: 100 iterations, covariant
+    update, hinge^2 margin 0.01) on the config-5 trajectories and world: one launch of
+    pk_traj_chomp_f32 for all 512 trajectories, then the ranking costs."""
+    import torch
+
+    import oracle
+    from pyroffi_amd.optimization_engines._chomp_optimization import ChompTrajOptConfig, chomp_run
+    from pyroffi_amd.optimization_engines._lbfgs_trajopt_optimization import TrajModel, flatten_trajs, traj_cost
+
+    robot, coll, world, trajs, lens, _, _, _ = trajopt_workload()
+    cfg = ChompTrajOptConfig()
+    model = TrajModel(robot, coll, world, cfg.collision_margin, mode=0)
+    x, start, max_len = flatten_trajs(trajs, model.device)
+    chomp_run(model, x, start, max_len, cfg)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        out, cost, _ = chomp_run(model, x, start, max_len, cfg)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gpu_ms = e0.elapsed_time(e1) / reps
+    w = (cfg.w_smooth, 0.0, cfg.w_acc, cfg.w_jerk, cfg.w_limits, cfg.w_collision_max)
+    c0, _ = traj_cost(model, x, start, max_len, w, grad=False)
+    init = (c0 @ torch.tensor([cfg.w_smooth, cfg.w_limits, cfg.w_collision_max], device=c0.device)).cpu().numpy()
+    final = cost.cpu().numpy()
+    t = oracle.Tables.of(robot)
+    om = oracle.TrajModel(t, oracle.padded_spheres(coll), coll.active_idx_i, coll.active_idx_j,
+                          world.astype(np.float64), cfg.collision_margin, 0)
+    kw = {k: getattr(cfg, k) for k in ("n_iters", "step_size", "w_smooth", "w_acc", "w_jerk", "w_collision",
+                                        "w_collision_max", "collision_penalty_scale", "w_limits",
+                                        "use_covariant_update", "smoothness_reg", "n_cg_iters", "cg_tol",
+                                        "grad_clip_norm", "max_delta_per_step", "early_stop_patience",
+                                        "min_cost_improve")}
+    threads = cpu_threads()
+    c_0 = time.perf_counter()
+    _, ocost, _ = oracle.traj_chomp(om, [tr.astype(np.float64) for tr in trajs[:cpu_sample]],
+                                    robot.joints.lower_limits, robot.joints.upper_limits, threads=threads, **kw)
+    cpu_dt = time.perf_counter() - c_0
+    return {
+        "workload": "panda_chomp_512traj_T32_64_128_world1024_100iters",
+        "trajectories": len(trajs),
+        "ms_per_solve": gpu_ms,
+        "trajectories_per_s": len(trajs) / (gpu_ms * 1e-3),
+        "median_cost_ratio_final_over_initial": float(np.median(final / init)),
+        "oracle_sample_median_cost_ratio_gpu_over_fp64": float(np.median(final[:cpu_sample] / ocost)),
+        "cpu_baseline": {"value": cpu_sample / cpu_dt, "unit": "trajectories/s", "cores": threads, "kind": "port",
+                         "sample": f"first {cpu_sample} trajectories, FP64 oracle, brute-force world",
+                         "cpu_model": cpu_model()},
+    }
+
+
+def sco_bench(reps: int = 3, cpu_sample: int = 4):
+    """SCO (sco_trajopt's solver, default ScoTrajOptConfig: 10 rounds of linearise + 30
+    inner L-BFGS steps) on the config-5 trajectories and world, one world group; the
+    20 launches and the ranking cost replay from one captured graph."""
+    import torch
+
+    import oracle
+    from pyroffi_amd.optimization_engines._sco_optimization import BatchedSco, ScoTrajOptConfig
+
+    robot, coll, world, trajs, lens, _, _, _ = trajopt_workload()
+    cfg = ScoTrajOptConfig()
+    opt = BatchedSco(robot, coll, world, trajs, cfg)
+    opt.capture()
+    opt.solve()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        _, cost = opt.solve()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gpu_ms = e0.elapsed_time(e1) / reps
+    final = cost.cpu().numpy()
+    t = oracle.Tables.of(robot)
+    om = oracle.TrajModel(t, oracle.padded_spheres(coll), coll.active_idx_i, coll.active_idx_j,
+                          world.astype(np.float64), cfg.collision_margin, 1)
+    kw = {k: getattr(cfg, k) for k in ("n_outer_iters", "n_inner_iters", "m_lbfgs", "w_smooth", "w_vel", "w_acc",
+                                        "w_jerk", "w_collision", "w_collision_max", "penalty_scale",
+                                        "collision_margin", "w_trust", "w_limits", "smooth_min_temperature")}
+    threads = cpu_threads()
+    c_0 = time.perf_counter()
+    _, ocost, _ = oracle.traj_sco(om, [tr.astype(np.float64) for tr in trajs[:cpu_sample]],
+                                  robot.joints.lower_limits, robot.joints.upper_limits,
+                                  np.array([0, len(world)], np.int32), threads=threads, **kw)
+    cpu_dt = time.perf_counter() - c_0
+    return {
+        "workload": "panda_sco_512traj_T32_64_128_world1024_10x30iters",
+        "trajectories": len(trajs),
+        "ms_per_solve": gpu_ms,
+        "trajectories_per_s": len(trajs) / (gpu_ms * 1e-3),
+        "oracle_sample_median_cost_ratio_gpu_over_fp64": float(np.median(final[:cpu_sample] / ocost)),
+        "cpu_baseline": {"value": cpu_sample / cpu_dt, "unit": "trajectories/s", "cores": threads, "kind": "port",
+                         "sample": f"first {cpu_sample} trajectories, FP64 oracle",
+                         "cpu_model": cpu_model()},
+    }
+
+
+def fk_small_bench(reps: int = 200):
+    """Config 1 FK: 1,024 Panda configurations to all link poses (one launch, launch-latency bound)."""
+    import torch
+
+    import pyroffi_amd as pk
+    from pyroffi_amd.kernels import ops
+
+    robot = pk.Robot.from_urdf(pk.bundled_urdf_path("panda"))
+    t = robot.device_tables()
+    rng = np.random.default_rng(11)
+    q = torch.as_tensor(rng.uniform(robot.joints.lower_limits, robot.joints.upper_limits, (1024, 7))
+                        .astype(np.float32), device="cuda")
+    bufs = t.robot_buffers()
+    for _ in range(20):
+        ops.fk_links(q, *bufs, t.link_parent_joint)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        ops.fk_links(q, *bufs, t.link_parent_joint)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"workload": "panda_fk_links_b1024", "configs": 1024, "ms_per_call": ms,
+            "configs_per_s": 1024 / (ms * 1e-3),
+            "note": "event time per call of back-to-back launches (host launch overhead included)"}
+
+
 def trajopt_workload():
     """Config 5 inputs: 512 Panda straight-line trajectories (T in {32, 64, 128}), 1024 world spheres."""
     import pyroffi_amd as pk
@@ -713,8 +844,8 @@ def _oracle_sweep(o, t, sp, coll, q, world, threads):
 def cpu_baseline(sample: int):
     """The oracle (C + OpenMP restatement) on `sample` configurations of the same workload:
     FP64 (the reference JAX path's arithmetic) on every core -- the reported value -- and
-    on one core, and the FP32 build on every core and on one core.  One-core legs run on
-    a sixteenth of the sample."""
+    on one core, and the FP32 build on every core and on one core, each the median of five
+    timed runs after a warm-up.  One-core legs run on a sixteenth of the sample."""
     import oracle
 
     robot, coll, q, world = make_inputs()
@@ -730,14 +861,17 @@ def cpu_baseline(sample: int):
             n = sample if cores == threads else max(64, sample // 16)
             qq = q[:n].astype(R)
             _oracle_sweep(o, t, sp, coll, qq[:64], w, cores)  # warm-up (thread pool, page-in)
-            t0 = time.perf_counter()
-            res = _oracle_sweep(o, t, sp, coll, qq, w, cores)
-            legs[f"{prec}_{cores}_cores"] = n / (time.perf_counter() - t0)
+            times = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                res = _oracle_sweep(o, t, sp, coll, qq, w, cores)
+                times.append(time.perf_counter() - t0)
+            legs[f"{prec}_{cores}_cores"] = n / float(np.median(times))
             if prec == "fp64" and cores == threads:
                 out = res
     return {"value": legs[f"fp64_{threads}_cores"], "unit": "configs/s", "cores": threads, "kind": "port",
             "sample": f"first {sample} of the {B} configurations (one-core legs: the first {max(64, sample // 16)}), "
-                      "oracle FK + world min + self pairs; value = FP64 on every core",
+                      "oracle FK + world min + self pairs; value = FP64 on every core; median of 5 runs",
             "legs_configs_per_s": legs, "cpu_model": cpu_model()}, out
 
 
@@ -777,6 +911,23 @@ def profile_workload(name: str):
             g.tune = tune
         for _ in range(3):
             solver.solve()
+    elif name == "chomp":
+        from pyroffi_amd.optimization_engines._chomp_optimization import ChompTrajOptConfig, chomp_run
+        from pyroffi_amd.optimization_engines._lbfgs_trajopt_optimization import TrajModel, flatten_trajs
+
+        robot, coll, world, trajs, *_ = trajopt_workload()
+        cfg = ChompTrajOptConfig()
+        model = TrajModel(robot, coll, world, cfg.collision_margin, mode=0)
+        x, start, max_len = flatten_trajs(trajs, model.device)
+        for _ in range(3):
+            chomp_run(model, x, start, max_len, cfg)
+    elif name == "sco":
+        from pyroffi_amd.optimization_engines._sco_optimization import BatchedSco, ScoTrajOptConfig
+
+        robot, coll, world, trajs, *_ = trajopt_workload()
+        opt = BatchedSco(robot, coll, world, trajs, ScoTrajOptConfig())
+        for _ in range(3):
+            opt.solve()
     else:
         *_, opt = trajopt_workload()
         for _ in range(3):
@@ -806,12 +957,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=B)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ik", action="store_true")
-    ap.add_argument("--profile", nargs="?", const="sweep", choices=("sweep", "grad", "ik", "ik_uneven", "trajopt"),
+    ap.add_argument("--profile", nargs="?", const="sweep", choices=("sweep", "grad", "ik", "ik_uneven", "trajopt", "chomp", "sco"),
                     help="run for rocprofv3: 'sweep' = the headline workload's 20 + 200 steps and 200 timed "
                          "launches of each kernel, plus a 256 MiB device copy (the byte-count calibration for the HBM counters); 'grad' = 20 launches "
                          "of the config-3 adjoint kernel plus the same copy; 'ik' = 20 config-1 solve calls; "
-                         "'ik_uneven' / 'trajopt' = 3 solves "
-                         "of the uneven LM-IK / trajectory workload")
+                         "'ik_uneven' / 'trajopt' / 'chomp' / 'sco' = 3 solves "
+                         "of the uneven LM-IK / trajectory workload (L-BFGS, CHOMP, SCO)")
     args = ap.parse_args()
     if args.profile == "sweep":
         profile_run()
@@ -876,9 +1027,11 @@ def main():
         line["cpu_baseline"] = cb
         _progress("cpu baseline done")
     if not args.no_ik:
-        for key, fn in (("ik", lambda: ik_bench(reps=10)), ("grad", lambda: grad_bench(reps=10, cpu_sample=256)),
+        for key, fn in (("fk_small", lambda: fk_small_bench()), ("ik", lambda: ik_bench(reps=10)),
+                        ("grad", lambda: grad_bench(reps=10, cpu_sample=256)),
                         ("ik_uneven", lambda: ik_uneven_bench(reps=5)), ("hjcd", lambda: hjcd_bench(reps=3)),
-                        ("trajopt", lambda: trajopt_bench(reps=5))):
+                        ("trajopt", lambda: trajopt_bench(reps=5)), ("chomp", lambda: chomp_bench()),
+                        ("sco", lambda: sco_bench())):
             line[key] = fn()
             _progress(f"{key} done")
     print(json.dumps(line), flush=True)

@@ -513,11 +513,13 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
   Pose P[MAXA];
   Pose Tee[NEE];
   float r[6 * NEE];
-  // G = 8 (8-wide instantiations; the 16-wide ones have no registers to
-  // spare): the kinematics and error of the iterate carried into the next
-  // iteration are the accepted line-search point's, computed by one lane of
-  // the group (the same arithmetic on the same q), so they are taken from that
-  // lane instead of being recomputed; a rejected step keeps the current ones.
+  // The error of the iterate carried into the next iteration (residuals and
+  // collision penalty) is the accepted line-search point's, evaluated by the
+  // same arithmetic on the same q, or the current one after a rejected step,
+  // so the next iteration does not evaluate the penalty again.  With G = 8
+  // (8-wide instantiations; the 16-wide ones have no registers to spare) the
+  // node poses, EE poses and residuals are taken from the lane that evaluated
+  // the accepted point as well, instead of being recomputed.
   constexpr bool kReuse = (G == 8) && (MAXA == 8);
   bool have = false;
   float cur_next = 0.f;
@@ -542,8 +544,8 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
     float fw[6 * NEE];
 #pragma unroll
     for (int i = 0; i < 6 * NEE; ++i) { fw[i] = r[i] * W[i % 6]; cur += fw[i] * fw[i]; }
-    if (kReuse && have) {
-      cur = cur_next;  // the same sum (and penalty) the accepting lane evaluated
+    if (have) {
+      cur = cur_next;  // the same sum (and penalty) the line search evaluated
     } else if (v.coll) {
       // With G lanes per seed every lane computes the same value; the work
       // count takes one lane's.
@@ -740,12 +742,10 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
         for (int e2 = 0; e2 < NEE; ++e2) Tee[e2] = shfl_pose(Tee2[e2], src);
 #pragma unroll
         for (int i = 0; i < 6 * NEE; ++i) r[i] = __shfl(r2[i], src, kWave);
-        cur_next = best_a_err;
-      } else {
-        cur_next = cur;
       }
-      have = true;
     }
+    cur_next = (best_a_err < cur * (1.f - 1e-4f)) ? best_a_err : cur;
+    have = true;
     const float al = ls_alpha(best_a);
     float qt[MAXA];
 #pragma unroll
