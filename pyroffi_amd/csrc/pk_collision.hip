@@ -174,15 +174,6 @@ __device__ __forceinline__ void link_sweep(int n, int g0, int g1, const float4* 
 // CU holds several times more waves than with one table copy per wave.
 constexpr int kSweepWaves = 4;
 
-// Orders one wave's LDS writes before its own later LDS reads from other
-// lanes.  LDS instructions of a wave execute in order, so a wavefront-scope
-// barrier only has to stop the compiler from moving accesses across it.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // SRC = 0: link poses + local spheres (fused path); SRC = 1: precomputed
 // world-frame sphere slots (the reference FFI layout).  T = obstacles per lane.
 // Six waves per SIMD (at most 80 VGPRs) for up to four obstacles per lane,

@@ -1,35 +1,66 @@
 // Forward kinematics for gfx950: one lane = one configuration.
 //
-// A 64-lane workgroup owns 64 consecutive configurations.  It stages the joint
-// records and the 64 x n_act configuration tile in LDS (coalesced loads); every
-// lane walks the joint tree for its configuration and keeps each joint's world
-// pose in LDS directly at its place in the lane's output row (a link pose is
-// the pose of its parent joint, and in a URDF every joint has exactly one
-// child link).  The finished tile -- 64 rows of n_out x 7 floats, the exact
-// image of the block's contiguous output -- is copied out with 16-byte loads
-// and non-temporal 16-byte stores, with no per-element index lookups.  Output
-// positions the walk does not write (root links: identity; a second link on
-// the same joint: a copy) are filled after the walk.  The row stride is odd
-// (the row padded by one float when 7 n_out is even), so the per-lane walk
-// accesses are free of bank conflicts.
+// A 64-lane workgroup owns 64 consecutive configurations.  Its prologue issues
+// every global load at once (the configuration block, the link -> joint map
+// and one joint's table entries per lane), then builds one walk record per
+// joint slot (topological position) in LDS through three barriers: the
+// joint -> slot map, the records and the link map, and the output positions.
+// Every lane then walks the slots for its configuration and keeps each
+// joint's world pose in LDS directly at its place in the lane's output row (a
+// link pose is the pose of its parent joint, and in a URDF every joint has
+// exactly one child link).  The finished tile -- 64 rows of n_out x 7 floats,
+// the exact image of the block's contiguous output -- is copied out with
+// 16-byte loads issued in batches of eight and non-temporal 16-byte stores.
+// Output positions the walk does not write are filled around it (root links:
+// the identity, before; a second link on the same joint: a copy, after).  The
+// row stride
+// is odd (the row padded by one float when 7 n_out is even), so the per-lane
+// walk accesses are free of bank conflicts.
 //
-// Measured phases for 65,536 Panda configurations (one wave per SIMD, run back
-// to back): staging ~4.9 us (kernel start plus three dependent table loads),
-// walk ~6.3 us, copy-out of 24 MB ~5.7 us with the earlier per-element lookups
-// costing another ~5 us on top of the store floor.
+// The walk is a dependency chain per lane with one wave per SIMD, where a
+// dependent instruction waits for its predecessor, so the work off the chain
+// is taken out of it: the revolute joints' half-angle cosines and sines are
+// computed first, two joints at a time (independent instruction streams the
+// wave interleaves), and the chain pass only composes.  It avoids LDS round
+// trips as well: the record, joint value and cosine/sine of slot i + 1 are
+// read while slot i is computed, a parent that is the previous slot is taken
+// from registers, and each record carries its own and its parent's row
+// offsets.  Records are specialised by joint kind: a revolute joint's local
+// rotation is c ptf.q + s (ptf.q * (0, axis)) for the half-angle cosine c and
+// sine s (ptf.t unchanged), a prismatic joint's local translation is
+// ptf.t + q R(ptf.q) v; only a general screw joint takes the full
+// exponential.
 //
 // Reference behaviour: FkCudaImpl / fk_kernel_warp
 // (src/pyroffi/cuda_kernels/_fk_cuda_kernel.cu:39-137, 236-454) and the link
-// gather of src/pyroffi/_robot.py:180-196.
+// gather of src/pyroffi/_robot.py:180-196.  A topological order that is not a
+// permutation of the joints gives NaN rows (the reference reads whatever
+// such a table indexes).
 #include "pk_model.h"
 
 namespace pk {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// Walk record of one joint slot, built by every FK workgroup from the tables.
+struct __attribute__((aligned(16))) FkRec {
+  float ptf[7];  // parent -> joint transform
+  float a[4];    // kind 4: ptf.q * (0, axis); kind 2: R(ptf.q) v; kind 3: v
+  float b[4];    // kind 4: b[0] = |omega|; kind 3: omega
+  float mul, off;
+  int src;       // actuated source, -1 for none
+  int kind;      // 0 constant, 2 prismatic, 3 general twist, 4 revolute
+  int pos;       // float offset of the slot's pose in the lane's row
+  int pad[3];
+};
+static_assert(sizeof(FkRec) == 24 * 4, "FkRec layout");
+
+// Raw table entries of one joint, staged by joint index.
+constexpr int kFkRaw = 18;  // twist 6, parent_tf 7, mul, off, src, parent, spare
+
 // LDS image of one FK workgroup (float offsets into the dynamic LDS array).
 struct FkLds {
-  int recs, slot_of, pos, fill, lmap, flag, cfg, tile, total;
+  int recs, raw, slot_of, first, fill, cfg, tile, total;
 };
 
 // n_extra: positions past the output row for joints without a child link.
@@ -38,15 +69,46 @@ __host__ __device__ inline int fk_row_stride(int n_out, int n_extra) { return (7
 __host__ __device__ inline FkLds fk_lds_layout(int J, int n_act, int n_out, int n_extra) {
   FkLds o;
   o.recs = 0;
-  o.slot_of = o.recs + static_cast<int>(sizeof(JointRec) / 4) * J;
-  o.pos = o.slot_of + J;
-  o.fill = o.pos + J;
-  o.lmap = o.fill + n_out;
-  o.flag = o.lmap + n_out;
-  o.cfg = o.flag + 1;
+  o.raw = o.recs + 24 * J;
+  o.slot_of = o.raw + kFkRaw * J;
+  o.first = o.slot_of + J;
+  o.fill = o.first + J;
+  o.cfg = o.fill + n_out;
   o.tile = ((o.cfg + n_act * kLaneStride) + 3) & ~3;  // 16-byte aligned
   o.total = o.tile + kWave * fk_row_stride(n_out, n_extra);
   return o;
+}
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Largest |x| the branch-free sine/cosine below is used for.
+constexpr float kFkTrigMax = 1.0e4f;
+
+// sin(x) and cos(x) without branches, for |x| <= kFkTrigMax: Cody-Waite
+// reduction x = k pi/2 + r with pi/2 split into three floats (each the float
+// nearest to what the previous ones leave), |r| <= pi/4, then the Taylor
+// polynomials of sin r to r^9 and cos r to r^10 (truncation below 2e-9 on
+// |r| <= pi/4) and the quadrant's swap and signs.  Error on every float in
+// [-8, 8] and every 256th float up to 1e4: at most 1.5 ulp, 7.2e-8 absolute.
+__device__ __forceinline__ void sincos_cw(float x, float& s, float& c) {
+  const float k = rintf(x * 0.636619772367581343f);  // x 2/pi
+  float r = fmaf(-k, 1.5707963705062866f, x);
+  r = fmaf(-k, -4.371138828673793e-08f, r);
+  r = fmaf(-k, -1.7151245100058819e-15f, r);
+  const float r2 = r * r;
+  float ps = fmaf(r2, 2.7557319223985893e-06f, -1.984126984126984e-04f);  // 1/9!, -1/7!
+  ps = fmaf(r2, ps, 8.333333333333333e-03f);                               // 1/5!
+  ps = fmaf(r2, ps, -1.6666666666666666e-01f);                             // -1/3!
+  const float sr = fmaf(r * r2, ps, r);
+  float pc = fmaf(r2, -2.755731922398589e-07f, 2.48015873015873e-05f);     // -1/10!, 1/8!
+  pc = fmaf(r2, pc, -1.388888888888889e-03f);                              // -1/6!
+  pc = fmaf(r2, pc, 4.1666666666666664e-02f);                              // 1/4!
+  pc = fmaf(r2, pc, -0.5f);
+  const float cr = fmaf(r2, pc, 1.0f);
+  const int qd = static_cast<int>(k) & 3;
+  const float so = (qd & 1) ? cr : sr, co = (qd & 1) ? sr : cr;
+  s = (qd & 2) ? -so : so;
+  c = ((qd + 1) & 2) ? -co : co;
 }
 
 // out_mode 0: joint poses (B, J, 7) in joint order; 1: link poses (B, L, 7).
@@ -59,26 +121,29 @@ __global__ __launch_bounds__(64) void fk_kernel(
     const int* __restrict__ link_parent_joint, int n_links, int out_mode, int n_extra,
     float* __restrict__ out) {
   extern __shared__ float smem[];
-  const int J = tables.n_joints;
+  const int J = tables.n_joints;  // <= 64: one slot and one joint per lane
   const int n_out = out_mode ? n_links : J;
   const int row = 7 * n_out;
   const int stride = fk_row_stride(n_out, n_extra);
   const FkLds lay = fk_lds_layout(J, n_act, n_out, n_extra);
-  JointRec* recs = reinterpret_cast<JointRec*>(smem + lay.recs);
+  FkRec* recs = reinterpret_cast<FkRec*>(smem + lay.recs);
+  float* raw = smem + lay.raw;                          // [joint * kFkRaw + field]
   int* slot_of = reinterpret_cast<int*>(smem + lay.slot_of);
-  int* pos = reinterpret_cast<int*>(smem + lay.pos);    // per slot: float offset of its pose in a row
-  int* fill = reinterpret_cast<int*>(smem + lay.fill);  // per output position: -1 walk, -2 identity, else copy source
-  int* lmap = reinterpret_cast<int*>(smem + lay.lmap);  // per output position: its joint slot, or -1
-  int* flag = reinterpret_cast<int*>(smem + lay.flag);
+  int* first = reinterpret_cast<int*>(smem + lay.first);  // per slot: its first output position
+  int* fill = reinterpret_cast<int*>(smem + lay.fill);  // repeated positions: row offset of the pose they copy
   float* cfg_tile = smem + lay.cfg;                     // [a][lane], stride 65
   float* tile = smem + lay.tile;                        // [lane * stride + offset]
 
+  // The workgroup is one wave: LDS hand-offs between lanes need only
+  // wave_sync(), and global loads stay in flight until their first use.
   const int lane = threadIdx.x;
   const long long b0 = static_cast<long long>(blockIdx.x) * kWave;
   const int n_valid = static_cast<int>(min(static_cast<long long>(kWave), batch - b0));
+  constexpr int kNoPos = 0x7fffffff;
 
-  // The configuration block and the link -> joint map are loaded into
-  // registers first, so their latency overlaps the table staging.
+  // A. Every global load of the prologue at once: the configuration block,
+  // the link -> joint map, this lane's slot's joint and this lane's joint's
+  // table entries.
   const float* cfg_blk = cfg + b0 * n_act;
   const int n_cfg = n_valid * n_act;
   float cv[kMaxAct];
@@ -94,101 +159,268 @@ __global__ __launch_bounds__(64) void fk_kernel(
     const int o = lane + i * kWave;
     pjv[i] = (o < n_out) ? (out_mode ? link_parent_joint[o] : o) : -1;
   }
-  if (lane == 0) *flag = 0;
-  for (int i = lane; i < J; i += kWave) pos[i] = -1;
-  stage_joint_records(tables, recs, slot_of);  // ends with a barrier
+  const bool is_slot = lane < J;
+  int jt = -1;
+  if (is_slot) {
+    jt = tables.topo_inv[lane];
+    float e[kFkRaw];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) e[c] = tables.twists[lane * 6 + c];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) e[6 + c] = tables.parent_tf[lane * 7 + c];
+    e[13] = tables.mimic_mul[lane];
+    e[14] = tables.mimic_off[lane];
+    const int m = tables.mimic_act_idx[lane];
+    const int ai = tables.act_idx[lane];
+    int src = (m != -1) ? m : ai;
+    if (src < 0 || src >= tables.n_act) src = -1;
+    e[15] = __int_as_float(src);
+    e[16] = __int_as_float(tables.parent_idx[lane]);
+    e[17] = 0.f;
+#pragma unroll
+    for (int c = 0; c < kFkRaw; ++c) raw[lane * kFkRaw + c] = e[c];
+    first[lane] = kNoPos;
+    if (jt >= 0 && jt < J) slot_of[jt] = lane;
+  }
+  wave_sync();
+
+  // B. The slot order is valid when every slot reads its own index back (J
+  // distinct joints).  Records without positions and the link map (first
+  // output position per slot by an LDS atomic minimum).
+  const bool own = !is_slot || (jt >= 0 && jt < J && slot_of[jt] == lane);
+  const bool perm = __all(own);
+  int ps = -1, my_kind = -1;
+  if (is_slot && perm) {
+    const float* e = raw + jt * kFkRaw;
+    FkRec r;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) r.ptf[c] = e[6 + c];
+    const float v[3] = {e[0], e[1], e[2]}, w[3] = {e[3], e[4], e[5]};
+    r.mul = e[13];
+    r.off = e[14];
+    r.src = __float_as_int(e[15]);
+    const int p = __float_as_int(e[16]);
+    ps = (p >= 0 && p < J) ? slot_of[p] : -1;
+    if (ps >= lane) ps = -1;  // not topologically before us: invalid table, treated as a root
+    const float l2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const float a2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    int kind;
+    if (r.src == -1 && r.off == 0.f) kind = 0;
+    else if (a2 == 0.f) kind = 2;
+    else if (l2 == 0.f) kind = 4;
+    else kind = 3;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) { r.a[c] = 0.f; r.b[c] = 0.f; }
+    if (kind == 4) {
+      const float na = sqrtf(a2);
+      const float inv = 1.f / na;
+      quat_mul(r.ptf[0], r.ptf[1], r.ptf[2], r.ptf[3], 0.f, w[0] * inv, w[1] * inv, w[2] * inv,
+               r.a[0], r.a[1], r.a[2], r.a[3]);
+      r.b[0] = na;
+    } else if (kind == 2) {
+      quat_rotate(r.ptf[0], r.ptf[1], r.ptf[2], r.ptf[3], v[0], v[1], v[2], r.a[0], r.a[1], r.a[2]);
+    } else if (kind == 3) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { r.a[c] = v[c]; r.b[c] = w[c]; }
+    }
+    r.kind = kind;
+    my_kind = kind;
+    r.pos = 0;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    recs[lane] = r;
+  }
+  int lm[kOutIters];
 #pragma unroll
   for (int i = 0; i < kOutIters; ++i) {
     const int o = lane + i * kWave;
-    if (o < n_out) {
-      const int pj = pjv[i];
-      lmap[o] = (pj >= 0 && pj < J) ? slot_of[pj] : -1;
+    const int pj = pjv[i];
+    lm[i] = (perm && o < n_out && pj >= 0 && pj < J) ? slot_of[pj] : -1;
+    if (lm[i] >= 0) atomicMin(&first[lm[i]], o);
+  }
+  wave_sync();
+
+  // C. Row offsets: a slot's first output position, or the k-th spare
+  // position past the row for the k-th slot without one; each slot's own
+  // offset, its parent's and whether the parent is the previous slot stay in
+  // that slot's lane (read back with readlane).  Output positions without a
+  // joint (root links) get the identity now, in every row; positions that
+  // repeat another position's joint are copied after the walk.
+  const unsigned long long miss = __ballot(is_slot && perm && first[lane] == kNoPos);
+  const bool bad = !perm || __popcll(miss) > n_extra;
+  const unsigned long long screw = __ballot(my_kind == 3);
+  auto slot_pos = [&](int s) {
+    const int f = first[s];
+    return (f != kNoPos) ? 7 * f : 7 * (n_out + __popcll(miss & ((1ull << s) - 1ull)));
+  };
+  int link = 0;  // pos | (parent pos + 1) << 12 | (parent is the previous slot) << 24
+  if (is_slot && !bad) {
+    const int pos = slot_pos(lane);
+    const int ppos = (ps >= 0) ? slot_pos(ps) : -1;
+    recs[lane].pos = pos;
+    link = pos | ((ppos + 1) << 12) | ((ps >= 0 && ps == lane - 1) ? (1 << 24) : 0);
+  }
+  unsigned long long ident[kOutIters], dup[kOutIters];
+#pragma unroll
+  for (int i = 0; i < kOutIters; ++i) {
+    const int o = lane + i * kWave;
+    const int s = lm[i];
+    const bool is_dup = o < n_out && s >= 0 && first[s] != o;
+    if (is_dup) fill[o] = 7 * first[s];
+    ident[i] = __ballot(o < n_out && s < 0);
+    dup[i] = __ballot(is_dup);
+  }
+  float* trow = tile + lane * stride;
+  if (lane < n_valid && !bad) {
+#pragma unroll
+    for (int i = 0; i < kOutIters; ++i) {
+      for (unsigned long long m = ident[i]; m; m &= m - 1ull) {
+        float* d = trow + 7 * (i * kWave + __builtin_ctzll(m));
+        d[0] = 1.f;
+#pragma unroll
+        for (int c = 1; c < 7; ++c) d[c] = 0.f;
+      }
     }
   }
+  // The configuration tile [a][lane]: element f of the block is
+  // (configuration f / n_act, joint f % n_act); the quotient is a multiply
+  // and shift (exact for f < 2^10, n_act <= 16).
+  const unsigned div_m = (65536u + static_cast<unsigned>(n_act) - 1u) / static_cast<unsigned>(max(n_act, 1));
 #pragma unroll
   for (int i = 0; i < kMaxAct; ++i) {
     const int f = lane + i * kWave;
     if (f < n_cfg) {
-      const int l = f / n_act, a = f - l * n_act;
+      const int l = static_cast<int>((static_cast<unsigned>(f) * div_m) >> 16), a = f - l * n_act;
       cfg_tile[a * kLaneStride + l] = cv[i];
     }
   }
-  __syncthreads();
-  // Positions: the first output position of a slot is where the walk writes
-  // its pose; later positions of the same slot copy it; positions without a
-  // slot read the identity.
-#pragma unroll
-  for (int i = 0; i < kOutIters; ++i) {
-    const int o = lane + i * kWave;
-    if (o < n_out) {
-      const int s = lmap[o];
-      int first = o;
-      for (int o2 = 0; o2 < o; ++o2)
-        if (s >= 0 && lmap[o2] == s) { first = o2; break; }
-      fill[o] = (s < 0) ? -2 : (first == o ? -1 : 7 * first);
-      if (s >= 0 && first == o) pos[s] = 7 * o;
-    }
-  }
-  __syncthreads();
-  // Slots without an output position take the spare positions past the row.
-  for (int i = lane; i < J; i += kWave) {
-    if (pos[i] < 0) {
-      int k = 0;
-      for (int i2 = 0; i2 < i; ++i2) k += (pos[i2] < 0) ? 1 : 0;
-      if (k < n_extra) {
-        pos[i] = 7 * (n_out + k);
-      } else {
-        pos[i] = 0;  // no room: the rows are overwritten with NaN below
-        *flag = 1;
-      }
-    }
-  }
-  __syncthreads();
+  wave_sync();
 
-  float* trow = tile + lane * stride;
   if (lane < n_valid) {
-    for (int i = 0; i < J; ++i) {
-      const JointRec& r = recs[i];
-      const float qref = (r.src >= 0) ? cfg_tile[r.src * kLaneStride + lane] : 0.f;
-      const float q = qref * r.mul + r.off;
-      const Pose tpc = joint_local(r, q);
-      const Pose tw = (r.parent_slot < 0) ? tpc : compose(load_pose(trow + pos[r.parent_slot]), tpc);
-      float* d = trow + pos[i];
-      d[0] = tw.w; d[1] = tw.x; d[2] = tw.y; d[3] = tw.z;
-      d[4] = tw.tx; d[5] = tw.ty; d[6] = tw.tz;
-    }
-    const bool bad = *flag != 0;
-    for (int o = 0; o < n_out; ++o) {
-      const int fo = fill[o];
-      if (fo == -1 && !bad) continue;
-      float* d = trow + 7 * o;
-      if (bad) {
+    if (!bad && J > 0) {
+      // Local transforms of every slot, four slots at a time (independent
+      // work the wave interleaves), into the slot's row position; the chain
+      // pass below reads each one before it writes the world pose there.
+      // Kinds 0, 2 and 4 share one branch-free form: th = 0 and tq = 0
+      // reduce it to the constant transform exactly.
+      for (int base = 0; base < J; base += 4) {
+        float lc[4][7];
+        int lp[4];
+        bool big = false;
 #pragma unroll
-        for (int c = 0; c < 7; ++c) d[c] = __builtin_nanf("");
-      } else if (fo == -2) {
-        d[0] = 1.f;
+        for (int k = 0; k < 4; ++k) {
+          const FkRec& r = recs[min(base + k, J - 1)];
+          const int src = uniform_i(r.src);
+          const float qv = cfg_tile[max(src, 0) * kLaneStride + lane];
+          const float q = ((src >= 0) ? qv : 0.f) * r.mul + r.off;
+          const float x = 0.5f * (r.b[0] * q);
+          big |= !(fabsf(x) <= kFkTrigMax);
+          float s, c;
+          sincos_cw(x, s, c);
+          if (fabsf(x) < 0.5e-6f) { s = 0.f; c = 1.f; }  // the exponential's small-angle rule, |th| < 1e-6
+          const float tq = (uniform_i(r.kind) == 2) ? q : 0.f;
+          lc[k][0] = c * r.ptf[0] + s * r.a[0];
+          lc[k][1] = c * r.ptf[1] + s * r.a[1];
+          lc[k][2] = c * r.ptf[2] + s * r.a[2];
+          lc[k][3] = c * r.ptf[3] + s * r.a[3];
+          lc[k][4] = r.ptf[4] + tq * r.a[0];
+          lc[k][5] = r.ptf[5] + tq * r.a[1];
+          lc[k][6] = r.ptf[6] + tq * r.a[2];
+          lp[k] = uniform_i(r.pos);
+        }
+        if (__any(big)) {
+          // Joint values beyond the branch-free range: the library sine/cosine.
 #pragma unroll
-        for (int c = 1; c < 7; ++c) d[c] = 0.f;
-      } else {
-        float v[7];
+          for (int k = 0; k < 4; ++k) {
+            const FkRec& r = recs[min(base + k, J - 1)];
+            const int src = uniform_i(r.src);
+            const float q = ((src >= 0) ? cfg_tile[src * kLaneStride + lane] : 0.f) * r.mul + r.off;
+            const float x = 0.5f * (r.b[0] * q);
+            float s, c;
+            sincosf(x, &s, &c);
+            if (fabsf(x) < 0.5e-6f) { s = 0.f; c = 1.f; }
+            lc[k][0] = c * r.ptf[0] + s * r.a[0];
+            lc[k][1] = c * r.ptf[1] + s * r.a[1];
+            lc[k][2] = c * r.ptf[2] + s * r.a[2];
+            lc[k][3] = c * r.ptf[3] + s * r.a[3];
+          }
+        }
 #pragma unroll
-        for (int c = 0; c < 7; ++c) v[c] = trow[fo + c];
+        for (int k = 0; k < 4; ++k) {
+          float* d = trow + lp[k];
 #pragma unroll
-        for (int c = 0; c < 7; ++c) d[c] = v[c];
+          for (int c = 0; c < 7; ++c) d[c] = lc[k][c];
+        }
       }
+      // General screw joints (rare): the full exponential.
+      for (unsigned long long m = screw; m; m &= m - 1ull) {
+        const FkRec& r = recs[__builtin_ctzll(m)];
+        const int src = uniform_i(r.src);
+        const float q = ((src >= 0) ? cfg_tile[src * kLaneStride + lane] : 0.f) * r.mul + r.off;
+        const Pose ptf = load_pose(r.ptf);
+        const Pose loc = compose(ptf, joint_exp(3, q, r.a[0], r.a[1], r.a[2], r.b[0], r.b[1], r.b[2]));
+        float* d = trow + uniform_i(r.pos);
+        d[0] = loc.w; d[1] = loc.x; d[2] = loc.y; d[3] = loc.z;
+        d[4] = loc.tx; d[5] = loc.ty; d[6] = loc.tz;
+      }
+      // The chain: world = parent o local, in slot order.  The next slot's
+      // local transform is read while this slot is composed.
+      Pose prev = pose_identity();
+      Pose loc = load_pose(trow + (__builtin_amdgcn_readlane(link, 0) & 0xfff));
+      for (int i = 0; i < J; ++i) {
+        const int lk = __builtin_amdgcn_readlane(link, i);
+        const int lkn = __builtin_amdgcn_readlane(link, min(i + 1, J - 1));
+        const Pose nxt = load_pose(trow + (lkn & 0xfff));
+        const int ppos = ((lk >> 12) & 0xfff) - 1;
+        Pose tw = loc;
+        if (ppos >= 0) {
+          const Pose par = (lk >> 24) ? prev : load_pose(trow + ppos);
+          tw = compose(par, loc);
+        }
+        float* d = trow + (lk & 0xfff);
+        d[0] = tw.w; d[1] = tw.x; d[2] = tw.y; d[3] = tw.z;
+        d[4] = tw.tx; d[5] = tw.ty; d[6] = tw.tz;
+        prev = tw;
+        loc = nxt;
+      }
+#pragma unroll
+      for (int i = 0; i < kOutIters; ++i) {
+        for (unsigned long long m = dup[i]; m; m &= m - 1ull) {
+          const int o = i * kWave + __builtin_ctzll(m);
+          const int fo = uniform_i(fill[o]);
+          float v[7];
+#pragma unroll
+          for (int c = 0; c < 7; ++c) v[c] = trow[fo + c];
+          float* d = trow + 7 * o;
+#pragma unroll
+          for (int c = 0; c < 7; ++c) d[c] = v[c];
+        }
+      }
+    } else if (bad) {
+      for (int f = 0; f < row; ++f) trow[f] = __builtin_nanf("");
     }
   }
-  __syncthreads();
+  wave_sync();
 
   // Copy-out of the block's n_valid * row contiguous floats.
   float* out_blk = out + b0 * row;
   const int total = n_valid * row;
   if (stride == row) {
-    // The tile is the output image: 16-byte loads and non-temporal stores
-    // (the block starts 16-byte aligned: 64 rows of 7 n_out floats).
+    // The tile is the output image (the block starts 16-byte aligned: 64 rows
+    // of 7 n_out floats).  Eight 16-byte LDS loads are issued before their
+    // stores, so the wave waits for LDS once per eight stores.
     const int n4 = total / 4;
     const v4f* t4 = reinterpret_cast<const v4f*>(tile);
-    for (int f4 = lane; f4 < n4; f4 += kWave) __builtin_nontemporal_store(t4[f4], reinterpret_cast<v4f*>(out_blk) + f4);
+    v4f* o4 = reinterpret_cast<v4f*>(out_blk);
+    for (int base = lane; base < n4; base += 8 * kWave) {
+      v4f v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = t4[min(base + k * kWave, n4 - 1)];  // in the tile; stored only when < n4
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int f4 = base + k * kWave;
+        if (f4 < n4) __builtin_nontemporal_store(v[k], o4 + f4);
+      }
+    }
     for (int f = 4 * n4 + lane; f < total; f += kWave) out_blk[f] = tile[f];
     return;
   }

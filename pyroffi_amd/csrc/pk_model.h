@@ -155,6 +155,15 @@ __device__ __forceinline__ void fk_walk(const JointRec* recs, int J, CfgFn cfg_l
   }
 }
 
+// Orders one wave's LDS writes before its own later LDS reads from other
+// lanes.  LDS instructions of a wave execute in order, so a wavefront-scope
+// barrier only has to stop the compiler from moving accesses across it.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Reductions with DPP lane moves (no LDS traffic).  Quad swaps and the
 // half-row / row mirrors combine lanes inside 16-lane rows; row_bcast15 and
 // row_bcast31 fold rows 0-2 into row 3.

@@ -44,6 +44,26 @@ def test_fk_joints_matches_oracle(oracle_mod, name):
     np.testing.assert_allclose(got, ref, atol=TOL, rtol=0)
 
 
+@pytest.mark.parametrize("name", ("panda", "ur5"))
+def test_fk_joint_values_far_outside_limits(oracle_mod, name):
+    """Revolute joint values of any size: the kernel's branch-free half-angle
+    sine/cosine (Cody-Waite reduction, |q/2| <= 1e4) and, above that range,
+    the library sine/cosine, mixed within one workgroup.  Both robots have
+    unit joint axes, so the half angle is exact in float32 and the FP64
+    oracle sees the same angle."""
+    robot, _ = load_robot(name)
+    rng = np.random.default_rng(5)
+    A = robot.joints.num_actuated_joints
+    parts = [rng.uniform(-60, 60, (64, A)), rng.uniform(-3000, 3000, (64, A)),
+             rng.choice([-1, 1], (64, A)) * rng.uniform(2.5e4, 5e4, (64, A))]
+    mixed = rng.uniform(-3, 3, (64, A))
+    mixed[::5] *= 1.5e4  # a few far configurations among ordinary ones
+    q = np.concatenate(parts + [mixed]).astype(np.float32)
+    got = robot.forward_kinematics(q)
+    ref = oracle_mod.fk_links(oracle_mod.Tables.of(robot), q.astype(np.float64))
+    np.testing.assert_allclose(got, ref, atol=TOL, rtol=0)
+
+
 def test_fk_batch_shapes_and_empty():
     robot, _ = load_robot("panda")
     q = torch.zeros((2, 3, 7), device="cuda")
