@@ -17,19 +17,20 @@
 // is odd (the row padded by one float when 7 n_out is even), so the per-lane
 // walk accesses are free of bank conflicts.
 //
-// The walk is a dependency chain per lane with one wave per SIMD, where a
-// dependent instruction waits for its predecessor, so the work off the chain
-// is taken out of it: the revolute joints' half-angle cosines and sines are
-// computed first, two joints at a time (independent instruction streams the
-// wave interleaves), and the chain pass only composes.  It avoids LDS round
-// trips as well: the record, joint value and cosine/sine of slot i + 1 are
-// read while slot i is computed, a parent that is the previous slot is taken
-// from registers, and each record carries its own and its parent's row
-// offsets.  Records are specialised by joint kind: a revolute joint's local
-// rotation is c ptf.q + s (ptf.q * (0, axis)) for the half-angle cosine c and
-// sine s (ptf.t unchanged), a prismatic joint's local translation is
-// ptf.t + q R(ptf.q) v; only a general screw joint takes the full
-// exponential.
+// The walk is a dependency chain per lane with one wave per SIMD, where an
+// instruction issues about every 8 cycles, so the work off the chain is taken
+// out of it: the local transforms of the revolute and prismatic joints are
+// computed first, four joints at a time (independent instruction streams the
+// wave interleaves) with a branch-free sine/cosine, and the chain pass only
+// composes (world = parent o local).  It avoids LDS round trips as well: each
+// slot's own and parent row offsets live in the slot's lane and are read
+// with readlane, the next slot's local transform is read while the current
+// one is composed, and a parent that is the previous slot stays in
+// registers.  A revolute joint's local rotation is c ptf.q + s (ptf.q *
+// (0, axis)) for the half-angle cosine c and sine s (ptf.t unchanged), a
+// prismatic joint's local translation ptf.t + q R(ptf.q) v, a constant
+// joint's local transform its parent_tf; only a general screw joint takes the
+// full exponential.
 //
 // Reference behaviour: FkCudaImpl / fk_kernel_warp
 // (src/pyroffi/cuda_kernels/_fk_cuda_kernel.cu:39-137, 236-454) and the link
@@ -248,16 +249,18 @@ __global__ __launch_bounds__(64) void fk_kernel(
   const unsigned long long miss = __ballot(is_slot && perm && first[lane] == kNoPos);
   const bool bad = !perm || __popcll(miss) > n_extra;
   const unsigned long long screw = __ballot(my_kind == 3);
+  const unsigned long long turn_slide = __ballot(my_kind == 2 || my_kind == 4);
   auto slot_pos = [&](int s) {
     const int f = first[s];
     return (f != kNoPos) ? 7 * f : 7 * (n_out + __popcll(miss & ((1ull << s) - 1ull)));
   };
-  int link = 0;  // pos | (parent pos + 1) << 12 | (parent is the previous slot) << 24
+  // pos | (parent pos + 1) << 12 | (parent is the previous slot) << 24 | (constant joint) << 25
+  int link = 0;
   if (is_slot && !bad) {
     const int pos = slot_pos(lane);
     const int ppos = (ps >= 0) ? slot_pos(ps) : -1;
     recs[lane].pos = pos;
-    link = pos | ((ppos + 1) << 12) | ((ps >= 0 && ps == lane - 1) ? (1 << 24) : 0);
+    link = pos | ((ppos + 1) << 12) | ((ps >= 0 && ps == lane - 1) ? (1 << 24) : 0) | ((my_kind == 0) ? (1 << 25) : 0);
   }
   unsigned long long ident[kOutIters], dup[kOutIters];
 #pragma unroll
@@ -297,18 +300,24 @@ __global__ __launch_bounds__(64) void fk_kernel(
 
   if (lane < n_valid) {
     if (!bad && J > 0) {
-      // Local transforms of every slot, four slots at a time (independent
-      // work the wave interleaves), into the slot's row position; the chain
-      // pass below reads each one before it writes the world pose there.
-      // Kinds 0, 2 and 4 share one branch-free form: th = 0 and tq = 0
-      // reduce it to the constant transform exactly.
-      for (int base = 0; base < J; base += 4) {
+      // Local transforms of the revolute and prismatic slots, four slots at
+      // a time (independent work the wave interleaves), into the slot's row
+      // position; the chain pass below reads each one before it writes the
+      // world pose there (a constant joint's local transform is its record's
+      // parent_tf).  Both kinds share one branch-free form: th = 0 (prismatic)
+      // and tq = 0 (revolute) reduce it to the other kind exactly.
+      for (unsigned long long m = turn_slide; m;) {
         float lc[4][7];
-        int lp[4];
+        int lp[4], ii[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          ii[k] = m ? __builtin_ctzll(m) : ii[k > 0 ? k - 1 : 0];  // m != 0 for k = 0
+          if (m) m &= m - 1ull;
+        }
         bool big = false;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const FkRec& r = recs[min(base + k, J - 1)];
+          const FkRec& r = recs[ii[k]];
           const int src = uniform_i(r.src);
           const float qv = cfg_tile[max(src, 0) * kLaneStride + lane];
           const float q = ((src >= 0) ? qv : 0.f) * r.mul + r.off;
@@ -331,7 +340,7 @@ __global__ __launch_bounds__(64) void fk_kernel(
           // Joint values beyond the branch-free range: the library sine/cosine.
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
-            const FkRec& r = recs[min(base + k, J - 1)];
+            const FkRec& r = recs[ii[k]];
             const int src = uniform_i(r.src);
             const float q = ((src >= 0) ? cfg_tile[src * kLaneStride + lane] : 0.f) * r.mul + r.off;
             const float x = 0.5f * (r.b[0] * q);
@@ -363,17 +372,21 @@ __global__ __launch_bounds__(64) void fk_kernel(
         d[4] = loc.tx; d[5] = loc.ty; d[6] = loc.tz;
       }
       // The chain: world = parent o local, in slot order.  The next slot's
-      // local transform is read while this slot is composed.
+      // local transform (from its row position, or a constant joint's
+      // record) is read while this slot is composed.
+      auto local_src = [&](int i, int lk) -> const float* {
+        return ((lk >> 25) & 1) ? recs[i].ptf : trow + (lk & 0xfff);
+      };
       Pose prev = pose_identity();
-      Pose loc = load_pose(trow + (__builtin_amdgcn_readlane(link, 0) & 0xfff));
+      Pose loc = load_pose(local_src(0, __builtin_amdgcn_readlane(link, 0)));
       for (int i = 0; i < J; ++i) {
         const int lk = __builtin_amdgcn_readlane(link, i);
-        const int lkn = __builtin_amdgcn_readlane(link, min(i + 1, J - 1));
-        const Pose nxt = load_pose(trow + (lkn & 0xfff));
+        const int in = min(i + 1, J - 1);
+        const Pose nxt = load_pose(local_src(in, __builtin_amdgcn_readlane(link, in)));
         const int ppos = ((lk >> 12) & 0xfff) - 1;
         Pose tw = loc;
         if (ppos >= 0) {
-          const Pose par = (lk >> 24) ? prev : load_pose(trow + ppos);
+          const Pose par = ((lk >> 24) & 1) ? prev : load_pose(trow + ppos);
           tw = compose(par, loc);
         }
         float* d = trow + (lk & 0xfff);
