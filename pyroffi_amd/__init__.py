@@ -9,7 +9,9 @@ Public API (same names and arguments as pyroffi):
 * ``ls_ik_solve`` (the JAX-convention LM), ``hjcd_solve`` /
   ``hjcd_solve_batch``, ``ls_ik_solve_cuda`` / ``ls_ik_solve_cuda_batch``;
   ``UnevenLsIkSolver`` for uneven multi-robot batches in one graph
-* ``costs.pose_hinge_cost`` (differentiable, hand-written adjoint)
+* ``costs.pose_cost`` / ``costs.self_collision_cost`` /
+  ``costs.world_collision_cost`` (the residual factories; jaxls is optional) and
+  ``costs.pose_hinge_cost`` (differentiable, hand-written adjoint)
 * ``lbfgs_trajopt`` / ``LbfgsTrajOptConfig`` (direct L-BFGS trajectory
   optimisation) and ``lbfgs_trajopt_uneven``; ``chomp_trajopt`` /
   ``ChompTrajOptConfig`` (CHOMP with the covariant update);
