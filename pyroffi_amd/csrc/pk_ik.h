@@ -501,6 +501,29 @@ __device__ __forceinline__ float weighted_error(const IkView<MAXA, NEE>& v, cons
   return e;
 }
 
+// Weighted Jacobian contribution of one node to its column for one EE:
+// [z x (p_ee - p_node); z] (revolute) or [z; 0] (prismatic), times the mimic
+// multiplier and the residual weights.  Explicit fmaf fixes the rounding of
+// each cross-product term, so every kernel that calls this gets the same bits
+// whatever registers or LDS its operands come from.
+__device__ __forceinline__ void jac_column(const Pose& Pn, const float* ax, int jt, float tex, float tey,
+                                           float tez, float mul, const float* W, float* out6) {
+  float zx, zy, zz;
+  quat_rotate(Pn.w, Pn.x, Pn.y, Pn.z, ax[0], ax[1], ax[2], zx, zy, zz);
+  float g[6];
+  if (jt == 1) {
+    const float dx = tex - Pn.tx, dy = tey - Pn.ty, dz = tez - Pn.tz;
+    g[0] = fmaf(zy, dz, -(zz * dy));
+    g[1] = fmaf(zz, dx, -(zx * dz));
+    g[2] = fmaf(zx, dy, -(zy * dx));
+    g[3] = zx; g[4] = zy; g[5] = zz;
+  } else {
+    g[0] = zx; g[1] = zy; g[2] = zz; g[3] = 0.f; g[4] = 0.f; g[5] = 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) out6[k] = (mul * g[k]) * W[k];
+}
+
 // LM iterations [it0, it1) of one seed (the algorithm in the file header).
 // G = 1: one lane per seed.  G = 8: the 8 lanes of an aligned group hold the
 // same seed, compute the same step, and lane k evaluates line-search step k
@@ -566,27 +589,18 @@ __device__ __forceinline__ void lm_iterate(const IkView<MAXA, NEE>& v, const flo
         const NodeRec& nr = sh.nodes[n];
         const int jt = __builtin_amdgcn_readfirstlane(nr.jac_type);
         if (jt != 0) {
-          float zx, zy, zz;
-          quat_rotate(P[n].w, P[n].x, P[n].y, P[n].z, nr.ax[0], nr.ax[1], nr.ax[2], zx, zy, zz);
           const int src = __builtin_amdgcn_readfirstlane(nr.src);
           const unsigned em = __builtin_amdgcn_readfirstlane(nr.ee_mask);
-          const float mul = nr.mul;
 #pragma unroll
           for (int e = 0; e < NEE; ++e) {
             if (em & (1u << e)) {
-              float g[6];
-              if (jt == 1) {
-                const float ax = Tee[e].tx - P[n].tx, ay = Tee[e].ty - P[n].ty, az = Tee[e].tz - P[n].tz;
-                g[0] = zy * az - zz * ay; g[1] = zz * ax - zx * az; g[2] = zx * ay - zy * ax;
-                g[3] = zx; g[4] = zy; g[5] = zz;
-              } else {
-                g[0] = zx; g[1] = zy; g[2] = zz; g[3] = 0.f; g[4] = 0.f; g[5] = 0.f;
-              }
+              float g6[6];
+              jac_column(P[n], nr.ax, jt, Tee[e].tx, Tee[e].ty, Tee[e].tz, nr.mul, W, g6);
 #pragma unroll
               for (int c = 0; c < MAXA; ++c)
                 if (src == c) {
 #pragma unroll
-                  for (int k = 0; k < 6; ++k) Jm[6 * e + k][c] += mul * g[k] * W[k];
+                  for (int k = 0; k < 6; ++k) Jm[6 * e + k][c] += g6[k];
                 }
             }
           }
@@ -869,6 +883,300 @@ __global__ __launch_bounds__(64) void ls_ik_kernel(IkArgs a) {
   }
 }
 
+// One seed per wave for small launches of the 8-wide instantiations: the
+// line search's steps run on lanes 0-7 as in the 8-lane kernel, and the step
+// in between is spread over the wave through LDS -- one lane per node for the
+// Jacobian contributions, one lane per column for the column sums and
+// scaling, one lane per entry of the packed normal matrix and of the
+// right-hand side -- instead of every lane repeating it.  The Cholesky and
+// the two triangular solves (a dependency chain) run in every lane on the
+// matrix read back from LDS.  Each quantity is computed by the same
+// expressions, in the same order, as in lm_iterate; the compiler's choices of
+// which products to fuse into FMAs differ between the two schedules, so the
+// iterates agree with the one-lane kernel's to rounding (about 1e-6 after one
+// iteration) rather than bit for bit, and diverge over many iterations like
+// any two FP32 orderings of LM.
+template <int MAXA, int NEE>
+struct WaveLds {
+  float P[MAXA][7];         // node poses of the current iterate
+  float Tee[NEE][7];        // its EE poses
+  float r[6 * NEE];         // its residuals
+  float Jn[MAXA][6 * NEE];  // per node: its weighted Jacobian contribution
+  float Js[MAXA][6 * NEE];  // per column: the scaled column
+  float ics[MAXA];          // per column: 1 / (|column| + 1e-8)
+  float A[MAXA * (MAXA + 1) / 2];
+  float rhs[MAXA];
+  int cnt[MAXA];            // nodes driving column c ...
+  int drv[MAXA][MAXA];      // ... in node order
+};
+
+// Threshold (problems x seeds) up to which the 8-wide instantiations run one
+// seed per wave: four waves per SIMD.
+constexpr long long kWaveSeeds = 4096;
+
+template <int MAXA, int NEE>
+__device__ __forceinline__ void store_iterate(WaveLds<MAXA, NEE>& wl, const Pose* P, const Pose* Tee, const float* r) {
+#pragma unroll
+  for (int n = 0; n < MAXA; ++n) pose_to(wl.P[n], P[n]);
+#pragma unroll
+  for (int e = 0; e < NEE; ++e) pose_to(wl.Tee[e], Tee[e]);
+#pragma unroll
+  for (int i = 0; i < 6 * NEE; ++i) wl.r[i] = r[i];
+}
+
+template <int MAXA, int NEE>
+__device__ __forceinline__ void lm_iterate_wave(const IkView<MAXA, NEE>& v, const float (&tg)[NEE][7],
+                                                const float* W, const IkArgs& a, LmState<MAXA>& st,
+                                                WaveLds<MAXA, NEE>& wl, float cur_next) {
+  const IkShared<MAXA, NEE>& sh = *v.sh;
+  const GridArgs& grid = a.grid;
+  const int lane = threadIdx.x;
+  constexpr int NT = MAXA * (MAXA + 1) / 2;
+  for (int it = 0; it < a.max_iter && !st.done; ++it) {
+    wave_sync();
+    float r[6 * NEE];
+#pragma unroll
+    for (int i = 0; i < 6 * NEE; ++i) r[i] = wl.r[i];
+    bool conv = true;
+    float max_p = 0.f, max_o = 0.f;
+#pragma unroll
+    for (int e = 0; e < NEE; ++e) {
+      const float np = sqrtf(r[6 * e] * r[6 * e] + r[6 * e + 1] * r[6 * e + 1] + r[6 * e + 2] * r[6 * e + 2]);
+      const float no = sqrtf(r[6 * e + 3] * r[6 * e + 3] + r[6 * e + 4] * r[6 * e + 4] + r[6 * e + 5] * r[6 * e + 5]);
+      if (e < a.n_ee && (np >= a.eps_pos || no >= a.eps_ori)) conv = false;
+      max_p = fmaxf(max_p, np);
+      max_o = fmaxf(max_o, no);
+    }
+    if (conv) { st.done = 1; break; }
+    float fw[6 * NEE];
+#pragma unroll
+    for (int i = 0; i < 6 * NEE; ++i) fw[i] = r[i] * W[i % 6];
+    const float cur = cur_next;  // the accepted point's error (residuals and penalty)
+
+    // Jacobian contribution of node `lane` to its column.
+    if (lane < MAXA) {
+      float g6[6 * NEE];
+#pragma unroll
+      for (int i = 0; i < 6 * NEE; ++i) g6[i] = 0.f;
+      if (lane < sh.n_nodes) {
+        const NodeRec& nr = sh.nodes[lane];
+        const int jt = nr.jac_type;
+        if (jt != 0) {
+          const Pose Pn = load_pose(wl.P[lane]);
+          const unsigned em = nr.ee_mask;
+#pragma unroll
+          for (int e = 0; e < NEE; ++e)
+            if (em & (1u << e)) jac_column(Pn, nr.ax, jt, wl.Tee[e][4], wl.Tee[e][5], wl.Tee[e][6], nr.mul, W, g6 + 6 * e);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 6 * NEE; ++i) wl.Jn[lane][i] = g6[i];
+    }
+    wave_sync();
+    // Column `lane`: its nodes' contributions summed in node order, then scaled.
+    if (lane < MAXA) {
+      float col[6 * NEE];
+#pragma unroll
+      for (int i = 0; i < 6 * NEE; ++i) col[i] = 0.f;
+      const int cnt = wl.cnt[lane];
+      for (int k = 0; k < cnt; ++k) {
+        const int n = wl.drv[lane][k];
+#pragma unroll
+        for (int i = 0; i < 6 * NEE; ++i) col[i] += wl.Jn[n][i];
+      }
+      float sq = 0.f;
+#pragma unroll
+      for (int i = 0; i < 6 * NEE; ++i) sq += col[i] * col[i];
+      const float ic = 1.f / (sqrtf(sq) + 1e-8f);
+#pragma unroll
+      for (int i = 0; i < 6 * NEE; ++i) wl.Js[lane][i] = col[i] * ic;
+      wl.ics[lane] = ic;
+    }
+    wave_sync();
+    // Packed normal matrix entry (i, j) on lane i (i + 1) / 2 + j, right-hand
+    // side i on lane NT + i; damping and frozen joints as in lm_iterate.
+    for (int t = lane; t < NT + MAXA; t += kWave) {
+      if (t < NT) {
+        int i = 0;
+        while ((i + 1) * (i + 2) / 2 <= t) ++i;
+        const int j = t - i * (i + 1) / 2;
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6 * NEE; ++k) acc += wl.Js[i][k] * wl.Js[j][k];
+        if (i == j) acc += fmaxf(st.lam, kDampFloor);
+        if (sh.fixed[i] || sh.fixed[j]) acc = (i == j) ? 1.f : 0.f;
+        wl.A[t] = acc;
+      } else {
+        const int i = t - NT;
+        float rb = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6 * NEE; ++k) rb += wl.Js[i][k] * fw[k];
+        wl.rhs[i] = sh.fixed[i] ? 0.f : -rb;
+      }
+    }
+    wave_sync();
+    float A[NT];
+    float rhs[MAXA];
+    float ics[MAXA];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) A[t] = wl.A[t];
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i) { rhs[i] = wl.rhs[i]; ics[i] = wl.ics[i]; }
+    bool ok = true;
+    float idg[MAXA];
+#pragma unroll
+    for (int k = 0; k < MAXA; ++k) {
+      float sd = A[k * (k + 1) / 2 + k];
+#pragma unroll
+      for (int pp = 0; pp < k; ++pp) sd -= A[k * (k + 1) / 2 + pp] * A[k * (k + 1) / 2 + pp];
+      if (sd <= 0.f) ok = false;
+      const float lkk = sqrtf(fmaxf(sd, 1e-30f));
+      A[k * (k + 1) / 2 + k] = lkk;
+      const float inv = 1.f / lkk;
+      idg[k] = inv;
+#pragma unroll
+      for (int i = k + 1; i < MAXA; ++i) {
+        float t = A[i * (i + 1) / 2 + k];
+#pragma unroll
+        for (int pp = 0; pp < k; ++pp) t -= A[i * (i + 1) / 2 + pp] * A[k * (k + 1) / 2 + pp];
+        A[i * (i + 1) / 2 + k] = t * inv;
+      }
+    }
+    float y[MAXA];
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i) {
+      float t = rhs[i];
+#pragma unroll
+      for (int pp = 0; pp < i; ++pp) t -= A[i * (i + 1) / 2 + pp] * y[pp];
+      y[i] = t * idg[i];
+    }
+    float delta[MAXA];
+#pragma unroll
+    for (int i = MAXA - 1; i >= 0; --i) {
+      float t = y[i];
+#pragma unroll
+      for (int pp = i + 1; pp < MAXA; ++pp) t -= A[pp * (pp + 1) / 2 + i] * delta[pp];
+      delta[i] = t * idg[i];
+    }
+    float dn = 0.f;
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i) {
+      delta[i] = ok ? delta[i] * ics[i] : 0.f;
+      dn += delta[i] * delta[i];
+    }
+    float R;
+    if (max_p > 1e-2f || max_o > 0.6f) R = 0.38f;
+    else if (max_p > 1e-3f || max_o > 0.25f) R = 0.22f;
+    else if (max_p > 2e-4f || max_o > 0.08f) R = 0.12f;
+    else R = 0.05f;
+    dn = sqrtf(dn);
+    if (dn > R) {
+      const float sc = R / (dn + 1e-18f);
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) delta[i] *= sc;
+    }
+
+    // Line search: lane k < 8 evaluates step min(k, 4); an argmin over lanes
+    // 0-7 gives the sequential pick (first index on ties).
+    Pose P2[MAXA], Tee2[NEE];
+    float r2[6 * NEE];
+    float e = INFINITY;
+    int idx = 0;
+    if (lane < 8) {
+      const int ai = min(lane, 4);
+      float qt[MAXA];
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + ls_alpha(ai) * delta[i]));
+      unsigned wk = 0;
+      const float et = weighted_error(v, tg, W, qt, P2, r2, Tee2, grid, wk);
+      e = (et < 1e30f) ? et : INFINITY;
+      idx = ai;
+      ls_argmin_step<0xb1>(e, idx);   // quad_perm [1,0,3,2]
+      ls_argmin_step<0x4e>(e, idx);   // quad_perm [2,3,0,1]
+      ls_argmin_step<0x141>(e, idx);  // row_half_mirror
+    }
+    const float e0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), 0));
+    const int i0 = __builtin_amdgcn_readlane(idx, 0);
+    float best_a_err = 1e30f;
+    int best_a = 0;
+    if (e0 != INFINITY) { best_a_err = e0; best_a = i0; }
+    const bool accept = best_a_err < cur * (1.f - 1e-4f);
+    if (accept && lane == best_a) store_iterate<MAXA, NEE>(wl, P2, Tee2, r2);
+    cur_next = accept ? best_a_err : cur;
+    const float al = ls_alpha(best_a);
+    float qt[MAXA];
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i) qt[i] = fmaxf(sh.lo[i], fminf(sh.hi[i], st.q[i] + al * delta[i]));
+    if (accept) {
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) st.q[i] = qt[i];
+      st.lam = fmaxf(st.lam * 0.5f, 1e-10f);
+    } else {
+      st.lam = fminf(st.lam * 3.f, 1e6f);
+    }
+    if (best_a_err < st.best_err) {
+      st.best_err = best_a_err;
+#pragma unroll
+      for (int i = 0; i < MAXA; ++i) st.best_q[i] = qt[i];
+    }
+  }
+}
+
+template <int MAXA, int NEE>
+__global__ __launch_bounds__(64) void ls_ik_wave_kernel(IkArgs a) {
+  extern __shared__ float4 smem4[];
+  char* base = reinterpret_cast<char*>(smem4);
+  const IkView<MAXA, NEE> v = ik_prologue<MAXA, NEE>(a, base);
+  using Sh = IkShared<MAXA, NEE>;
+  const int Kin = a.enable_collision ? a.n_robot_spheres : 0;
+  const IkLds off = ik_lds_layout(sizeof(Sh), a.tables.n_joints, Kin, a.enable_collision ? a.n_world : 0,
+                                  ik_other_floats(a));
+  WaveLds<MAXA, NEE>& wl = *reinterpret_cast<WaveLds<MAXA, NEE>*>(base + align16(off.total));
+  const int lane = threadIdx.x;
+  const int p = blockIdx.y;
+  const int s = blockIdx.x;
+  const long long gs = static_cast<long long>(p) * a.n_seeds + s;
+  if (v.sh->too_many_nodes) {  // reported to the caller as NaN errors, seeds unchanged
+    if (lane == 0) {
+      for (int i = 0; i < a.n_act; ++i) a.out_cfgs[gs * a.n_act + i] = a.seeds[gs * a.n_act + i];
+      a.out_err[gs] = NAN;
+    }
+    return;
+  }
+  if (lane < MAXA) {  // the nodes driving column `lane`, in node order
+    int c = 0;
+    for (int n = 0; n < v.sh->n_nodes; ++n) {
+      const NodeRec& nr = v.sh->nodes[n];
+      if (nr.jac_type != 0 && nr.src == lane) wl.drv[lane][c++] = n;
+    }
+    wl.cnt[lane] = c;
+  }
+  float tg[NEE][7];
+  load_targets<MAXA, NEE>(a, p, tg);
+  const float W[6] = {a.pos_weight, a.pos_weight, a.pos_weight, a.ori_weight, a.ori_weight, a.ori_weight};
+  LmState<MAXA> st;
+#pragma unroll
+  for (int i = 0; i < MAXA; ++i) {
+    st.q[i] = (i < a.n_act) ? a.seeds[gs * a.n_act + i] : 0.f;
+    st.best_q[i] = st.q[i];
+  }
+  Pose P[MAXA];
+  Pose Tee[NEE];
+  float r[6 * NEE];
+  st.work = 0;
+  st.best_err = weighted_error(v, tg, W, st.q, P, r, Tee, a.grid, st.work);
+  if (lane == 0) store_iterate<MAXA, NEE>(wl, P, Tee, r);
+  st.lam = a.lambda_init;
+  st.done = 0;
+  lm_iterate_wave<MAXA, NEE>(v, tg, W, a, st, wl, st.best_err);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i)
+      if (i < a.n_act) a.out_cfgs[gs * a.n_act + i] = st.best_q[i];
+    a.out_err[gs] = st.best_err;
+  }
+}
+
 // Uneven batch: seeds of all problems in one flat list (problem p owns
 // [problem_seed_start[p], problem_seed_start[p+1])).  A workgroup takes tiles
 // of 64 consecutive seeds round-robin (tiles may straddle problems: the target
@@ -963,9 +1271,20 @@ inline int launch_ik(void* stream, const IkArgs& a, bool batched, int grid_waves
     return check_launch("ls_ik_batched_kernel");
   }
   // Small batches leave most of the GPU idle and run at the latency of one
-  // seed's iterations: give each seed 8 lanes (parallel line search) while
-  // the lanes still fit one pass of the CUs at two waves per SIMD.
-  const long long lanes8 = static_cast<long long>(a.n_problems) * a.n_seeds * 8;
+  // seed's iterations: the 8-wide instantiations give each seed a wave while
+  // there are at most four waves per SIMD, and every instantiation gives
+  // each seed 8 lanes (parallel line search) while the lanes still fit one
+  // pass of the CUs at two waves per SIMD.
+  const long long n_all = static_cast<long long>(a.n_problems) * a.n_seeds;
+  if (MAXA == 8 && n_all <= kWaveSeeds) {
+    const size_t lds_w = align16(lds) + sizeof(WaveLds<MAXA, NEE>);
+    PK_REQUIRE(lds_w <= 150 * 1024, "IK tables need %zu bytes of LDS (limit 150 KiB)", lds_w);
+    dim3 grid(a.n_seeds, a.n_problems);
+    hipLaunchKernelGGL((ls_ik_wave_kernel<MAXA, NEE>), grid, dim3(kWave), lds_w,
+                       static_cast<hipStream_t>(stream), a);
+    return check_launch("ls_ik_wave_kernel");
+  }
+  const long long lanes8 = n_all * 8;
   if (lanes8 <= kSmallBatchLanes) {
     constexpr int G = 8;
     dim3 grid((a.n_seeds + kWave / G - 1) / (kWave / G), a.n_problems);
