@@ -49,6 +49,7 @@ struct NodeRec {
   int jac_type;     // 1 revolute, 2 prismatic, 0 no Jacobian column
   float ax[3];      // unit joint axis in the joint frame (Jacobian)
   unsigned ee_mask; // bit e set when the node is an ancestor of EE e
+  float qa[4];      // kind 4: ptf.q * (0, unit axis), so the local rotation is c ptf.q + s qa
   int pad[2];
 };
 
@@ -137,6 +138,9 @@ __device__ void build_problem(const IkArgs& a, const JointRec* recs, const int* 
       pose_to(n.ptf, compose(bt, load_pose(r.ptf)));
       for (int c = 0; c < 3; ++c) { n.lin[c] = r.lin[c]; n.ang[c] = r.ang[c]; }
       n.mul = r.mul; n.off = r.off; n.src = r.src; n.parent = base; n.kind = r.kind;
+      // kind 4 stages the unit axis in ang: the folded constant times (0, axis)
+      quat_mul(n.ptf[0], n.ptf[1], n.ptf[2], n.ptf[3], 0.f, r.ang[0], r.ang[1], r.ang[2], n.qa[0], n.qa[1],
+               n.qa[2], n.qa[3]);
       const float a2 = r.ang[0] * r.ang[0] + r.ang[1] * r.ang[1] + r.ang[2] * r.ang[2];
       const float l2 = r.lin[0] * r.lin[0] + r.lin[1] * r.lin[1] + r.lin[2] * r.lin[2];
       n.jac_type = 0;
@@ -208,7 +212,11 @@ __device__ void build_problem(const IkArgs& a, const JointRec* recs, const int* 
 
 // Forward kinematics of the node tree for one lane.  Node parents are
 // wave-uniform, so the parent lookup is a chain of scalar branches.
-template <int MAXA, int NEE>
+// kRevoluteFast: a revolute node's local transform as c ptf.q + s qa (eight
+// FMAs) instead of composing ptf with the joint exponential (the same
+// transform, other rounding); the JAX-convention and HJCD kernels keep the
+// composed form.
+template <int MAXA, int NEE, bool kRevoluteFast = true>
 __device__ __forceinline__ void node_fk(const IkShared<MAXA, NEE>& sh, const float* q,
                                         Pose* P) {
 #pragma unroll
@@ -222,9 +230,22 @@ __device__ __forceinline__ void node_fk(const IkShared<MAXA, NEE>& sh, const flo
         if (src == a) qs = q[a];
       const float qj = qs * r.mul + r.off;
       const int kind = __builtin_amdgcn_readfirstlane(r.kind);
-      const Pose tl = compose(load_pose(r.ptf),
-                              joint_exp<true>(kind, qj, r.lin[0], r.lin[1], r.lin[2], r.ang[0],
-                                              r.ang[1], r.ang[2]));
+      Pose tl;
+      if (kRevoluteFast && kind == 4) {
+        // Revolute: ptf o (cos(th/2), sin(th/2) axis) = (c ptf.q + s qa, ptf.t),
+        // the identity rotation below |th| = 1e-6 as in joint_exp.
+        const float th = r.lin[0] * qj;
+        float sn = __sinf(0.5f * th), cs = __cosf(0.5f * th);
+        if (fabsf(th) < 1e-6f) { sn = 0.f; cs = 1.f; }
+        tl.w = cs * r.ptf[0] + sn * r.qa[0];
+        tl.x = cs * r.ptf[1] + sn * r.qa[1];
+        tl.y = cs * r.ptf[2] + sn * r.qa[2];
+        tl.z = cs * r.ptf[3] + sn * r.qa[3];
+        tl.tx = r.ptf[4]; tl.ty = r.ptf[5]; tl.tz = r.ptf[6];
+      } else {
+        tl = compose(load_pose(r.ptf), joint_exp<true>(kind, qj, r.lin[0], r.lin[1], r.lin[2], r.ang[0],
+                                                       r.ang[1], r.ang[2]));
+      }
       const int par = __builtin_amdgcn_readfirstlane(r.parent);
       Pose pp = pose_identity();
       bool has_parent = false;

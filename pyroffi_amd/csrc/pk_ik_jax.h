@@ -243,7 +243,7 @@ __device__ __forceinline__ float jlm_error(const IkShared<MAXA, NEE>& sh, const 
                                            const float* w) {
   Pose P[MAXA];
   float f[6 * NEE];
-  node_fk(sh, q, P);
+  node_fk<MAXA, NEE, false>(sh, q, P);
   jax_res_jac<MAXA, NEE, false>(sh, P, tg, f, nullptr);
   float e = 0.f;
 #pragma unroll
@@ -311,7 +311,7 @@ __device__ __forceinline__ float jlm_seed(const IkShared<MAXA, NEE>& sh, const I
   float w[6 * NEE], f[6 * NEE];
   {
     Pose P[MAXA];
-    node_fk(sh, c, P);
+    node_fk<MAXA, NEE, false>(sh, c, P);
     jax_res_jac<MAXA, NEE, false>(sh, P, tg, f, nullptr);
   }
   jlm_weights<NEE>(a, o, f, w);
@@ -324,7 +324,7 @@ __device__ __forceinline__ float jlm_seed(const IkShared<MAXA, NEE>& sh, const I
     float Jm[6 * NEE][MAXA];
     {
       Pose P[MAXA];
-      node_fk(sh, c, P);
+      node_fk<MAXA, NEE, false>(sh, c, P);
       jax_res_jac<MAXA, NEE, true>(sh, P, tg, f, Jm);
     }
     jlm_weights<NEE>(a, o, f, w);
@@ -433,7 +433,7 @@ __device__ __forceinline__ float jlm_seed(const IkShared<MAXA, NEE>& sh, const I
   // Winner metric at the best configuration.
   {
     Pose P[MAXA];
-    node_fk(sh, best_q, P);
+    node_fk<MAXA, NEE, false>(sh, best_q, P);
     jax_res_jac<MAXA, NEE, false>(sh, P, tg, f, nullptr);
   }
   float e = 0.f;
@@ -460,7 +460,7 @@ __device__ __forceinline__ float hjcd_coarse_seed(const IkShared<MAXA, NEE>& sh,
     float Jm[6 * NEE][MAXA];
     {
       Pose P[MAXA];
-      node_fk(sh, c, P);
+      node_fk<MAXA, NEE, false>(sh, c, P);
       jax_res_jac<MAXA, NEE, true>(sh, P, tg, f, Jm);
     }
     // Rows of this seed's EE.
@@ -512,7 +512,7 @@ __device__ __forceinline__ float hjcd_coarse_seed(const IkShared<MAXA, NEE>& sh,
   }
   {
     Pose P[MAXA];
-    node_fk(sh, c, P);
+    node_fk<MAXA, NEE, false>(sh, c, P);
     jax_res_jac<MAXA, NEE, false>(sh, P, tg, f, nullptr);
   }
   float e = 0.f;

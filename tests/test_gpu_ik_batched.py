@@ -136,7 +136,7 @@ def test_batched_kernel_reproduces_single_problem_kernel(oracle_mod):
     robot, path = load_robot("panda")
     links = [robot.links.names.index("panda_hand")]
     t = oracle_mod.Tables.of(robot)
-    P, S = 6, 40
+    P, S = 6, 700  # 4200 seeds: the single-problem launch gives each seed 8 lanes (the arithmetic of one lane)
     tgt = oracle_mod.fk_links(t, random_cfgs(robot, P, seed=21))[:, links[0]][:, None]
     seeds = np.random.default_rng(22).uniform(robot.joints.lower_limits, robot.joints.upper_limits,
                                               (P, S, 7)).astype(np.float32)
