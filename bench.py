@@ -209,10 +209,10 @@ def ik_bench(reps: int):
         "solves_per_s": 16 / dt,
         "ms_per_solve_call": dt * 1e3,
         "gpu_event_ms_per_call": gpu_ms,
-        "roofline": {"bound": "valu", "kernel": "ls_ik_kernel (pk_ls_ik_f32) + select", "achieved": achieved,
+        "roofline": {"bound": "valu", "kernel": "ls_ik_wave_kernel (pk_ls_ik_f32) + select", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "flop_per_seed_iteration": IK_FLOP_PER_SEED_ITER,
-                     "note": "latency-bound: 1024 seeds (128 waves of 8-lane seed groups) on 256 CUs"},
+                     "note": "latency-bound: 1024 seeds, one wave each (line search on 8 lanes, the step spread over the wave), one wave per SIMD"},
         "success_1mm_0.01rad": float(win01.mean()),
         "success_1mm_0.05rad": float(win05.mean()),
         "seed_success_1mm_0.01rad": float(seed01.mean()),
