@@ -1188,10 +1188,11 @@ __global__ __launch_bounds__(kMaxTrajThreads) void traj_chomp_kernel(ChompArgs a
 // of the JAX distances (closest same-slot pair / closest sphere of the link,
 // +1e-6 bias) and its gradient: softmax weights times each distance's gradient
 // at its closest sphere(s).  Links without spheres take no part (the
-// reference's padded distances of about 1e9 have weight 0).  Two passes per
-// group over the group's lanes: the minimum, then the weights, accumulated per
-// link as a wrench (F, c x F) and pulled back to the link's ancestor nodes
-// once per link: dq_a += v_a . F + w_a . (tau - p_a x F).
+// reference's padded distances of about 1e9 have weight 0).  The world groups
+// take one pass with a running minimum (weights rescaled when it drops), the
+// self group two (the minimum, then the weights); the weights' distance
+// gradients are accumulated per link as a wrench (F, c x F) and pulled back to
+// the link's ancestor nodes once per link: dq_a += v_a . F + w_a . (tau - p_a x F).
 constexpr int kMaxScoGroups = 8;
 
 __device__ __forceinline__ float group8_min(float v) {
@@ -1258,27 +1259,16 @@ __device__ __forceinline__ void group_smooth_min(const TrajModelDev& md, const T
   for (int a = 0; a < MAXN; ++a) gn[a] = 0.f;
   float bx, by, bz, bn;
   float4 bc, bc2;
-  // Pass 1: the group minimum.
-  float dmin = INFINITY;
+  float z = 0.f, dmin;
   if (gi == 0) {
+    // Self pairs (a few dozen): pass 1 the group minimum, pass 2 the weights.
+    dmin = INFINITY;
     for (int p = g; p < md.n_pairs; p += kGroup) dmin = fminf(dmin, pair_dist(L, gc, p, bx, by, bz, bn, bc, bc2));
-  } else {
-    const int m0 = gstart[gi - 1], m1 = gstart[gi];
-    for (int l = 0; l < md.n_links; ++l) {
-      const int k0 = L.link_sph[l], k1 = L.link_sph[l + 1];
-      if (k0 >= k1) continue;
-      for (int m = m0 + g; m < m1; m += kGroup)
-        dmin = fminf(dmin, link_world_dist(gc, k0, k1, L.world[m], bx, by, bz, bn, bc));
+    dmin = group8_min(dmin);
+    if (!(dmin < INFINITY)) {  // empty group
+      dk = INFINITY;
+      return;
     }
-  }
-  dmin = group8_min(dmin);
-  if (!(dmin < INFINITY)) {  // empty group
-    dk = INFINITY;
-    return;
-  }
-  // Pass 2: weights and their gradients.
-  float z = 0.f;
-  if (gi == 0) {
     for (int p = g; p < md.n_pairs; p += kGroup) {
       const float d = pair_dist(L, gc, p, bx, by, bz, bn, bc, bc2);
       const float w = __expf(-(d - dmin) / tau);
@@ -1291,14 +1281,30 @@ __device__ __forceinline__ void group_smooth_min(const TrajModelDev& md, const T
       pull_point<MAXN, true>(K, ni >= 0 ? L.anc[ni] : 0, bc.x, bc.y, bc.z, -k * bx, -k * by, -k * bz, gn);
     }
   } else {
+    // World group: one pass.  Each lane keeps its own running minimum mloc
+    // and its weights relative to it, exp(-(d - mloc) / tau); when mloc
+    // drops, the sum, the link's wrench and the node gradients gathered so
+    // far are rescaled by exp(-(mloc_old - mloc_new) / tau).  The lanes'
+    // sums are brought to the group minimum before they are added: the same
+    // sums as weighting by the group minimum directly, in other rounding.
     const int m0 = gstart[gi - 1], m1 = gstart[gi];
+    float mloc = INFINITY;
     for (int l = 0; l < md.n_links; ++l) {
       const int k0 = L.link_sph[l], k1 = L.link_sph[l + 1];
       if (k0 >= k1) continue;
       float F[3] = {0.f, 0.f, 0.f}, tq[3] = {0.f, 0.f, 0.f};
       for (int m = m0 + g; m < m1; m += kGroup) {
         const float d = link_world_dist(gc, k0, k1, L.world[m], bx, by, bz, bn, bc);
-        const float w = __expf(-(d - dmin) / tau);
+        if (d < mloc) {
+          const float r = (mloc < INFINITY) ? __expf(-(mloc - d) / tau) : 0.f;
+          z *= r;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) { F[c] *= r; tq[c] *= r; }
+#pragma unroll
+          for (int a = 0; a < MAXN; ++a) gn[a] *= r;
+          mloc = d;
+        }
+        const float w = __expf(-(d - mloc) / tau);
         if (!(w > 0.f)) continue;
         z += w;
         const float k = -w / bn;  // d(distance)/dc = -(o - c + e) / |o - c + e|
@@ -1311,6 +1317,15 @@ __device__ __forceinline__ void group_smooth_min(const TrajModelDev& md, const T
       const int node = L.link_node[l];
       pull_wrench<MAXN>(K, node >= 0 ? L.anc[node] : 0, F, tq, gn);
     }
+    dmin = group8_min(mloc);
+    if (!(dmin < INFINITY)) {  // empty group
+      dk = INFINITY;
+      return;
+    }
+    const float r = (mloc < INFINITY) ? __expf(-(mloc - dmin) / tau) : 0.f;
+    z *= r;
+#pragma unroll
+    for (int a = 0; a < MAXN; ++a) gn[a] *= r;
   }
   z = group_sum(z);
   dk = dmin - tau * __logf(z);
