@@ -931,9 +931,20 @@ struct WaveLds {
   int drv[MAXA][MAXA];      // ... in node order
 };
 
-// Threshold (problems x seeds) up to which the 8-wide instantiations run one
-// seed per wave: four waves per SIMD.
-constexpr long long kWaveSeeds = 4096;
+// Seeds (problems x seeds) up to which the 8-wide instantiations run one seed
+// per wave: one wave per SIMD (four per CU).  With two waves sharing a SIMD
+// each runs at about 0.7 of its speed alone, and the 8-lane kernel (one wave
+// per 8 seeds) is as fast.
+inline long long wave_seed_limit() {
+  static const long long limit = [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || cus <= 0)
+      cus = 256;
+    return 4LL * cus;
+  }();
+  return limit;
+}
 
 template <int MAXA, int NEE>
 __device__ __forceinline__ void store_iterate(WaveLds<MAXA, NEE>& wl, const Pose* P, const Pose* Tee, const float* r) {
@@ -1293,11 +1304,11 @@ inline int launch_ik(void* stream, const IkArgs& a, bool batched, int grid_waves
   }
   // Small batches leave most of the GPU idle and run at the latency of one
   // seed's iterations: the 8-wide instantiations give each seed a wave while
-  // there are at most four waves per SIMD, and every instantiation gives
+  // every SIMD holds at most one of them, and every instantiation gives
   // each seed 8 lanes (parallel line search) while the lanes still fit one
   // pass of the CUs at two waves per SIMD.
   const long long n_all = static_cast<long long>(a.n_problems) * a.n_seeds;
-  if (MAXA == 8 && n_all <= kWaveSeeds) {
+  if (MAXA == 8 && n_all <= wave_seed_limit()) {
     const size_t lds_w = align16(lds) + sizeof(WaveLds<MAXA, NEE>);
     PK_REQUIRE(lds_w <= 150 * 1024, "IK tables need %zu bytes of LDS (limit 150 KiB)", lds_w);
     dim3 grid(a.n_seeds, a.n_problems);

@@ -198,7 +198,7 @@ def test_public_api_solves(oracle_mod):
 @pytest.mark.parametrize("name,n_small", [("panda", 3000), ("baxter", 29)])
 def test_parallel_line_search_matches_one_lane_per_seed(oracle_mod, name, n_small):
     """Launches of up to 16384 seeds that do not get a wave per seed (more
-    than 4096 seeds, or a 16-wide robot such as Baxter) give each seed 8 lanes
+    than 1024 seeds, or a 16-wide robot such as Baxter) give each seed 8 lanes
     (one line-search step per lane); larger launches one lane.  The same seeds
     give bit-identical results both ways."""
     robot, path = load_robot(name)
@@ -222,7 +222,7 @@ def test_parallel_line_search_matches_one_lane_per_seed(oracle_mod, name, n_smal
 
 @pytest.mark.parametrize("with_coll", [False, True])
 def test_wave_per_seed_kernel(oracle_mod, with_coll):
-    """Launches of at most 4096 seeds of an 8-wide robot give each seed a wave
+    """Launches of at most 1024 seeds (one wave per SIMD) of an 8-wide robot give each seed a wave
     (line search on lanes 0-7; Jacobian contributions, column scaling, normal
     matrix and right-hand side spread over the wave).  One iteration lands on
     the one-lane kernel's iterate to rounding (the same step; a logic slip
@@ -236,7 +236,7 @@ def test_wave_per_seed_kernel(oracle_mod, with_coll):
     lo, hi = robot.joints.lower_limits, robot.joints.upper_limits
     tgt = oracle_mod.fk_links(t, random_cfgs(robot, 8, seed=72))[:, links]
     big = rng.uniform(lo, hi, (8, 2100, len(lo))).astype(np.float32)   # 16800 seeds: one lane each
-    small = np.ascontiguousarray(big[:, :256])                          # 2048 seeds: a wave each
+    small = np.ascontiguousarray(big[:, :128])                          # 1024 seeds: a wave each
     kw = dict(lambda_init=1e-2)
     if with_coll:
         coll = RobotCollisionSpherized.from_urdf(path)
@@ -245,18 +245,18 @@ def test_wave_per_seed_kernel(oracle_mod, with_coll):
         kw["collision"] = (rs, rj, ws, 1e4, 0.02)
     cb, eb = ls_ik_run(robot, big, tgt, links, max_iter=1, **kw)
     cs, es = ls_ik_run(robot, small, tgt, links, max_iter=1, **kw)
-    d = (cs - cb[:, :256]).abs().max().item()
+    d = (cs - cb[:, :128]).abs().max().item()
     print(f"one iteration: max |wave - one lane| {d:.2e} rad", flush=True)
     assert d <= 2e-5
-    torch.testing.assert_close(es, eb[:, :256], rtol=1e-4, atol=1e-6)
+    torch.testing.assert_close(es, eb[:, :128], rtol=1e-4, atol=1e-6)
     cb, eb = ls_ik_run(robot, big, tgt, links, max_iter=30, **kw)
     cs, es = ls_ik_run(robot, small, tgt, links, max_iter=30, **kw)
-    sub = np.ascontiguousarray(big[:, :256]).astype(np.float64)
+    sub = np.ascontiguousarray(big[:, :128]).astype(np.float64)
     ocfgs, _ = oracle_mod.ls_ik(t, sub, *ee_tables(robot, links), tgt, lo, hi, max_iter=30, lambda_init=1e-2,
                                 **({"robot_spheres": rs, "sphere_joint": rj, "world": ws, "enable_collision": True,
                                     "collision_weight": 1e4, "collision_margin": 0.02} if with_coll else {}))
     ok = []
-    for c in (cs.cpu().numpy(), cb[:, :256].cpu().numpy(), ocfgs):
+    for c in (cs.cpu().numpy(), cb[:, :128].cpu().numpy(), ocfgs):
         pos, rot = _errors(oracle_mod, t, c, tgt, links)
         ok.append(((pos <= 1e-3) & (rot <= 0.01)).mean())
     print(f"30 iterations: seed success wave {ok[0]:.4f}, one lane {ok[1]:.4f}, oracle {ok[2]:.4f}", flush=True)
