@@ -1218,7 +1218,9 @@ __device__ __forceinline__ void pull_wrench(const WaypointNodes<MAXN, true>& K, 
 }
 
 // Closest sphere of link spheres [k0, k1) to world sphere o: distance and the
-// (unnormalised) direction o - c + e and its length.
+// (unnormalised) direction o - c + e and its length.  The hardware square
+// root (at most 1 ulp) instead of the correctly rounded one: this loop runs
+// for every (link sphere, world sphere) pair of every waypoint.
 __device__ __forceinline__ float link_world_dist(const float4* gc, int k0, int k1, float4 o, float& bx, float& by,
                                                  float& bz, float& bn, float4& bc) {
   const float e = 1e-6f;
@@ -1226,7 +1228,7 @@ __device__ __forceinline__ float link_world_dist(const float4* gc, int k0, int k
   for (int s = k0; s < k1; ++s) {
     const float4 c = gc[s];
     const float dx = o.x - c.x + e, dy = o.y - c.y + e, dz = o.z - c.z + e;
-    const float n = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float n = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);  // v_sqrt_f32, <= 1 ulp
     const float d = n - c.w - o.w;
     if (d < best) { best = d; bx = dx; by = dy; bz = dz; bn = n; bc = c; }
   }
@@ -1242,7 +1244,7 @@ __device__ __forceinline__ float pair_dist(const TrajLds& L, const float4* gc, i
     const float4 ci = gc[L.pair_items[2 * it]];
     const float4 cj = gc[L.pair_items[2 * it + 1]];
     const float dx = cj.x - ci.x + e, dy = cj.y - ci.y + e, dz = cj.z - ci.z + e;
-    const float n = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float n = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
     const float d = n - ci.w - cj.w;
     if (d < best) { best = d; bx = dx; by = dy; bz = dz; bn = n; bci = ci; bcj = cj; }
   }
