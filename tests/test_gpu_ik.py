@@ -88,12 +88,17 @@ def test_initial_error_and_penalty(oracle_mod, name, with_coll):
     np.testing.assert_allclose(errs, oerrs, rtol=1e-4, atol=1e-6)
 
 
-def test_baxter_two_end_effectors_initial_error(oracle_mod):
-    robot, _ = load_robot("baxter")
+@pytest.mark.parametrize("with_coll", [False, True])
+def test_baxter_two_end_effectors_initial_error(oracle_mod, with_coll):
+    """With collision, Baxter's five world-fixed spheres (torso and pedestal,
+    radii up to 0.5 m; their penalty is summed once per workgroup) carry most
+    of the penalty."""
+    robot, path = load_robot("baxter")
     names = robot.links.names
     links = (names.index("left_gripper_base"), names.index("right_gripper_base"))
     t, tgt, seeds = _problems(oracle_mod, robot, links, 3, 64, seed=5)
-    cfgs, errs, _, oerrs = _run_both(oracle_mod, robot, links, tgt, seeds, t, max_iter=0)
+    coll = _collision(robot, path, 256, 4) if with_coll else None
+    cfgs, errs, _, oerrs = _run_both(oracle_mod, robot, links, tgt, seeds, t, max_iter=0, collision=coll)
     np.testing.assert_allclose(errs, oerrs, rtol=1e-4, atol=1e-6)
 
 
