@@ -266,6 +266,22 @@ def test_autotune_picks_a_candidate(oracle_mod):
     assert len(report) == 1
 
 
+def test_autotune_lead_group_workgroups(oracle_mod):
+    """With several groups the longest one's workgroup count is re-timed with
+    every group running; any choice gives the same winners (each seed's
+    iterate does not depend on which workgroup runs it)."""
+    world = _world(256, 23)
+    groups = [_group(oracle_mod, "baxter", 24, 61, world)[0], _group(oracle_mod, "panda", 24, 62, world)[0]]
+    solver = UnevenLsIkSolver(groups, max_iter=10)
+    ref = [(w.clone(), i.clone()) for w, i in solver.solve()]
+    report = solver.autotune(reps=1, cache=False)
+    lead = max(groups, key=lambda g: g.tune_ms)
+    assert "joint" in report and lead.tune == tuple(report["joint"]["choice"])
+    assert len(report["joint"]["ms"]) >= 2
+    for (w, i), (rw, ri) in zip(solver.solve(), ref):
+        assert torch.equal(i, ri) and torch.equal(w, rw)
+
+
 def test_select_segments_matches_numpy():
     rng = np.random.default_rng(3)
     counts = np.array([1, 17, 64, 65, 3, 130])
