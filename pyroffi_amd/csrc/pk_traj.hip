@@ -1217,22 +1217,23 @@ __device__ __forceinline__ void pull_wrench(const WaypointNodes<MAXN, true>& K, 
   }
 }
 
-// Closest sphere of link spheres [k0, k1) to world sphere o: distance and the
-// (unnormalised) direction o - c + e and its length.  The hardware square
-// root (at most 1 ulp) instead of the correctly rounded one: this loop runs
-// for every (link sphere, world sphere) pair of every waypoint.
-__device__ __forceinline__ float link_world_dist(const float4* gc, int k0, int k1, float4 o, float& bx, float& by,
-                                                 float& bz, float& bn, float4& bc) {
+// Distance between robot sphere c and world sphere o, |o - c + e| - r_c - r_o.
+// The hardware square root (at most 1 ulp) instead of the correctly rounded
+// one: this runs for every (link sphere, world sphere) pair of every waypoint.
+__device__ __forceinline__ float sphere_world_dist(const float4& c, const float4& o) {
   const float e = 1e-6f;
-  float best = INFINITY;
-  for (int s = k0; s < k1; ++s) {
-    const float4 c = gc[s];
-    const float dx = o.x - c.x + e, dy = o.y - c.y + e, dz = o.z - c.z + e;
-    const float n = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);  // v_sqrt_f32, <= 1 ulp
-    const float d = n - c.w - o.w;
-    if (d < best) { best = d; bx = dx; by = dy; bz = dz; bn = n; bc = c; }
-  }
-  return best;
+  const float dx = o.x - c.x + e, dy = o.y - c.y + e, dz = o.z - c.z + e;
+  return __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - c.w - o.w;  // v_sqrt_f32, <= 1 ulp
+}
+
+// Direction o - c + e from sphere c to obstacle o, its length and the centre
+// (the same expressions as sphere_world_dist, so the same bits).
+__device__ __forceinline__ void closest_dir(const float4& c, const float4& o, float& bx, float& by, float& bz,
+                                            float& bn, float4& bc) {
+  const float e = 1e-6f;
+  bx = o.x - c.x + e; by = o.y - c.y + e; bz = o.z - c.z + e;
+  bn = __builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz);
+  bc = c;
 }
 
 // Closest same-slot pair of active pair p: distance, direction cj - ci + e, its length, the centres.
@@ -1290,31 +1291,60 @@ __device__ __forceinline__ void group_smooth_min(const TrajModelDev& md, const T
     // sums are brought to the group minimum before they are added: the same
     // sums as weighting by the group minimum directly, in other rounding.
     const int m0 = gstart[gi - 1], m1 = gstart[gi];
+    const float inv_tau = 1.f / tau;
     float mloc = INFINITY;
     for (int l = 0; l < md.n_links; ++l) {
       const int k0 = L.link_sph[l], k1 = L.link_sph[l + 1];
       if (k0 >= k1) continue;
       float F[3] = {0.f, 0.f, 0.f}, tq[3] = {0.f, 0.f, 0.f};
-      for (int m = m0 + g; m < m1; m += kGroup) {
-        const float d = link_world_dist(gc, k0, k1, L.world[m], bx, by, bz, bn, bc);
-        if (d < mloc) {
-          const float r = (mloc < INFINITY) ? __expf(-(mloc - d) / tau) : 0.f;
-          z *= r;
+      // kObs obstacles per pass share each sphere centre read from LDS (and
+      // give the wave independent work); their terms are then added in
+      // obstacle order, the order of one obstacle at a time.
+      constexpr int kObs = 8;
+      for (int mb = m0 + g; mb < m1; mb += kGroup * kObs) {
+        float4 o[kObs];
+        float best[kObs];
+        int bs[kObs];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) { F[c] *= r; tq[c] *= r; }
-#pragma unroll
-          for (int a = 0; a < MAXN; ++a) gn[a] *= r;
-          mloc = d;
+        for (int u = 0; u < kObs; ++u) {
+          const int m = mb + u * kGroup;
+          o[u] = L.world[m < m1 ? m : mb];
+          best[u] = INFINITY;
+          bs[u] = -1;
         }
-        const float w = __expf(-(d - mloc) / tau);
-        if (!(w > 0.f)) continue;
-        z += w;
-        const float k = -w / bn;  // d(distance)/dc = -(o - c + e) / |o - c + e|
-        const float gx = k * bx, gy = k * by, gz = k * bz;
-        F[0] += gx; F[1] += gy; F[2] += gz;
-        tq[0] += bc.y * gz - bc.z * gy;
-        tq[1] += bc.z * gx - bc.x * gz;
-        tq[2] += bc.x * gy - bc.y * gx;
+        for (int sp = k0; sp < k1; ++sp) {
+          const float4 c = gc[sp];
+#pragma unroll
+          for (int u = 0; u < kObs; ++u) {
+            const float d = sphere_world_dist(c, o[u]);
+            if (d < best[u]) { best[u] = d; bs[u] = sp; }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kObs; ++u) {
+          if (mb + u * kGroup >= m1) break;
+          const float d = best[u];
+          if (bs[u] >= 0) closest_dir(gc[bs[u]], o[u], bx, by, bz, bn, bc);
+          if (d < mloc) {
+            const float r = (mloc < INFINITY) ? __expf((d - mloc) * inv_tau) : 0.f;
+            z *= r;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { F[c] *= r; tq[c] *= r; }
+#pragma unroll
+            for (int a = 0; a < MAXN; ++a) gn[a] *= r;
+            mloc = d;
+          }
+          const float w = __expf((mloc - d) * inv_tau);
+          if (w > 0.f) {
+            z += w;
+            const float k = -w * __builtin_amdgcn_rcpf(bn);  // d(distance)/dc = -(o - c + e) / |o - c + e|
+            const float gx = k * bx, gy = k * by, gz = k * bz;
+            F[0] += gx; F[1] += gy; F[2] += gz;
+            tq[0] += bc.y * gz - bc.z * gy;
+            tq[1] += bc.z * gx - bc.x * gz;
+            tq[2] += bc.x * gy - bc.y * gx;
+          }
+        }
       }
       const int node = L.link_node[l];
       pull_wrench<MAXN>(K, node >= 0 ? L.anc[node] : 0, F, tq, gn);
@@ -1421,14 +1451,15 @@ struct ScoInnerArgs {
 
 __host__ __device__ inline size_t sco_inner_lds_floats(int max_len, int A, int G, int m) {
   const size_t n = static_cast<size_t>(max_len) * A;
-  return (9 + 2 * static_cast<size_t>(m)) * n + static_cast<size_t>(max_len) * G * (A + 1) + 2 * A +
+  return (9 + 2 * static_cast<size_t>(m)) * n + static_cast<size_t>(max_len) * G * (A + 2) + 2 * A +
          static_cast<size_t>(kMaxTrajWaves) * kRedSlots;
 }
 
-// Inner cost at y (LDS) and, with grad, its gradient into gout (endpoint rows zeroed).
+// Inner cost at y (LDS) and, with grad, its gradient into gout (endpoint rows
+// zeroed); hk (T, G) holds each row's hinge slope -2 w_coll h for the gradient.
 __device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const float* y, const float* qk,
                                                 const float* dk, const float* Jk, const float* lo, const float* hi,
-                                                int T, float* acc, float* gout, float* red, bool grad) {
+                                                int T, float* acc, float* hk, float* gout, float* red, bool grad) {
   const int A = a.n_act, G = a.G;
   const int n = T * A;
   const int tid = threadIdx.x;
@@ -1461,6 +1492,7 @@ __device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const flo
     for (int c = 0; c < A; ++c) dl += Jr[c] * (y[t * A + c] - qk[t * A + c]);
     const float h = a.margin - dl;
     if (h > 0.f) rest += a.w_coll * h * h;
+    if (grad) hk[r] = (h > 0.f) ? -2.f * a.w_coll * h : 0.f;
   }
   __syncthreads();
   for (int i = tid; i < na * A; i += blockDim.x) {
@@ -1471,17 +1503,14 @@ __device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const flo
     }
   }
   if (grad) {
-    // hinge gradient, one (waypoint, group) row per thread-iteration, added per row element
+    // hinge gradient: each element sums its waypoint's row slopes times the
+    // rows' Jacobian entries
     for (int i = tid; i < n; i += blockDim.x) {
       const int t = i / A, c = i % A;
       float gh = 0.f;
       for (int gg = 0; gg < G; ++gg) {
         const int r = t * G + gg;
-        const float* Jr = Jk + static_cast<long long>(r) * A;
-        float dl = dk[r];
-        for (int cc = 0; cc < A; ++cc) dl += Jr[cc] * (y[t * A + cc] - qk[t * A + cc]);
-        const float h = a.margin - dl;
-        if (h > 0.f) gh -= 2.f * a.w_coll * h * Jr[c];
+        gh += hk[r] * Jk[static_cast<long long>(r) * A + c];
       }
       gout[i] += gh;
     }
@@ -1538,7 +1567,8 @@ __global__ __launch_bounds__(256) void traj_sco_inner_kernel(ScoInnerArgs a) {
   float* yb = sb + m * cap;         // (m, cap)
   float* dk = yb + m * cap;         // (max_len, G)
   float* Jk = dk + a.max_len * G;   // (max_len, G, A)
-  float* lo = Jk + a.max_len * G * A;
+  float* hk = Jk + a.max_len * G * A;  // (max_len, G)
+  float* lo = hk + a.max_len * G;
   float* hi = lo + A;
   float* red = hi + A;
   for (int i = tid; i < n; i += blockDim.x) {
@@ -1553,11 +1583,11 @@ __global__ __launch_bounds__(256) void traj_sco_inner_kernel(ScoInnerArgs a) {
 #pragma unroll
   for (int h = 0; h < kMaxHist; ++h) rho[h] = 0.f;
   int m_used = 0, newest = 0;
-  float best_cost = sco_inner_eval(a, x, qk, dk, Jk, lo, hi, T, acc, g, red, true);
+  float best_cost = sco_inner_eval(a, x, qk, dk, Jk, lo, hi, T, acc, hk, g, red, true);
   for (int i = tid; i < n; i += blockDim.x) { best[i] = x[i]; xp[i] = x[i]; gp[i] = g[i]; }
   __syncthreads();
   for (int k = 0; k < a.n_inner; ++k) {
-    const float cost = sco_inner_eval(a, x, qk, dk, Jk, lo, hi, T, acc, g, red, true);
+    const float cost = sco_inner_eval(a, x, qk, dk, Jk, lo, hi, T, acc, hk, g, red, true);
     float syy[2] = {0.f, 0.f};
     for (int i = tid; i < n; i += blockDim.x) {
       const float sv = x[i] - xp[i], yv = g[i] - gp[i];
@@ -1629,7 +1659,7 @@ __global__ __launch_bounds__(256) void traj_sco_inner_kernel(ScoInnerArgs a) {
       const float al = traj_alpha(kk);
       for (int i = tid; i < n; i += blockDim.x) xt[i] = x[i] + al * dir[i];
       __syncthreads();
-      const float c = sco_inner_eval(a, xt, qk, dk, Jk, lo, hi, T, acc, nullptr, red, false);
+      const float c = sco_inner_eval(a, xt, qk, dk, Jk, lo, hi, T, acc, hk, nullptr, red, false);
       if (kk == 0 || c < pick_cost) { pick_cost = c; first_best = kk; }
       if (c < thresh) { pick = kk; pick_cost = c; break; }
     }
