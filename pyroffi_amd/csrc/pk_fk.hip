@@ -38,6 +38,7 @@
 // permutation of the joints gives NaN rows (the reference reads whatever
 // such a table indexes).
 #include "pk_model.h"
+#include <type_traits>
 
 namespace pk {
 
@@ -306,17 +307,20 @@ __global__ __launch_bounds__(64) void fk_kernel(
       // world pose there (a constant joint's local transform is its record's
       // parent_tf).  Both kinds share one branch-free form: th = 0 (prismatic)
       // and tq = 0 (revolute) reduce it to the other kind exactly.
-      for (unsigned long long m = turn_slide; m;) {
-        float lc[4][7];
-        int lp[4], ii[4];
+      // K slots per pass, K = 4 while four remain, then 2 and 1 for the rest
+      // (a padded last pass of four would compute repeated slots).
+      auto local_pass = [&](auto width, unsigned long long& m) {
+        constexpr int K = decltype(width)::value;
+        float lc[K][7];
+        int lp[K], ii[K];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          ii[k] = m ? __builtin_ctzll(m) : ii[k > 0 ? k - 1 : 0];  // m != 0 for k = 0
-          if (m) m &= m - 1ull;
+        for (int k = 0; k < K; ++k) {  // m holds at least K slots
+          ii[k] = __builtin_ctzll(m);
+          m &= m - 1ull;
         }
         bool big = false;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < K; ++k) {
           const FkRec& r = recs[ii[k]];
           const int src = uniform_i(r.src);
           const float qv = cfg_tile[max(src, 0) * kLaneStride + lane];
@@ -339,7 +343,7 @@ __global__ __launch_bounds__(64) void fk_kernel(
         if (__any(big)) {
           // Joint values beyond the branch-free range: the library sine/cosine.
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
+          for (int k = 0; k < K; ++k) {
             const FkRec& r = recs[ii[k]];
             const int src = uniform_i(r.src);
             const float q = ((src >= 0) ? cfg_tile[src * kLaneStride + lane] : 0.f) * r.mul + r.off;
@@ -354,12 +358,16 @@ __global__ __launch_bounds__(64) void fk_kernel(
           }
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < K; ++k) {
           float* d = trow + lp[k];
 #pragma unroll
           for (int c = 0; c < 7; ++c) d[c] = lc[k][c];
         }
-      }
+      };
+      unsigned long long m = turn_slide;
+      while (__popcll(m) >= 4) local_pass(std::integral_constant<int, 4>{}, m);
+      if (__popcll(m) >= 2) local_pass(std::integral_constant<int, 2>{}, m);
+      if (m) local_pass(std::integral_constant<int, 1>{}, m);
       // General screw joints (rare): the full exponential.
       for (unsigned long long m = screw; m; m &= m - 1ull) {
         const FkRec& r = recs[__builtin_ctzll(m)];
@@ -379,6 +387,9 @@ __global__ __launch_bounds__(64) void fk_kernel(
       };
       Pose prev = pose_identity();
       Pose loc = load_pose(local_src(0, __builtin_amdgcn_readlane(link, 0)));
+      // Unrolled by two so the carried poses alternate between two register
+      // sets instead of being moved back at the end of every step.
+#pragma unroll 2
       for (int i = 0; i < J; ++i) {
         const int lk = __builtin_amdgcn_readlane(link, i);
         const int in = min(i + 1, J - 1);
