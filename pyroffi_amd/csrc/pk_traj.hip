@@ -977,8 +977,8 @@ __global__ __launch_bounds__(kMaxTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a
 // CHOMP (_chomp_optimization.py:208-330): per iteration the gradient of the
 // cost at the current collision weight, clipped to a global norm, endpoint
 // rows zeroed; the covariant direction -M^-1 g, M the interior second-
-// difference precision operator per joint, by truncated CG (all joints in
-// lockstep, one LDS reduction of A sums per CG half-step); the six trials
+// difference precision operator per joint, by truncated CG (each joint's
+// system on one wave, wave-level sums); the six trials
 // x + clip(step alpha d, +-max_delta) evaluated in full and the cheapest (first
 // on ties) kept when it beats the current cost by min_cost_improve; stall count,
 // early stop and collision-weight continuation.  The alpha = 0 trial is x itself,
@@ -1065,77 +1065,53 @@ __global__ __launch_bounds__(kMaxTrajThreads) void traj_chomp_kernel(ChompArgs a
     __syncthreads();
     if (a.covariant && T > 2) {
       // CG on every joint's interior timeline: x = 0 (vd), r = b (vt), p = b (vg).
-      float rr[MAXN], tol2[MAXN];
-      bool act[MAXN];
-#pragma unroll
-      for (int c = 0; c < MAXN; ++c) rr[c] = 0.f;
-      for (int i = tid; i < n; i += blockDim.x) {
-        const float gv = L.vg[i];
-        L.vd[i] = 0.f;
-        L.vt[i] = gv;
-        add_at(rr, i % A, gv * gv);
-      }
-      block_sum<MAXN>(rr, L.red);
-#pragma unroll
-      for (int c = 0; c < MAXN; ++c) {
-        const float tb = a.cg_tol * sqrtf(rr[c] + 1e-18f) + 1e-12f;
-        tol2[c] = tb * tb;
-        act[c] = c < A;
-      }
-      for (int it = 0; it < a.n_cg; ++it) {
-        bool any = false;
-#pragma unroll
-        for (int c = 0; c < MAXN; ++c) any = any || act[c];
-        if (!any) break;  // uniform: every thread holds the same flags
-        float pap[MAXN];
-#pragma unroll
-        for (int c = 0; c < MAXN; ++c) pap[c] = 0.f;
-        for (int i = tid; i < n; i += blockDim.x) {
-          const int t = i / A;
-          if (t == 0 || t == T - 1) continue;
-          add_at(pap, i % A, L.vg[i] * chomp_mp(L.vg, i, t, T, A, a.reg));
+      // The joints' systems are independent, so each wave solves whole
+      // joints (joint c on wave c mod W) with wave-level sums and wavefront
+      // fences: no workgroup barrier inside the CG.
+      const int n_waves = static_cast<int>(blockDim.x) / kWave;
+      const int wv = tid / kWave, ln = tid % kWave;
+      for (int c = wv; c < A; c += n_waves) {
+        float rr = 0.f;
+        for (int t = ln; t < T; t += kWave) {
+          const int i = t * A + c;
+          const float gv = L.vg[i];
+          L.vd[i] = 0.f;
+          L.vt[i] = gv;
+          rr += gv * gv;
         }
-        block_sum<MAXN>(pap, L.red);
-        float al[MAXN];
-#pragma unroll
-        for (int c = 0; c < MAXN; ++c) al[c] = act[c] ? rr[c] / (pap[c] + 1e-9f) : 0.f;
-        float rn[MAXN];
-#pragma unroll
-        for (int c = 0; c < MAXN; ++c) rn[c] = 0.f;
-        for (int i = tid; i < n; i += blockDim.x) {
-          const int t = i / A;
-          if (t == 0 || t == T - 1) continue;
-          const int c = i % A;
-          const float alc = get_at(al, c);
-          const float mp = chomp_mp(L.vg, i, t, T, A, a.reg);
-          L.vd[i] += alc * L.vg[i];
-          const float r = L.vt[i] - alc * mp;
-          L.vt[i] = r;
-          add_at(rn, c, r * r);
-        }
-        block_sum<MAXN>(rn, L.red);  // its barriers also order the p update after every M p read
-        float be[MAXN];
-#pragma unroll
-        for (int c = 0; c < MAXN; ++c) be[c] = rn[c] / (rr[c] + 1e-9f);
-        for (int i = tid; i < n; i += blockDim.x) {
-          const int t = i / A;
-          if (t == 0 || t == T - 1) continue;
-          const int c = i % A;
-          bool ac = false;
-#pragma unroll
-          for (int cc = 0; cc < MAXN; ++cc)
-            if (cc == c) ac = act[cc];
-          if (ac) L.vg[i] = L.vt[i] + get_at(be, c) * L.vg[i];
-        }
-#pragma unroll
-        for (int c = 0; c < MAXN; ++c) {
-          if (act[c]) {
-            rr[c] = rn[c];
-            act[c] = rn[c] > tol2[c];
+        rr = traj_wave_sum(rr);
+        const float tb = a.cg_tol * sqrtf(rr + 1e-18f) + 1e-12f;
+        const float tol2 = tb * tb;
+        wave_sync();
+        for (int it = 0; it < a.n_cg; ++it) {
+          float pap = 0.f;
+          for (int t = 1 + ln; t < T - 1; t += kWave) {
+            const int i = t * A + c;
+            pap += L.vg[i] * chomp_mp(L.vg, i, t, T, A, a.reg);
           }
+          const float al = rr / (traj_wave_sum(pap) + 1e-9f);
+          float rn = 0.f;
+          for (int t = 1 + ln; t < T - 1; t += kWave) {
+            const int i = t * A + c;
+            const float mp = chomp_mp(L.vg, i, t, T, A, a.reg);
+            L.vd[i] += al * L.vg[i];
+            const float r = L.vt[i] - al * mp;
+            L.vt[i] = r;
+            rn += r * r;
+          }
+          rn = traj_wave_sum(rn);
+          wave_sync();  // every M p read of this step before the p update
+          const float be = rn / (rr + 1e-9f);
+          for (int t = 1 + ln; t < T - 1; t += kWave) {
+            const int i = t * A + c;
+            L.vg[i] = L.vt[i] + be * L.vg[i];
+          }
+          wave_sync();
+          rr = rn;
+          if (!(rn > tol2)) break;  // uniform in the wave
         }
-        __syncthreads();
       }
+      __syncthreads();
       // The residual and search vectors of an inactive joint froze with it; the
       // solution is complete.  Direction = -x.
       for (int i = tid; i < n; i += blockDim.x) L.vd[i] = -L.vd[i];
