@@ -1193,21 +1193,21 @@ __device__ __forceinline__ void pull_wrench(const WaypointNodes<MAXN, true>& K, 
   }
 }
 
-// Distance between robot sphere c and world sphere o, |o - c + e| - r_c - r_o.
-// The hardware square root (at most 1 ulp) instead of the correctly rounded
-// one: this runs for every (link sphere, world sphere) pair of every waypoint.
-__device__ __forceinline__ float sphere_world_dist(const float4& c, const float4& o) {
-  const float e = 1e-6f;
-  const float dx = o.x - c.x + e, dy = o.y - c.y + e, dz = o.z - c.z + e;
-  return __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - c.w - o.w;  // v_sqrt_f32, <= 1 ulp
+// |o_e - c| - r_c for robot sphere c and a world sphere centre o_e already
+// shifted by e = 1e-6 per component (the world sphere's radius is subtracted
+// once the closest robot sphere is known).  The hardware square root (at most
+// 1 ulp) instead of the correctly rounded one: this runs for every (link
+// sphere, world sphere) pair of every waypoint.
+__device__ __forceinline__ float sphere_world_dist(const float4& c, const float4& oe) {
+  const float dx = oe.x - c.x, dy = oe.y - c.y, dz = oe.z - c.z;
+  return __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) - c.w;  // v_sqrt_f32, <= 1 ulp
 }
 
-// Direction o - c + e from sphere c to obstacle o, its length and the centre
-// (the same expressions as sphere_world_dist, so the same bits).
-__device__ __forceinline__ void closest_dir(const float4& c, const float4& o, float& bx, float& by, float& bz,
+// Direction o_e - c from sphere c to the shifted obstacle centre, its length
+// and the centre (the same expressions as sphere_world_dist, so the same bits).
+__device__ __forceinline__ void closest_dir(const float4& c, const float4& oe, float& bx, float& by, float& bz,
                                             float& bn, float4& bc) {
-  const float e = 1e-6f;
-  bx = o.x - c.x + e; by = o.y - c.y + e; bz = o.z - c.z + e;
+  bx = oe.x - c.x; by = oe.y - c.y; bz = oe.z - c.z;
   bn = __builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz);
   bc = c;
 }
@@ -1284,7 +1284,8 @@ __device__ __forceinline__ void group_smooth_min(const TrajModelDev& md, const T
 #pragma unroll
         for (int u = 0; u < kObs; ++u) {
           const int m = mb + u * kGroup;
-          o[u] = L.world[m < m1 ? m : mb];
+          const float4 w = L.world[m < m1 ? m : mb];
+          o[u] = make_float4(w.x + 1e-6f, w.y + 1e-6f, w.z + 1e-6f, w.w);  // centre + e
           best[u] = INFINITY;
           bs[u] = -1;
         }
@@ -1299,7 +1300,7 @@ __device__ __forceinline__ void group_smooth_min(const TrajModelDev& md, const T
 #pragma unroll
         for (int u = 0; u < kObs; ++u) {
           if (mb + u * kGroup >= m1) break;
-          const float d = best[u];
+          const float d = best[u] - o[u].w;
           if (bs[u] >= 0) closest_dir(gc[bs[u]], o[u], bx, by, bz, bn, bc);
           if (d < mloc) {
             const float r = (mloc < INFINITY) ? __expf((d - mloc) * inv_tau) : 0.f;
