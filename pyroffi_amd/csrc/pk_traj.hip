@@ -90,7 +90,22 @@ struct TrajModelDev {
   const float* upper;
   float margin;
   int mode;
+  unsigned a_magic;  // quot_magic(n_act)
 };
+
+// i / d for i >= 0 and i * d < 2^32 without a division: with m = ceil(2^32 / d)
+// the quotient is the high half of i * m (exact under that bound: the arrays
+// indexed this way live in LDS, at most 150 KiB, so i < 40 k, and d is the
+// joint count or the collision group count, both at most 64); d = 1 is the
+// identity.  The trajectory loops index element
+// i = t * A + a of a (T, A) array and need t and a per element.
+__host__ __device__ inline unsigned quot_magic(int d) {
+  return d > 1 ? static_cast<unsigned>((0x100000000ull + static_cast<unsigned>(d) - 1u) / static_cast<unsigned>(d))
+               : 0u;
+}
+__device__ __forceinline__ int quot(int i, int d, unsigned m) {
+  return d > 1 ? static_cast<int>(__umulhi(static_cast<unsigned>(i), m)) : i;
+}
 
 struct TrajWeights {
   float w_smooth, w_vel, w_acc, w_jerk, w_limits;
@@ -608,7 +623,7 @@ __device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const Tr
       });
   const int na = T - 4;
   for (int i = tid; i < n; i += blockDim.x) {
-    const int a = i % A;
+    const int a = i - quot(i, A, md.a_magic) * A;
     const float xv = X[i];
     const float v = fmaxf(xv - L.hi[a], 0.f) + fmaxf(L.lo[a] - xv, 0.f);
     lim += v * v;
@@ -627,7 +642,7 @@ __device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const Tr
   }
   __syncthreads();
   for (int i = tid; i < na * A; i += blockDim.x) {
-    const int t = i / A;
+    const int t = quot(i, A, md.a_magic);
     const float acc = L.vt[i];
     if (t + 1 < na) {
       const float jk = L.vt[i + A] - acc;
@@ -644,7 +659,7 @@ __device__ __forceinline__ void traj_value_grad(const TrajModelDev& md, const Tr
     __syncthreads();
     const float c5[5] = {-1.f / 12.f, 16.f / 12.f, -30.f / 12.f, 16.f / 12.f, -1.f / 12.f};
     for (int i = tid; i < n; i += blockDim.x) {
-      const int t = i / A;
+      const int t = quot(i, A, md.a_magic);
       float gs = 0.f;
       if (t >= 1) gs += 2.f * w.w_vel * (X[i] - X[i - A]);
       if (t + 1 < T) gs -= 2.f * w.w_vel * (X[i + A] - X[i]);
@@ -688,8 +703,8 @@ __device__ __forceinline__ int traj_line_search(const TrajModelDev& md, const Tr
 #pragma unroll
   for (int k = 0; k < 3 + kNumAlphas; ++k) v[k] = 0.f;
   for (int i = tid; i < n; i += blockDim.x) {
-    const int a = i % A;
-    const int t = i / A;
+    const int t = quot(i, A, md.a_magic);
+    const int a = i - t * A;
     const float xv = L.vx[i], dv = L.vd[i];
 #pragma unroll
     for (int k = 0; k < kNumAlphas; ++k) {
@@ -942,7 +957,7 @@ __global__ __launch_bounds__(kMaxTrajThreads) void traj_lbfgs_kernel(LbfgsArgs a
       for (int i = tid; i < n; i += blockDim.x) L.vd[i] = -L.vg[i] * inv;
     }
     for (int i = tid; i < n; i += blockDim.x) {
-      const int t = i / A;
+      const int t = quot(i, A, a.md.a_magic);
       if (t == 0 || t == T - 1) L.vd[i] = 0.f;
     }
     __syncthreads();
@@ -1059,7 +1074,7 @@ __global__ __launch_bounds__(kMaxTrajThreads) void traj_chomp_kernel(ChompArgs a
     const float gn = sqrtf(block_sum1(g2, L.red));
     const float cs = (a.clip > 0.f && gn > a.clip) ? a.clip / (gn + 1e-12f) : 1.f;
     for (int i = tid; i < n; i += blockDim.x) {
-      const int t = i / A;
+      const int t = quot(i, A, a.md.a_magic);
       L.vg[i] = (t == 0 || t == T - 1) ? 0.f : L.vg[i] * cs;
     }
     __syncthreads();
@@ -1424,6 +1439,7 @@ struct ScoInnerArgs {
   TrajWeights w;
   float w_coll, margin, w_trust;
   float* out_x;         // (W, A)
+  unsigned a_magic, g_magic;  // quot_magic(n_act), quot_magic(G)
 };
 
 __host__ __device__ inline size_t sco_inner_lds_floats(int max_len, int A, int G, int m) {
@@ -1443,7 +1459,7 @@ __device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const flo
   const int na = T - 4;
   float sm = 0.f, rest = 0.f;
   for (int i = tid; i < n; i += blockDim.x) {
-    const int c = i % A;
+    const int c = i - quot(i, A, a.a_magic) * A;
     const float xv = y[i];
     const float v = fmaxf(xv - hi[c], 0.f) + fmaxf(lo[c] - xv, 0.f);
     rest += a.w.w_limits * v * v;
@@ -1463,7 +1479,7 @@ __device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const flo
   }
   // Linearised hinge per (waypoint, group).
   for (int r = tid; r < T * G; r += blockDim.x) {
-    const int t = r / G;
+    const int t = quot(r, G, a.g_magic);
     const float* Jr = Jk + static_cast<long long>(r) * A;
     float dl = dk[r];
     for (int c = 0; c < A; ++c) dl += Jr[c] * (y[t * A + c] - qk[t * A + c]);
@@ -1473,7 +1489,7 @@ __device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const flo
   }
   __syncthreads();
   for (int i = tid; i < na * A; i += blockDim.x) {
-    const int t = i / A;
+    const int t = quot(i, A, a.a_magic);
     if (t + 1 < na) {
       const float jk = acc[i + A] - acc[i];
       sm += a.w.w_jerk * jk * jk;
@@ -1483,7 +1499,7 @@ __device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const flo
     // hinge gradient: each element sums its waypoint's row slopes times the
     // rows' Jacobian entries
     for (int i = tid; i < n; i += blockDim.x) {
-      const int t = i / A, c = i % A;
+      const int t = quot(i, A, a.a_magic), c = i - t * A;
       float gh = 0.f;
       for (int gg = 0; gg < G; ++gg) {
         const int r = t * G + gg;
@@ -1494,7 +1510,7 @@ __device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const flo
     __syncthreads();
     const float c5[5] = {-1.f / 12.f, 16.f / 12.f, -30.f / 12.f, 16.f / 12.f, -1.f / 12.f};
     for (int i = tid; i < n; i += blockDim.x) {
-      const int t = i / A;
+      const int t = quot(i, A, a.a_magic);
       float gs = 0.f;
       if (t >= 1) gs += 2.f * a.w.w_vel * (y[i] - y[i - A]);
       if (t + 1 < T) gs -= 2.f * a.w.w_vel * (y[i + A] - y[i]);
@@ -1624,7 +1640,7 @@ __global__ __launch_bounds__(256) void traj_sco_inner_kernel(ScoInnerArgs a) {
       for (int i = tid; i < n; i += blockDim.x) dir[i] = -g[i] * inv;
     }
     for (int i = tid; i < n; i += blockDim.x) {
-      const int t = i / A;
+      const int t = quot(i, A, a.a_magic);
       if (t == 0 || t == T - 1) dir[i] = 0.f;
     }
     __syncthreads();
@@ -1682,6 +1698,7 @@ int fill_traj_model(const pk_traj_model* m, pk::TrajModelDev& d, int& maxn) {
   d.pair_links = m->pair_links; d.pair_start = m->pair_start; d.pair_items = m->pair_items;
   d.n_pairs = m->n_pairs; d.n_items = m->n_items; d.world = m->world; d.n_world = m->n_world;
   d.lower = m->lower; d.upper = m->upper; d.margin = m->margin; d.mode = m->mode;
+  d.a_magic = pk::quot_magic(m->n_act);
   const int gs = pk::fill_grid_args(m->grids, m->n_grids, m->margin, d.grid);
   if (gs != PK_OK) return gs;
   const int width = std::max(m->n_nodes, m->n_act);
@@ -1931,6 +1948,8 @@ extern "C" int pk_traj_sco_inner_f32(void* stream, const float* x, const int64_t
   a.max_len = max_len;
   a.n_act = n_act;
   a.G = n_groups;
+  a.a_magic = pk::quot_magic(n_act);
+  a.g_magic = pk::quot_magic(n_groups);
   a.m_hist = cfg->m_lbfgs;
   a.n_inner = cfg->n_inner_iters;
   a.n_waypoints = n_waypoints;
