@@ -1535,7 +1535,7 @@ __device__ __forceinline__ float sco_inner_eval(const ScoInnerArgs& a, const flo
   return a.w.w_smooth * v2[0] + v2[1];
 }
 
-__global__ __launch_bounds__(256) void traj_sco_inner_kernel(ScoInnerArgs a) {
+__global__ __launch_bounds__(512) void traj_sco_inner_kernel(ScoInnerArgs a) {
   extern __shared__ float4 smem4[];
   const int b = blockIdx.x;
   if (b >= a.n_traj) return;  // uniform per block
@@ -1962,6 +1962,6 @@ extern "C" int pk_traj_sco_inner_f32(void* stream, const float* x, const int64_t
   a.margin = cfg->collision_margin;
   a.w_trust = cfg->w_trust;
   a.out_x = out_x;
-  hipLaunchKernelGGL(pk::traj_sco_inner_kernel, dim3(n_traj), dim3(256), lds, static_cast<hipStream_t>(stream), a);
+  hipLaunchKernelGGL(pk::traj_sco_inner_kernel, dim3(n_traj), dim3(512), lds, static_cast<hipStream_t>(stream), a);
   return pk::check_launch("traj_sco_inner_kernel");
 }
