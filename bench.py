@@ -245,7 +245,7 @@ def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32,
     n_seeds = int(sum(g.seeds.shape[0] for g in groups))
     solver = UnevenLsIkSolver(groups, max_iter=50)
     report = solver.autotune(reps=1, cache=False)
-    solver.capture()
+    launch_mode = solver.capture(keep_faster=True)
     solver.solve()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -264,6 +264,7 @@ def ik_uneven_bench(reps: int, per_robot: int = 1024, plain_per_robot: int = 32,
     padded_dt = time.perf_counter() - t0
     res = _uneven_report(groups, samples, winners, fused_dt, n_seeds, report, per_robot, plain_per_robot,
                          cpu_per_robot)
+    res["launch_mode"] = launch_mode  # "graph": replay of the captured graph; "eager": direct launches (faster here)
     res.update(_uneven_roofline(groups, work, fused_dt, solver.kw["max_iter"], list(UNEVEN_EE)))
     res.update(_padded_report(groups, winners, padded, padded_dt, batch_per_robot, per_robot,
                               res["fused_solves_per_s"]))

@@ -241,14 +241,33 @@ class UnevenLsIkSolver:
         saved[key] = list(lead.tune)
         return {"choice": lead.tune, "ms": {f"{k[0]}/{k[1]}": v * 1e3 for k, v in timings.items()}}
 
-    def capture(self) -> None:
-        """Capture every group's chunk launches and selections into one graph."""
+    def capture(self, keep_faster: bool = False, reps: int = 3) -> str:
+        """Capture every group's chunk launches and selections into one graph.
+
+        With ``keep_faster`` the replay is timed against the same launches made
+        directly and the graph is dropped when it is slower: the direct
+        launches start the groups longest-first, and a graph's replay need
+        not launch its independent branches in that order.  Returns the mode
+        ``solve`` will use, ``"graph"`` or ``"eager"``.
+        """
         self.run_eager()  # warm-up outside the capture
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self.run_eager()
         self.graph = g
+        if keep_faster and len(self.groups) > 1:
+            def timed(fn):
+                fn()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    fn()
+                torch.cuda.synchronize()
+                return time.perf_counter() - t0
+            if timed(self.run_eager) < timed(g.replay):
+                self.graph = None
+        return "graph" if self.graph is not None else "eager"
 
     def count_work(self) -> list[int]:
         """Per group, the sphere-pair tests its collision penalty executes in one

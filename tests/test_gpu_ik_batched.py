@@ -280,6 +280,12 @@ def test_autotune_lead_group_workgroups(oracle_mod):
     assert len(report["joint"]["ms"]) >= 2
     for (w, i), (rw, ri) in zip(solver.solve(), ref):
         assert torch.equal(i, ri) and torch.equal(w, rw)
+    # capture keeping the faster of graph replay and direct launches: either
+    # mode gives the same winners
+    mode = solver.capture(keep_faster=True)
+    assert mode in ("graph", "eager") and (solver.graph is not None) == (mode == "graph")
+    for (w, i), (rw, ri) in zip(solver.solve(), ref):
+        assert torch.equal(i, ri) and torch.equal(w, rw)
 
 
 def test_select_segments_matches_numpy():
