@@ -454,7 +454,8 @@ def trajopt_bench(reps: int, cpu_sample: int = 16):
 
 def chomp_bench(reps: int = 3, cpu_sample: int = 4):
     """CHOMP (chomp_trajopt's solver, default ChompTrajOptConfig: 100 iterations, covariant
-    update, hinge^2 margin 0.01) on the config-5 trajectories and world: one launch of
+    update, hinge^2 margin 0.01) on the 512-trajectory workload (T in {32, 64, 128}, 1024 world
+    spheres): one launch of
     pk_traj_chomp_f32 for all 512 trajectories, then the ranking costs."""
     import torch
 
@@ -508,7 +509,7 @@ def chomp_bench(reps: int = 3, cpu_sample: int = 4):
 
 def sco_bench(reps: int = 3, cpu_sample: int = 4):
     """SCO (sco_trajopt's solver, default ScoTrajOptConfig: 10 rounds of linearise + 30
-    inner L-BFGS steps) on the config-5 trajectories and world, one world group; the
+    inner L-BFGS steps) on the 512-trajectory workload and its 1024-sphere world, one world group; the
     20 launches and the ranking cost replay from one captured graph."""
     import torch
 
@@ -899,8 +900,9 @@ def load_traffic():
 
 
 def profile_workload(name: str):
-    """Short runs for rocprofv3: 20 adjoint launches of the config-3 workload (plus the copy
-    calibration), or a few solves of the config-4 or config-5 workload."""
+    """Short runs for rocprofv3: 20 adjoint launches of the pose + hinge cost workload (16,384
+    configurations, 512 world spheres; plus the copy calibration), or a few solves of the uneven
+    LM-IK workload (4096 targets over four robots) or of a 512-trajectory workload."""
     import torch
 
     if name == "grad":
@@ -973,7 +975,7 @@ def main():
     ap.add_argument("--profile", nargs="?", const="sweep", choices=("sweep", "grad", "ik", "ik_uneven", "trajopt", "chomp", "sco"),
                     help="run for rocprofv3: 'sweep' = the headline workload's 20 + 200 steps and 200 timed "
                          "launches of each kernel, plus a 256 MiB device copy (the byte-count calibration for the HBM counters); 'grad' = 20 launches "
-                         "of the config-3 adjoint kernel plus the same copy; 'ik' = 20 config-1 solve calls; "
+                         "of the pose + hinge adjoint kernel plus the same copy; 'ik' = 20 solve calls of 16 targets x 64 seeds; "
                          "'ik_uneven' / 'trajopt' / 'chomp' / 'sco' = 3 solves "
                          "of the uneven LM-IK / trajectory workload (L-BFGS, CHOMP, SCO)")
     args = ap.parse_args()
